@@ -1,0 +1,148 @@
+/*
+ * libqtt -- MI355X (gfx950) engine for the quantics tensor-network hot path of
+ * ITensorNumericalAnalysis.jl (JoeyT1994, v0.2.1).  Plain C ABI: no C++ or
+ * torch types cross it.  A Julia `ccall` shim and the Python ctypes harness
+ * both bind exactly these symbols.
+ *
+ * Network convention (identical for every entry point):
+ *   - a rooted tree of nv vertices renumbered so that parent[0] = -1 and
+ *     parent[v] < v (a chain is parent[v] = v - 1);
+ *   - vertex v carries at most one digit site of dimension sdim[v] (2, or 1
+ *     for a vertex without digit); the digit is number vdigit[v] (1 = most
+ *     significant) of coordinate vcoord[v] (0-based, -1 = none); coordinate c
+ *     has ndigits[c] digits;
+ *   - bond[v] is the dimension of edge (v, parent[v]); bond[0] = 1;
+ *   - core v is a dense row-major float64 array
+ *       [bond[v]][sdim[v]][bond[c1]]...[bond[cm]]   (children c1 < ... < cm);
+ *   - an operator core is [bond[v]][s_in][s_out][bond[c1]]... and acts as
+ *       phi[s_out] = sum_{s_in} O[s_in, s_out] psi[s_in].
+ * Julia/ITensors are column-major: the shim permutes each ITensor into this
+ * order.
+ *
+ * Status codes: QTT_OK, QTT_EINVAL (shape/topology/argument), QTT_ERANGE
+ * (a coordinate < 0 or NaN -- the reference never terminates there; x >= 1
+ * saturates every digit to 1 exactly as the reference), QTT_ENOMEM,
+ * QTT_EHIP (a HIP runtime error), QTT_ENOCONV (SVD did not converge).
+ * The message of the last failure on a context is qtt_last_error().
+ *
+ * Threading: one context owns one HIP stream; calls on a context are
+ * serialised by the caller.  Device pointers passed to the *_dev / async entry
+ * points must be valid on the context's device; host buffers are only touched
+ * during the call.
+ */
+#ifndef QTT_H
+#define QTT_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QTT_OK 0
+#define QTT_EINVAL 1
+#define QTT_ERANGE 2
+#define QTT_ENOMEM 3
+#define QTT_EHIP 4
+#define QTT_ENOCONV 5
+
+/* arithmetic of the bulk contractions (truncation is always FP64) */
+#define QTT_F64 0       /* float64 everywhere */
+#define QTT_F32_SPLIT 1 /* float32 hi/lo split operands, float-float accumulation */
+#define QTT_F32 2       /* plain float32 operands and accumulation */
+
+typedef struct qtt_ctx qtt_ctx;
+typedef struct qtt_net qtt_net;
+typedef struct qtt_mpo qtt_mpo;
+
+typedef struct {
+  int32_t nv;
+  const int32_t* parent;
+  const int32_t* sdim;
+  const int32_t* vcoord;
+  const int32_t* vdigit;
+  int32_t ncoord;
+  const int32_t* ndigits;
+} qtt_topology;
+
+/* ABI version (major * 100 + minor). */
+int qtt_version(void);
+
+/* Context on HIP device `device`. */
+int qtt_ctx_create(int device, qtt_ctx** out);
+void qtt_ctx_destroy(qtt_ctx* ctx);
+const char* qtt_last_error(const qtt_ctx* ctx);
+/* Run subsequent work on an existing HIP stream (hipStream_t, NULL = the
+ * context's own stream).  The caller keeps the stream alive. */
+int qtt_ctx_set_stream(qtt_ctx* ctx, void* hip_stream);
+/* Wait for queued work; reports QTT_ERANGE if an asynchronous evaluate met an
+ * invalid coordinate since the last sync. */
+int qtt_sync(qtt_ctx* ctx);
+
+/* ---- networks -----------------------------------------------------------
+ * Replaces the ITensorNetworkFunction storage (src/itensornetworkfunction.jl:18-57):
+ * cores are uploaded once and stay resident in HBM. */
+int qtt_net_create(qtt_ctx* ctx, const qtt_topology* topo, const int64_t* bond,
+                   const double* const* cores, qtt_net** out);
+int qtt_net_bonds(qtt_ctx* ctx, const qtt_net* net, int64_t* bond_out);
+int qtt_net_read(qtt_ctx* ctx, const qtt_net* net, double* const* cores_out);
+int qtt_net_copy(qtt_ctx* ctx, const qtt_net* net, qtt_net** out);
+void qtt_net_destroy(qtt_net* net);
+
+/* Operator networks (the TTN returned by ttn(::OpSum), src/elementary_operators.jl:151). */
+int qtt_mpo_create(qtt_ctx* ctx, const qtt_topology* topo, const int64_t* bond,
+                   const double* const* cores, qtt_mpo** out);
+void qtt_mpo_destroy(qtt_mpo* mpo);
+
+/* ---- evaluate ------------------------------------------------------------
+ * Replaces evaluate(f, xs, dims; alg) (src/itensornetworkfunction.jl:96-106:
+ * calculate_ind_values + project + scalar), batched over points.
+ * xs: device pointer, row-major [npts][ncoord]; out: device pointer [npts].
+ * Asynchronous on the context stream; invalid points yield NaN and a
+ * QTT_ERANGE at the next qtt_sync. */
+int qtt_eval(qtt_ctx* ctx, const qtt_net* net, const double* xs, int64_t npts, double* out,
+             int precision);
+/* Same with host buffers; synchronous, returns QTT_ERANGE directly. */
+int qtt_eval_host(qtt_ctx* ctx, const qtt_net* net, const double* xs, int64_t npts,
+                  double* out, int precision);
+
+/* ---- integrate / inner ---------------------------------------------------
+ * integrate(f; take_sum) (src/integration.jl:6-17): weight 0.5 per binary digit
+ * gives the left Riemann sum, weight 1.0 the plain sum.  Synchronous. */
+int qtt_integrate(qtt_ctx* ctx, const qtt_net* net, double weight, double* out);
+/* <a|op|b> with op = NULL meaning identity; integrate(op, f)
+ * (src/integration.jl:20-32) is qtt_integrate of qtt_apply(op, f) without
+ * truncation, or qtt_inner with a uniform product state.  Synchronous. */
+int qtt_inner(qtt_ctx* ctx, const qtt_net* a, const qtt_mpo* op, const qtt_net* b, double* out);
+
+/* ---- operate / truncate --------------------------------------------------
+ * operate(op, f; cutoff, maxdim) (src/elementary_operators.jl:294-299):
+ * exact operator application followed by the truncation sweep below.
+ * cutoff < 0 disables truncation (exact product, bonds multiply).
+ * *out receives a new device-resident network; trunc_err (nullable) the summed
+ * discarded relative weight. */
+int qtt_apply(qtt_ctx* ctx, const qtt_mpo* op, const qtt_net* net, double cutoff,
+              int64_t maxdim, int precision, qtt_net** out, double* trunc_err);
+/* truncate(f; cutoff, maxdim) (src/itensornetworkfunction.jl:114-118 ->
+ * ITensorNetworks.truncate): for every edge in post-order from vertex 0,
+ * orthogonalise toward the child (Householder QR moves), SVD the edge
+ * (one-sided Jacobi, FP64) and keep singular values by the ITensors rule on
+ * squared values (relative cutoff, then maxdim).  In place. */
+int qtt_truncate(qtt_ctx* ctx, qtt_net* net, double cutoff, int64_t maxdim, double* trunc_err);
+
+/* ---- adjoint (no reference counterpart: the rrule of evaluate) ------------
+ * grad = sum_m ybar[m] d f(x_m) / d cores, written as the concatenation of
+ * all cores in vertex order (same sizes as the network's cores).
+ * xs, ybar, grad: device pointers.  Asynchronous. */
+int qtt_eval_grad(qtt_ctx* ctx, const qtt_net* net, const double* xs, const double* ybar,
+                  int64_t npts, double* grad, int precision);
+/* Host-buffer variant, synchronous; grads[v] receives core v's gradient. */
+int qtt_eval_grad_host(qtt_ctx* ctx, const qtt_net* net, const double* xs, const double* ybar,
+                       int64_t npts, double* const* grads, int precision);
+
+/* number of float64 elements of all cores of a network (for gradient buffers) */
+int64_t qtt_net_size(const qtt_net* net);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,33 @@
+"""MI355X engine for the quantics tensor-network hot path of ITensorNumericalAnalysis.jl.
+
+Host-side API mirroring the reference package (src/ITensorNumericalAnalysis.jl:20-66):
+construction (continuous_siteinds, *_itn builders, operators) runs on the host;
+evaluate / integrate / inner / operate / truncate / gradients run as HIP kernels
+in libqtt.so (csrc/), called through its C-ABI (include/qtt.h).  There is no
+CPU fallback for those: without the built library they raise.
+"""
+from .graphs import NamedGraph, named_comb_tree, named_grid, named_path_graph
+from .indsnetworkmap import (DigitMapError, IndsNetworkMap, continuous_siteinds,
+                             default_dimension_vertices)
+from .network import ITensorNetworkFunction, Topology, TreeOperator
+from .builders import (add, const_itn, cos_itn, cosh_itn, exp_itn, multiply, random_itn, scale,
+                       sin_itn, sinh_itn)
+from .operators import (add_operators, backward_shift_op, first_derivative_operator,
+                        forward_shift_op, fourth_derivative_operator, fredholm_operator,
+                        identity_operator, laplacian_operator, second_derivative_operator, stencil,
+                        third_derivative_operator)
+from .api import (Precision, engine_context, eval_grad, evaluate, inner, integrate, on_stream, operate,
+                  truncate)
+
+real_continuous_siteinds = continuous_siteinds
+
+__all__ = [
+    "NamedGraph", "named_grid", "named_comb_tree", "named_path_graph", "IndsNetworkMap",
+    "continuous_siteinds", "real_continuous_siteinds", "default_dimension_vertices",
+    "DigitMapError", "ITensorNetworkFunction", "TreeOperator", "Topology", "const_itn", "exp_itn",
+    "sin_itn", "cos_itn", "cosh_itn", "sinh_itn", "random_itn", "add", "multiply", "scale",
+    "stencil", "forward_shift_op", "backward_shift_op", "first_derivative_operator",
+    "second_derivative_operator", "third_derivative_operator", "fourth_derivative_operator",
+    "laplacian_operator", "identity_operator", "fredholm_operator", "add_operators", "evaluate",
+    "integrate", "inner", "operate", "truncate", "eval_grad", "Precision", "engine_context",
+]

@@ -1,0 +1,189 @@
+"""The hot path: evaluate / integrate / inner / operate / truncate / eval_grad.
+
+Each call runs in libqtt on the GPU (there is no host fallback).  Names and
+argument meaning follow the reference:
+  evaluate(f, xs, dims)          src/itensornetworkfunction.jl:96-112
+  integrate(f; take_sum)         src/integration.jl:6-17
+  integrate(op, f; take_sum)     src/integration.jl:20-32
+  operate(op, f; cutoff, maxdim) src/elementary_operators.jl:294-319
+  truncate(f; cutoff, maxdim)    src/itensornetworkfunction.jl:114-118
+`evaluate` is batched: xs may be an (npts, ndims) array (or a torch CUDA tensor,
+in which case the values stay on the device).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from contextlib import contextmanager
+
+import numpy as np
+
+from . import _capi
+from .indsnetworkmap import DigitMapError
+from .network import ITensorNetworkFunction, TreeOperator
+
+
+class Precision:
+    F64 = 0        # float64 everywhere
+    F32_SPLIT = 1  # float32 (hi, lo) operands with float-float accumulation
+    F32 = 2        # plain float32
+
+
+def engine_context():
+    """The process-wide libqtt context (device 0)."""
+    return _capi.context()
+
+
+@contextmanager
+def on_stream(stream_ptr):
+    """Run engine work on an existing HIP stream (e.g. torch.cuda.Stream().cuda_stream)."""
+    ctx = engine_context()
+    ctx.set_stream(stream_ptr)
+    try:
+        yield ctx
+    finally:
+        ctx.set_stream(None)
+
+
+def device_network(f: ITensorNetworkFunction) -> _capi.DeviceNetwork:
+    """Upload f's cores once and keep the handle on f."""
+    if f.device is None:
+        f.attach(_capi.DeviceNetwork.upload(engine_context(), f.topo, f.cores))
+    return f.device
+
+
+def device_operator(op: TreeOperator) -> _capi.DeviceOperator:
+    if op._device is None:
+        op._device = _capi.DeviceOperator(engine_context(), op.topo, op.cores)
+    return op._device
+
+
+def _points(f, xs, dims):
+    nd = f.dimension()
+    a = np.asarray(xs, dtype=np.float64)
+    if a.ndim == 0:
+        a = a.reshape(1, 1)
+        single = True
+    elif a.ndim == 1:
+        a = a.reshape(1, -1)
+        single = True
+    else:
+        single = False
+    if dims is not None:
+        dims = list(dims)
+        if sorted(dims) != list(range(1, nd + 1)):
+            raise ValueError("dims must list every dimension of the function once")
+        full = np.empty((a.shape[0], nd))
+        for j, d in enumerate(dims):
+            full[:, d - 1] = a[:, j]
+        a = full
+    if a.shape[1] != nd:
+        raise ValueError(f"points have {a.shape[1]} coordinates, the function has {nd} dimensions")
+    return np.ascontiguousarray(a), single
+
+
+def evaluate(f: ITensorNetworkFunction, xs, dims=None, precision: int = Precision.F64, out=None):
+    """f at one point (scalar result) or at a batch of points (array result)."""
+    dn = device_network(f)
+    ctx = dn.ctx
+    try:
+        import torch
+        is_torch = isinstance(xs, torch.Tensor)
+    except ImportError:  # pragma: no cover
+        is_torch = False
+    if is_torch:
+        if not xs.is_cuda or xs.dtype != torch.float64 or xs.dim() != 2 or not xs.is_contiguous():
+            raise ValueError("device points must be a contiguous float64 CUDA tensor [npts, ndims]")
+        if xs.shape[1] != f.dimension():
+            raise ValueError("wrong number of coordinates")
+        n = xs.shape[0]
+        if out is None:
+            out = torch.empty(n, dtype=torch.float64, device=xs.device)
+        ctx.check(_capi.lib().qtt_eval(ctx.handle, dn.handle, C.c_void_p(xs.data_ptr()), n,
+                                       C.c_void_p(out.data_ptr()), int(precision)))
+        return out
+    pts, single = _points(f, xs, dims)
+    res = np.empty(pts.shape[0])
+    rc = _capi.lib().qtt_eval_host(ctx.handle, dn.handle, _capi._dptr(pts), pts.shape[0],
+                                   _capi._dptr(res), int(precision))
+    if rc == _capi.QTT_ERANGE:
+        raise DigitMapError("a coordinate is negative or NaN (the digit map is defined for x >= 0)")
+    ctx.check(rc)
+    return float(res[0]) if single else res
+
+
+def _sigma(f: ITensorNetworkFunction, take_sum: bool):
+    """The uniform product state used by integrate(op, f) (src/integration.jl:30)."""
+    c = 1.0 if take_sum else 1.0 / f.s.base
+    cores = [np.full(f.topo.core_shape(v, np.ones(f.topo.nv, dtype=np.int64)), c)
+             for v in range(f.topo.nv)]
+    return ITensorNetworkFunction(f.s, cores, f.topo)
+
+
+def integrate(op_or_f, f: ITensorNetworkFunction | None = None, take_sum: bool = False):
+    """integrate(f) or integrate(op, f)."""
+    if f is None:
+        f = op_or_f
+        dn = device_network(f)
+        out = C.c_double(0.0)
+        weight = 1.0 if take_sum else 1.0 / f.s.base
+        dn.ctx.check(_capi.lib().qtt_integrate(dn.ctx.handle, dn.handle, weight, C.byref(out)))
+        return out.value
+    return inner(_sigma(f, take_sum), f, op=op_or_f)
+
+
+def inner(a: ITensorNetworkFunction, b: ITensorNetworkFunction, op: TreeOperator | None = None):
+    """<a|op|b> (op = None: <a|b>)."""
+    da, db = device_network(a), device_network(b)
+    dop = device_operator(op) if op is not None else None
+    out = C.c_double(0.0)
+    da.ctx.check(_capi.lib().qtt_inner(da.ctx.handle, da.handle, dop.handle if dop else None, db.handle,
+                                       C.byref(out)))
+    return out.value
+
+
+def _cut(cutoff):
+    return 0.0 if cutoff is None else float(cutoff)
+
+
+def operate(op, f: ITensorNetworkFunction, cutoff=None, maxdim=None, precision: int = Precision.F64,
+            truncate: bool = True, return_error: bool = False):
+    """op applied to f, truncated with (cutoff, maxdim); a list of operators is
+    applied in order (src/elementary_operators.jl:305-319)."""
+    if isinstance(op, (list, tuple)):
+        g = f
+        err = 0.0
+        for o in op:
+            g, e = operate(o, g, cutoff, maxdim, precision, truncate, True)
+            err += e
+        return (g, err) if return_error else g
+    dn = device_network(f)
+    dop = device_operator(op)
+    h = C.c_void_p()
+    err = C.c_double(0.0)
+    cut = _cut(cutoff) if truncate else -1.0
+    dn.ctx.check(_capi.lib().qtt_apply(dn.ctx.handle, dop.handle, dn.handle, cut, int(maxdim or 0),
+                                       int(precision), C.byref(h), C.byref(err)))
+    g = ITensorNetworkFunction(f.s, None, f.topo, device=_capi.DeviceNetwork(dn.ctx, f.topo, h))
+    return (g, err.value) if return_error else g
+
+
+def truncate(f: ITensorNetworkFunction, cutoff=None, maxdim=None, return_error: bool = False):
+    """A truncated copy of f (the input is left unchanged)."""
+    dn = device_network(f).copy()
+    err = C.c_double(0.0)
+    dn.ctx.check(_capi.lib().qtt_truncate(dn.ctx.handle, dn.handle, _cut(cutoff), int(maxdim or 0),
+                                          C.byref(err)))
+    g = ITensorNetworkFunction(f.s, None, f.topo, device=dn)
+    return (g, err.value) if return_error else g
+
+
+def eval_grad(f: ITensorNetworkFunction, xs, ybar, precision: int = Precision.F64):
+    """Cotangent-weighted gradient sum_m ybar[m] d f(x_m) / d cores (list of arrays)."""
+    dn = device_network(f)
+    pts, _ = _points(f, xs, None)
+    yb = np.ascontiguousarray(np.broadcast_to(np.asarray(ybar, dtype=np.float64), (pts.shape[0],)))
+    b = dn.bonds()
+    grads = [np.empty(f.topo.core_shape(v, b)) for v in range(f.topo.nv)]
+    dn.ctx.check(_capi.lib().qtt_eval_grad_host(dn.ctx.handle, dn.handle, _capi._dptr(pts), _capi._dptr(yb),
+                                                pts.shape[0], _capi._dpp(grads), int(precision)))
+    return grads

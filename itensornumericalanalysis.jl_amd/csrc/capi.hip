@@ -1,0 +1,582 @@
+// C-ABI of libqtt: contexts, device-resident networks and operators, and the
+// entry points declared in include/qtt.h.  Host-side C++; the arithmetic runs
+// in the kernels of eval.hip, sweep.hip, factor.hip and grad.hip.
+#include <algorithm>
+#include <cstring>
+#include <functional>
+#include <new>
+
+#include "qtt_internal.h"
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+int qtt_fail(qtt_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+int qtt_hip_check(qtt_ctx* ctx, hipError_t e, const char* what) {
+  if (e == hipSuccess) return QTT_OK;
+  if (e == hipErrorOutOfMemory) {
+    (void)hipGetLastError();
+    return qtt_fail(ctx, QTT_ENOMEM, std::string(what) + ": out of device memory");
+  }
+  return qtt_fail(ctx, QTT_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(ctx, expr)                                   \
+  do {                                                      \
+    int _rc = qtt_hip_check((ctx), (expr), #expr);          \
+    if (_rc) return _rc;                                    \
+  } while (0)
+
+double* qtt_scratch(qtt_ctx* ctx, size_t bytes) {
+  if (bytes <= ctx->scratch_bytes) return ctx->d_scratch;
+  if (ctx->d_scratch) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(ctx->d_scratch);
+    ctx->d_scratch = nullptr;
+    ctx->scratch_bytes = 0;
+  }
+  size_t sz = std::max<size_t>(bytes, size_t(1) << 20);
+  if (hipMalloc(&ctx->d_scratch, sz) != hipSuccess) {
+    (void)hipGetLastError();
+    ctx->d_scratch = nullptr;
+    return nullptr;
+  }
+  ctx->scratch_bytes = sz;
+  return ctx->d_scratch;
+}
+
+// ---------------------------------------------------------------------------
+// topology
+// ---------------------------------------------------------------------------
+static int build_topo(qtt_ctx* ctx, const qtt_topology* tp, HostTopo& t) {
+  if (!tp || tp->nv < 1 || !tp->parent || !tp->sdim || !tp->vcoord || !tp->vdigit)
+    return qtt_fail(ctx, QTT_EINVAL, "topology: missing arrays or nv < 1");
+  if (tp->ncoord < 0 || (tp->ncoord > 0 && !tp->ndigits))
+    return qtt_fail(ctx, QTT_EINVAL, "topology: bad ncoord");
+  int nv = tp->nv;
+  t.nv = nv;
+  t.ncoord = tp->ncoord;
+  t.parent.assign(tp->parent, tp->parent + nv);
+  t.sdim.assign(tp->sdim, tp->sdim + nv);
+  t.vcoord.assign(tp->vcoord, tp->vcoord + nv);
+  t.vdigit.assign(tp->vdigit, tp->vdigit + nv);
+  t.ndigits.assign(tp->ndigits, tp->ndigits + tp->ncoord);
+  if (t.parent[0] != -1) return qtt_fail(ctx, QTT_EINVAL, "topology: parent[0] must be -1");
+  for (int v = 1; v < nv; v++)
+    if (t.parent[v] < 0 || t.parent[v] >= v)
+      return qtt_fail(ctx, QTT_EINVAL, "topology: parent[v] must satisfy 0 <= parent[v] < v");
+  for (int c = 0; c < t.ncoord; c++)
+    if (t.ndigits[c] < 1 || t.ndigits[c] > 62)
+      return qtt_fail(ctx, QTT_EINVAL, "topology: digits per coordinate must be in 1..62");
+  for (int v = 0; v < nv; v++) {
+    if (t.sdim[v] < 1) return qtt_fail(ctx, QTT_EINVAL, "topology: sdim < 1");
+    if (t.vcoord[v] >= 0) {
+      if (t.vcoord[v] >= t.ncoord || t.vdigit[v] < 1 || t.vdigit[v] > t.ndigits[t.vcoord[v]])
+        return qtt_fail(ctx, QTT_EINVAL, "topology: digit out of range");
+      if (t.sdim[v] != 2) return qtt_fail(ctx, QTT_EINVAL, "topology: digit sites must be binary");
+    }
+  }
+  t.nch.assign(nv, 0);
+  for (int v = 1; v < nv; v++) t.nch[t.parent[v]]++;
+  t.choff.assign(nv, 0);
+  int o = 0;
+  for (int v = 0; v < nv; v++) {
+    t.choff[v] = o;
+    o += t.nch[v];
+    if (2 + t.nch[v] > QTT_MAX_NDIM) return qtt_fail(ctx, QTT_EINVAL, "topology: vertex degree too large");
+  }
+  t.chs.assign(std::max(o, 1), 0);
+  std::vector<int> fill(nv, 0);
+  for (int v = 1; v < nv; v++) {
+    int p = t.parent[v];
+    t.chs[t.choff[p] + fill[p]++] = v;
+  }
+  t.chain = true;
+  for (int v = 1; v < nv; v++)
+    if (t.parent[v] != v - 1) t.chain = false;
+  return QTT_OK;
+}
+
+static bool same_topo(const HostTopo& a, const HostTopo& b) {
+  return a.nv == b.nv && a.parent == b.parent && a.sdim == b.sdim && a.vcoord == b.vcoord &&
+         a.vdigit == b.vdigit;
+}
+
+static int upload_topo(qtt_ctx* ctx, const HostTopo& t, int** d_topo) {
+  int nv = t.nv;
+  size_t n = size_t(7) * nv + std::max(t.ncoord, 1);
+  std::vector<int> h(n, 0);
+  std::copy(t.parent.begin(), t.parent.end(), h.begin() + 0 * nv);
+  std::copy(t.sdim.begin(), t.sdim.end(), h.begin() + 1 * nv);
+  std::copy(t.vcoord.begin(), t.vcoord.end(), h.begin() + 2 * nv);
+  std::copy(t.vdigit.begin(), t.vdigit.end(), h.begin() + 3 * nv);
+  std::copy(t.nch.begin(), t.nch.end(), h.begin() + 4 * nv);
+  std::copy(t.choff.begin(), t.choff.end(), h.begin() + 5 * nv);
+  std::copy(t.chs.begin(), t.chs.begin() + std::max(0, nv - 1), h.begin() + 6 * nv);
+  std::copy(t.ndigits.begin(), t.ndigits.end(), h.begin() + 7 * nv);
+  HIPCHK(ctx, hipMalloc(d_topo, n * sizeof(int)));
+  HIPCHK(ctx, hipMemcpy(*d_topo, h.data(), n * sizeof(int), hipMemcpyHostToDevice));
+  return QTT_OK;
+}
+
+static TopoView make_view(const int* d, const HostTopo& t) {
+  TopoView v;
+  int nv = t.nv;
+  v.parent = d;
+  v.sdim = d + nv;
+  v.vcoord = d + 2 * nv;
+  v.vdigit = d + 3 * nv;
+  v.nch = d + 4 * nv;
+  v.choff = d + 5 * nv;
+  v.chs = d + 6 * nv;
+  v.ndigits = d + 7 * nv;
+  v.nv = nv;
+  v.ncoord = t.ncoord;
+  return v;
+}
+
+TopoView qtt_net::view() const { return make_view(d_topo, t); }
+
+int64_t qtt_net::core_size(int v) const {
+  int64_t s = bond[v] * t.sdim[v];
+  for (int k = 0; k < t.nch[v]; k++) s *= bond[t.chs[t.choff[v] + k]];
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// truncation schedule: post-order edges from vertex 0; orthogonalise toward
+// the first edge source from no gauge (farthest vertices first), then for each
+// edge (c -> parent) move the centre to c along the tree path and SVD the edge.
+// ---------------------------------------------------------------------------
+std::vector<int> truncation_schedule(const HostTopo& t) {
+  std::vector<int> steps;
+  int nv = t.nv;
+  if (nv < 2) return steps;
+  std::vector<int> src;
+  std::vector<int> stack{0}, it(nv, 0);
+  while (!stack.empty()) {
+    int v = stack.back();
+    if (it[v] < t.nch[v]) {
+      stack.push_back(t.chs[t.choff[v] + it[v]++]);
+    } else {
+      stack.pop_back();
+      if (v != 0) src.push_back(v);
+    }
+  }
+  int target = src[0];
+  std::vector<int> order{target}, via(nv, -1), seen(nv, 0);
+  seen[target] = 1;
+  for (size_t h = 0; h < order.size(); h++) {
+    int v = order[h];
+    std::vector<int> nb;
+    if (t.parent[v] >= 0) nb.push_back(t.parent[v]);
+    for (int k = 0; k < t.nch[v]; k++) nb.push_back(t.chs[t.choff[v] + k]);
+    for (int u : nb)
+      if (!seen[u]) {
+        seen[u] = 1;
+        via[u] = v;
+        order.push_back(u);
+      }
+  }
+  for (int i = nv - 1; i >= 1; i--) {
+    steps.push_back(order[i]);
+    steps.push_back(via[order[i]]);
+    steps.push_back(0);
+  }
+  int center = target;
+  for (int c : src) {
+    while (center != c) {
+      int x = c;
+      bool below = false;
+      while (x != -1) {
+        if (t.parent[x] == center) {
+          below = true;
+          break;
+        }
+        x = t.parent[x];
+      }
+      int step = below ? x : t.parent[center];
+      steps.push_back(center);
+      steps.push_back(step);
+      steps.push_back(0);
+      center = step;
+    }
+    steps.push_back(c);
+    steps.push_back(t.parent[c]);
+    steps.push_back(1);
+    center = t.parent[c];
+  }
+  return steps;
+}
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int qtt_version(void) { return 100; }
+
+int qtt_ctx_create(int device, qtt_ctx** out) {
+  if (!out) return QTT_EINVAL;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) {
+    (void)hipGetLastError();
+    return QTT_EHIP;
+  }
+  qtt_ctx* c = new (std::nothrow) qtt_ctx;
+  if (!c) return QTT_ENOMEM;
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->own_stream) != hipSuccess ||
+      hipMalloc(&c->d_flag, 4 * sizeof(int)) != hipSuccess ||
+      hipMemset(c->d_flag, 0, 4 * sizeof(int)) != hipSuccess) {
+    (void)hipGetLastError();
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+    delete c;
+    return QTT_EHIP;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cus = prop.multiProcessorCount;
+  c->stream = c->own_stream;
+  *out = c;
+  return QTT_OK;
+}
+
+void qtt_ctx_destroy(qtt_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->d_scratch) (void)hipFree(ctx->d_scratch);
+  if (ctx->d_flag) (void)hipFree(ctx->d_flag);
+  if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+  delete ctx;
+}
+
+const char* qtt_last_error(const qtt_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int qtt_ctx_set_stream(qtt_ctx* ctx, void* s) {
+  if (!ctx) return QTT_EINVAL;
+  ctx->stream = s ? (hipStream_t)s : ctx->own_stream;
+  return QTT_OK;
+}
+
+int qtt_sync(qtt_ctx* ctx) {
+  if (!ctx) return QTT_EINVAL;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  int h[4];
+  HIPCHK(ctx, hipMemcpy(h, ctx->d_flag, sizeof(h), hipMemcpyDeviceToHost));
+  if (h[0] || h[1]) {
+    HIPCHK(ctx, hipMemset(ctx->d_flag, 0, sizeof(h)));
+    if (h[1]) return qtt_fail(ctx, QTT_ENOCONV, "SVD did not converge");
+    return qtt_fail(ctx, QTT_ERANGE, "a coordinate was negative or NaN");
+  }
+  return QTT_OK;
+}
+
+static int alloc_net(qtt_ctx* ctx, const HostTopo& t, const std::vector<int64_t>& bond,
+                     const std::vector<int64_t>& cap, qtt_net** out) {
+  qtt_net* n = new (std::nothrow) qtt_net;
+  if (!n) return qtt_fail(ctx, QTT_ENOMEM, "host allocation");
+  n->ctx = ctx;
+  n->t = t;
+  n->bond = bond;
+  n->cap = cap;
+  n->off.resize(t.nv);
+  int64_t o = 0;
+  for (int v = 0; v < t.nv; v++) {
+    n->off[v] = o;
+    o += (cap[v] + 1) & ~int64_t(1);  // 16-byte aligned cores
+  }
+  n->total = o;
+  int rc = QTT_OK;
+  std::vector<int> b32(t.nv);
+  for (int v = 0; v < t.nv; v++) b32[v] = (int)bond[v];
+  if ((rc = qtt_hip_check(ctx, hipMalloc(&n->d_data, std::max<int64_t>(o, 2) * sizeof(double)), "hipMalloc cores")) ||
+      (rc = qtt_hip_check(ctx, hipMalloc(&n->d_off, t.nv * sizeof(int64_t)), "hipMalloc offsets")) ||
+      (rc = qtt_hip_check(ctx, hipMalloc(&n->d_bond, t.nv * sizeof(int)), "hipMalloc bonds")) ||
+      (rc = upload_topo(ctx, t, &n->d_topo)) ||
+      (rc = qtt_hip_check(ctx, hipMemcpy(n->d_off, n->off.data(), t.nv * sizeof(int64_t), hipMemcpyHostToDevice), "offsets")) ||
+      (rc = qtt_hip_check(ctx, hipMemcpy(n->d_bond, b32.data(), t.nv * sizeof(int), hipMemcpyHostToDevice), "bonds"))) {
+    qtt_net_destroy(n);
+    return rc;
+  }
+  *out = n;
+  return QTT_OK;
+}
+
+int qtt_net_create(qtt_ctx* ctx, const qtt_topology* topo, const int64_t* bond,
+                   const double* const* cores, qtt_net** out) {
+  if (!ctx || !out || !bond || !cores) return qtt_fail(ctx, QTT_EINVAL, "null argument");
+  *out = nullptr;
+  HostTopo t;
+  int rc = build_topo(ctx, topo, t);
+  if (rc) return rc;
+  if (bond[0] != 1) return qtt_fail(ctx, QTT_EINVAL, "bond[0] must be 1");
+  std::vector<int64_t> b(bond, bond + t.nv), cap(t.nv);
+  for (int v = 0; v < t.nv; v++) {
+    if (b[v] < 1 || b[v] > (1 << 20)) return qtt_fail(ctx, QTT_EINVAL, "bond out of range");
+    if (!cores[v]) return qtt_fail(ctx, QTT_EINVAL, "null core");
+  }
+  for (int v = 0; v < t.nv; v++) {
+    int64_t s = b[v] * t.sdim[v];
+    for (int k = 0; k < t.nch[v]; k++) s *= b[t.chs[t.choff[v] + k]];
+    cap[v] = s;
+  }
+  qtt_net* n = nullptr;
+  if ((rc = alloc_net(ctx, t, b, cap, &n))) return rc;
+  for (int v = 0; v < t.nv; v++) {
+    rc = qtt_hip_check(ctx, hipMemcpy(n->d_data + n->off[v], cores[v], cap[v] * sizeof(double), hipMemcpyHostToDevice), "upload core");
+    if (rc) {
+      qtt_net_destroy(n);
+      return rc;
+    }
+  }
+  *out = n;
+  return QTT_OK;
+}
+
+int qtt_net_bonds(qtt_ctx* ctx, const qtt_net* net, int64_t* bond_out) {
+  if (!net || !bond_out) return qtt_fail(ctx, QTT_EINVAL, "null argument");
+  std::copy(net->bond.begin(), net->bond.end(), bond_out);
+  return QTT_OK;
+}
+
+int64_t qtt_net_size(const qtt_net* net) {
+  if (!net) return 0;
+  int64_t s = 0;
+  for (int v = 0; v < net->t.nv; v++) s += net->core_size(v);
+  return s;
+}
+
+int qtt_net_read(qtt_ctx* ctx, const qtt_net* net, double* const* cores_out) {
+  if (!ctx || !net || !cores_out) return qtt_fail(ctx, QTT_EINVAL, "null argument");
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  for (int v = 0; v < net->t.nv; v++)
+    HIPCHK(ctx, hipMemcpy(cores_out[v], net->d_data + net->off[v], net->core_size(v) * sizeof(double),
+                          hipMemcpyDeviceToHost));
+  return QTT_OK;
+}
+
+int qtt_net_copy(qtt_ctx* ctx, const qtt_net* net, qtt_net** out) {
+  if (!ctx || !net || !out) return qtt_fail(ctx, QTT_EINVAL, "null argument");
+  qtt_net* n = nullptr;
+  int rc = alloc_net(ctx, net->t, net->bond, net->cap, &n);
+  if (rc) return rc;
+  rc = qtt_hip_check(ctx, hipMemcpyAsync(n->d_data, net->d_data, net->total * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream), "copy cores");
+  if (!rc) rc = qtt_hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+  if (rc) {
+    qtt_net_destroy(n);
+    return rc;
+  }
+  *out = n;
+  return QTT_OK;
+}
+
+void qtt_net_destroy(qtt_net* n) {
+  if (!n) return;
+  if (n->ctx) (void)hipStreamSynchronize(n->ctx->stream);
+  if (n->d_data) (void)hipFree(n->d_data);
+  if (n->d_off) (void)hipFree(n->d_off);
+  if (n->d_topo) (void)hipFree(n->d_topo);
+  if (n->d_bond) (void)hipFree(n->d_bond);
+  if (n->d_pack) (void)hipFree(n->d_pack);
+  if (n->d_pack_off) (void)hipFree(n->d_pack_off);
+  if (n->d_pack32) (void)hipFree(n->d_pack32);
+  delete n;
+}
+
+int qtt_mpo_create(qtt_ctx* ctx, const qtt_topology* topo, const int64_t* bond,
+                   const double* const* cores, qtt_mpo** out) {
+  if (!ctx || !out || !bond || !cores) return qtt_fail(ctx, QTT_EINVAL, "null argument");
+  *out = nullptr;
+  qtt_mpo* m = new (std::nothrow) qtt_mpo;
+  if (!m) return qtt_fail(ctx, QTT_ENOMEM, "host allocation");
+  int rc = build_topo(ctx, topo, m->t);
+  if (rc || bond[0] != 1) {
+    delete m;
+    return rc ? rc : qtt_fail(ctx, QTT_EINVAL, "bond[0] must be 1");
+  }
+  m->ctx = ctx;
+  int nv = m->t.nv;
+  m->bond.assign(bond, bond + nv);
+  m->off.resize(nv);
+  int64_t o = 0;
+  std::vector<int64_t> sz(nv);
+  for (int v = 0; v < nv; v++) {
+    if (m->bond[v] < 1) {
+      delete m;
+      return qtt_fail(ctx, QTT_EINVAL, "operator bond < 1");
+    }
+    int64_t s = m->bond[v] * m->t.sdim[v] * m->t.sdim[v];
+    for (int k = 0; k < m->t.nch[v]; k++) s *= m->bond[m->t.chs[m->t.choff[v] + k]];
+    sz[v] = s;
+    m->off[v] = o;
+    o += (s + 1) & ~int64_t(1);
+  }
+  m->total = o;
+  std::vector<int> b32(nv);
+  for (int v = 0; v < nv; v++) b32[v] = (int)m->bond[v];
+  if ((rc = qtt_hip_check(ctx, hipMalloc(&m->d_data, std::max<int64_t>(o, 2) * sizeof(double)), "hipMalloc operator")) ||
+      (rc = qtt_hip_check(ctx, hipMalloc(&m->d_off, nv * sizeof(int64_t)), "hipMalloc")) ||
+      (rc = qtt_hip_check(ctx, hipMalloc(&m->d_bond, nv * sizeof(int)), "hipMalloc")) ||
+      (rc = qtt_hip_check(ctx, hipMemcpy(m->d_off, m->off.data(), nv * sizeof(int64_t), hipMemcpyHostToDevice), "upload")) ||
+      (rc = qtt_hip_check(ctx, hipMemcpy(m->d_bond, b32.data(), nv * sizeof(int), hipMemcpyHostToDevice), "upload"))) {
+    qtt_mpo_destroy(m);
+    return rc;
+  }
+  for (int v = 0; v < nv; v++) {
+    if (!cores[v]) {
+      qtt_mpo_destroy(m);
+      return qtt_fail(ctx, QTT_EINVAL, "null operator core");
+    }
+    rc = qtt_hip_check(ctx, hipMemcpy(m->d_data + m->off[v], cores[v], sz[v] * sizeof(double), hipMemcpyHostToDevice), "upload operator core");
+    if (rc) {
+      qtt_mpo_destroy(m);
+      return rc;
+    }
+  }
+  *out = m;
+  return QTT_OK;
+}
+
+void qtt_mpo_destroy(qtt_mpo* m) {
+  if (!m) return;
+  if (m->ctx) (void)hipStreamSynchronize(m->ctx->stream);
+  if (m->d_data) (void)hipFree(m->d_data);
+  if (m->d_off) (void)hipFree(m->d_off);
+  if (m->d_bond) (void)hipFree(m->d_bond);
+  delete m;
+}
+
+static int check_precision(qtt_ctx* ctx, int precision) {
+  if (precision != QTT_F64 && precision != QTT_F32_SPLIT && precision != QTT_F32)
+    return qtt_fail(ctx, QTT_EINVAL, "unknown precision");
+  return QTT_OK;
+}
+
+int qtt_eval(qtt_ctx* ctx, const qtt_net* net, const double* xs, int64_t npts, double* out,
+             int precision) {
+  if (!ctx || !net || (!xs && npts) || (!out && npts) || npts < 0)
+    return qtt_fail(ctx, QTT_EINVAL, "null argument");
+  int rc = check_precision(ctx, precision);
+  if (rc) return rc;
+  if (npts == 0) return QTT_OK;
+  return launch_eval(ctx, const_cast<qtt_net*>(net), xs, npts, out, precision);
+}
+
+int qtt_eval_host(qtt_ctx* ctx, const qtt_net* net, const double* xs, int64_t npts, double* out,
+                  int precision) {
+  if (!ctx || !net || (!xs && npts) || (!out && npts) || npts < 0)
+    return qtt_fail(ctx, QTT_EINVAL, "null argument");
+  if (npts == 0) return QTT_OK;
+  int nc = std::max(net->t.ncoord, 1);
+  double* d = nullptr;
+  HIPCHK(ctx, hipMalloc(&d, size_t(npts) * (nc + 1) * sizeof(double)));
+  int rc = qtt_hip_check(ctx, hipMemcpyAsync(d, xs, size_t(npts) * nc * sizeof(double), hipMemcpyHostToDevice, ctx->stream), "upload points");
+  if (!rc) rc = qtt_eval(ctx, net, d, npts, d + size_t(npts) * nc, precision);
+  if (!rc) rc = qtt_hip_check(ctx, hipMemcpyAsync(out, d + size_t(npts) * nc, npts * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "download values");
+  int rs = qtt_sync(ctx);
+  (void)hipFree(d);
+  return rc ? rc : rs;
+}
+
+int qtt_integrate(qtt_ctx* ctx, const qtt_net* net, double weight, double* out) {
+  if (!ctx || !net || !out) return qtt_fail(ctx, QTT_EINVAL, "null argument");
+  return launch_integrate(ctx, net, weight, out);
+}
+
+int qtt_inner(qtt_ctx* ctx, const qtt_net* a, const qtt_mpo* op, const qtt_net* b, double* out) {
+  if (!ctx || !a || !b || !out) return qtt_fail(ctx, QTT_EINVAL, "null argument");
+  if (!same_topo(a->t, b->t)) return qtt_fail(ctx, QTT_EINVAL, "inner: networks on different layouts");
+  if (!op) return launch_inner(ctx, a, b, out);
+  qtt_net* wb = nullptr;
+  int rc = qtt_apply(ctx, op, b, -1.0, 0, QTT_F64, &wb, nullptr);
+  if (rc) return rc;
+  rc = launch_inner(ctx, a, wb, out);
+  qtt_net_destroy(wb);
+  return rc;
+}
+
+int qtt_apply(qtt_ctx* ctx, const qtt_mpo* op, const qtt_net* net, double cutoff, int64_t maxdim,
+              int precision, qtt_net** out, double* trunc_err) {
+  if (!ctx || !op || !net || !out) return qtt_fail(ctx, QTT_EINVAL, "null argument");
+  *out = nullptr;
+  int rc = check_precision(ctx, precision);
+  if (rc) return rc;
+  if (!same_topo(op->t, net->t)) return qtt_fail(ctx, QTT_EINVAL, "apply: operator and network layouts differ");
+  const HostTopo& t = net->t;
+  std::vector<int64_t> b(t.nv), cap(t.nv);
+  for (int v = 0; v < t.nv; v++) b[v] = op->bond[v] * net->bond[v];
+  for (int v = 0; v < t.nv; v++) {
+    int64_t s = b[v] * t.sdim[v];
+    for (int k = 0; k < t.nch[v]; k++) s *= b[t.chs[t.choff[v] + k]];
+    cap[v] = s;
+  }
+  qtt_net* n = nullptr;
+  if ((rc = alloc_net(ctx, t, b, cap, &n))) return rc;
+  if ((rc = launch_contract(ctx, op, net, n))) {
+    qtt_net_destroy(n);
+    return rc;
+  }
+  if (trunc_err) *trunc_err = 0.0;
+  if (cutoff >= 0.0) {
+    rc = launch_truncate(ctx, n, cutoff, maxdim > 0 ? maxdim : (int64_t(1) << 40), trunc_err);
+    if (rc) {
+      qtt_net_destroy(n);
+      return rc;
+    }
+  }
+  *out = n;
+  return QTT_OK;
+}
+
+int qtt_truncate(qtt_ctx* ctx, qtt_net* net, double cutoff, int64_t maxdim, double* trunc_err) {
+  if (!ctx || !net) return qtt_fail(ctx, QTT_EINVAL, "null argument");
+  if (maxdim < 0) return qtt_fail(ctx, QTT_EINVAL, "maxdim < 0");
+  return launch_truncate(ctx, net, cutoff < 0 ? 0.0 : cutoff, maxdim > 0 ? maxdim : (int64_t(1) << 40),
+                         trunc_err);
+}
+
+int qtt_eval_grad(qtt_ctx* ctx, const qtt_net* net, const double* xs, const double* ybar,
+                  int64_t npts, double* grad, int precision) {
+  if (!ctx || !net || !grad || npts < 0 || (npts && (!xs || !ybar)))
+    return qtt_fail(ctx, QTT_EINVAL, "null argument");
+  int rc = check_precision(ctx, precision);
+  if (rc) return rc;
+  return launch_grad(ctx, net, xs, ybar, npts, grad, precision);
+}
+
+int qtt_eval_grad_host(qtt_ctx* ctx, const qtt_net* net, const double* xs, const double* ybar,
+                       int64_t npts, double* const* grads, int precision) {
+  if (!ctx || !net || !grads || npts < 0 || (npts && (!xs || !ybar)))
+    return qtt_fail(ctx, QTT_EINVAL, "null argument");
+  int nc = std::max(net->t.ncoord, 1);
+  int64_t gsz = qtt_net_size(net);
+  double* d = nullptr;
+  size_t bytes = (size_t(npts) * (nc + 1) + gsz) * sizeof(double);
+  HIPCHK(ctx, hipMalloc(&d, std::max<size_t>(bytes, 16)));
+  int rc = QTT_OK;
+  double* dx = d;
+  double* dy = d + size_t(npts) * nc;
+  double* dg = dy + npts;
+  if (npts) {
+    rc = qtt_hip_check(ctx, hipMemcpyAsync(dx, xs, size_t(npts) * nc * sizeof(double), hipMemcpyHostToDevice, ctx->stream), "upload");
+    if (!rc) rc = qtt_hip_check(ctx, hipMemcpyAsync(dy, ybar, size_t(npts) * sizeof(double), hipMemcpyHostToDevice, ctx->stream), "upload");
+  }
+  if (!rc) rc = qtt_eval_grad(ctx, net, dx, dy, npts, dg, precision);
+  if (!rc) rc = qtt_sync(ctx);
+  int64_t o = 0;
+  for (int v = 0; v < net->t.nv && !rc; v++) {
+    int64_t s = net->core_size(v);
+    rc = qtt_hip_check(ctx, hipMemcpy(grads[v], dg + o, s * sizeof(double), hipMemcpyDeviceToHost), "download");
+    o += s;
+  }
+  (void)hipFree(d);
+  return rc;
+}
+
+}  // extern "C"

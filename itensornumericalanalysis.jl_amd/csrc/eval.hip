@@ -1,0 +1,405 @@
+// Batched point evaluation: f(x) = product of the core slices selected by the
+// digits of x (src/itensornetworkfunction.jl:96-106 -- calculate_ind_values,
+// project, scalar -- for a whole batch of points in one launch).
+//
+// Chain kernel (the hot path): one lane per point.  The digit map is fused
+// into the prologue as integer bit extraction of floor(x 2^L)
+// (src/IndexMaps/abstractindexmap.jl:121-138 for base 2).  The chain is
+// reduced from the last site to the first: w[l] = sum_r T[l][d][r] v[r].
+// Core entries are wave-uniform, so they are read through the scalar data
+// path from a packed copy [r][lpad][s] (8 doubles = one 64-byte scalar load
+// per r and 4 rows); each lane forms both digit slices with FMAs and keeps
+// the one its digit selects.  The per-lane vectors live in LDS ([i][lane]
+// layout, conflict free).
+#include "qtt_internal.h"
+
+namespace {
+
+__device__ inline uint64_t digit_word(double x, int L, bool& bad) {
+  if (!(x >= 0.0)) {  // negative or NaN
+    bad = true;
+    return 0;
+  }
+  if (x >= 1.0) return (uint64_t(1) << L) - 1;  // the greedy map saturates at 1
+  return (uint64_t)(x * (double)(uint64_t(1) << L));  // exact scaling, truncation = floor
+}
+
+// branch-free select of a (mask 0) or b (mask all ones), kept in registers
+__device__ inline double bitsel(double a, double b, uint64_t mask) {
+  const uint64_t ua = (uint64_t)__double_as_longlong(a), ub = (uint64_t)__double_as_longlong(b);
+  return __longlong_as_double((long long)((ua & ~mask) | (ub & mask)));
+}
+
+__device__ inline float bitself(float a, float b, uint32_t mask) {
+  return __uint_as_float((__float_as_uint(a) & ~mask) | (__float_as_uint(b) & mask));
+}
+
+// pack chain site k as P[r][lpad][2] (lpad = bond[k] rounded up to 4, zero padded)
+__global__ void pack_chain_kernel(const double* __restrict__ data, const int64_t* __restrict__ off,
+                                  const int* __restrict__ bond, const int* __restrict__ sdim,
+                                  const int64_t* __restrict__ poff, int nv,
+                                  double* __restrict__ pack) {
+  const int k = blockIdx.y;
+  const int chil = bond[k];
+  const int chir = (k == nv - 1) ? 1 : bond[k + 1];
+  const int lpad = (chil + 3) & ~3;
+  const int sd = sdim[k];
+  const int64_t n = (int64_t)chir * lpad * 2;
+  const double* T = data + off[k];
+  double* P = pack + poff[k];
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    int s = (int)(e & 1);
+    int64_t rest = e >> 1;
+    int l = (int)(rest % lpad);
+    int r = (int)(rest / lpad);
+    double val = 0.0;
+    if (l < chil) {
+      int ss = sd == 1 ? 0 : s;
+      val = T[((int64_t)l * sd + ss) * chir + r];
+    }
+    P[e] = val;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+eval_chain_f64_kernel(const double* __restrict__ pack, const int64_t* __restrict__ poff,
+                      const int* __restrict__ bond, TopoView t, const double* __restrict__ xs,
+                      int64_t npts, double* __restrict__ out, int* __restrict__ flag, int vstride) {
+  extern __shared__ double lds[];
+  const int tid = threadIdx.x;
+  const int bs = blockDim.x;
+  const int64_t p = (int64_t)blockIdx.x * bs + tid;
+  const bool active = p < npts;
+  const int nc = t.ncoord;
+  bool bad = false;
+  uint64_t kw[QTT_MAX_COORD] = {0, 0, 0, 0};
+#pragma unroll
+  for (int c = 0; c < QTT_MAX_COORD; ++c)
+    if (c < nc) kw[c] = digit_word(active ? xs[p * nc + c] : 0.0, t.ndigits[c], bad);
+  double* v = lds + tid;
+  double* w = lds + (size_t)vstride * bs + tid;
+  v[0] = 1.0;
+  for (int k = t.nv - 1; k >= 0; --k) {
+    const int chil = bond[k];
+    const int chir = (k == t.nv - 1) ? 1 : bond[k + 1];
+    const int c = t.vcoord[k];
+    int d = 0;
+    if (c >= 0) {
+      const uint64_t kk = c == 0 ? kw[0] : c == 1 ? kw[1] : c == 2 ? kw[2] : kw[3];
+      d = (int)((kk >> (t.ndigits[c] - t.vdigit[k])) & 1u);
+    }
+    const uint64_t dmask = 0 - (uint64_t)d;
+    const double* __restrict__ P = pack + poff[k];
+    const int lpad = (chil + 3) & ~3;
+    for (int l0 = 0; l0 < chil; l0 += 4) {
+      double a0[4] = {0.0, 0.0, 0.0, 0.0}, a1[4] = {0.0, 0.0, 0.0, 0.0};
+      const double* __restrict__ q = P + 2 * l0;
+      for (int r = 0; r < chir; ++r) {
+        const double vr = v[(size_t)r * bs];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          a0[i] = __builtin_fma(q[2 * i], vr, a0[i]);
+          a1[i] = __builtin_fma(q[2 * i + 1], vr, a1[i]);
+        }
+        q += 2 * lpad;
+      }
+      double sel[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) sel[i] = bitsel(a0[i], a1[i], dmask);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (l0 + i < chil) w[(size_t)(l0 + i) * bs] = sel[i];
+    }
+    double* tmp = v;
+    v = w;
+    w = tmp;
+  }
+  if (active) out[p] = bad ? __builtin_nan("") : v[0];
+  if (bad) flag[0] = 1;
+}
+
+// float32 variants of the chain kernel.  SPLIT keeps each operand as a
+// (hi, lo) float pair and accumulates in float-float (Dekker/Knuth error-free
+// transformations), PLAIN is ordinary float32.
+struct ff {
+  float h, l;
+};
+
+__device__ inline ff two_sum(float a, float b) {
+  float s = a + b;
+  float bb = s - a;
+  float e = (a - (s - bb)) + (b - bb);
+  return {s, e};
+}
+
+__device__ inline ff ff_fma(float ah, float al, float bh, float bl, ff c) {
+  // c + (ah + al) * (bh + bl), dropping al * bl
+  float p = ah * bh;
+  float pe = __builtin_fmaf(ah, bh, -p);
+  pe = __builtin_fmaf(ah, bl, pe);
+  pe = __builtin_fmaf(al, bh, pe);
+  ff s = two_sum(c.h, p);
+  float e = s.l + c.l + pe;
+  float h = s.h + e;
+  return {h, e - (h - s.h)};
+}
+
+template <bool SPLIT>
+__global__ void __launch_bounds__(256)
+eval_chain_f32_kernel(const float* __restrict__ pack, const int64_t* __restrict__ poff,
+                      const int* __restrict__ bond, TopoView t, const double* __restrict__ xs,
+                      int64_t npts, double* __restrict__ out, int* __restrict__ flag, int vstride) {
+  extern __shared__ float ldsf[];
+  const int tid = threadIdx.x;
+  const int bs = blockDim.x;
+  const int64_t p = (int64_t)blockIdx.x * bs + tid;
+  const bool active = p < npts;
+  const int nc = t.ncoord;
+  bool bad = false;
+  uint64_t kw[QTT_MAX_COORD] = {0, 0, 0, 0};
+#pragma unroll
+  for (int c = 0; c < QTT_MAX_COORD; ++c)
+    if (c < nc) kw[c] = digit_word(active ? xs[p * nc + c] : 0.0, t.ndigits[c], bad);
+  // vectors as (hi, lo) pairs: [i][lane][2]
+  float* v = ldsf + 2 * tid;
+  float* w = ldsf + (size_t)2 * vstride * bs + 2 * tid;
+  v[0] = 1.0f;
+  v[1] = 0.0f;
+  for (int k = t.nv - 1; k >= 0; --k) {
+    const int chil = bond[k];
+    const int chir = (k == t.nv - 1) ? 1 : bond[k + 1];
+    const int c = t.vcoord[k];
+    int d = 0;
+    if (c >= 0) {
+      const uint64_t kk = c == 0 ? kw[0] : c == 1 ? kw[1] : c == 2 ? kw[2] : kw[3];
+      d = (int)((kk >> (t.ndigits[c] - t.vdigit[k])) & 1u);
+    }
+    // packed float layout: [r][lpad][s][hi, lo]
+    const float* __restrict__ P = pack + 2 * poff[k];
+    const int lpad = (chil + 3) & ~3;
+    for (int l0 = 0; l0 < chil; l0 += 4) {
+      ff a0[4], a1[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a0[i] = a1[i] = {0.0f, 0.0f};
+      const float* __restrict__ q = P + 4 * l0;
+      for (int r = 0; r < chir; ++r) {
+        const float vh = v[(size_t)2 * r * bs];
+        const float vl = SPLIT ? v[(size_t)2 * r * bs + 1] : 0.0f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          if (SPLIT) {
+            a0[i] = ff_fma(q[4 * i], q[4 * i + 1], vh, vl, a0[i]);
+            a1[i] = ff_fma(q[4 * i + 2], q[4 * i + 3], vh, vl, a1[i]);
+          } else {
+            a0[i].h = __builtin_fmaf(q[4 * i], vh, a0[i].h);
+            a1[i].h = __builtin_fmaf(q[4 * i + 2], vh, a1[i].h);
+          }
+        }
+        q += 4 * lpad;
+      }
+      const uint32_t fmask = 0u - (uint32_t)d;
+      float sh[4], sl[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        sh[i] = bitself(a0[i].h, a1[i].h, fmask);
+        sl[i] = bitself(a0[i].l, a1[i].l, fmask);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (l0 + i < chil) {
+          w[(size_t)2 * (l0 + i) * bs] = sh[i];
+          w[(size_t)2 * (l0 + i) * bs + 1] = sl[i];
+        }
+    }
+    float* tmp = v;
+    v = w;
+    w = tmp;
+  }
+  if (active) out[p] = bad ? __builtin_nan("") : (double)v[0] + (double)v[1];
+  if (bad) flag[0] = 1;
+}
+
+// float32 (hi, lo) pack of the float64 pack: [r][lpad][s] -> [r][lpad][s][2]
+__global__ void pack_split_kernel(const double* __restrict__ pack, int64_t n, float* __restrict__ out) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    double a = pack[e];
+    float h = (float)a;
+    float l = (float)(a - (double)h);
+    out[2 * e] = h;
+    out[2 * e + 1] = l;
+  }
+}
+
+// Generic tree evaluation: one thread per point, per-vertex vectors in a
+// global buffer laid out [vector element][thread] (coalesced).  Used for tree
+// topologies; the vertex contraction sums over all child index combinations.
+__global__ void __launch_bounds__(256)
+eval_tree_kernel(const double* __restrict__ data, const int64_t* __restrict__ off,
+                 const int* __restrict__ bond, TopoView t, const int64_t* __restrict__ voff,
+                 const double* __restrict__ xs, int64_t npts, double* __restrict__ out,
+                 int* __restrict__ flag, double* __restrict__ vs, int64_t nthr) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= nthr) return;
+  const int nc = t.ncoord;
+  for (int64_t p = gid; p < npts; p += nthr) {
+    bool bad = false;
+    for (int c = 0; c < nc; ++c) {
+      double x = xs[p * nc + c];
+      if (!(x >= 0.0)) bad = true;
+    }
+    if (bad) {
+      out[p] = __builtin_nan("");
+      flag[0] = 1;
+      continue;
+    }
+    for (int v = t.nv - 1; v >= 0; --v) {
+      const int c = t.vcoord[v];
+      int d = 0;
+      if (c >= 0) {
+        bool b2 = false;
+        uint64_t kk = digit_word(xs[p * nc + c], t.ndigits[c], b2);
+        d = (int)((kk >> (t.ndigits[c] - t.vdigit[v])) & 1u);
+      }
+      const int m = t.nch[v];
+      const int* ch = t.chs + t.choff[v];
+      int64_t pc = 1;
+      for (int k = 0; k < m; ++k) pc *= bond[ch[k]];
+      const int sd = t.sdim[v];
+      for (int pp = 0; pp < bond[v]; ++pp) {
+        const double* base = data + off[v] + ((int64_t)pp * sd + d) * pc;
+        double acc = 0.0;
+        for (int64_t j = 0; j < pc; ++j) {
+          double wgt = base[j];
+          int64_t r = j;
+          for (int k = m - 1; k >= 0 && wgt != 0.0; --k) {
+            const int b = bond[ch[k]];
+            const int idx = (int)(r % b);
+            r /= b;
+            wgt *= vs[(voff[ch[k]] + idx) * nthr + gid];
+          }
+          acc += wgt;
+        }
+        vs[(voff[v] + pp) * nthr + gid] = acc;
+      }
+    }
+    out[p] = vs[voff[0] * nthr + gid];
+  }
+}
+
+}  // namespace
+
+static int ensure_pack(qtt_ctx* ctx, qtt_net* net) {
+  if (net->pack_valid) return QTT_OK;
+  const int nv = net->t.nv;
+  std::vector<int64_t> poff(nv);
+  int64_t o = 0;
+  for (int k = 0; k < nv; k++) {
+    int64_t chil = net->bond[k];
+    int64_t chir = (k == nv - 1) ? 1 : net->bond[k + 1];
+    int64_t lpad = (chil + 3) & ~int64_t(3);
+    poff[k] = o;
+    o += chir * lpad * 2;
+  }
+  if (net->d_pack && o > net->pack_elems) {
+    (void)hipFree(net->d_pack);
+    net->d_pack = nullptr;
+    if (net->d_pack32) (void)hipFree(net->d_pack32);
+    net->d_pack32 = nullptr;
+  }
+  if (!net->d_pack) {
+    int rc = qtt_hip_check(ctx, hipMalloc(&net->d_pack, std::max<int64_t>(o, 1) * sizeof(double)), "hipMalloc pack");
+    if (rc) return rc;
+    net->pack_elems = o;
+  }
+  if (!net->d_pack_off) {
+    int rc = qtt_hip_check(ctx, hipMalloc(&net->d_pack_off, nv * sizeof(int64_t)), "hipMalloc pack offsets");
+    if (rc) return rc;
+  }
+  int rc = qtt_hip_check(ctx, hipMemcpyAsync(net->d_pack_off, poff.data(), nv * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream), "pack offsets");
+  if (rc) return rc;
+  TopoView tv = net->view();
+  dim3 grid(64, nv);
+  pack_chain_kernel<<<grid, 256, 0, ctx->stream>>>(net->d_data, net->d_off, net->d_bond, tv.sdim,
+                                                   net->d_pack_off, nv, net->d_pack);
+  rc = qtt_hip_check(ctx, hipGetLastError(), "pack_chain_kernel");
+  if (rc) return rc;
+  // the host copy of poff must outlive the async copy
+  rc = qtt_hip_check(ctx, hipStreamSynchronize(ctx->stream), "pack sync");
+  if (rc) return rc;
+  net->pack_valid = true;
+  net->pack32_valid = false;
+  return QTT_OK;
+}
+
+int qtt_ensure_pack(qtt_ctx* ctx, qtt_net* net) { return ensure_pack(ctx, net); }
+
+static int ensure_pack32(qtt_ctx* ctx, qtt_net* net) {
+  int rc = ensure_pack(ctx, net);
+  if (rc) return rc;
+  if (net->pack32_valid) return QTT_OK;
+  if (!net->d_pack32) {
+    rc = qtt_hip_check(ctx, hipMalloc(&net->d_pack32, std::max<int64_t>(net->pack_elems, 1) * 2 * sizeof(float)), "hipMalloc pack32");
+    if (rc) return rc;
+  }
+  pack_split_kernel<<<256, 256, 0, ctx->stream>>>(net->d_pack, net->pack_elems, net->d_pack32);
+  rc = qtt_hip_check(ctx, hipGetLastError(), "pack_split_kernel");
+  if (rc) return rc;
+  net->pack32_valid = true;
+  return QTT_OK;
+}
+
+int launch_eval(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t npts, double* out,
+                int precision) {
+  const HostTopo& t = net->t;
+  TopoView tv = net->view();
+  int64_t maxb = 1;
+  for (int v = 0; v < t.nv; v++) maxb = std::max(maxb, net->bond[v]);
+  if (t.chain && t.ncoord <= QTT_MAX_COORD) {
+    int rc = precision == QTT_F64 ? ensure_pack(ctx, net) : ensure_pack32(ctx, net);
+    if (rc) return rc;
+    // LDS: two vectors of maxb entries per lane (double, or a float pair)
+    const size_t per_lane = 2 * size_t(maxb) * sizeof(double);
+    int bs = 256;
+    while (bs > 64 && per_lane * bs > 160 * 1024) bs /= 2;
+    if (per_lane * bs > 160 * 1024)
+      return qtt_fail(ctx, QTT_EINVAL, "evaluate: bond dimension too large for the chain kernel");
+    const int64_t nb = (npts + bs - 1) / bs;
+    if (nb > INT32_MAX) return qtt_fail(ctx, QTT_EINVAL, "evaluate: too many points");
+    const size_t lds = per_lane * bs;
+    if (precision == QTT_F64) {
+      eval_chain_f64_kernel<<<dim3((unsigned)nb), bs, lds, ctx->stream>>>(
+          net->d_pack, net->d_pack_off, net->d_bond, tv, xs, npts, out, ctx->d_flag, (int)maxb);
+    } else if (precision == QTT_F32_SPLIT) {
+      eval_chain_f32_kernel<true><<<dim3((unsigned)nb), bs, lds, ctx->stream>>>(
+          (const float*)net->d_pack32, net->d_pack_off, net->d_bond, tv, xs, npts, out, ctx->d_flag, (int)maxb);
+    } else {
+      eval_chain_f32_kernel<false><<<dim3((unsigned)nb), bs, lds, ctx->stream>>>(
+          (const float*)net->d_pack32, net->d_pack_off, net->d_bond, tv, xs, npts, out, ctx->d_flag, (int)maxb);
+    }
+    return qtt_hip_check(ctx, hipGetLastError(), "eval_chain kernel");
+  }
+  if (precision != QTT_F64)
+    return qtt_fail(ctx, QTT_EINVAL, "evaluate: float32 paths are implemented for chains only");
+  // generic tree path
+  std::vector<int64_t> voff(t.nv);
+  int64_t tot = 0;
+  for (int v = 0; v < t.nv; v++) {
+    voff[v] = tot;
+    tot += net->bond[v];
+  }
+  int64_t nthr = std::min<int64_t>(npts, 65536);
+  while (nthr > 256 && (size_t)tot * nthr * sizeof(double) > (size_t(1) << 31)) nthr /= 2;
+  size_t bytes = (size_t)tot * nthr * sizeof(double) + t.nv * sizeof(int64_t) + 64;
+  double* scr = qtt_scratch(ctx, bytes);
+  if (!scr) return qtt_fail(ctx, QTT_ENOMEM, "evaluate: scratch");
+  int64_t* d_voff = (int64_t*)(scr + (size_t)tot * nthr);
+  int rc = qtt_hip_check(ctx, hipMemcpyAsync(d_voff, voff.data(), t.nv * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream), "voff");
+  if (rc) return rc;
+  int bs = 256;
+  int64_t nb = (nthr + bs - 1) / bs;
+  eval_tree_kernel<<<dim3((unsigned)nb), bs, 0, ctx->stream>>>(net->d_data, net->d_off, net->d_bond, tv, d_voff, xs, npts, out,
+                                                               ctx->d_flag, scr, nthr);
+  rc = qtt_hip_check(ctx, hipGetLastError(), "eval_tree_kernel");
+  if (rc) return rc;
+  return qtt_hip_check(ctx, hipStreamSynchronize(ctx->stream), "eval_tree sync");
+}

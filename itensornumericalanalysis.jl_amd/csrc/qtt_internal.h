@@ -1,0 +1,96 @@
+// Internal declarations shared by the libqtt translation units (gfx950 only).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/qtt.h"
+
+#define QTT_MAX_COORD 4     // coordinates handled by the fused digit prologue
+#define QTT_MAX_NDIM 10     // core rank (parent + site + up to 8 children)
+#define QTT_LDS_DOUBLES 20224  // 158 KiB of the 160 KiB LDS per CU
+
+struct qtt_ctx {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int* d_flag = nullptr;  // [0] = invalid coordinate seen, [1] = SVD non-convergence
+  double* d_scratch = nullptr;
+  size_t scratch_bytes = 0;
+  int cus = 256;
+};
+
+// Topology view passed by value to kernels (device pointers).
+struct TopoView {
+  const int* parent;
+  const int* sdim;
+  const int* vcoord;
+  const int* vdigit;
+  const int* nch;
+  const int* choff;
+  const int* chs;
+  const int* ndigits;
+  int nv;
+  int ncoord;
+};
+
+struct HostTopo {
+  int nv = 0, ncoord = 0;
+  std::vector<int> parent, sdim, vcoord, vdigit, ndigits, nch, choff, chs;
+  bool chain = false;
+};
+
+struct qtt_net {
+  qtt_ctx* ctx = nullptr;
+  HostTopo t;
+  std::vector<int64_t> bond;  // host mirror of the current bonds
+  std::vector<int64_t> cap;   // capacity (elements) per vertex
+  std::vector<int64_t> off;   // element offset per vertex
+  int64_t total = 0;
+  double* d_data = nullptr;
+  int64_t* d_off = nullptr;
+  int* d_topo = nullptr;  // packed int arrays, see topo_view()
+  int* d_bond = nullptr;  // int32 bonds on device
+  // packed copy for the chain evaluate kernel: site k as [r][lpad][s]
+  double* d_pack = nullptr;
+  int64_t* d_pack_off = nullptr;
+  int64_t pack_elems = 0;
+  bool pack_valid = false;
+  // float32 copies for the mixed-precision evaluate
+  float* d_pack32 = nullptr;
+  bool pack32_valid = false;
+  TopoView view() const;
+  int64_t core_size(int v) const;
+};
+
+struct qtt_mpo {
+  qtt_ctx* ctx = nullptr;
+  HostTopo t;
+  std::vector<int64_t> bond;
+  std::vector<int64_t> off;
+  int64_t total = 0;
+  double* d_data = nullptr;
+  int64_t* d_off = nullptr;
+  int* d_bond = nullptr;
+};
+
+// helpers implemented in capi.hip
+int qtt_fail(qtt_ctx* ctx, int code, const std::string& msg);
+int qtt_hip_check(qtt_ctx* ctx, hipError_t e, const char* what);
+double* qtt_scratch(qtt_ctx* ctx, size_t bytes);
+
+// kernel launchers (eval.hip, sweep.hip, factor.hip, grad.hip)
+int launch_eval(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t npts, double* out,
+                int precision);
+int launch_integrate(qtt_ctx* ctx, const qtt_net* net, double weight, double* out_host);
+int launch_inner(qtt_ctx* ctx, const qtt_net* a, const qtt_net* b, double* out_host);
+int launch_contract(qtt_ctx* ctx, const qtt_mpo* op, const qtt_net* net, qtt_net* out);
+int launch_truncate(qtt_ctx* ctx, qtt_net* net, double cutoff, int64_t maxdim, double* err);
+int launch_grad(qtt_ctx* ctx, const qtt_net* net, const double* xs, const double* ybar,
+                int64_t npts, double* grad, int precision);
+
+// schedule of the truncation sweep (capi.hip): triples (x, u, truncate)
+std::vector<int> truncation_schedule(const HostTopo& t);

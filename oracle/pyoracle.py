@@ -1,0 +1,221 @@
+"""ctypes wrapper of the C oracle (liboracle.so).  TEST INFRASTRUCTURE ONLY:
+imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+
+Every function takes the raw network description (parent, sdim, cores in the
+[parent][site][children...] layout) so it never depends on the engine's classes.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        dp = C.POINTER(C.c_double)
+        ip = C.POINTER(C.c_int)
+        lp = C.POINTER(C.c_long)
+        dpp = C.POINTER(dp)
+        L.or_digit.argtypes = [C.c_double, C.c_int, C.c_int]
+        L.or_eval.argtypes = [C.c_int, ip, ip, lp, dpp, ip, ip, C.c_int, ip, dp, C.c_long, dp, C.c_int]
+        L.or_integrate.argtypes = [C.c_int, ip, ip, lp, dpp, C.c_double, dp]
+        L.or_inner.argtypes = [C.c_int, ip, ip, lp, dpp, lp, dpp, dp]
+        L.or_contract.argtypes = [C.c_int, ip, ip, lp, dpp, lp, dpp, dpp]
+        L.or_truncate.argtypes = [C.c_int, ip, ip, lp, dpp, C.c_double, C.c_long, dp]
+        L.or_eval_grad.argtypes = [C.c_int, ip, ip, lp, dpp, ip, ip, C.c_int, ip, dp, C.c_long, dp, dpp,
+                                   C.c_int]
+        L.or_svd.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp]
+        L.or_qr.argtypes = [C.c_int, C.c_int, dp, dp, dp]
+        L.or_trunc_rule.argtypes = [dp, C.c_long, C.c_double, C.c_long, dp]
+        L.or_trunc_rule.restype = C.c_long
+        _LIB = L
+    return _LIB
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _l(a):
+    a = np.ascontiguousarray(a, dtype=np.int64)
+    return a, a.ctypes.data_as(C.POINTER(C.c_long))
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _dpp(arrs):
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in arrs]
+    ptrs = (C.POINTER(C.c_double) * len(arrs))(*[a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs])
+    return arrs, ptrs
+
+
+def bonds(cores):
+    return np.array([c.shape[0] for c in cores], dtype=np.int64)
+
+
+def digit(x, L, j):
+    return lib().or_digit(float(x), int(L), int(j))
+
+
+def evaluate(parent, sdim, cores, vcoord, vdigit, Lc, xs, nthreads=None):
+    xs = np.ascontiguousarray(np.atleast_2d(np.asarray(xs, dtype=np.float64)))
+    if xs.shape[1] != len(Lc) and xs.shape[0] == len(Lc) and xs.shape[1] != 1:
+        raise ValueError("xs must be [npts, ncoord]")
+    npts = xs.shape[0]
+    out = np.empty(npts)
+    p, pp = _i(parent)
+    s, sp = _i(sdim)
+    b, bp = _l(bonds(cores))
+    cs, cp = _dpp(cores)
+    vc, vcp = _i(vcoord)
+    vd, vdp = _i(vdigit)
+    lc, lcp = _i(Lc)
+    x, xp = _d(xs)
+    o, op = _d(out)
+    nt = nthreads or min(16, len(os.sched_getaffinity(0)))
+    rc = lib().or_eval(len(p), pp, sp, bp, cp, vcp, vdp, len(lc), lcp, xp, npts, op, nt)
+    return o, rc
+
+
+def integrate(parent, sdim, cores, weight=0.5):
+    p, pp = _i(parent)
+    s, sp = _i(sdim)
+    b, bp = _l(bonds(cores))
+    cs, cp = _dpp(cores)
+    out = C.c_double(0.0)
+    rc = lib().or_integrate(len(p), pp, sp, bp, cp, float(weight), C.byref(out))
+    if rc:
+        raise RuntimeError(f"or_integrate failed: {rc}")
+    return out.value
+
+
+def inner(parent, sdim, coresA, coresB):
+    p, pp = _i(parent)
+    s, sp = _i(sdim)
+    ba, bap = _l(bonds(coresA))
+    bb, bbp = _l(bonds(coresB))
+    ca, cap = _dpp(coresA)
+    cb, cbp = _dpp(coresB)
+    out = C.c_double(0.0)
+    rc = lib().or_inner(len(p), pp, sp, bap, cap, bbp, cbp, C.byref(out))
+    if rc:
+        raise RuntimeError(f"or_inner failed: {rc}")
+    return out.value
+
+
+def _children(parent):
+    ch = [[] for _ in parent]
+    for v in range(1, len(parent)):
+        ch[parent[v]].append(v)
+    return ch
+
+
+def contract(parent, sdim, coresW, coresA):
+    """Exact operator application; returns the new cores."""
+    ch = _children(parent)
+    bw, ba = bonds(coresW), bonds(coresA)
+    outs = []
+    for v in range(len(parent)):
+        shape = (bw[v] * ba[v], int(sdim[v])) + tuple(int(bw[c] * ba[c]) for c in ch[v])
+        outs.append(np.zeros(shape))
+    p, pp = _i(parent)
+    s, sp = _i(sdim)
+    b1, b1p = _l(bw)
+    b2, b2p = _l(ba)
+    cw, cwp = _dpp(coresW)
+    ca, cap = _dpp(coresA)
+    oo, op = _dpp(outs)
+    rc = lib().or_contract(len(p), pp, sp, b1p, cwp, b2p, cap, op)
+    if rc:
+        raise RuntimeError(f"or_contract failed: {rc}")
+    return oo
+
+
+def truncate(parent, sdim, cores, cutoff=0.0, maxdim=1 << 30):
+    """Returns (new cores, truncation error)."""
+    ch = _children(parent)
+    b = bonds(cores).copy()
+    bufs = [np.ascontiguousarray(c, dtype=np.float64).ravel().copy() for c in cores]
+    p, pp = _i(parent)
+    s, sp = _i(sdim)
+    bl, blp = _l(b)
+    arrs, cp = _dpp(bufs)
+    err = C.c_double(0.0)
+    rc = lib().or_truncate(len(p), pp, sp, blp, cp, float(cutoff), int(maxdim), C.byref(err))
+    if rc:
+        raise RuntimeError(f"or_truncate failed: {rc}")
+    out = []
+    for v in range(len(p)):
+        shape = (int(bl[v]), int(sdim[v])) + tuple(int(bl[c]) for c in ch[v])
+        out.append(arrs[v][: int(np.prod(shape))].reshape(shape).copy())
+    return out, err.value
+
+
+def eval_grad(parent, sdim, cores, vcoord, vdigit, Lc, xs, ybar, nthreads=None):
+    xs = np.ascontiguousarray(np.atleast_2d(np.asarray(xs, dtype=np.float64)))
+    npts = xs.shape[0]
+    grads = [np.zeros_like(np.asarray(c, dtype=np.float64)) for c in cores]
+    p, pp = _i(parent)
+    s, sp = _i(sdim)
+    b, bp = _l(bonds(cores))
+    cs, cp = _dpp(cores)
+    vc, vcp = _i(vcoord)
+    vd, vdp = _i(vdigit)
+    lc, lcp = _i(Lc)
+    x, xp = _d(xs)
+    yb, ybp = _d(ybar)
+    gg, gp = _dpp(grads)
+    nt = nthreads or min(16, len(os.sched_getaffinity(0)))
+    rc = lib().or_eval_grad(len(p), pp, sp, bp, cp, vcp, vdp, len(lc), lcp, xp, npts, ybp, gp, nt)
+    if rc:
+        raise RuntimeError(f"or_eval_grad failed: {rc}")
+    return gg
+
+
+def svd(M):
+    M = np.ascontiguousarray(M, dtype=np.float64)
+    r, c = M.shape
+    k = min(r, c)
+    U, S, Vt = np.zeros((r, k)), np.zeros(k), np.zeros((k, c))
+    _, mp = _d(M)
+    rc = lib().or_svd(r, c, mp, U.ctypes.data_as(C.POINTER(C.c_double)),
+                      S.ctypes.data_as(C.POINTER(C.c_double)), Vt.ctypes.data_as(C.POINTER(C.c_double)))
+    if rc:
+        raise RuntimeError(f"or_svd failed: {rc}")
+    return U, S, Vt
+
+
+def qr(M):
+    M = np.ascontiguousarray(M, dtype=np.float64)
+    r, c = M.shape
+    k = min(r, c)
+    Q, R = np.zeros((r, k)), np.zeros((k, c))
+    _, mp = _d(M)
+    lib().or_qr(r, c, mp, Q.ctypes.data_as(C.POINTER(C.c_double)), R.ctypes.data_as(C.POINTER(C.c_double)))
+    return Q, R
+
+
+def trunc_rule(p, cutoff, maxdim):
+    pa, pp = _d(p)
+    err = C.c_double(0.0)
+    m = lib().or_trunc_rule(pp, len(pa), float(cutoff), int(maxdim), C.byref(err))
+    return int(m), err.value

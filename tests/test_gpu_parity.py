@@ -1,0 +1,222 @@
+"""Engine (HIP, through the C-ABI) against the CPU oracle on identical inputs.
+
+Bars: float64 results within 1e-12 relative of the oracle (norm-wise over the
+points, and per value where the values are O(1)); integer digit work exact
+(checked through values at grid points and the invalid-coordinate status).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import grid_of, oracle_eval, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def eng(q):
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return q.engine_context()
+
+
+def _oracle_apply_truncate(po, f, op, cutoff, maxdim):
+    t = f.topo
+    C = po.contract(t.parent, t.sdim, op.cores, f.cores)
+    if cutoff is None or cutoff < 0:
+        return C, 0.0
+    return po.truncate(t.parent, t.sdim, C, cutoff, maxdim or (1 << 30))
+
+
+# ------------------------------------------------------------------ evaluate
+def test_eval_config1(q, po, eng):
+    from itensornumericalanalysis_jl_amd import workloads
+    w = workloads.config1(npts=20000)
+    y = q.evaluate(w.f, w.xs)
+    ref = oracle_eval(po, w.f, w.xs)
+    assert rel_l2(y, ref) < TOL
+    assert np.max(np.abs(y - ref)) < TOL * np.max(np.abs(ref))
+    # and the closed form on the grid
+    xg = grid_of(w.xs[:, 0], 12)
+    assert np.allclose(y, workloads.config1_function(xg), rtol=1e-12, atol=1e-12)
+
+
+def test_eval_config2(q, po, eng):
+    from itensornumericalanalysis_jl_amd import workloads
+    w = workloads.config2(npts=20000)
+    y = q.evaluate(w.f, w.xs)
+    ref = oracle_eval(po, w.f, w.xs)
+    assert rel_l2(y, ref) < TOL
+    xg = grid_of(w.xs, 20)
+    closed = workloads.config2_function(xg, w.extra["a"], w.extra["w"], w.extra["b"])
+    assert rel_l2(y, closed) < 1e-12
+
+
+@pytest.mark.parametrize("bond", [1, 3, 8, 17, 32, 64])
+def test_eval_random_chain_bonds(q, po, eng, rng, bond):
+    s = q.continuous_siteinds(q.named_grid((18, 1)), map_dimension=3)
+    f = q.random_itn(s, link_space=bond, rng=rng, std=1.0 / math.sqrt(bond))
+    xs = rng.uniform(0, 1, (3001, 3))
+    assert rel_l2(q.evaluate(f, xs), oracle_eval(po, f, xs)) < TOL
+
+
+def test_eval_tree(q, po, eng, rng):
+    s = q.continuous_siteinds(q.named_comb_tree((3, 5)), map_dimension=2)
+    f = q.random_itn(s, link_space=4, rng=rng, std=0.5)
+    xs = rng.uniform(0, 1, (2000, 2))
+    assert rel_l2(q.evaluate(f, xs), oracle_eval(po, f, xs)) < TOL
+
+
+def test_eval_edge_points(q, po, eng):
+    L = 10
+    s = q.continuous_siteinds(q.named_grid((L, 1)))
+    f = q.exp_itn(s, k=1.7, a=-0.3, c=0.9)
+    h = 2.0 ** -L
+    xs = np.array([[0.0], [h], [0.5], [1 - h], [np.nextafter(1.0, 0)], [1.0], [5.0], [h * 0.999]])
+    y = q.evaluate(f, xs)
+    xg = np.minimum(grid_of(xs[:, 0], L), 1 - h)
+    assert np.allclose(y, 0.9 * np.exp(1.7 * xg - 0.3), rtol=1e-13)
+    assert q.evaluate(f, 0.625) == pytest.approx(0.9 * math.exp(1.7 * 0.625 - 0.3), rel=1e-13)
+    assert q.evaluate(f, np.zeros((0, 1))).shape == (0,)
+
+
+def test_eval_invalid_coordinate(q, eng):
+    s = q.continuous_siteinds(q.named_grid((8, 1)))
+    f = q.exp_itn(s)
+    with pytest.raises(q.DigitMapError):
+        q.evaluate(f, np.array([[0.5], [-0.25]]))
+    with pytest.raises(q.DigitMapError):
+        q.evaluate(f, np.array([[float("nan")]]))
+    # the status is cleared: a valid call afterwards succeeds
+    assert q.evaluate(f, 0.5) == pytest.approx(math.exp(0.5), rel=1e-13)
+
+
+def test_eval_device_tensors_deterministic(q, po, eng, rng):
+    import torch
+    s = q.continuous_siteinds(q.named_grid((30, 1)), map_dimension=2)
+    f = q.random_itn(s, link_space=16, rng=rng, std=0.25)
+    xs = rng.uniform(0, 1, (100000, 2))
+    xd = torch.tensor(xs, device="cuda")
+    y1 = q.evaluate(f, xd).cpu().numpy()
+    y2 = q.evaluate(f, xd).cpu().numpy()
+    assert np.array_equal(y1, y2)
+    assert rel_l2(y1, oracle_eval(po, f, xs)) < TOL
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-10), (2, 1e-5)])
+def test_eval_float32_paths(q, po, eng, prec, tol):
+    from itensornumericalanalysis_jl_amd import workloads
+    w = workloads.config2(npts=20000)
+    y = q.evaluate(w.f, w.xs, precision=prec)
+    ref = oracle_eval(po, w.f, w.xs)
+    err = rel_l2(y, ref)
+    print(f"precision {prec}: rel L2 error vs float64 oracle {err:.3e}")
+    assert err < tol
+
+
+# ------------------------------------------------------------------ integrate / inner
+def test_integrate_chain_and_tree(q, po, eng, rng):
+    from itensornumericalanalysis_jl_amd import workloads
+    w = workloads.config1(npts=10)
+    t = w.f.topo
+    for take_sum, wt in ((False, 0.5), (True, 1.0)):
+        got = q.integrate(w.f, take_sum=take_sum)
+        ref = po.integrate(t.parent, t.sdim, w.f.cores, wt)
+        assert got == pytest.approx(ref, rel=TOL)
+    s = q.continuous_siteinds(q.named_comb_tree((3, 10)), map_dimension=2)
+    f = q.exp_itn(s, dim=1) * q.exp_itn(s, dim=2)
+    assert abs(q.integrate(f) - (math.e - 1) ** 2) < 1e-4  # test/test_integration.jl:28-38
+    g = q.random_itn(s, link_space=5, rng=rng)
+    assert q.integrate(g) == pytest.approx(po.integrate(g.topo.parent, g.topo.sdim, g.cores, 0.5), rel=TOL)
+
+
+def test_inner_and_integrate_operator(q, po, eng, rng):
+    s = q.continuous_siteinds(q.named_comb_tree((2, 6)))
+    f = q.random_itn(s, link_space=4, rng=rng)
+    g = q.random_itn(s, link_space=3, rng=rng)
+    t = f.topo
+    assert q.inner(f, g) == pytest.approx(po.inner(t.parent, t.sdim, f.cores, g.cores), rel=TOL)
+    op = q.second_derivative_operator(s)
+    C = po.contract(t.parent, t.sdim, op.cores, f.cores)
+    ref = po.integrate(t.parent, t.sdim, C, 0.5)
+    assert q.integrate(op, f) == pytest.approx(ref, rel=1e-11)
+
+
+# ------------------------------------------------------------------ operate / truncate
+def _compare_networks(q, po, g_dev, ref_cores, f, rng, tol=TOL, npts=4000):
+    xs = rng.uniform(0, 1, (npts, f.dimension()))
+    y = q.evaluate(g_dev, xs)
+    t = f.topo
+    yr, _ = po.evaluate(t.parent, t.sdim, ref_cores, t.vcoord, t.vdigit, t.Lc, xs)
+    return rel_l2(y, yr)
+
+
+def test_apply_exact_matches_oracle(q, po, eng, rng):
+    s = q.continuous_siteinds(q.named_comb_tree((2, 5)))
+    f = q.random_itn(s, link_space=3, rng=rng)
+    op = q.first_derivative_operator(s, left_boundary="Periodic", right_boundary="Neumann")
+    g = q.operate(op, f, truncate=False)
+    C, _ = _oracle_apply_truncate(po, f, op, -1, 0)
+    assert _compare_networks(q, po, g, C, f, rng) < TOL
+    assert list(g.bonds()) == [c.shape[0] for c in C]
+
+
+def test_operate_truncate_config1(q, po, eng, rng):
+    from itensornumericalanalysis_jl_amd import workloads
+    w = workloads.config1(npts=10)
+    g, err = q.operate(w.op, w.f, cutoff=w.cutoff, return_error=True)
+    ref, rerr = _oracle_apply_truncate(po, w.f, w.op, w.cutoff, 0)
+    assert list(g.bonds()) == [c.shape[0] for c in ref]
+    assert _compare_networks(q, po, g, ref, w.f, rng) < 1e-11
+    assert err == pytest.approx(rerr, rel=1e-6, abs=1e-20)
+    # telescoping identity of the Dirichlet second derivative
+    h = 2.0 ** -12
+    fx = workloads.config1_function
+    assert q.integrate(g) == pytest.approx(-(fx(1 - h) + fx(0.0)) / h, rel=1e-9)
+
+
+def test_operate_truncate_config2(q, po, eng, rng):
+    from itensornumericalanalysis_jl_amd import workloads
+    w = workloads.config2(npts=10)
+    fc = q.truncate(w.f, cutoff=1e-16, maxdim=64)
+    t = w.f.topo
+    fref, _ = po.truncate(t.parent, t.sdim, w.f.cores, 1e-16, 64)
+    assert list(fc.bonds()) == [c.shape[0] for c in fref]
+    assert _compare_networks(q, po, fc, fref, w.f, rng) < TOL
+    g = q.operate(w.op, fc, cutoff=w.cutoff, maxdim=w.maxdim)
+    C = po.contract(t.parent, t.sdim, w.op.cores, fref)
+    gref, _ = po.truncate(t.parent, t.sdim, C, w.cutoff, w.maxdim)
+    assert list(g.bonds()) == [c.shape[0] for c in gref]
+    assert _compare_networks(q, po, g, gref, w.f, rng) < 1e-11
+
+
+@pytest.mark.parametrize("graph", ["chain", "comb"])
+def test_truncate_random(q, po, eng, rng, graph):
+    g = q.named_grid((14, 1)) if graph == "chain" else q.named_comb_tree((3, 5))
+    s = q.continuous_siteinds(g)
+    f = q.random_itn(s, link_space=12, rng=rng)
+    tr, err = q.truncate(f, cutoff=1e-4, maxdim=6, return_error=True)
+    ref, rerr = po.truncate(f.topo.parent, f.topo.sdim, f.cores, 1e-4, 6)
+    assert list(tr.bonds()) == [c.shape[0] for c in ref]
+    assert _compare_networks(q, po, tr, ref, f, rng) < 1e-11
+    assert err == pytest.approx(rerr, rel=1e-9)
+
+
+# ------------------------------------------------------------------ gradient
+def test_gradient_matches_oracle(q, po, eng, rng):
+    s = q.continuous_siteinds(q.named_grid((12, 1)), map_dimension=2)
+    f = q.random_itn(s, link_space=8, rng=rng, std=0.4)
+    xs = rng.uniform(0, 1, (5000, 2))
+    yb = rng.standard_normal(5000)
+    g = q.eval_grad(f, xs, yb)
+    t = f.topo
+    ref = po.eval_grad(t.parent, t.sdim, f.cores, t.vcoord, t.vdigit, t.Lc, xs, yb)
+    num = math.sqrt(sum(np.sum((a - b) ** 2) for a, b in zip(g, ref)))
+    den = math.sqrt(sum(np.sum(b ** 2) for b in ref))
+    assert num / den < TOL
+    g2 = q.eval_grad(f, xs, yb)
+    assert all(np.array_equal(a, b) for a, b in zip(g, g2))  # deterministic reduction

@@ -1,0 +1,449 @@
+"""Pins the CPU oracle (oracle/oracle.c) before it is trusted as the checker.
+
+Each check is independent of the oracle's own formulas: closed forms, brute
+force over the whole grid (dense contraction, src/utils.jl:28-39), numpy's
+LAPACK, torch float64 autograd, and the reference's own known answers
+(test/test_indexmaps.jl, test/test_realitensorfunction.jl,
+test/test_integration.jl, test/test_operators.jl).
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+from hypothesis import given, settings
+from hypothesis import strategies as st
+
+from oracle import dense
+from helpers import grid_of, oracle_args, oracle_eval, rel_l2
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# --------------------------------------------------------------------------- digits
+def test_digit_reference_roundtrip(po):
+    # test/test_indexmaps.jl:27-30: x = 0.625 maps to digits whose calculate_p is x exactly
+    L = 16
+    digs = [po.digit(0.625, L, j) for j in range(1, L + 1)]
+    assert sum(d * 2.0 ** -(j + 1) for j, d in enumerate(digs)) == 0.625
+    assert digs[:3] == [1, 0, 1]
+
+
+def test_digit_saturation_and_invalid(po):
+    assert all(po.digit(1.0, 8, j) == 1 for j in range(1, 9))
+    assert all(po.digit(7.5, 8, j) == 1 for j in range(1, 9))
+    assert po.digit(-1e-300, 8, 1) == -1
+    assert po.digit(float("nan"), 8, 1) == -1
+    # largest double below 1 with L = 40: k = 2^40 - 1
+    x = np.nextafter(1.0, 0.0)
+    assert all(po.digit(x, 40, j) == 1 for j in range(1, 41))
+
+
+@settings(max_examples=200, deadline=None)
+@given(st.integers(min_value=1, max_value=52), st.data())
+def test_digit_grid_points_exact(po, L, data):
+    k = data.draw(st.integers(min_value=0, max_value=2 ** L - 1))
+    x = k / 2.0 ** L
+    digs = [po.digit(x, L, j) for j in range(1, L + 1)]
+    assert int("".join(map(str, digs)), 2) == k
+
+
+def _greedy(x, L):
+    """The reference's greedy loop (abstractindexmap.jl:121-138), base 2, in floats."""
+    out = []
+    xr = x
+    for j in range(1, L + 1):
+        v = 1
+        while True:
+            if xr >= v * 2.0 ** -j:
+                out.append(v)
+                xr -= v * 2.0 ** -j
+                break
+            v -= 1
+    return out
+
+
+@settings(max_examples=200, deadline=None)
+@given(st.floats(min_value=0.0, max_value=4.0, allow_nan=False), st.integers(min_value=1, max_value=60))
+def test_digit_matches_greedy(po, x, L):
+    assert [po.digit(x, L, j) for j in range(1, L + 1)] == _greedy(x, L)
+
+
+# --------------------------------------------------------------------------- factorizations
+@pytest.mark.parametrize("shape", [(1, 1), (5, 3), (3, 5), (40, 17), (17, 40), (64, 64)])
+def test_svd_against_lapack(po, rng, shape):
+    M = rng.standard_normal(shape)
+    U, S, Vt = po.svd(M)
+    s_ref = np.linalg.svd(M, compute_uv=False)
+    assert np.allclose(S, s_ref, rtol=1e-13, atol=1e-14 * s_ref[0])
+    assert np.allclose((U * S) @ Vt, M, atol=1e-13 * s_ref[0])
+    k = min(shape)
+    assert np.allclose(U.T @ U, np.eye(k), atol=1e-13)
+
+
+def test_svd_graded_spectrum(po, rng):
+    # singular values 1 .. 1e-12: the small ones must keep high relative accuracy
+    n = 30
+    Q1, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    Q2, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    s = np.logspace(0, -12, n)
+    M = (Q1 * s) @ Q2.T
+    _, S, _ = po.svd(M)
+    assert np.allclose(S, s, rtol=1e-6)
+
+
+def test_svd_rank_deficient(po, rng):
+    A = rng.standard_normal((20, 3)) @ rng.standard_normal((3, 12))
+    U, S, Vt = po.svd(A)
+    assert np.all(S[3:] < 1e-13 * S[0])
+    assert np.allclose(U.T @ U, np.eye(12), atol=1e-12)
+
+
+@pytest.mark.parametrize("shape", [(6, 4), (4, 6), (30, 30), (1, 5), (5, 1)])
+def test_qr(po, rng, shape):
+    M = rng.standard_normal(shape)
+    Q, R = po.qr(M)
+    assert np.allclose(Q @ R, M, atol=1e-13)
+    assert np.allclose(Q.T @ Q, np.eye(min(shape)), atol=1e-13)
+    assert np.allclose(np.tril(R, -1), 0)
+
+
+def test_trunc_rule(po):
+    p = np.array([4.0, 2.0, 1.0, 0.5, 0.25])
+    tot = p.sum()
+    assert po.trunc_rule(p, 0.0, 100) == (5, 0.0)
+    m, e = po.trunc_rule(p, 0.25 / tot, 100)  # exactly the last weight
+    assert m == 4 and math.isclose(e, 0.25 / tot)
+    m, e = po.trunc_rule(p, 0.74 / tot, 100)  # 0.25 + 0.5 > 0.74
+    assert m == 4
+    m, e = po.trunc_rule(p, 0.75 / tot, 100)
+    assert m == 3 and math.isclose(e, 0.75 / tot)
+    m, e = po.trunc_rule(p, 0.0, 2)
+    assert m == 2 and math.isclose(e, 1.75 / tot)
+    m, _ = po.trunc_rule(p, 1.0, 100)  # mindim 1
+    assert m == 1
+    assert po.trunc_rule(np.array([1.0, 0.0, 0.0]), 0.0, 10)[0] == 1  # exact zeros dropped
+
+
+# --------------------------------------------------------------------------- evaluate / integrate
+def test_eval_brute_force_tree(q, po, rng):
+    g = q.named_comb_tree((3, 3))
+    s = q.continuous_siteinds(g, map_dimension=2)
+    f = q.random_itn(s, link_space=3, rng=rng)
+    t = f.topo
+    T = dense.dense_tensor(t.parent, f.cores)
+    # every grid point: coordinates from the digits of each vertex
+    idx = np.array(list(np.ndindex(*T.shape)))
+    xs = np.zeros((len(idx), 2))
+    for v in range(t.nv):
+        c, j = t.vcoord[v], t.vdigit[v]
+        xs[:, c] += idx[:, v] * 2.0 ** -j
+    y = oracle_eval(po, f, xs)
+    assert np.allclose(y, T.ravel(), rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("name,k,a,c", [("exp", 0.7, 0.3, 1.3), ("sin", 2.1, 0.4, 0.8),
+                                         ("cos", 3.3, -0.2, 1.1), ("cosh", 0.9, 0.1, 0.6),
+                                         ("sinh", 1.7, 0.25, 2.0)])
+def test_builders_reference_values(q, po, name, k, a, c):
+    # test/test_realitensorfunction.jl:58-81: comb tree (2,3), x = 0.625, base 2
+    g = q.named_comb_tree((2, 3))
+    s = q.continuous_siteinds(g)
+    builder = getattr(q, f"{name}_itn")
+    f = builder(s, k=k, a=a, c=c)
+    func = {"exp": math.exp, "sin": math.sin, "cos": math.cos, "cosh": math.cosh, "sinh": math.sinh}[name]
+    y = oracle_eval(po, f, [[0.625]])[0]
+    assert math.isclose(y, c * func(k * 0.625 + a), rel_tol=1e-12)
+
+
+def test_const_linkdim(q, po):
+    # test/test_realitensorfunction.jl:41-56
+    g = q.named_grid((3, 3))
+    s = q.continuous_siteinds(q.named_comb_tree((3, 3)))
+    f = q.const_itn(s, c=1.5, linkdim=4)
+    assert math.isclose(oracle_eval(po, f, [[0.5]])[0], 1.5, rel_tol=1e-13)
+    assert g.nv() == 9
+
+
+def test_eval_closed_form_chain_2d(q, po, rng):
+    # interleaved 2D chain (src/digit_inds.jl:17-21): f(x, y) = sin(5x) exp(-2y) + cos(3y)
+    s = q.continuous_siteinds(q.named_grid((20, 1)), map_dimension=2)
+    f = q.sin_itn(s, k=5.0, dim=1) * q.exp_itn(s, k=-2.0, dim=2) + q.cos_itn(s, k=3.0, dim=2)
+    xs = rng.uniform(0, 1, (500, 2))
+    xg = grid_of(xs, 10)
+    ref = np.sin(5 * xg[:, 0]) * np.exp(-2 * xg[:, 1]) + np.cos(3 * xg[:, 1])
+    assert np.allclose(oracle_eval(po, f, xs), ref, rtol=1e-12, atol=1e-13)
+
+
+def test_integrate_geometric_series(q, po):
+    # left Riemann sum of c exp(k x + a): c e^a (e^k - 1) / (N (e^{k/N} - 1))
+    L, k, a, c = 14, 1.3, 0.2, 0.7
+    s = q.continuous_siteinds(q.named_grid((L, 1)))
+    f = q.exp_itn(s, k=k, a=a, c=c)
+    N = 2 ** L
+    ref = c * math.exp(a) * math.expm1(k) / (N * math.expm1(k / N))
+    p, sd, cores, *_ = oracle_args(f)
+    assert math.isclose(po.integrate(p, sd, cores, 0.5), ref, rel_tol=1e-13)
+    assert math.isclose(po.integrate(p, sd, cores, 1.0), ref * N, rel_tol=1e-13)
+
+
+def test_integrate_reference_2d(q, po):
+    # test/test_integration.jl:28-38: L = 30 comb (3, 10), exp(x) exp(y), (e-1)^2 to 1e-4
+    g = q.named_comb_tree((3, 10))
+    s = q.continuous_siteinds(g, map_dimension=2)
+    f = q.exp_itn(s, dim=1) * q.exp_itn(s, dim=2)
+    p, sd, cores, *_ = oracle_args(f)
+    assert abs(po.integrate(p, sd, cores, 0.5) - (math.e - 1) ** 2) < 1e-4
+
+
+def test_integrate_and_inner_brute_force(q, po, rng):
+    s = q.continuous_siteinds(q.named_comb_tree((3, 3)))
+    f = q.random_itn(s, link_space=2, rng=rng)
+    g = q.random_itn(s, link_space=3, rng=rng)
+    Tf = dense.dense_tensor(f.topo.parent, f.cores)
+    Tg = dense.dense_tensor(g.topo.parent, g.cores)
+    assert math.isclose(po.integrate(f.topo.parent, f.topo.sdim, f.cores, 0.5), Tf.mean(), rel_tol=1e-12)
+    assert math.isclose(po.inner(f.topo.parent, f.topo.sdim, f.cores, g.cores), float((Tf * Tg).sum()),
+                        rel_tol=1e-12)
+
+
+def test_integrate_operator_reference(q, po):
+    # test/test_integration.jl:15-26 with a Hadamard square instead of operator_proj:
+    # integral of exp(x)^2 over [0,1) = (e^2 - 1)/2 to 1e-4 (L = 20, comb (2, 10))
+    s = q.continuous_siteinds(q.named_comb_tree((2, 10)))
+    f = q.exp_itn(s)
+    sq = f * f
+    p, sd, cores, *_ = oracle_args(sq)
+    assert abs(po.integrate(p, sd, cores, 0.5) - (math.e ** 2 - 1) / 2) < 1e-4
+
+
+# --------------------------------------------------------------------------- operators
+def _shift_matrix(N, delta, boundary):
+    """M[s_in, s_out] of phi(k) = psi(k + delta) on N points."""
+    M = np.zeros((N, N))
+    for k in range(N):
+        j = k + delta
+        if 0 <= j < N:
+            M[j, k] = 1.0
+        elif boundary == "Periodic":
+            M[j % N, k] = 1.0
+        elif boundary == "Neumann":
+            M[k, k] = 1.0
+    return M
+
+
+@pytest.mark.parametrize("boundary", ["Dirichlet", "Periodic", "Neumann"])
+@pytest.mark.parametrize("n", [0, 1])
+@pytest.mark.parametrize("fwd", [True, False])
+def test_shift_operators_dense(q, boundary, n, fwd):
+    # src/elementary_operators.jl:32-103; checked against the definition
+    # phi(x) = psi(x +- (n+1) h) with the three boundary rules (A.6)
+    L = 6
+    s = q.continuous_siteinds(q.named_comb_tree((2, 3)))
+    op = (q.forward_shift_op if fwd else q.backward_shift_op)(s, boundary=boundary, n=n)
+    t = op.topo
+    M = dense.dense_operator(t.parent, op.cores)
+    # reorder vertex axes to digit order (MSB first) on both sides
+    order = np.argsort(t.vdigit)
+    M = M.reshape((2,) * (2 * L))
+    M = np.transpose(M, list(order) + [L + o for o in order]).reshape(2 ** L, 2 ** L)
+    ref = _shift_matrix(2 ** L, (n + 1) if fwd else -(n + 1), boundary)
+    assert np.allclose(M, ref, atol=1e-12)
+
+
+@pytest.mark.parametrize("order,coefs", [(1, [0, .5, 0, -.5, 0]), (2, [0, 1, -2, 1, 0]),
+                                         (3, [.5, -1, 0, 1, -.5]), (4, [1, -4, 6, -4, 1])])
+def test_derivative_operators_dense(q, order, coefs):
+    L = 7
+    s = q.continuous_siteinds(q.named_grid((L, 1)))
+    op = getattr(q, ["first", "second", "third", "fourth"][order - 1] + "_derivative_operator")(s)
+    M = dense.dense_operator(op.topo.parent, op.cores)
+    N = 2 ** L
+    ref = sum(c * _shift_matrix(N, sh, "Dirichlet") for c, sh in zip(coefs, [2, 1, 0, -1, -2]) if c)
+    ref = ref * float(2 ** (order * L))
+    assert np.allclose(M, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+def test_laplacian_2d_dense(q):
+    L = 4  # per dimension, interleaved on an 8-site chain
+    s = q.continuous_siteinds(q.named_grid((2 * L, 1)), map_dimension=2)
+    op = q.laplacian_operator(s)
+    M = dense.dense_operator(op.topo.parent, op.cores)
+    N = 2 ** L
+    D2 = sum(c * _shift_matrix(N, sh, "Dirichlet") for c, sh in zip([1, -2, 1], [1, 0, -1])) * float(4 ** L)
+    # axis order: x1 y1 x2 y2 ... -> reshape to (x bits, y bits)
+    M = M.reshape((2,) * (4 * L))
+    perm = [2 * i for i in range(L)] + [2 * i + 1 for i in range(L)]
+    perm = perm + [4 * L // 2 + p for p in perm]
+    M = np.transpose(M, perm).reshape(N * N, N * N)
+    ref = np.kron(D2, np.eye(N)) + np.kron(np.eye(N), D2)
+    assert np.allclose(M, ref, rtol=1e-12, atol=1e-10)
+
+
+def test_fredholm_operator_dense(q):
+    L = 6
+    s = q.continuous_siteinds(q.named_grid((L, 1)))
+    op = q.fredholm_operator(s)
+    M = dense.dense_operator(op.topo.parent, op.cores)
+    x = np.arange(2 ** L) / 2 ** L
+    ref = np.exp(-np.abs(x[:, None] - x[None, :])) / 2 ** L  # [s_in = y, s_out = x]
+    assert np.allclose(M, ref, rtol=1e-13)
+
+
+def test_contract_matches_dense(q, po, rng):
+    s = q.continuous_siteinds(q.named_comb_tree((2, 3)))
+    f = q.random_itn(s, link_space=3, rng=rng)
+    op = q.second_derivative_operator(s, left_boundary="Periodic", right_boundary="Neumann")
+    C = po.contract(f.topo.parent, f.topo.sdim, op.cores, f.cores)
+    M = dense.dense_operator(op.topo.parent, op.cores)
+    vf = dense.dense_tensor(f.topo.parent, f.cores).ravel()
+    vc = dense.dense_tensor(f.topo.parent, C).ravel()
+    assert np.allclose(vc, M.T @ vf, rtol=1e-11, atol=1e-9)
+
+
+def test_telescoping_second_derivative(q, po):
+    # integral of the Dirichlet D^2 f = -(f(1-h) + f(0)) / h  (exact identity)
+    L = 12
+    s = q.continuous_siteinds(q.named_grid((L, 1)))
+    f = q.sin_itn(s, k=2 * math.pi) + q.cos_itn(s, k=6 * math.pi, c=0.5) + q.exp_itn(s, k=-1.0)
+    op = q.second_derivative_operator(s)
+    C = po.contract(f.topo.parent, f.topo.sdim, op.cores, f.cores)
+    h = 2.0 ** -L
+    fx = lambda x: math.sin(2 * math.pi * x) + 0.5 * math.cos(6 * math.pi * x) + math.exp(-x)
+    ref = -(fx(1 - h) + fx(0.0)) / h
+    got = po.integrate(f.topo.parent, f.topo.sdim, C, 0.5)
+    assert math.isclose(got, ref, rel_tol=1e-9)
+
+
+def test_derivative_accuracy_reference(q, po):
+    # test/test_operators.jl:16-46 (L = 9, periodic, rtol 1e-3 on 1 + f')
+    s = q.continuous_siteinds(q.named_grid((9, 1)))
+    f = q.sin_itn(s, k=2 * math.pi)
+    xs = np.array([[0.0], [0.25], [0.625], [0.875], [1 - 2.0 ** -9]])
+    for order, ref in [(1, lambda x: 2 * math.pi * math.cos(2 * math.pi * x)),
+                       (2, lambda x: -(2 * math.pi) ** 2 * math.sin(2 * math.pi * x))]:
+        op = getattr(q, ["first", "second"][order - 1] + "_derivative_operator")(
+            s, left_boundary="Periodic", right_boundary="Periodic")
+        C = po.contract(f.topo.parent, f.topo.sdim, op.cores, f.cores)
+        C, _ = po.truncate(f.topo.parent, f.topo.sdim, C, 1e-8)
+        y, _ = po.evaluate(f.topo.parent, f.topo.sdim, C, f.topo.vcoord, f.topo.vdigit, f.topo.Lc, xs)
+        for yi, x in zip(y, xs[:, 0]):
+            assert math.isclose(1 + yi, 1 + ref(x), rel_tol=1e-3)
+
+
+# --------------------------------------------------------------------------- truncation
+def test_truncate_single_edge_is_dense_svd(q, po, rng):
+    s = q.continuous_siteinds(q.named_grid((2, 1)))
+    A = rng.standard_normal((1, 2, 2))
+    B = rng.standard_normal((2, 2))
+    f = q.ITensorNetworkFunction(s, [A, B])
+    M = np.einsum("asb,bt->st", A, B)
+    out, err = po.truncate(f.topo.parent, f.topo.sdim, f.cores, 0.0, 1)
+    S = np.linalg.svd(M, compute_uv=False)
+    U, Sv, Vt = np.linalg.svd(M)
+    ref = Sv[0] * np.outer(U[:, 0], Vt[0])
+    got = np.einsum("asb,bt->st", out[0], out[1])
+    assert np.allclose(got, ref, atol=1e-13)
+    assert math.isclose(err, S[1] ** 2 / (S ** 2).sum(), rel_tol=1e-12)
+
+
+def test_truncate_lossless_on_low_rank(q, po, rng):
+    s = q.continuous_siteinds(q.named_grid((16, 1)), map_dimension=2)
+    f = q.sin_itn(s, k=7.0, dim=1) * q.exp_itn(s, k=-1.5, dim=2) + q.cos_itn(s, k=2.0, dim=2)
+    f = f + f  # bond 8; true rank <= 4 (sin/cos of x times exp of y, plus cos/sin of y)
+    out, err = po.truncate(f.topo.parent, f.topo.sdim, f.cores, 1e-16)
+    assert max(c.shape[0] for c in out) <= 4
+    xs = rng.uniform(0, 1, (300, 2))
+    y0 = oracle_eval(po, f, xs)
+    y1, _ = po.evaluate(f.topo.parent, f.topo.sdim, out, f.topo.vcoord, f.topo.vdigit, f.topo.Lc, xs)
+    assert rel_l2(y1, y0) < 1e-13
+    assert err < 1e-15
+
+
+@pytest.mark.parametrize("graph", ["chain", "comb"])
+def test_truncate_error_bound(q, po, rng, graph):
+    g = q.named_grid((10, 1)) if graph == "chain" else q.named_comb_tree((3, 4))
+    s = q.continuous_siteinds(g)
+    f = q.random_itn(s, link_space=6, rng=rng)
+    Tf = dense.dense_tensor(f.topo.parent, f.cores)
+    out, err = po.truncate(f.topo.parent, f.topo.sdim, f.cores, 1e-3, 4)
+    Tg = dense.dense_tensor(f.topo.parent, out)
+    assert max(c.shape[0] for c in out) <= 4
+    # sequential canonical truncation: ||f - g||^2 / ||f||^2 <= sum of discarded weights
+    assert np.sum((Tf - Tg) ** 2) / np.sum(Tf ** 2) <= err * (1 + 1e-9) + 1e-14
+    # gauge: every non-root edge produced by an SVD keeps the cut-side isometric;
+    # norms agree up to the discarded weight
+    assert np.sum(Tg ** 2) <= np.sum(Tf ** 2) * (1 + 1e-12)
+
+
+# --------------------------------------------------------------------------- gradient
+def test_gradient_vs_torch_autograd(q, po, rng):
+    import torch
+    s = q.continuous_siteinds(q.named_comb_tree((2, 3)), map_dimension=2)
+    f = q.random_itn(s, link_space=3, rng=rng)
+    xs = rng.uniform(0, 1, (64, 2))
+    yb = rng.standard_normal(64)
+    t = f.topo
+    grads = po.eval_grad(t.parent, t.sdim, f.cores, t.vcoord, t.vdigit, t.Lc, xs, yb)
+    # independent: torch float64 autograd of the dense tensor contraction
+    cores = [torch.tensor(c, dtype=torch.float64, requires_grad=True) for c in f.cores]
+    T = _torch_dense(t, cores)
+    idx = np.zeros((64, t.nv), dtype=np.int64)
+    for v in range(t.nv):
+        c, j, L = t.vcoord[v], t.vdigit[v], t.Lc[t.vcoord[v]]
+        k = np.floor(xs[:, c] * 2.0 ** L).astype(np.int64)
+        idx[:, v] = (k >> (L - j)) & 1
+    vals = T[tuple(torch.tensor(idx[:, v]) for v in range(t.nv))]
+    (vals * torch.tensor(yb)).sum().backward()
+    for g, c in zip(grads, cores):
+        assert np.allclose(g, c.grad.numpy(), rtol=1e-12, atol=1e-12)
+
+
+def _torch_dense(t, cores):
+    import torch
+    sub = {}
+    for v in reversed(range(t.nv)):
+        T = cores[v]
+        verts = [v]
+        for c in t.children[v]:
+            Tc, vc = sub.pop(c)
+            ax = 1 + len(verts)
+            T = torch.tensordot(T, Tc, dims=([ax], [0]))
+            nc = Tc.dim() - 1
+            T = torch.movedim(T, list(range(T.dim() - nc, T.dim())), list(range(ax, ax + nc)))
+            verts += vc
+        sub[v] = (T, verts)
+    T, verts = sub[0]
+    T = T[0]
+    return T.permute(*np.argsort(verts).tolist())
+
+
+def test_gradient_finite_differences(q, po, rng):
+    s = q.continuous_siteinds(q.named_grid((8, 1)), map_dimension=2)
+    f = q.random_itn(s, link_space=4, rng=rng, std=0.5)
+    xs = rng.uniform(0, 1, (32, 2))
+    yb = rng.standard_normal(32)
+    t = f.topo
+    grads = po.eval_grad(t.parent, t.sdim, f.cores, t.vcoord, t.vdigit, t.Lc, xs, yb)
+    h = 1e-6
+    for v in (0, 3, 7):
+        core = f.cores[v]
+        for flat in rng.choice(core.size, 4, replace=False):
+            cp = [c.copy() for c in f.cores]
+            cm = [c.copy() for c in f.cores]
+            cp[v].flat[flat] += h
+            cm[v].flat[flat] -= h
+            yp, _ = po.evaluate(t.parent, t.sdim, cp, t.vcoord, t.vdigit, t.Lc, xs)
+            ym, _ = po.evaluate(t.parent, t.sdim, cm, t.vcoord, t.vdigit, t.Lc, xs)
+            fd = float(yb @ (yp - ym)) / (2 * h)
+            assert math.isclose(grads[v].flat[flat], fd, rel_tol=1e-6, abs_tol=1e-8)
+
+
+def test_reference_known_answers_file():
+    """The golden file lists the reference's known-answer tests used above."""
+    with open(os.path.join(GOLDEN, "reference_known_answers.json")) as fh:
+        cases = json.load(fh)
+    assert {c["source"].split(":")[0] for c in cases} >= {
+        "test/test_indexmaps.jl", "test/test_integration.jl", "test/test_realitensorfunction.jl",
+        "test/test_operators.jl"}
