@@ -1,0 +1,179 @@
+"""Benchmark of the quantics hot path on the 2D Laplacian workload (one GPU).
+
+Workload: 2D function, 20 binary digits per coordinate interleaved on a
+40-site chain, f = sum of 8 separable sin(w x) exp(-b y) terms (seed 1,
+bond 16, compressed losslessly to <= 64), Laplacian operator (Dirichlet),
+operate + truncate at cutoff 1e-10 / maxdim 128, then evaluation of the
+result at 1e6 uniform random points that stay resident in HBM.
+
+One step = one evaluation pass over the 1e6 points (the headline points/s).
+operate+truncate is timed in the same run (ms per call).  float64 and the
+float32 hi/lo split path are both timed; `value` is the faster path whose
+relative L2 error against the float64 result is <= 1e-10.  The CPU oracle
+(C, OpenMP) is timed on a bounded sample on the same host.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+HBM_PEAK_GBS = 8000.0
+
+
+def chain_eval_flops(bonds):
+    """Useful flops per point of the chain reduction: 2 sum_k chi_k chi_{k+1}."""
+    b = list(bonds) + [1]
+    return 2 * sum(int(b[k]) * int(b[k + 1]) for k in range(len(bonds)))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--npts", type=int, default=1_000_000)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); the engine has no "
+              "multi-GPU path in this build.", file=sys.stderr)
+        sys.exit(1)
+
+    import torch
+
+    import __graft_entry__ as ge
+    q = ge._load_package()
+    from itensornumericalanalysis_jl_amd import workloads
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    ctx = q.engine_context()
+    ctx.set_stream(stream.cuda_stream)
+
+    w = workloads.config2(npts=args.npts)
+    # setup (not timed): lossless compression of f, then one operate to get the result
+    fc = q.truncate(w.f, cutoff=1e-16, maxdim=w.extra["compress_maxdim"])
+    g, terr = q.operate(w.op, fc, cutoff=w.cutoff, maxdim=w.maxdim, return_error=True)
+    bonds = [int(b) for b in g.bonds()]
+    flops_pt = chain_eval_flops(bonds)
+    xs_d = torch.tensor(w.xs, device=dev)
+    out64 = torch.empty(args.npts, dtype=torch.float64, device=dev)
+    out32 = torch.empty(args.npts, dtype=torch.float64, device=dev)
+
+    def timed_eval(prec, out):
+        for _ in range(args.warmup):
+            q.evaluate(g, xs_d, precision=prec, out=out)
+        ctx.sync()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for a, b in evs:
+            a.record(stream)
+            q.evaluate(g, xs_d, precision=prec, out=out)
+            b.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ctx.sync()
+        kms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+        return wall, kms
+
+    wall64, k64 = timed_eval(0, out64)
+    wall32, k32 = timed_eval(1, out32)
+    y64 = out64.cpu().numpy()
+    y32 = out32.cpu().numpy()
+    err_split = float(np.linalg.norm(y32 - y64) / np.linalg.norm(y64))
+
+    # operate + truncate (device-resident input and output), float64 truncation
+    op_times = []
+    for i in range(args.warmup + args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        gi = q.operate(w.op, fc, cutoff=w.cutoff, maxdim=w.maxdim)
+        torch.cuda.synchronize()
+        if i >= args.warmup:
+            op_times.append((time.perf_counter() - t0) * 1e3)
+        del gi
+    op_ms = float(np.median(op_times))
+
+    pts64 = args.npts * args.steps / wall64
+    pts32 = args.npts * args.steps / wall32
+    use_split = err_split <= 1e-10 and pts32 > pts64
+    value = pts32 if use_split else pts64
+    kms = k32 if use_split else k64
+    achieved = flops_pt * args.npts / (k64 * 1e-3) / 1e12  # float64 kernel
+    bytes_pt = 2 * 8 + 8
+
+    # parity of the timed result against the CPU oracle on a sample of points
+    from oracle import pyoracle as po
+    t = g.topo
+    cores = g.cores
+    ns = 20000
+    yref, _ = po.evaluate(t.parent, t.sdim, cores, t.vcoord, t.vdigit, t.Lc, w.xs[:ns])
+    err_oracle = float(np.linalg.norm(y64[:ns] - yref) / np.linalg.norm(yref))
+
+    cpu = None
+    if not args.no_cpu:
+        ncores = min(16, len(os.sched_getaffinity(0)))
+        nsamp = min(args.npts, 1_000_000)
+        po.evaluate(t.parent, t.sdim, cores, t.vcoord, t.vdigit, t.Lc, w.xs[:1000], nthreads=ncores)
+        t0 = time.perf_counter()
+        reps = 0
+        while time.perf_counter() - t0 < 5.0 or reps == 0:
+            po.evaluate(t.parent, t.sdim, cores, t.vcoord, t.vdigit, t.Lc, w.xs[:nsamp], nthreads=ncores)
+            reps += 1
+        cpu_pts = nsamp * reps / (time.perf_counter() - t0)
+        # operate + truncate on the CPU: exact contraction + truncation sweep (1 thread)
+        fcc = fc.cores
+        t0 = time.perf_counter()
+        C = po.contract(t.parent, t.sdim, w.op.cores, fcc)
+        po.truncate(t.parent, t.sdim, C, w.cutoff, w.maxdim)
+        cpu_op_ms = (time.perf_counter() - t0) * 1e3
+        cpu = {"value": cpu_pts, "unit": "points/s", "cores": ncores, "kind": "port",
+               "sample": f"oracle or_eval over {nsamp} points x {reps} reps, {ncores} OpenMP threads; "
+                         f"operate+truncate (1 thread) {cpu_op_ms:.1f} ms",
+               "operate_truncate_ms": cpu_op_ms}
+
+    rec = {
+        "metric": "points/s + operate+truncate ms; mixed vs FP64 GPU at <=1e-10 rel err",
+        "value": value,
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * (wall32 if use_split else wall64) / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-split" if use_split else "f64",
+        "data": "synthetic (seeded numpy PCG64; no reference data)",
+        "config": {"workload": "config2: 2D Laplacian, 2x20 interleaved digits on a 40-site chain, "
+                               "operate+truncate cutoff 1e-10 maxdim 128, evaluate 1e6 points",
+                   "npts": args.npts, "result_bonds_max": max(bonds), "parallelism": "single GPU"},
+        "operate_truncate_ms": op_ms,
+        "fp64_points_per_s": pts64,
+        "split_points_per_s": pts32,
+        "split_rel_err_vs_fp64": err_split,
+        "fp64_rel_err_vs_oracle": err_oracle,
+        "trunc_err": terr,
+        "flops_per_point": flops_pt,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "eval_chain_f64_kernel", "kernel_ms": k64,
+                     "hbm_gbs": bytes_pt * args.npts / (k64 * 1e-3) / 1e9},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(rec))
+
+
+if __name__ == "__main__":
+    main()

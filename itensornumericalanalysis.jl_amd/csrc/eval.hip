@@ -125,7 +125,10 @@ struct ff {
   float h, l;
 };
 
+// The error-free transformations need every + and * rounded on its own:
+// contraction into FMA (hipcc's default) would silently drop the low word.
 __device__ inline ff two_sum(float a, float b) {
+#pragma clang fp contract(off)
   float s = a + b;
   float bb = s - a;
   float e = (a - (s - bb)) + (b - bb);
@@ -134,6 +137,7 @@ __device__ inline ff two_sum(float a, float b) {
 
 __device__ inline ff ff_fma(float ah, float al, float bh, float bl, ff c) {
   // c + (ah + al) * (bh + bl), dropping al * bl
+#pragma clang fp contract(off)
   float p = ah * bh;
   float pe = __builtin_fmaf(ah, bh, -p);
   pe = __builtin_fmaf(ah, bl, pe);
@@ -366,6 +370,14 @@ int launch_eval(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t npts, doub
     const int64_t nb = (npts + bs - 1) / bs;
     if (nb > INT32_MAX) return qtt_fail(ctx, QTT_EINVAL, "evaluate: too many points");
     const size_t lds = per_lane * bs;
+    if (lds > 64 * 1024) {
+      const void* fn = precision == QTT_F64 ? (const void*)eval_chain_f64_kernel
+                       : precision == QTT_F32_SPLIT ? (const void*)eval_chain_f32_kernel<true>
+                                                    : (const void*)eval_chain_f32_kernel<false>;
+      rc = qtt_hip_check(ctx, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                         "evaluate LDS attribute");
+      if (rc) return rc;
+    }
     if (precision == QTT_F64) {
       eval_chain_f64_kernel<<<dim3((unsigned)nb), bs, lds, ctx->stream>>>(
           net->d_pack, net->d_pack_off, net->d_bond, tv, xs, npts, out, ctx->d_flag, (int)maxb);
