@@ -242,6 +242,8 @@ int qtt_ctx_create(int device, qtt_ctx** out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cus = prop.multiProcessorCount;
   c->stream = c->own_stream;
+  const char* ek = getenv("QTT_EVAL_KERNEL");
+  c->force_lds_eval = ek && std::strcmp(ek, "lds") == 0;
   *out = c;
   return QTT_OK;
 }
@@ -386,6 +388,7 @@ void qtt_net_destroy(qtt_net* n) {
   if (n->d_pack) (void)hipFree(n->d_pack);
   if (n->d_pack_off) (void)hipFree(n->d_pack_off);
   if (n->d_pack32) (void)hipFree(n->d_pack32);
+  if (n->d_desc) (void)hipFree(n->d_desc);
   delete n;
 }
 

@@ -43,8 +43,9 @@ __global__ void pack_chain_kernel(const double* __restrict__ data, const int64_t
   const int chil = bond[k];
   const int chir = (k == nv - 1) ? 1 : bond[k + 1];
   const int lpad = (chil + 3) & ~3;
+  const int rpad = (chir + 3) & ~3;
   const int sd = sdim[k];
-  const int64_t n = (int64_t)chir * lpad * 2;
+  const int64_t n = (int64_t)rpad * lpad * 2;
   const double* T = data + off[k];
   double* P = pack + poff[k];
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
@@ -53,7 +54,7 @@ __global__ void pack_chain_kernel(const double* __restrict__ data, const int64_t
     int l = (int)(rest % lpad);
     int r = (int)(rest / lpad);
     double val = 0.0;
-    if (l < chil) {
+    if (l < chil && r < chir) {
       int ss = sd == 1 ? 0 : s;
       val = T[((int64_t)l * sd + ss) * chir + r];
     }
@@ -113,6 +114,76 @@ eval_chain_f64_kernel(const double* __restrict__ pack, const int64_t* __restrict
     double* tmp = v;
     v = w;
     w = tmp;
+  }
+  if (active) out[p] = bad ? __builtin_nan("") : v[0];
+  if (bad) flag[0] = 1;
+}
+
+// Register-resident chain kernel for bonds <= CHI (4, 8, 16 or 32).  Same
+// arithmetic as eval_chain_f64_kernel; the per-lane vectors stay in VGPRs
+// (fully unrolled, compile-time indices) and rows of the packed core are zero
+// padded to a multiple of 4, so four 64-byte scalar loads are issued back to
+// back for each group of 4 rows before their 32 FMAs.
+struct SiteDesc {
+  int chil, chir, coord, shift;  // coord < 0: no digit on this site
+};
+
+template <int CHI>
+__global__ void __launch_bounds__(256)
+eval_chain_reg_kernel(const double* __restrict__ pack, const int64_t* __restrict__ poff,
+                      const SiteDesc* __restrict__ desc, int nv, int nc, const int* __restrict__ ndigits,
+                      const double* __restrict__ xs, int64_t npts, double* __restrict__ out,
+                      int* __restrict__ flag) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = p < npts;
+  bool bad = false;
+  uint64_t kw[QTT_MAX_COORD] = {0, 0, 0, 0};
+#pragma unroll
+  for (int c = 0; c < QTT_MAX_COORD; ++c)
+    if (c < nc) kw[c] = digit_word(active ? xs[p * nc + c] : 0.0, ndigits[c], bad);
+  double v[CHI], w[CHI];
+#pragma unroll
+  for (int i = 0; i < CHI; ++i) v[i] = 0.0;
+  v[0] = 1.0;
+  for (int k = nv - 1; k >= 0; --k) {
+    const SiteDesc sd = desc[k];
+    int d = 0;
+    if (sd.coord >= 0) {
+      const uint64_t kk = sd.coord == 0 ? kw[0] : sd.coord == 1 ? kw[1] : sd.coord == 2 ? kw[2] : kw[3];
+      d = (int)((kk >> sd.shift) & 1u);
+    }
+    const uint64_t dmask = 0 - (uint64_t)d;
+    const double* __restrict__ P = pack + poff[k];
+    const int lpad = (sd.chil + 3) & ~3;
+#pragma unroll
+    for (int l0 = 0; l0 < CHI; l0 += 4) {
+      if (l0 < sd.chil) {
+        double a0[4] = {0.0, 0.0, 0.0, 0.0}, a1[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int r0 = 0; r0 < CHI; r0 += 4) {
+          if (r0 < sd.chir) {
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) {
+              const double* __restrict__ q = P + ((r0 + rr) * lpad + l0) * 2;
+              const double vr = v[r0 + rr];
+#pragma unroll
+              for (int i = 0; i < 4; ++i) {
+                a0[i] = __builtin_fma(q[2 * i], vr, a0[i]);
+                a1[i] = __builtin_fma(q[2 * i + 1], vr, a1[i]);
+              }
+            }
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) w[l0 + i] = bitsel(a0[i], a1[i], dmask);
+      } else {
+        // keep entries past the bond finite: the next site's padded rows multiply them by 0
+#pragma unroll
+        for (int i = 0; i < 4; ++i) w[l0 + i] = 0.0;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < CHI; ++i) v[i] = w[i];
   }
   if (active) out[p] = bad ? __builtin_nan("") : v[0];
   if (bad) flag[0] = 1;
@@ -296,13 +367,28 @@ static int ensure_pack(qtt_ctx* ctx, qtt_net* net) {
   if (net->pack_valid) return QTT_OK;
   const int nv = net->t.nv;
   std::vector<int64_t> poff(nv);
+  std::vector<int> desc(4 * nv);
   int64_t o = 0;
   for (int k = 0; k < nv; k++) {
     int64_t chil = net->bond[k];
     int64_t chir = (k == nv - 1) ? 1 : net->bond[k + 1];
     int64_t lpad = (chil + 3) & ~int64_t(3);
+    int64_t rpad = (chir + 3) & ~int64_t(3);
     poff[k] = o;
-    o += chir * lpad * 2;
+    o += rpad * lpad * 2;
+    const int c = net->t.vcoord[k];
+    desc[4 * k + 0] = (int)chil;
+    desc[4 * k + 1] = (int)chir;
+    desc[4 * k + 2] = c;
+    desc[4 * k + 3] = c >= 0 ? net->t.ndigits[c] - net->t.vdigit[k] : 0;
+  }
+  if (!net->d_desc) {
+    int rc = qtt_hip_check(ctx, hipMalloc(&net->d_desc, 4 * nv * sizeof(int)), "hipMalloc site descriptors");
+    if (rc) return rc;
+  }
+  {
+    int rc = qtt_hip_check(ctx, hipMemcpyAsync(net->d_desc, desc.data(), 4 * nv * sizeof(int), hipMemcpyHostToDevice, ctx->stream), "site descriptors");
+    if (rc) return rc;
   }
   if (net->d_pack && o > net->pack_elems) {
     (void)hipFree(net->d_pack);
@@ -361,6 +447,21 @@ int launch_eval(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t npts, doub
   if (t.chain && t.ncoord <= QTT_MAX_COORD) {
     int rc = precision == QTT_F64 ? ensure_pack(ctx, net) : ensure_pack32(ctx, net);
     if (rc) return rc;
+    if (precision == QTT_F64 && maxb <= 32 && !ctx->force_lds_eval) {
+      const int bsr = 256;
+      const int64_t nbr = (npts + bsr - 1) / bsr;
+      if (nbr > INT32_MAX) return qtt_fail(ctx, QTT_EINVAL, "evaluate: too many points");
+      const SiteDesc* dd = (const SiteDesc*)net->d_desc;
+#define QTT_REG_LAUNCH(C)                                                                          \
+  eval_chain_reg_kernel<C><<<dim3((unsigned)nbr), bsr, 0, ctx->stream>>>(                          \
+      net->d_pack, net->d_pack_off, dd, t.nv, t.ncoord, tv.ndigits, xs, npts, out, ctx->d_flag)
+      if (maxb <= 4) QTT_REG_LAUNCH(4);
+      else if (maxb <= 8) QTT_REG_LAUNCH(8);
+      else if (maxb <= 16) QTT_REG_LAUNCH(16);
+      else QTT_REG_LAUNCH(32);
+#undef QTT_REG_LAUNCH
+      return qtt_hip_check(ctx, hipGetLastError(), "eval_chain_reg_kernel");
+    }
     // LDS: two vectors of maxb entries per lane (double, or a float pair)
     const size_t per_lane = 2 * size_t(maxb) * sizeof(double);
     int bs = 256;

@@ -6,31 +6,64 @@
 //
 // truncate_kernel: the truncation sweep of ITensorNetworks.truncate(::TTN)
 //   (called from src/itensornetworkfunction.jl:114-118), run by one workgroup
-//   per network over a host-built schedule of moves.  A move (x -> u) views
-//   core x as a matrix (all other indices) x (bond to u) and
+//   per network over a host-built schedule of moves; a batch of networks is one
+//   launch with one workgroup each.  A move (x -> u) views core x as a matrix
+//   (all other indices) x (bond to u) and
 //     * QR move:  Householder QR, x <- Q, u <- R (x)_bond u;
 //     * SVD move: Householder QR if the matrix is tall, then one-sided Jacobi
 //       on the rows of the (square or wide) factor with the rotations kept in J,
 //       singular values sorted, the ITensors cutoff rule applied, x <- U (an
 //       exact product of orthogonal factors), u <- S V^T (x)_bond u.
 //   The matrix lives in LDS when it fits (158 KiB), otherwise in a global
-//   scratch area; all arithmetic is FP64.  Rounds of the Jacobi sweep use the
-//   round-robin (circle) pairing so every pair in a round is independent: one
-//   wavefront per pair, lanes over the row.
+//   scratch area; all arithmetic is FP64.
+//   Work is spread over lane groups of G lanes (G = 8 for matrices of up to
+//   256 rows, a whole wavefront above): one group per column in the QR, one
+//   group per row pair in each Jacobi round (round-robin pairing, all pairs of
+//   a round independent).  Group sums use DPP lane permutes, not LDS.
 #include <cfloat>
 
 #include "qtt_internal.h"
 
 namespace {
 
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+// ---------------------------------------------------------------------------
+// cross-lane sums with DPP (double = two 32-bit halves)
+// ---------------------------------------------------------------------------
+template <int CTRL>
+__device__ inline double dpp_d(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ inline double readlane_d(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+
+// sum over aligned groups of 8 lanes (result in every lane of the group)
+__device__ inline double sum8(double v) {
+  v += dpp_d<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_d<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_d<0x141>(v);  // row_half_mirror
   return v;
 }
 
+__device__ inline double sum64(double v) {
+  v = sum8(v);
+  v += dpp_d<0x140>(v);  // row_mirror: 16-lane sums
+  return readlane_d(v, 0) + readlane_d(v, 16) + readlane_d(v, 32) + readlane_d(v, 48);
+}
+
+template <int G>
+__device__ inline double gsum(double v) {
+  if constexpr (G == 8) return sum8(v);
+  else return sum64(v);
+}
+
 __device__ inline double block_sum(double v, double* red) {
-  v = wave_sum(v);
+  v = sum64(v);
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
   __syncthreads();
   if (lane == 0) red[wid] = v;
@@ -93,12 +126,11 @@ struct SweepArgs {
   int nsteps;
   double cutoff;
   long long maxdim;
-  double* gscr;      // global workspace (used when a matrix does not fit LDS)
-  int64_t gscr_ws;   // doubles available for the workspace part
-  double* gY;        // global buffer for the mode-product result
-  int ws_lds;        // doubles of LDS workspace
+  double* gscr;  // global workspace (used when a matrix does not fit LDS)
+  double* gY;    // global buffer for the mode-product result
+  int ws_lds;    // doubles of LDS workspace
   double* err_out;
-  int* status;       // [1] <- non-convergence
+  int* status;   // [1] <- non-convergence
 };
 
 __device__ inline int core_dims(const TopoView& t, const int* bond, int v, int64_t* dims) {
@@ -125,67 +157,144 @@ __device__ inline void split_dims(const int64_t* dims, int nd, int ax, int64_t& 
   for (int i = ax + 1; i < nd; ++i) B *= dims[i];
 }
 
+// Householder reflector of column j (LAPACK dlarfg), computed by one group:
+// v = [1; W[j+1:, j] * scal] stored below the diagonal, beta on the diagonal.
+// The strided loops below run 4 elements per iteration (loads first, then the
+// arithmetic) so several LDS/L2 loads are in flight per wait; the summation
+// order is fixed, so results stay deterministic.
+template <int G>
+__device__ inline void grp_reflector(double* W, int rows, int ld, int j, double* tau) {
+  const int gl = threadIdx.x & (G - 1);
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int i = j + 1 + gl;
+  for (; i + 3 * G < rows; i += 4 * G) {
+    const double x0 = W[(int64_t)i * ld + j], x1 = W[(int64_t)(i + G) * ld + j];
+    const double x2 = W[(int64_t)(i + 2 * G) * ld + j], x3 = W[(int64_t)(i + 3 * G) * ld + j];
+    s0 = __builtin_fma(x0, x0, s0);
+    s1 = __builtin_fma(x1, x1, s1);
+    s2 = __builtin_fma(x2, x2, s2);
+    s3 = __builtin_fma(x3, x3, s3);
+  }
+  for (; i < rows; i += G) {
+    const double x = W[(int64_t)i * ld + j];
+    s0 = __builtin_fma(x, x, s0);
+  }
+  const double xn2 = gsum<G>((s0 + s1) + (s2 + s3));
+  const double x0 = W[(int64_t)j * ld + j];
+  double tj = 0.0, beta = x0;
+  if (xn2 > 0.0) {
+    const double nrm = sqrt(x0 * x0 + xn2);
+    beta = x0 >= 0.0 ? -nrm : nrm;
+    tj = (beta - x0) / beta;
+    const double scal = 1.0 / (x0 - beta);
+    i = j + 1 + gl;
+    for (; i + 3 * G < rows; i += 4 * G) {
+      double* p0 = W + (int64_t)i * ld + j;
+      double* p1 = W + (int64_t)(i + G) * ld + j;
+      double* p2 = W + (int64_t)(i + 2 * G) * ld + j;
+      double* p3 = W + (int64_t)(i + 3 * G) * ld + j;
+      const double y0 = *p0, y1 = *p1, y2 = *p2, y3 = *p3;
+      *p0 = y0 * scal;
+      *p1 = y1 * scal;
+      *p2 = y2 * scal;
+      *p3 = y3 * scal;
+    }
+    for (; i < rows; i += G) W[(int64_t)i * ld + j] *= scal;
+  }
+  if (gl == 0) {
+    tau[j] = tj;
+    W[(int64_t)j * ld + j] = beta;
+  }
+}
+
+// apply H = I - tau v v^T (v[0] = 1 implicit, v[i] = vcol[i * vstride] for i > j) to column col
+template <int G>
+__device__ inline void grp_apply(double* W, int rows, int ld, int j, int col, const double* vcol,
+                                 int vstride, double tj) {
+  const int gl = threadIdx.x & (G - 1);
+  double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
+  int i = j + 1 + gl;
+  for (; i + 3 * G < rows; i += 4 * G) {
+    const double v0 = vcol[(int64_t)i * vstride], v1 = vcol[(int64_t)(i + G) * vstride];
+    const double v2 = vcol[(int64_t)(i + 2 * G) * vstride], v3 = vcol[(int64_t)(i + 3 * G) * vstride];
+    const double w0 = W[(int64_t)i * ld + col], w1 = W[(int64_t)(i + G) * ld + col];
+    const double w2 = W[(int64_t)(i + 2 * G) * ld + col], w3 = W[(int64_t)(i + 3 * G) * ld + col];
+    d0 = __builtin_fma(v0, w0, d0);
+    d1 = __builtin_fma(v1, w1, d1);
+    d2 = __builtin_fma(v2, w2, d2);
+    d3 = __builtin_fma(v3, w3, d3);
+  }
+  for (; i < rows; i += G) d0 = __builtin_fma(vcol[(int64_t)i * vstride], W[(int64_t)i * ld + col], d0);
+  const double d = (gsum<G>((d0 + d1) + (d2 + d3)) + W[(int64_t)j * ld + col]) * tj;
+  i = j + 1 + gl;
+  for (; i + 3 * G < rows; i += 4 * G) {
+    const double v0 = vcol[(int64_t)i * vstride], v1 = vcol[(int64_t)(i + G) * vstride];
+    const double v2 = vcol[(int64_t)(i + 2 * G) * vstride], v3 = vcol[(int64_t)(i + 3 * G) * vstride];
+    double* p0 = W + (int64_t)i * ld + col;
+    double* p1 = W + (int64_t)(i + G) * ld + col;
+    double* p2 = W + (int64_t)(i + 2 * G) * ld + col;
+    double* p3 = W + (int64_t)(i + 3 * G) * ld + col;
+    const double w0 = *p0, w1 = *p1, w2 = *p2, w3 = *p3;
+    *p0 = __builtin_fma(-d, v0, w0);
+    *p1 = __builtin_fma(-d, v1, w1);
+    *p2 = __builtin_fma(-d, v2, w2);
+    *p3 = __builtin_fma(-d, v3, w3);
+  }
+  for (; i < rows; i += G) W[(int64_t)i * ld + col] = __builtin_fma(-d, vcol[(int64_t)i * vstride], W[(int64_t)i * ld + col]);
+  if (gl == 0) W[(int64_t)j * ld + col] -= d;
+}
+
+__device__ inline int first_owned(int start, int gid, int ng) {
+  return start + ((gid - start % ng) + ng) % ng;
+}
+
 // Householder QR in place (LAPACK dgeqr2): W is rows x cols with leading
 // dimension ld; reflectors below the diagonal, R on and above it, tau[k].
-__device__ void geqr2(double* W, int rows, int cols, int ld, double* tau, double* red) {
-  const int tid = threadIdx.x, bs = blockDim.x;
-  const int lane = tid & 63, wid = tid >> 6, nw = bs >> 6;
+// Column c belongs to group c % ng; the owner of column j+1 updates it first
+// and builds the next reflector right away (look-ahead): one barrier per column.
+template <int G>
+__device__ void geqr2(double* W, int rows, int cols, int ld, double* tau) {
+  const int gid = threadIdx.x / G, ng = blockDim.x / G;
   const int k = rows < cols ? rows : cols;
+  if (k <= 0) return;
+  if (gid == 0) grp_reflector<G>(W, rows, ld, 0, tau);
+  __syncthreads();
   for (int j = 0; j < k; ++j) {
-    double s = 0.0;
-    for (int i = j + 1 + tid; i < rows; i += bs) {
-      const double x = W[(int64_t)i * ld + j];
-      s += x * x;
-    }
-    const double xn2 = block_sum(s, red);
-    const double x0 = W[(int64_t)j * ld + j];
-    double tj = 0.0, beta = x0, scal = 0.0;
-    if (xn2 > 0.0) {
-      const double nrm = sqrt(x0 * x0 + xn2);
-      beta = x0 >= 0.0 ? -nrm : nrm;
-      tj = (beta - x0) / beta;
-      scal = 1.0 / (x0 - beta);
-      for (int i = j + 1 + tid; i < rows; i += bs) W[(int64_t)i * ld + j] *= scal;
-    }
-    __syncthreads();
-    if (tj != 0.0) {
-      for (int col = j + 1 + wid; col < cols; col += nw) {
-        double d = 0.0;
-        for (int i = j + 1 + lane; i < rows; i += 64) d += W[(int64_t)i * ld + j] * W[(int64_t)i * ld + col];
-        d = (wave_sum(d) + W[(int64_t)j * ld + col]) * tj;
-        for (int i = j + 1 + lane; i < rows; i += 64) W[(int64_t)i * ld + col] -= d * W[(int64_t)i * ld + j];
-        if (lane == 0) W[(int64_t)j * ld + col] -= d;
-      }
-    }
-    if (tid == 0) {
-      tau[j] = tj;
-      W[(int64_t)j * ld + j] = beta;
+    const double tj = tau[j];
+    for (int col = first_owned(j + 1, gid, ng); col < cols; col += ng) {
+      if (tj != 0.0) grp_apply<G>(W, rows, ld, j, col, W + j, ld, tj);
+      if (col == j + 1 && j + 1 < k) grp_reflector<G>(W, rows, ld, j + 1, tau);
     }
     __syncthreads();
   }
 }
 
-// Form Q (rows x k) in place from the reflectors (LAPACK dorg2r).
-__device__ void org2r(double* W, int rows, int k, int ld, const double* tau) {
+// Form Q (rows x k) in place from the reflectors (LAPACK dorg2r).  The
+// reflector of step j is staged in vbuf (double buffered) so the owner of
+// column j can overwrite it with the Q column while the other groups still
+// apply H_j: one barrier per column.
+template <int G>
+__device__ void org2r(double* W, int rows, int k, int ld, const double* tau, double* vbuf) {
   const int tid = threadIdx.x, bs = blockDim.x;
-  const int lane = tid & 63, wid = tid >> 6, nw = bs >> 6;
+  const int gl = tid & (G - 1), gid = tid / G, ng = bs / G;
+  if (k <= 0) return;
+  for (int i = tid; i < rows; i += bs) vbuf[((k - 1) & 1) * rows + i] = W[(int64_t)i * ld + (k - 1)];
+  __syncthreads();
   for (int j = k - 1; j >= 0; --j) {
     const double tj = tau[j];
-    if (tj != 0.0) {
-      for (int col = j + 1 + wid; col < k; col += nw) {
-        double d = 0.0;
-        for (int i = j + 1 + lane; i < rows; i += 64) d += W[(int64_t)i * ld + j] * W[(int64_t)i * ld + col];
-        d = (wave_sum(d) + W[(int64_t)j * ld + col]) * tj;
-        for (int i = j + 1 + lane; i < rows; i += 64) W[(int64_t)i * ld + col] -= d * W[(int64_t)i * ld + j];
-        if (lane == 0) W[(int64_t)j * ld + col] -= d;
+    const double* v = vbuf + (j & 1) * rows;
+    for (int col = first_owned(j + 1, gid, ng); col < k; col += ng)
+      if (tj != 0.0) grp_apply<G>(W, rows, ld, j, col, v, 1, tj);
+    if (gid == j % ng) {
+      for (int i = gl; i < rows; i += G) {
+        double* a = W + (int64_t)i * ld + j;
+        if (i < j) *a = 0.0;
+        else if (i == j) *a = 1.0 - tj;
+        else *a = -tj * v[i];
       }
     }
-    __syncthreads();
-    for (int i = tid; i < rows; i += bs) {
-      double* a = W + (int64_t)i * ld + j;
-      if (i < j) *a = 0.0;
-      else if (i == j) *a = 1.0 - tj;
-      else *a *= -tj;
+    if (j > 0 && gid == (j - 1) % ng) {
+      for (int i = gl; i < rows; i += G) vbuf[((j - 1) & 1) * rows + i] = W[(int64_t)i * ld + (j - 1)];
     }
     __syncthreads();
   }
@@ -197,9 +306,10 @@ __device__ inline int rr_player(int pos, int t, int nn) {
 
 // One-sided Jacobi on the rows of X (n x b, ld b), rotations accumulated in J
 // (n x n).  Returns false if it did not converge.
+template <int G>
 __device__ bool jacobi_rows(double* X, int n, int b, double* J, int* flag, double* red) {
   const int tid = threadIdx.x, bs = blockDim.x;
-  const int lane = tid & 63, wid = tid >> 6, nw = bs >> 6;
+  const int gl = tid & (G - 1), gid = tid / G, ng = bs / G;
   for (int e = tid; e < n * n; e += bs) J[e] = (e / n == e % n) ? 1.0 : 0.0;
   __syncthreads();
   if (n < 2) return true;
@@ -214,43 +324,69 @@ __device__ bool jacobi_rows(double* X, int n, int b, double* J, int* flag, doubl
     if (tid == 0) *flag = 0;
     __syncthreads();
     for (int t = 0; t < nn - 1; ++t) {
-      for (int pi = wid; pi < nn / 2; pi += nw) {
+      for (int pi = gid; pi < nn / 2; pi += ng) {
         int p = rr_player(pi, t, nn), q = rr_player(nn - 1 - pi, t, nn);
         if (p > q) {
-          int tmp = p;
+          const int tmp = p;
           p = q;
           q = tmp;
         }
         if (q >= n) continue;
         double* xp = X + (int64_t)p * b;
         double* xq = X + (int64_t)q * b;
-        double a = 0.0, c = 0.0, g = 0.0;
-        for (int i = lane; i < b; i += 64) {
-          const double u = xp[i], w = xq[i];
-          a += u * u;
-          c += w * w;
-          g += u * w;
+        double a0 = 0.0, c0 = 0.0, g0 = 0.0, a1 = 0.0, c1 = 0.0, g1 = 0.0;
+        int i = gl;
+        for (; i + G < b; i += 2 * G) {
+          const double u0 = xp[i], w0 = xq[i], u1 = xp[i + G], w1 = xq[i + G];
+          a0 = __builtin_fma(u0, u0, a0);
+          c0 = __builtin_fma(w0, w0, c0);
+          g0 = __builtin_fma(u0, w0, g0);
+          a1 = __builtin_fma(u1, u1, a1);
+          c1 = __builtin_fma(w1, w1, c1);
+          g1 = __builtin_fma(u1, w1, g1);
         }
-        a = wave_sum(a);
-        c = wave_sum(c);
-        g = wave_sum(g);
+        if (i < b) {
+          const double u0 = xp[i], w0 = xq[i];
+          a0 = __builtin_fma(u0, u0, a0);
+          c0 = __builtin_fma(w0, w0, c0);
+          g0 = __builtin_fma(u0, w0, g0);
+        }
+        const double a = gsum<G>(a0 + a1);
+        const double c = gsum<G>(c0 + c1);
+        const double g = gsum<G>(g0 + g1);
         if (g != 0.0 && a > small && c > small && fabs(g) > tol * sqrt(a) * sqrt(c)) {
           const double zeta = (c - a) / (2.0 * g);
           const double tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
           const double cs = 1.0 / sqrt(1.0 + tt * tt), sn = cs * tt;
-          for (int i = lane; i < b; i += 64) {
-            const double u = xp[i], w = xq[i];
-            xp[i] = cs * u - sn * w;
-            xq[i] = sn * u + cs * w;
+          i = gl;
+          for (; i + G < b; i += 2 * G) {
+            const double u0 = xp[i], w0 = xq[i], u1 = xp[i + G], w1 = xq[i + G];
+            xp[i] = cs * u0 - sn * w0;
+            xq[i] = sn * u0 + cs * w0;
+            xp[i + G] = cs * u1 - sn * w1;
+            xq[i + G] = sn * u1 + cs * w1;
+          }
+          if (i < b) {
+            const double u0 = xp[i], w0 = xq[i];
+            xp[i] = cs * u0 - sn * w0;
+            xq[i] = sn * u0 + cs * w0;
           }
           double* jp = J + (int64_t)p * n;
           double* jq = J + (int64_t)q * n;
-          for (int i = lane; i < n; i += 64) {
-            const double u = jp[i], w = jq[i];
-            jp[i] = cs * u - sn * w;
-            jq[i] = sn * u + cs * w;
+          i = gl;
+          for (; i + G < n; i += 2 * G) {
+            const double u0 = jp[i], w0 = jq[i], u1 = jp[i + G], w1 = jq[i + G];
+            jp[i] = cs * u0 - sn * w0;
+            jq[i] = sn * u0 + cs * w0;
+            jp[i + G] = cs * u1 - sn * w1;
+            jq[i + G] = sn * u1 + cs * w1;
           }
-          if (lane == 0) *flag = 1;
+          if (i < n) {
+            const double u0 = jp[i], w0 = jq[i];
+            jp[i] = cs * u0 - sn * w0;
+            jq[i] = sn * u0 + cs * w0;
+          }
+          if (gl == 0) *flag = 1;
         }
       }
       __syncthreads();
@@ -262,11 +398,175 @@ __device__ bool jacobi_rows(double* X, int n, int b, double* J, int* flag, doubl
   return false;
 }
 
-__global__ void __launch_bounds__(512) truncate_kernel(SweepArgs a) {
+template <int G>
+__device__ inline void row_norms(const double* X, int n, int b, double* sig) {
+  const int gl = threadIdx.x & (G - 1), gid = threadIdx.x / G, ng = blockDim.x / G;
+  for (int i = gid; i < n; i += ng) {
+    double s = 0.0;
+    for (int c = gl; c < b; c += G) s += X[(int64_t)i * b + c] * X[(int64_t)i * b + c];
+    s = gsum<G>(s);
+    if (gl == 0) sig[i] = sqrt(s);
+  }
+}
+
+// One move of the sweep, with the workspace `ws` either in LDS or in global
+// scratch.  Two separate instantiations (one per memory) let the compiler use
+// LDS instructions on the LDS path instead of generic flat addressing.
+template <int G>
+__device__ __forceinline__ int run_move(double* ws, const int rows, const int b, const int tr, const int bx,
+                                        const int64_t Au, const int64_t nu, const int64_t Bu, double* Xc,
+                                        const double* Uc, double* Ud, double* gY, double cutoff, long long maxdim,
+                                        int* status, int* iflag, double* red, double* dbc) {
+  const int tid = threadIdx.x, bs = blockDim.x;
+  const int k = rows < b ? rows : b;
+  // workspace: W rows x b | R k x b | J k x k (SVD only) | tau, sig, perm | 2 staging rows
+  const int64_t jsz = tr ? (int64_t)k * k : 0;
+  double* W = ws;                               // rows x b
+  double* Rb = W + (int64_t)rows * b;           // k x b
+  double* Jb = Rb + (int64_t)k * b;             // k x k
+  double* tau = Jb + jsz;                       // k
+  double* sig = tau + k;                        // k
+  int* perm = (int*)(sig + k);                  // k ints (2k doubles reserved)
+  double* vbuf = sig + 3 * (int64_t)k;          // 2 rows
+  // gather the matrix: M[a*Bx + bb][i] = X[(a*b + i)*Bx + bb] (sizes < 2^31, checked on the host)
+  for (int e = tid; e < rows * b; e += bs) {
+    const int row = e / b, i = e - row * b;
+    const int aa = row / bx, bb = row - aa * bx;
+    W[e] = Xc[((int64_t)aa * b + i) * bx + bb];
+  }
+  __syncthreads();
+  int m, n;
+  const double* Rrows;
+  bool use_perm = false, tall = false;
+  if (!tr) {
+    geqr2<G>(W, rows, b, b, tau);
+    for (int e = tid; e < k * b; e += bs) {
+      const int i = e / b, c = e - i * b;
+      Rb[e] = c >= i ? W[(int64_t)i * b + c] : 0.0;
+    }
+    __syncthreads();
+    org2r<G>(W, rows, k, b, tau, vbuf);
+    m = k;
+    n = k;
+    Rrows = Rb;
+    // x <- Q (bond m)
+    for (int e = tid; e < rows * m; e += bs) {
+      const int row = e / m, j = e - row * m;
+      const int aa = row / bx, bb = row - aa * bx;
+      Xc[((int64_t)aa * m + j) * bx + bb] = W[(int64_t)row * b + j];
+    }
+  } else {
+    tall = rows > b;
+    double* X = W;
+    n = rows;
+    if (tall) {
+      geqr2<G>(W, rows, b, b, tau);
+      for (int e = tid; e < b * b; e += bs) {
+        const int i = e / b, c = e - i * b;
+        Rb[e] = c >= i ? W[(int64_t)i * b + c] : 0.0;
+      }
+      __syncthreads();
+      org2r<G>(W, rows, b, b, tau, vbuf);
+      X = Rb;
+      n = b;
+    }
+    if (!jacobi_rows<G>(X, n, b, Jb, iflag, red)) {
+      if (tid == 0) status[1] = 1;
+    }
+    row_norms<G>(X, n, b, sig);
+    __syncthreads();
+    for (int i = tid; i < n; i += bs) {
+      int r = 0;
+      const double si = sig[i];
+      for (int j = 0; j < n; ++j) r += (sig[j] > si || (sig[j] == si && j < i)) ? 1 : 0;
+      perm[r] = i;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      // ITensors/NDTensors truncate!!: relative cutoff on squared singular values
+      double scale = 0.0;
+      for (int i = 0; i < n; ++i) scale += sig[perm[i]] * sig[perm[i]];
+      int mm = n;
+      double terr = 0.0;
+      const long long md = maxdim < 1 ? 1 : maxdim;
+      while (mm > md) {
+        terr += sig[perm[mm - 1]] * sig[perm[mm - 1]];
+        --mm;
+      }
+      const double sc = scale == 0.0 ? 1.0 : scale;
+      if (cutoff >= 0.0) {
+        while (mm > 1) {
+          const double pm = sig[perm[mm - 1]] * sig[perm[mm - 1]];
+          if (terr + pm <= cutoff * sc) {
+            terr += pm;
+            --mm;
+          } else {
+            break;
+          }
+        }
+      }
+      iflag[1] = mm;
+      dbc[0] += terr / sc;
+    }
+    __syncthreads();
+    m = iflag[1];
+    Rrows = X;
+    use_perm = true;
+    // x <- U (bond m): U[row][i] = sum_j Q[row][j] J[perm[i]][j]  (or J[perm[i]][row])
+    for (int e = tid; e < rows * m; e += bs) {
+      const int row = e / m, i = e - row * m;
+      const double* jr = Jb + (int64_t)perm[i] * n;
+      double s;
+      if (tall) {
+        s = 0.0;
+        const double* wr = W + (int64_t)row * b;
+        for (int j = 0; j < n; ++j) s = __builtin_fma(wr[j], jr[j], s);
+      } else {
+        s = jr[row];
+      }
+      const int aa = row / bx, bb = row - aa * bx;
+      Xc[((int64_t)aa * m + i) * bx + bb] = s;
+    }
+  }
+  // u <- R (x)_{au} u : Y[(aa*m + i)*Bu + bb] = sum_c R[i][c] U[(aa*nu + c)*Bu + bb].
+  // One task = one (aa, bb) column and 8 output rows: u is read once per 8 rows.
+  const int ytot = (int)(Au * m * Bu);
+  {
+    const int ncol = (int)(Au * Bu), bu = (int)Bu, nuu = (int)nu;
+    const int nib = (m + 7) >> 3;
+    for (int task = tid; task < ncol * nib; task += bs) {
+      const int j = task % ncol, ib = task / ncol;
+      const int aa = j / bu, bb = j - aa * bu;
+      const double* rr[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int i = min(ib * 8 + q, m - 1);
+        rr[q] = Rrows + (int64_t)(use_perm ? perm[i] : i) * b;
+      }
+      double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      const double* up = Uc + (int64_t)aa * nuu * bu + bb;
+      for (int c = 0; c < nuu; ++c) {
+        const double uv = up[(int64_t)c * bu];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc[q] = __builtin_fma(rr[q][c], uv, acc[q]);
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int i = ib * 8 + q;
+        if (i < m) gY[((int64_t)aa * m + i) * bu + bb] = acc[q];
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < ytot; e += bs) Ud[e] = gY[e];
+  return m;
+}
+
+__global__ void __launch_bounds__(512) truncate_kernel(const SweepArgs* __restrict__ all) {
   extern __shared__ double lds[];
+  const SweepArgs& a = all[blockIdx.x];
   const TopoView& t = a.t;
   const int tid = threadIdx.x, bs = blockDim.x;
-  const int lane = tid & 63, wid = tid >> 6, nw = bs >> 6;
   // LDS layout: [misc 64 doubles][bonds: nv ints][workspace]
   double* red = lds;                      // 16 doubles
   int* iflag = (int*)(lds + 16);          // flags and broadcast ints
@@ -287,134 +587,28 @@ __global__ void __launch_bounds__(512) truncate_kernel(SweepArgs a) {
     split_dims(du, ndu, au, Au, nu, Bu);
     const int rows = (int)(Ax * Bx), b = (int)nx;
     const int k = rows < b ? rows : b;
-    const int64_t need = (int64_t)rows * b + (int64_t)k * b + (int64_t)k * k + 4 * (int64_t)k + 16;
-    double* ws = need <= a.ws_lds ? wsl : a.gscr;
-    double* W = ws;                               // rows x b
-    double* Rb = W + (int64_t)rows * b;           // k x b
-    double* Jb = Rb + (int64_t)k * b;             // k x k
-    double* tau = Jb + (int64_t)k * k;            // k
-    double* sig = tau + k;                        // k
-    int* perm = (int*)(sig + k);                  // k ints (2k doubles reserved)
+    const int64_t jsz = tr ? (int64_t)k * k : 0;
+    const int64_t need = (int64_t)rows * b + (int64_t)k * b + jsz + 4 * (int64_t)k + 2 * (int64_t)rows + 16;
     double* Xc = a.data + a.off[x];
     const double* Uc = a.data + a.off[u];
-    // gather the matrix: M[a*Bx + bb][i] = X[(a*nx + i)*Bx + bb]
-    for (int64_t e = tid; e < (int64_t)rows * b; e += bs) {
-      const int64_t row = e / b, i = e % b;
-      const int64_t aa = row / Bx, bb = row % Bx;
-      W[e] = Xc[(aa * nx + i) * Bx + bb];
-    }
-    __syncthreads();
-    int m;
-    const double* Rrows;  // m rows of length b pushed into u, row i = Rrows[rowidx(i) * b]
-    bool use_perm = false;
-    if (!tr) {
-      geqr2(W, rows, b, b, tau, red);
-      for (int64_t e = tid; e < (int64_t)k * b; e += bs) {
-        const int64_t i = e / b, c = e % b;
-        Rb[e] = c >= i ? W[i * b + c] : 0.0;
-      }
-      __syncthreads();
-      org2r(W, rows, k, b, tau);
-      m = k;
-      // x <- Q (bond m)
-      for (int64_t e = tid; e < (int64_t)rows * m; e += bs) {
-        const int64_t row = e / m, j = e % m;
-        const int64_t aa = row / Bx, bb = row % Bx;
-        Xc[(aa * m + j) * Bx + bb] = W[row * b + j];
-      }
-      Rrows = Rb;
-    } else {
-      const bool tall = rows > b;
-      double* X = W;
-      int n = rows;
-      if (tall) {
-        geqr2(W, rows, b, b, tau, red);
-        for (int64_t e = tid; e < (int64_t)b * b; e += bs) {
-          const int64_t i = e / b, c = e % b;
-          Rb[e] = c >= i ? W[i * b + c] : 0.0;
-        }
-        __syncthreads();
-        org2r(W, rows, b, b, tau);
-        X = Rb;
-        n = b;
-      }
-      if (!jacobi_rows(X, n, b, Jb, iflag, red)) {
-        if (tid == 0) a.status[1] = 1;
-      }
-      // singular values = row norms; descending order with index tie-break
-      for (int i = wid; i < n; i += nw) {
-        double s = 0.0;
-        for (int c = lane; c < b; c += 64) s += X[(int64_t)i * b + c] * X[(int64_t)i * b + c];
-        s = wave_sum(s);
-        if (lane == 0) sig[i] = sqrt(s);
-      }
-      __syncthreads();
-      for (int i = tid; i < n; i += bs) {
-        int r = 0;
-        const double si = sig[i];
-        for (int j = 0; j < n; ++j) r += (sig[j] > si || (sig[j] == si && j < i)) ? 1 : 0;
-        perm[r] = i;
-      }
-      __syncthreads();
-      if (tid == 0) {
-        // ITensors/NDTensors truncate!!: relative cutoff on squared singular values
-        double scale = 0.0;
-        for (int i = 0; i < n; ++i) scale += sig[perm[i]] * sig[perm[i]];
-        int mm = n;
-        double terr = 0.0;
-        const long long md = a.maxdim < 1 ? 1 : a.maxdim;
-        while (mm > md) {
-          terr += sig[perm[mm - 1]] * sig[perm[mm - 1]];
-          --mm;
-        }
-        const double sc = scale == 0.0 ? 1.0 : scale;
-        if (a.cutoff >= 0.0) {
-          while (mm > 1) {
-            const double pm = sig[perm[mm - 1]] * sig[perm[mm - 1]];
-            if (terr + pm <= a.cutoff * sc) {
-              terr += pm;
-              --mm;
-            } else {
-              break;
-            }
-          }
-        }
-        iflag[1] = mm;
-        dbc[0] += terr / sc;
-      }
-      __syncthreads();
-      m = iflag[1];
-      // x <- U (bond m): U[row][i] = sum_j Q[row][j] J[perm[i]][j]  (or J[perm[i]][row])
-      for (int64_t e = tid; e < (int64_t)rows * m; e += bs) {
-        const int64_t row = e / m, i = e % m;
-        const double* jr = Jb + (int64_t)perm[i] * n;
-        double s;
-        if (tall) {
-          s = 0.0;
-          for (int j = 0; j < n; ++j) s += W[row * b + j] * jr[j];
-        } else {
-          s = jr[row];
-        }
-        const int64_t aa = row / Bx, bb = row % Bx;
-        Xc[(aa * m + i) * Bx + bb] = s;
-      }
-      Rrows = X;
-      use_perm = true;
-    }
-    // u <- R (x)_{au} u : Y[(aa*m + i)*Bu + bb] = sum_c R[i][c] U[(aa*nu + c)*Bu + bb]
-    const int64_t ytot = Au * m * Bu;
-    for (int64_t e = tid; e < ytot; e += bs) {
-      const int64_t bb = e % Bu;
-      const int64_t rest = e / Bu;
-      const int64_t i = rest % m, aa = rest / m;
-      const double* rr = Rrows + (int64_t)(use_perm ? perm[i] : i) * b;
-      double s = 0.0;
-      for (int64_t c = 0; c < nu; ++c) s += rr[c] * Uc[(aa * nu + c) * Bu + bb];
-      a.gY[e] = s;
-    }
-    __syncthreads();
     double* Ud = a.data + a.off[u];
-    for (int64_t e = tid; e < ytot; e += bs) Ud[e] = a.gY[e];
+    const bool small = rows <= 256 && b <= 256;
+    int m;
+    if (need <= a.ws_lds) {
+      if (small)
+        m = run_move<8>(wsl, rows, b, tr, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+                        iflag, red, dbc);
+      else
+        m = run_move<64>(wsl, rows, b, tr, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+                         iflag, red, dbc);
+    } else {
+      if (small)
+        m = run_move<8>(a.gscr, rows, b, tr, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+                        iflag, red, dbc);
+      else
+        m = run_move<64>(a.gscr, rows, b, tr, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+                         iflag, red, dbc);
+    }
     if (tid == 0) {
       if (t.parent[x] == u) bond[x] = m; else bond[u] = m;
     }
@@ -445,63 +639,94 @@ int launch_contract(qtt_ctx* ctx, const qtt_mpo* op, const qtt_net* net, qtt_net
   return qtt_hip_check(ctx, hipStreamSynchronize(ctx->stream), "contract sync");
 }
 
-int launch_truncate(qtt_ctx* ctx, qtt_net* net, double cutoff, int64_t maxdim, double* err) {
-  const HostTopo& t = net->t;
-  if (err) *err = 0.0;
-  if (t.nv < 2) return QTT_OK;
-  std::vector<int> steps = truncation_schedule(t);
-  const int nsteps = (int)steps.size() / 3;
-  int64_t maxcap = 1, maxb = 1;
-  for (int v = 0; v < t.nv; v++) {
-    maxcap = std::max(maxcap, net->cap[v]);
-    maxb = std::max(maxb, net->bond[v]);
+// Truncate a batch of networks, one workgroup each, in one launch.
+int launch_truncate_batch(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double cutoff, int64_t maxdim,
+                          double* errs) {
+  if (nnet <= 0) return QTT_OK;
+  std::vector<std::vector<int>> steps(nnet);
+  std::vector<int64_t> ws(nnet), soff(nnet);
+  int64_t tot_doubles = 0, tot_steps = 0;
+  int maxnv = 1;
+  for (int i = 0; i < nnet; i++) {
+    const qtt_net* net = nets[i];
+    steps[i] = truncation_schedule(net->t);
+    int64_t maxcap = 1, maxb = 1;
+    for (int v = 0; v < net->t.nv; v++) {
+      maxcap = std::max(maxcap, net->cap[v]);
+      maxb = std::max(maxb, net->bond[v]);
+    }
+    if (maxcap >= (int64_t(1) << 31) / 8)
+      return qtt_fail(ctx, QTT_EINVAL, "truncate: a core is too large for the sweep kernel");
+    // workspace bound: rows*b <= core size, k <= b, staging rows <= core size; plus the Y buffer
+    ws[i] = 3 * maxcap + 2 * maxb * maxb + 4 * maxb + 64 + maxcap;
+    soff[i] = tot_steps;
+    tot_doubles += ws[i];
+    tot_steps += (int64_t)steps[i].size();
+    maxnv = std::max(maxnv, net->t.nv);
   }
-  // workspace bound: rows*b <= core size; k <= b
-  const int64_t ws = maxcap + 2 * maxb * maxb + 4 * maxb + 64;
-  const size_t bytes = (ws + maxcap + 4) * sizeof(double) + steps.size() * sizeof(int) + 64;
+  const size_t args_bytes = (size_t)nnet * sizeof(SweepArgs);
+  const size_t bytes = (tot_doubles + nnet + 8) * sizeof(double) + args_bytes + tot_steps * sizeof(int) + 256;
   double* scr = qtt_scratch(ctx, bytes);
   if (!scr) return qtt_fail(ctx, QTT_ENOMEM, "truncate scratch");
-  double* gws = scr;
-  double* gY = gws + ws;
-  double* d_err = gY + maxcap;
-  int* d_steps = (int*)(d_err + 4);
-  int rc = qtt_hip_check(ctx, hipMemcpyAsync(d_steps, steps.data(), steps.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream), "steps");
-  if (rc) return rc;
-  SweepArgs sa;
-  sa.data = net->d_data;
-  sa.off = net->d_off;
-  sa.bond_g = net->d_bond;
-  sa.t = net->view();
-  sa.steps = d_steps;
-  sa.nsteps = nsteps;
-  sa.cutoff = cutoff;
-  sa.maxdim = (long long)maxdim;
-  sa.gscr = gws;
-  sa.gscr_ws = ws;
-  sa.gY = gY;
-  sa.err_out = d_err;
-  sa.status = ctx->d_flag;
+  double* d_err = scr;
+  double* wsp = scr + nnet + 8;
+  SweepArgs* d_args = (SweepArgs*)(wsp + tot_doubles);
+  int* d_steps = (int*)((char*)d_args + ((args_bytes + 255) & ~size_t(255)));
+  std::vector<int> allsteps;
+  allsteps.reserve(tot_steps);
+  for (int i = 0; i < nnet; i++) allsteps.insert(allsteps.end(), steps[i].begin(), steps[i].end());
   const int lds_doubles = QTT_LDS_DOUBLES;
-  sa.ws_lds = lds_doubles - 64 - (t.nv + 1) / 2;
-  if (sa.ws_lds < 0) return qtt_fail(ctx, QTT_EINVAL, "truncate: too many vertices");
+  std::vector<SweepArgs> hargs(nnet);
+  double* w = wsp;
+  for (int i = 0; i < nnet; i++) {
+    qtt_net* net = nets[i];
+    SweepArgs& sa = hargs[i];
+    sa.data = net->d_data;
+    sa.off = net->d_off;
+    sa.bond_g = net->d_bond;
+    sa.t = net->view();
+    sa.steps = d_steps + soff[i];
+    sa.nsteps = (int)steps[i].size() / 3;
+    sa.cutoff = cutoff;
+    sa.maxdim = (long long)maxdim;
+    sa.gscr = w;
+    int64_t maxcap = 1;
+    for (int v = 0; v < net->t.nv; v++) maxcap = std::max(maxcap, net->cap[v]);
+    sa.gY = w + ws[i] - maxcap;
+    w += ws[i];
+    sa.ws_lds = lds_doubles - 64 - (net->t.nv + 1) / 2;
+    sa.err_out = d_err + i;
+    sa.status = ctx->d_flag;
+    if (sa.ws_lds < 0) return qtt_fail(ctx, QTT_EINVAL, "truncate: too many vertices");
+  }
+  int rc = qtt_hip_check(ctx, hipMemcpyAsync(d_args, hargs.data(), args_bytes, hipMemcpyHostToDevice, ctx->stream), "sweep args");
+  if (!rc && tot_steps)
+    rc = qtt_hip_check(ctx, hipMemcpyAsync(d_steps, allsteps.data(), tot_steps * sizeof(int), hipMemcpyHostToDevice, ctx->stream), "steps");
+  if (rc) return rc;
   const size_t lds_bytes = (size_t)lds_doubles * sizeof(double);
   rc = qtt_hip_check(ctx, hipFuncSetAttribute((const void*)truncate_kernel,
                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes),
                      "truncate LDS attribute");
   if (rc) return rc;
-  truncate_kernel<<<1, 512, lds_bytes, ctx->stream>>>(sa);
+  truncate_kernel<<<nnet, 512, lds_bytes, ctx->stream>>>(d_args);
   rc = qtt_hip_check(ctx, hipGetLastError(), "truncate_kernel");
   if (rc) return rc;
-  std::vector<int> b32(t.nv);
-  double herr = 0.0;
-  rc = qtt_hip_check(ctx, hipMemcpyAsync(b32.data(), net->d_bond, t.nv * sizeof(int), hipMemcpyDeviceToHost, ctx->stream), "bonds");
-  if (!rc) rc = qtt_hip_check(ctx, hipMemcpyAsync(&herr, d_err, sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "err");
+  std::vector<double> herr(nnet);
+  rc = qtt_hip_check(ctx, hipMemcpyAsync(herr.data(), d_err, nnet * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "err");
+  if (rc) return rc;
+  std::vector<std::vector<int>> b32(nnet);
+  for (int i = 0; i < nnet && !rc; i++) {
+    b32[i].resize(nets[i]->t.nv);
+    rc = qtt_hip_check(ctx, hipMemcpyAsync(b32[i].data(), nets[i]->d_bond, nets[i]->t.nv * sizeof(int), hipMemcpyDeviceToHost, ctx->stream), "bonds");
+  }
   if (!rc) rc = qtt_hip_check(ctx, hipStreamSynchronize(ctx->stream), "truncate sync");
   if (rc) return rc;
-  for (int v = 0; v < t.nv; v++) net->bond[v] = b32[v];
-  net->pack_valid = false;
-  net->pack32_valid = false;
-  if (err) *err = herr;
+  for (int i = 0; i < nnet; i++) {
+    for (int v = 0; v < nets[i]->t.nv; v++) nets[i]->bond[v] = b32[i][v];
+    nets[i]->pack_valid = false;
+    nets[i]->pack32_valid = false;
+    if (errs) errs[i] = herr[i];
+  }
   int fl[2];
   rc = qtt_hip_check(ctx, hipMemcpy(fl, ctx->d_flag, sizeof(fl), hipMemcpyDeviceToHost), "flags");
   if (rc) return rc;
@@ -510,4 +735,13 @@ int launch_truncate(qtt_ctx* ctx, qtt_net* net, double cutoff, int64_t maxdim, d
     return qtt_fail(ctx, QTT_ENOCONV, "truncate: Jacobi SVD did not converge");
   }
   return QTT_OK;
+}
+
+int launch_truncate(qtt_ctx* ctx, qtt_net* net, double cutoff, int64_t maxdim, double* err) {
+  if (err) *err = 0.0;
+  if (net->t.nv < 2) return QTT_OK;
+  double e = 0.0;
+  int rc = launch_truncate_batch(ctx, &net, 1, cutoff, maxdim, &e);
+  if (err) *err = e;
+  return rc;
 }

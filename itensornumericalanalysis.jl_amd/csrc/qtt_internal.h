@@ -21,6 +21,7 @@ struct qtt_ctx {
   double* d_scratch = nullptr;
   size_t scratch_bytes = 0;
   int cus = 256;
+  bool force_lds_eval = false;  // QTT_EVAL_KERNEL=lds selects the LDS-vector chain kernel
 };
 
 // Topology view passed by value to kernels (device pointers).
@@ -58,6 +59,7 @@ struct qtt_net {
   double* d_pack = nullptr;
   int64_t* d_pack_off = nullptr;
   int64_t pack_elems = 0;
+  int* d_desc = nullptr;  // per site {chil, chir, coord, shift} for the register kernel
   bool pack_valid = false;
   // float32 copies for the mixed-precision evaluate
   float* d_pack32 = nullptr;
@@ -89,6 +91,8 @@ int launch_integrate(qtt_ctx* ctx, const qtt_net* net, double weight, double* ou
 int launch_inner(qtt_ctx* ctx, const qtt_net* a, const qtt_net* b, double* out_host);
 int launch_contract(qtt_ctx* ctx, const qtt_mpo* op, const qtt_net* net, qtt_net* out);
 int launch_truncate(qtt_ctx* ctx, qtt_net* net, double cutoff, int64_t maxdim, double* err);
+int launch_truncate_batch(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double cutoff, int64_t maxdim,
+                          double* errs);
 int launch_grad(qtt_ctx* ctx, const qtt_net* net, const double* xs, const double* ybar,
                 int64_t npts, double* grad, int precision);
 
