@@ -42,6 +42,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--npts", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-others", action="store_true", help="skip the 1D and gradient workloads")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); the engine has no "
@@ -143,6 +144,8 @@ def main():
                          f"operate+truncate (1 thread) {cpu_op_ms:.1f} ms",
                "operate_truncate_ms": cpu_op_ms}
 
+    others = {} if args.no_others else other_configs(q, po, ctx, stream, args)
+
     rec = {
         "metric": "points/s + operate+truncate ms; mixed vs FP64 GPU at <=1e-10 rel err",
         "value": value,
@@ -171,8 +174,76 @@ def main():
                      "kernel": "eval_chain_f64_kernel", "kernel_ms": k64,
                      "hbm_gbs": bytes_pt * args.npts / (k64 * 1e-3) / 1e9},
         "cpu_baseline": cpu,
+        "other_configs": others,
     }
     print(json.dumps(rec))
+
+
+def _time_gpu(fn, reps, torch, stream):
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def other_configs(q, po, ctx, stream, args):
+    """The 1D chain workload and the gradient workload, timed in the same run."""
+    import torch
+    from itensornumericalanalysis_jl_amd import workloads
+    out = {}
+    # 1D: 12 digits, D2 Dirichlet, cutoff 1e-12, 1e5 points, integral
+    w = workloads.config1()
+    t = w.f.topo
+    g = q.operate(w.op, w.f, cutoff=w.cutoff)
+    xd = torch.tensor(w.xs, device="cuda")
+    y = torch.empty(len(w.xs), dtype=torch.float64, device="cuda")
+    ev = _time_gpu(lambda: q.evaluate(g, xd, out=y), 20, torch, stream)
+    op = _time_gpu(lambda: q.operate(w.op, w.f, cutoff=w.cutoff), 10, torch, stream)
+    integ = q.integrate(g)
+    C = po.contract(t.parent, t.sdim, w.op.cores, w.f.cores)
+    gref, _ = po.truncate(t.parent, t.sdim, C, w.cutoff)
+    iref = po.integrate(t.parent, t.sdim, gref, 0.5)
+    yr, _ = po.evaluate(t.parent, t.sdim, gref, t.vcoord, t.vdigit, t.Lc, w.xs[:20000])
+    yy = y.cpu().numpy()[:20000]
+    out["config1"] = {"points_per_s": len(w.xs) / ev, "operate_truncate_ms": op * 1e3,
+                      "integral": integ, "integral_rel_err_vs_oracle": abs(integ - iref) / abs(iref),
+                      "eval_rel_err_vs_oracle": float(np.linalg.norm(yy - yr) / np.linalg.norm(yr)),
+                      "result_bonds": [int(b) for b in g.bonds()]}
+    # gradient: 2 x 14 digits, bond 32, 2^20 points, MSE loss
+    w4 = workloads.config4()
+    f4 = w4.f
+    xs = torch.tensor(w4.xs, device="cuda")
+    yv = torch.empty(len(w4.xs), dtype=torch.float64, device="cuda")
+    q.evaluate(f4, xs, out=yv)
+    target = torch.tensor(w4.extra["target"], device="cuda")
+    ybar = (2.0 / len(w4.xs)) * (yv - target)
+    dn = q.api.device_network(f4)
+    grad = torch.empty(dn.size(), dtype=torch.float64, device="cuda")
+    import ctypes
+    from itensornumericalanalysis_jl_amd import _capi
+
+    def run_grad():
+        ctx.check(_capi.lib().qtt_eval_grad(ctx.handle, dn.handle, ctypes.c_void_p(xs.data_ptr()),
+                                            ctypes.c_void_p(ybar.data_ptr()), len(w4.xs),
+                                            ctypes.c_void_p(grad.data_ptr()), 0))
+    gt = _time_gpu(run_grad, 5, torch, stream)
+    flops_fwd = chain_eval_flops([int(b) for b in f4.bonds()])
+    # parity on a subset against the oracle's adjoint
+    ns = 1 << 14
+    sub = q.eval_grad(f4, w4.xs[:ns], ybar[:ns].cpu().numpy())
+    t4 = f4.topo
+    ref = po.eval_grad(t4.parent, t4.sdim, f4.cores, t4.vcoord, t4.vdigit, t4.Lc, w4.xs[:ns],
+                       ybar[:ns].cpu().numpy())
+    num = sum(float(np.sum((a - b) ** 2)) for a, b in zip(sub, ref)) ** 0.5
+    den = sum(float(np.sum(b ** 2)) for b in ref) ** 0.5
+    out["config4"] = {"gradient_ms": gt * 1e3, "points": len(w4.xs),
+                      "forward_flops_per_point": flops_fwd,
+                      "gradient_tflops_3x_forward": 3 * flops_fwd * len(w4.xs) / gt / 1e12,
+                      "grad_rel_err_vs_oracle_2^14_subset": num / den}
+    return out
 
 
 if __name__ == "__main__":

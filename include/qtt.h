@@ -142,6 +142,16 @@ int qtt_eval_grad_host(qtt_ctx* ctx, const qtt_net* net, const double* xs, const
 /* number of float64 elements of all cores of a network (for gradient buffers) */
 int64_t qtt_net_size(const qtt_net* net);
 
+/* ---- batches (no reference counterpart: the reference applies operators to
+ * one function at a time, src/elementary_operators.jl:305-319) -------------
+ * n independent operate(op[i], nets[i]; cutoff, maxdim): the exact
+ * contractions, then ONE truncation launch with one workgroup per network.
+ * outs[i] receives new networks; errs (nullable) the truncation errors. */
+int qtt_apply_batch(qtt_ctx* ctx, int n, const qtt_mpo* const* ops, const qtt_net* const* nets,
+                    double cutoff, int64_t maxdim, int precision, qtt_net** outs, double* errs);
+/* n integrals in one launch (one workgroup per network); out[i] on the host. */
+int qtt_integrate_batch(qtt_ctx* ctx, int n, const qtt_net* const* nets, double weight, double* out);
+
 #ifdef __cplusplus
 }
 #endif

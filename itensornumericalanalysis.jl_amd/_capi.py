@@ -20,7 +20,7 @@ EXPORTS = [
     "qtt_sync", "qtt_net_create", "qtt_net_bonds", "qtt_net_read", "qtt_net_copy",
     "qtt_net_destroy", "qtt_mpo_create", "qtt_mpo_destroy", "qtt_eval", "qtt_eval_host",
     "qtt_integrate", "qtt_inner", "qtt_apply", "qtt_truncate", "qtt_eval_grad",
-    "qtt_eval_grad_host", "qtt_net_size",
+    "qtt_eval_grad_host", "qtt_net_size", "qtt_apply_batch", "qtt_integrate_batch",
 ]
 
 
@@ -76,6 +76,9 @@ def lib():
         L.qtt_truncate.argtypes = [vp, vp, C.c_double, C.c_int64, dp]
         L.qtt_eval_grad.argtypes = [vp, vp, vp, vp, C.c_int64, vp, C.c_int]
         L.qtt_eval_grad_host.argtypes = [vp, vp, dp, dp, C.c_int64, dpp, C.c_int]
+        L.qtt_apply_batch.argtypes = [vp, C.c_int, C.POINTER(vp), C.POINTER(vp), C.c_double, C.c_int64, C.c_int,
+                                      C.POINTER(vp), dp]
+        L.qtt_integrate_batch.argtypes = [vp, C.c_int, C.POINTER(vp), C.c_double, dp]
         _lib = L
     return _lib
 

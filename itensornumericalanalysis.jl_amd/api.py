@@ -187,3 +187,36 @@ def eval_grad(f: ITensorNetworkFunction, xs, ybar, precision: int = Precision.F6
     dn.ctx.check(_capi.lib().qtt_eval_grad_host(dn.ctx.handle, dn.handle, _capi._dptr(pts), _capi._dptr(yb),
                                                 pts.shape[0], _capi._dpp(grads), int(precision)))
     return grads
+
+
+def operate_batch(ops, fs, cutoff=None, maxdim=None, precision: int = Precision.F64):
+    """operate(ops[i], fs[i]; cutoff, maxdim) for a batch of independent functions:
+    the truncation sweeps of the whole batch run in one launch."""
+    n = len(fs)
+    if len(ops) != n:
+        raise ValueError("one operator per function")
+    dns = [device_network(f) for f in fs]
+    dops = [device_operator(o) for o in ops]
+    ctx = engine_context()
+    arr_op = (C.c_void_p * n)(*[d.handle.value for d in dops])
+    arr_f = (C.c_void_p * n)(*[d.handle.value for d in dns])
+    outs = (C.c_void_p * n)()
+    errs = np.zeros(n)
+    ctx.check(_capi.lib().qtt_apply_batch(ctx.handle, n, arr_op, arr_f, _cut(cutoff), int(maxdim or 0),
+                                          int(precision), outs, _capi._dptr(errs)))
+    res = [ITensorNetworkFunction(f.s, None, f.topo, device=_capi.DeviceNetwork(ctx, f.topo, C.c_void_p(h)))
+           for f, h in zip(fs, outs)]
+    return res, errs
+
+
+def integrate_batch(fs, take_sum: bool = False):
+    """Integrals of a batch of functions in one launch."""
+    n = len(fs)
+    dns = [device_network(f) for f in fs]
+    ctx = engine_context()
+    arr = (C.c_void_p * n)(*[d.handle.value for d in dns])
+    out = np.zeros(n)
+    base = fs[0].s.base if n else 2
+    ctx.check(_capi.lib().qtt_integrate_batch(ctx.handle, n, arr, 1.0 if take_sum else 1.0 / base,
+                                              _capi._dptr(out)))
+    return out

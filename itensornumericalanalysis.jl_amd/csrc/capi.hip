@@ -537,6 +537,52 @@ int qtt_apply(qtt_ctx* ctx, const qtt_mpo* op, const qtt_net* net, double cutoff
   return QTT_OK;
 }
 
+int qtt_apply_batch(qtt_ctx* ctx, int n, const qtt_mpo* const* ops, const qtt_net* const* nets, double cutoff,
+                    int64_t maxdim, int precision, qtt_net** outs, double* errs) {
+  if (!ctx || n < 0 || (n && (!ops || !nets || !outs))) return qtt_fail(ctx, QTT_EINVAL, "null argument");
+  int rc = check_precision(ctx, precision);
+  if (rc) return rc;
+  for (int i = 0; i < n; i++) {
+    outs[i] = nullptr;
+    if (!ops[i] || !nets[i]) return qtt_fail(ctx, QTT_EINVAL, "null operator or network in batch");
+    if (!same_topo(ops[i]->t, nets[i]->t))
+      return qtt_fail(ctx, QTT_EINVAL, "apply batch: operator and network layouts differ");
+  }
+  std::vector<qtt_net*> made(n, nullptr);
+  auto cleanup = [&]() {
+    for (auto* m : made) qtt_net_destroy(m);
+  };
+  for (int i = 0; i < n && !rc; i++) {
+    const HostTopo& t = nets[i]->t;
+    std::vector<int64_t> b(t.nv), cap(t.nv);
+    for (int v = 0; v < t.nv; v++) b[v] = ops[i]->bond[v] * nets[i]->bond[v];
+    for (int v = 0; v < t.nv; v++) {
+      int64_t s = b[v] * t.sdim[v];
+      for (int k = 0; k < t.nch[v]; k++) s *= b[t.chs[t.choff[v] + k]];
+      cap[v] = s;
+    }
+    rc = alloc_net(ctx, t, b, cap, &made[i]);
+    if (!rc) rc = launch_contract(ctx, ops[i], nets[i], made[i]);
+  }
+  if (!rc && cutoff >= 0.0)
+    rc = launch_truncate_batch(ctx, made.data(), n, cutoff, maxdim > 0 ? maxdim : (int64_t(1) << 40), errs);
+  else if (!rc && errs)
+    for (int i = 0; i < n; i++) errs[i] = 0.0;
+  if (rc) {
+    cleanup();
+    return rc;
+  }
+  for (int i = 0; i < n; i++) outs[i] = made[i];
+  return QTT_OK;
+}
+
+int qtt_integrate_batch(qtt_ctx* ctx, int n, const qtt_net* const* nets, double weight, double* out) {
+  if (!ctx || n < 0 || (n && (!nets || !out))) return qtt_fail(ctx, QTT_EINVAL, "null argument");
+  for (int i = 0; i < n; i++)
+    if (!nets[i]) return qtt_fail(ctx, QTT_EINVAL, "null network in batch");
+  return launch_integrate_batch(ctx, n, nets, weight, out);
+}
+
 int qtt_truncate(qtt_ctx* ctx, qtt_net* net, double cutoff, int64_t maxdim, double* trunc_err) {
   if (!ctx || !net) return qtt_fail(ctx, QTT_EINVAL, "null argument");
   if (maxdim < 0) return qtt_fail(ctx, QTT_EINVAL, "maxdim < 0");

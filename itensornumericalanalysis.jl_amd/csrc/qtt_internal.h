@@ -89,6 +89,7 @@ int launch_eval(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t npts, doub
                 int precision);
 int launch_integrate(qtt_ctx* ctx, const qtt_net* net, double weight, double* out_host);
 int launch_inner(qtt_ctx* ctx, const qtt_net* a, const qtt_net* b, double* out_host);
+int launch_integrate_batch(qtt_ctx* ctx, int n, const qtt_net* const* nets, double weight, double* out_host);
 int launch_contract(qtt_ctx* ctx, const qtt_mpo* op, const qtt_net* net, qtt_net* out);
 int launch_truncate(qtt_ctx* ctx, qtt_net* net, double cutoff, int64_t maxdim, double* err);
 int launch_truncate_batch(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double cutoff, int64_t maxdim,

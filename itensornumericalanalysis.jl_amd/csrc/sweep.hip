@@ -18,10 +18,35 @@ __device__ inline double wave_sum(double v) {
 
 // Post-order contraction of all cores with the site weight, one workgroup.
 // Wave per output row p, lanes over the flattened child index j.
+__device__ void integrate_body(const double* __restrict__ data, const int64_t* __restrict__ off,
+                               const int* __restrict__ bond, const TopoView& t, double weight,
+                               const int64_t* __restrict__ voff, double* vec, double* out);
+
+struct IntegrateArgs {
+  const double* data;
+  const int64_t* off;
+  const int* bond;
+  TopoView t;
+  const int64_t* voff;
+  double* vec;
+  double* out;
+};
+
+__global__ void __launch_bounds__(512) integrate_batch_kernel(const IntegrateArgs* __restrict__ args, double weight) {
+  const IntegrateArgs& a = args[blockIdx.x];
+  integrate_body(a.data, a.off, a.bond, a.t, weight, a.voff, a.vec, a.out);
+}
+
 __global__ void __launch_bounds__(512)
 integrate_tree_kernel(const double* __restrict__ data, const int64_t* __restrict__ off,
                       const int* __restrict__ bond, TopoView t, double weight,
                       const int64_t* __restrict__ voff, double* vec, double* out) {
+  integrate_body(data, off, bond, t, weight, voff, vec, out);
+}
+
+__device__ void integrate_body(const double* __restrict__ data, const int64_t* __restrict__ off,
+                               const int* __restrict__ bond, const TopoView& t, double weight,
+                               const int64_t* __restrict__ voff, double* vec, double* out) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
   for (int v = t.nv - 1; v >= 0; --v) {
     const int m = t.nch[v];
@@ -136,6 +161,52 @@ int launch_integrate(qtt_ctx* ctx, const qtt_net* net, double weight, double* ou
   rc = qtt_hip_check(ctx, hipMemcpyAsync(out_host, d_out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "integral");
   if (rc) return rc;
   return qtt_hip_check(ctx, hipStreamSynchronize(ctx->stream), "integrate sync");
+}
+
+int launch_integrate_batch(qtt_ctx* ctx, int n, const qtt_net* const* nets, double weight, double* out_host) {
+  if (n <= 0) return QTT_OK;
+  std::vector<int64_t> voff;
+  std::vector<int64_t> vbase(n), obase(n), vtot(n);
+  int64_t vall = 0, oall = 0;
+  for (int i = 0; i < n; i++) {
+    const qtt_net* net = nets[i];
+    obase[i] = oall;
+    oall += net->t.nv;
+    vbase[i] = vall;
+    int64_t tot = 0;
+    for (int v = 0; v < net->t.nv; v++) {
+      voff.push_back(tot);
+      tot += net->bond[v];
+    }
+    vtot[i] = tot;
+    vall += tot;
+  }
+  const size_t args_bytes = (size_t)n * sizeof(IntegrateArgs);
+  const size_t bytes = (n + vall + 8) * sizeof(double) + oall * sizeof(int64_t) + args_bytes + 256;
+  double* scr = qtt_scratch(ctx, bytes);
+  if (!scr) return qtt_fail(ctx, QTT_ENOMEM, "integrate scratch");
+  double* d_out = scr;
+  double* vec = scr + n + 8;
+  int64_t* d_voff = (int64_t*)(vec + vall);
+  IntegrateArgs* d_args = (IntegrateArgs*)(d_voff + oall);
+  std::vector<IntegrateArgs> h(n);
+  for (int i = 0; i < n; i++) {
+    h[i].data = nets[i]->d_data;
+    h[i].off = nets[i]->d_off;
+    h[i].bond = nets[i]->d_bond;
+    h[i].t = nets[i]->view();
+    h[i].voff = d_voff + obase[i];
+    h[i].vec = vec + vbase[i];
+    h[i].out = d_out + i;
+  }
+  int rc = qtt_hip_check(ctx, hipMemcpyAsync(d_voff, voff.data(), oall * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream), "voff");
+  if (!rc) rc = qtt_hip_check(ctx, hipMemcpyAsync(d_args, h.data(), args_bytes, hipMemcpyHostToDevice, ctx->stream), "args");
+  if (rc) return rc;
+  integrate_batch_kernel<<<n, 512, 0, ctx->stream>>>(d_args, weight);
+  rc = qtt_hip_check(ctx, hipGetLastError(), "integrate_batch_kernel");
+  if (!rc) rc = qtt_hip_check(ctx, hipMemcpyAsync(out_host, d_out, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "integrals");
+  if (rc) return rc;
+  return qtt_hip_check(ctx, hipStreamSynchronize(ctx->stream), "integrate batch sync");
 }
 
 int launch_inner(qtt_ctx* ctx, const qtt_net* a, const qtt_net* b, double* out_host) {

@@ -223,6 +223,40 @@ def test_truncate_random(q, po, eng, rng, graph):
     assert err == pytest.approx(rerr, rel=1e-9)
 
 
+# ------------------------------------------------------------------ comb tree (config-3 shape, reduced)
+def test_comb_operate_and_integrate(q, po, eng, rng):
+    from itensornumericalanalysis_jl_amd import workloads
+    w = workloads.config3(bond=8, digits=6)
+    f = w.f
+    t = f.topo
+    assert q.integrate(f) == pytest.approx(po.integrate(t.parent, t.sdim, f.cores, 0.5), rel=TOL)
+    g = q.operate(w.op, f, cutoff=w.cutoff, maxdim=w.maxdim)
+    C = po.contract(t.parent, t.sdim, w.op.cores, f.cores)
+    gref, _ = po.truncate(t.parent, t.sdim, C, w.cutoff, w.maxdim)
+    assert list(g.bonds()) == [c.shape[0] for c in gref]
+    assert _compare_networks(q, po, g, gref, f, rng) < 1e-11
+    assert q.integrate(g) == pytest.approx(po.integrate(t.parent, t.sdim, gref, 0.5), rel=1e-11)
+
+
+# ------------------------------------------------------------------ batches (config-5 shape, reduced)
+def test_operate_batch_and_integrate_batch(q, po, eng, rng):
+    from itensornumericalanalysis_jl_amd import workloads
+    ws = workloads.config5(nfun=12, npts=64, seed=5)
+    for w in ws:  # keep the oracle fast: cap the bond
+        if w.f.maxlinkdim() > 24:
+            w.f = q.random_itn(w.s, link_space=24, rng=rng)
+    gs, errs = q.operate_batch([w.op for w in ws], [w.f for w in ws], cutoff=1e-10)
+    ints = q.integrate_batch(gs)
+    for w, g, e, integ in zip(ws, gs, errs, ints):
+        t = w.f.topo
+        C = po.contract(t.parent, t.sdim, w.op.cores, w.f.cores)
+        gref, rerr = po.truncate(t.parent, t.sdim, C, 1e-10)
+        assert list(g.bonds()) == [c.shape[0] for c in gref]
+        assert _compare_networks(q, po, g, gref, w.f, rng, npts=512) < 1e-11
+        assert integ == pytest.approx(po.integrate(t.parent, t.sdim, gref, 0.5), rel=1e-11)
+        assert e == pytest.approx(rerr, rel=1e-6, abs=1e-18)
+
+
 # ------------------------------------------------------------------ gradient
 def test_gradient_matches_oracle(q, po, eng, rng):
     s = q.continuous_siteinds(q.named_grid((12, 1)), map_dimension=2)
