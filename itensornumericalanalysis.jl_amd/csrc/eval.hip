@@ -124,9 +124,6 @@ eval_chain_f64_kernel(const double* __restrict__ pack, const int64_t* __restrict
 // (fully unrolled, compile-time indices) and rows of the packed core are zero
 // padded to a multiple of 4, so four 64-byte scalar loads are issued back to
 // back for each group of 4 rows before their 32 FMAs.
-struct SiteDesc {
-  int chil, chir, coord, shift;  // coord < 0: no digit on this site
-};
 
 template <int CHI>
 __global__ void __launch_bounds__(256)

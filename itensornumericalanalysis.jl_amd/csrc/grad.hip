@@ -3,20 +3,26 @@
 // where L_k is the product of the slices left of site k and R_k the product
 // right of it.
 //
-// Chain kernel: one lane per point, tiles of 128 points per workgroup.
-//   1. backward pass (last site to first) computes every R_k and parks it in
-//      a per-workgroup global buffer (the environments are recomputed per
-//      tile, never stored for the whole batch);
-//   2. forward pass rebuilds L_k site by site; the tile's outer products are
-//      reduced over its 128 points in LDS and added to the workgroup's own
-//      partial gradient (owner-computes, no atomics);
+// Register kernel (chains of binary sites, bonds <= 32), tiles of 256 points
+// per workgroup, one lane per point:
+//   1. backward pass (last site to first) with the vector in registers; every
+//      R_k is parked in the workgroup's own global buffer (environments are
+//      recomputed per tile, never stored for the whole batch);
+//   2. forward pass rebuilds L_k in registers.  At each site the tile's L_k and
+//      ybar R_k are staged in LDS and reduced over the points: wavefront w
+//      takes site value s = w & 1 and half of the parent-bond rows, walks only
+//      the points whose digit is s (a scalar loop over the ballot of digits)
+//      and keeps a small block of (l, r) sums per lane; the block is added to
+//      the workgroup's own partial gradient (owner-computes, no atomics);
 //   3. a second kernel sums the partials in a fixed order, so the result is
 //      bitwise reproducible run to run.
+// A general kernel (LDS vectors, any bond) covers the other chains.
 #include "qtt_internal.h"
 
 namespace {
 
-constexpr int GB = 128;  // points per tile = threads per workgroup
+constexpr int GB = 128;  // general kernel: points per tile = threads per workgroup
+constexpr int GT = 256;  // register kernel: points per tile = threads per workgroup
 
 __device__ inline uint64_t digit_word_g(double x, int L, bool& bad) {
   if (!(x >= 0.0)) {
@@ -148,6 +154,232 @@ grad_chain_kernel(const double* __restrict__ pack, const int64_t* __restrict__ p
   }
 }
 
+__device__ inline double bsel(double a, double b, uint64_t mask) {
+  const uint64_t ua = (uint64_t)__double_as_longlong(a), ub = (uint64_t)__double_as_longlong(b);
+  return __longlong_as_double((long long)((ua & ~mask) | (ub & mask)));
+}
+
+__device__ inline uint64_t uniform64(uint64_t x) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ inline uint64_t site_digit_mask(const SiteDesc& sd, const uint64_t (&kw)[QTT_MAX_COORD]) {
+  if (sd.coord < 0) return 0;
+  const uint64_t kk = sd.coord == 0 ? kw[0] : sd.coord == 1 ? kw[1] : sd.coord == 2 ? kw[2] : kw[3];
+  return 0 - ((kk >> sd.shift) & 1u);
+}
+
+// v <- T[:, d, :] v over the child bond (packed site P[r][lpad][2]); entries
+// past the parent bond are left at zero.
+template <int CHI>
+__device__ __forceinline__ void step_to_parent(const double* __restrict__ P, const SiteDesc& sd, uint64_t dmask,
+                                               double (&v)[CHI]) {
+  const int lpad = (sd.chil + 3) & ~3;
+  double w[CHI];
+#pragma unroll
+  for (int l0 = 0; l0 < CHI; l0 += 4) {
+    if (l0 < sd.chil) {
+      double a0[4] = {0.0, 0.0, 0.0, 0.0}, a1[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int r0 = 0; r0 < CHI; r0 += 4) {
+        if (r0 < sd.chir) {
+#pragma unroll
+          for (int rr = 0; rr < 4; ++rr) {
+            const double* __restrict__ q = P + ((r0 + rr) * lpad + l0) * 2;
+            const double vr = v[r0 + rr];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              a0[i] = __builtin_fma(q[2 * i], vr, a0[i]);
+              a1[i] = __builtin_fma(q[2 * i + 1], vr, a1[i]);
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) w[l0 + i] = bsel(a0[i], a1[i], dmask);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) w[l0 + i] = 0.0;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < CHI; ++i) v[i] = w[i];
+}
+
+// L <- L T[:, d, :] over the parent bond; entries past the child bond stay zero.
+template <int CHI>
+__device__ __forceinline__ void step_to_child(const double* __restrict__ P, const SiteDesc& sd, uint64_t dmask,
+                                              double (&L)[CHI]) {
+  const int lpad = (sd.chil + 3) & ~3;
+  double o[CHI];
+#pragma unroll
+  for (int r0 = 0; r0 < CHI; r0 += 4) {
+    if (r0 < sd.chir) {
+      double a0[4] = {0.0, 0.0, 0.0, 0.0}, a1[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int l0 = 0; l0 < CHI; l0 += 4) {
+        if (l0 < sd.chil) {
+#pragma unroll
+          for (int rr = 0; rr < 4; ++rr) {
+            const double* __restrict__ q = P + ((r0 + rr) * lpad + l0) * 2;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              a0[rr] = __builtin_fma(q[2 * i], L[l0 + i], a0[rr]);
+              a1[rr] = __builtin_fma(q[2 * i + 1], L[l0 + i], a1[rr]);
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) o[r0 + rr] = bsel(a0[rr], a1[rr], dmask);
+    } else {
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) o[r0 + rr] = 0.0;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < CHI; ++i) L[i] = o[i];
+}
+
+// Reduction block of one lane: BL parent-bond rows x BR child-bond columns.
+template <int CHI>
+struct RedBlock {
+  static constexpr int BR = CHI >= 16 ? 2 : 1;
+  static constexpr int BL = CHI >= 32 ? 4 : 1;
+  static constexpr int NRB = CHI / BR;  // lanes along the child bond
+};
+
+template <int CHI>
+__global__ void __launch_bounds__(GT)
+grad_chain_reg_kernel(const double* __restrict__ pack, const int64_t* __restrict__ poff,
+                      const SiteDesc* __restrict__ desc, const int* __restrict__ sdim, int nv, int nc,
+                      const int* __restrict__ ndigits, const int64_t* __restrict__ goff,
+                      const int64_t* __restrict__ roff, int64_t rtot, const double* __restrict__ xs,
+                      const double* __restrict__ ybar, int64_t npts, int64_t gsize, double* __restrict__ Rscr,
+                      double* __restrict__ gpart, int* __restrict__ flag) {
+  constexpr int LS = GT + 1;  // odd row stride: the rows read together sit in different banks
+  constexpr int BL = RedBlock<CHI>::BL, BR = RedBlock<CHI>::BR, NRB = RedBlock<CHI>::NRB;
+  extern __shared__ double lds[];
+  double* Ls = lds;                                    // [CHI][LS]  L_k per point
+  double* Rs = lds + CHI * LS;                         // [CHI][LS]  ybar R_k per point
+  uint64_t* bits = (uint64_t*)(lds + 2 * CHI * LS);    // [0, 4): digit ballots, [4, 8): active ballots
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int s_w = wv & 1;
+  const int lb = lane / NRB, rb = lane % NRB;
+  const bool lane_on = lb * BL < CHI / 2;
+  const int lrow0 = lane_on ? (wv >> 1) * (CHI / 2) + lb * BL : 0;
+  const int rcol0 = rb * BR;
+  double* R = Rscr + (size_t)blockIdx.x * rtot * GT;
+  double* G = gpart + (size_t)blockIdx.x * gsize;
+  for (int64_t e = tid; e < gsize; e += GT) G[e] = 0.0;
+  if (tid < CHI) {  // column GT is a zero point, paired with the last point of an odd count
+    Ls[tid * LS + GT] = 0.0;
+    Rs[tid * LS + GT] = 0.0;
+  }
+  const int64_t ntiles = (npts + GT - 1) / GT;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t p = tile * GT + tid;
+    const bool active = p < npts;
+    bool bad = false;
+    uint64_t kw[QTT_MAX_COORD] = {0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < QTT_MAX_COORD; ++c)
+      if (c < nc) kw[c] = digit_word_g(active ? xs[p * nc + c] : 0.0, ndigits[c], bad);
+    if (bad) flag[0] = 1;
+    const double yb = (active && !bad) ? ybar[p] : 0.0;
+    // ---- backward: R_k over the child bond of site k, parked per tile ----
+    double v[CHI];
+#pragma unroll
+    for (int i = 0; i < CHI; ++i) v[i] = 0.0;
+    v[0] = 1.0;
+    for (int k = nv - 1; k >= 0; --k) {
+      const SiteDesc sd = desc[k];
+      double* Rk = R + roff[k] * GT + tid;
+#pragma unroll
+      for (int r = 0; r < CHI; ++r)
+        if (r < sd.chir) Rk[(int64_t)r * GT] = v[r];
+      if (k == 0) break;
+      step_to_parent<CHI>(pack + poff[k], sd, site_digit_mask(sd, kw), v);
+    }
+    // ---- forward: stage, reduce, advance L ----
+    double L[CHI];
+#pragma unroll
+    for (int i = 0; i < CHI; ++i) L[i] = 0.0;
+    L[0] = 1.0;
+    for (int k = 0; k < nv; ++k) {
+      const SiteDesc sd = desc[k];
+      const uint64_t dmask = site_digit_mask(sd, kw);
+      const double* Rk = R + roff[k] * GT + tid;
+#pragma unroll
+      for (int i = 0; i < CHI; ++i) {
+        Ls[i * LS + tid] = L[i];
+        Rs[i * LS + tid] = i < sd.chir ? yb * Rk[(int64_t)i * GT] : 0.0;
+      }
+      const uint64_t db = __ballot(dmask != 0 && active);
+      const uint64_t ab = __ballot(active);
+      if (lane == 0) {
+        bits[wv] = db;
+        bits[4 + wv] = ab;
+      }
+      __syncthreads();
+      const int sk = sdim[k];
+      if (s_w < sk) {
+        double acc[BL][BR];
+#pragma unroll
+        for (int i = 0; i < BL; ++i)
+#pragma unroll
+          for (int q = 0; q < BR; ++q) acc[i][q] = 0.0;
+        for (int c = 0; c < GT / 64; ++c) {
+          uint64_t m = bits[4 + c];
+          if (sk == 2) m = s_w ? (m & bits[c]) : (m & ~bits[c]);
+          m = uniform64(m);
+          while (m) {
+            const int ja = c * 64 + __builtin_ctzll(m);
+            m &= m - 1;
+            int jb = GT;
+            if (m) {
+              jb = c * 64 + __builtin_ctzll(m);
+              m &= m - 1;
+            }
+            double la[BL], lq[BL], ra[BR], rq[BR];
+#pragma unroll
+            for (int i = 0; i < BL; ++i) {
+              la[i] = Ls[(lrow0 + i) * LS + ja];
+              lq[i] = Ls[(lrow0 + i) * LS + jb];
+            }
+#pragma unroll
+            for (int q = 0; q < BR; ++q) {
+              ra[q] = Rs[(rcol0 + q) * LS + ja];
+              rq[q] = Rs[(rcol0 + q) * LS + jb];
+            }
+#pragma unroll
+            for (int i = 0; i < BL; ++i)
+#pragma unroll
+              for (int q = 0; q < BR; ++q) {
+                acc[i][q] = __builtin_fma(la[i], ra[q], acc[i][q]);
+                acc[i][q] = __builtin_fma(lq[i], rq[q], acc[i][q]);
+              }
+          }
+        }
+        if (lane_on) {
+          double* Gk = G + goff[k];
+#pragma unroll
+          for (int i = 0; i < BL; ++i)
+#pragma unroll
+            for (int q = 0; q < BR; ++q) {
+              const int l = lrow0 + i, r = rcol0 + q;
+              if (l < sd.chil && r < sd.chir) Gk[((int64_t)l * sk + s_w) * sd.chir + r] += acc[i][q];
+            }
+        }
+      }
+      if (k + 1 < nv) step_to_child<CHI>(pack + poff[k], sd, dmask, L);
+      __syncthreads();
+    }
+  }
+}
+
 __global__ void grad_reduce_kernel(const double* __restrict__ gpart, int nparts, int64_t gsize,
                                    double* __restrict__ grad) {
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < gsize;
@@ -183,30 +415,53 @@ int launch_grad(qtt_ctx* ctx, const qtt_net* cnet, const double* xs, const doubl
     rtot += (k == nv - 1) ? 1 : net->bond[k + 1];
     maxb = std::max(maxb, net->bond[k]);
   }
-  const size_t lds = (size_t)3 * maxb * GB * sizeof(double) + GB * sizeof(int);
+  bool binary = true;
+  for (int k = 0; k < nv; k++) binary = binary && t.sdim[k] <= 2;
+  const int chi = maxb <= 8 ? 8 : maxb <= 16 ? 16 : 32;
+  const bool reg = binary && maxb <= 32;
+  const int tilep = reg ? GT : GB;
+  const size_t lds = reg ? ((size_t)2 * chi * (GT + 1) + 8) * sizeof(double)
+                         : (size_t)3 * maxb * GB * sizeof(double) + GB * sizeof(int);
   if (lds > 160 * 1024) return qtt_fail(ctx, QTT_EINVAL, "gradient: bond dimension too large");
-  const int64_t ntiles = (npts + GB - 1) / GB;
-  int nwg = (int)std::max<int64_t>(1, std::min<int64_t>(ntiles, 2 * ctx->cus));
-  size_t bytes = ((size_t)nwg * (rtot * GB + gsize) + 2 * nv + 16) * sizeof(double);
+  const int64_t ntiles = (npts + tilep - 1) / tilep;
+  const int per_cu = reg ? std::max(1, std::min(4, (int)((160 * 1024) / lds))) : 2;
+  int nwg = (int)std::max<int64_t>(1, std::min<int64_t>(ntiles, (int64_t)per_cu * ctx->cus));
+  size_t bytes = ((size_t)nwg * (rtot * tilep + gsize) + 2 * nv + 16) * sizeof(double);
   double* scr = qtt_scratch(ctx, bytes);
   if (!scr) return qtt_fail(ctx, QTT_ENOMEM, "gradient scratch");
   double* Rscr = scr;
-  double* gpart = Rscr + (size_t)nwg * rtot * GB;
+  double* gpart = Rscr + (size_t)nwg * rtot * tilep;  // each workgroup parks rtot x tilep values
   int64_t* d_goff = (int64_t*)(gpart + (size_t)nwg * gsize);
   int64_t* d_roff = d_goff + nv;
   rc = qtt_hip_check(ctx, hipMemcpyAsync(d_goff, goff.data(), nv * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream), "goff");
   if (!rc) rc = qtt_hip_check(ctx, hipMemcpyAsync(d_roff, roff.data(), nv * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream), "roff");
   if (rc) return rc;
+  const TopoView tv = net->view();
+  const void* fn = !reg ? (const void*)grad_chain_kernel
+                   : chi == 8 ? (const void*)grad_chain_reg_kernel<8>
+                   : chi == 16 ? (const void*)grad_chain_reg_kernel<16>
+                               : (const void*)grad_chain_reg_kernel<32>;
   if (lds > 64 * 1024) {
-    rc = qtt_hip_check(ctx, hipFuncSetAttribute((const void*)grad_chain_kernel,
-                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+    rc = qtt_hip_check(ctx, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
                        "grad LDS attribute");
     if (rc) return rc;
   }
   if (npts > 0) {
-    grad_chain_kernel<<<nwg, GB, lds, ctx->stream>>>(net->d_pack, net->d_pack_off, net->d_bond, net->view(), d_goff,
-                                                     d_roff, rtot, xs, ybar, npts, gsize, Rscr, gpart, ctx->d_flag,
-                                                     (int)maxb);
+    if (reg) {
+      const SiteDesc* dd = (const SiteDesc*)net->d_desc;
+#define QTT_GRAD_LAUNCH(C)                                                                                   \
+  grad_chain_reg_kernel<C><<<nwg, GT, lds, ctx->stream>>>(net->d_pack, net->d_pack_off, dd, tv.sdim, nv,     \
+                                                          tv.ncoord, tv.ndigits, d_goff, d_roff, rtot, xs, ybar, \
+                                                          npts, gsize, Rscr, gpart, ctx->d_flag)
+      if (chi == 8) QTT_GRAD_LAUNCH(8);
+      else if (chi == 16) QTT_GRAD_LAUNCH(16);
+      else QTT_GRAD_LAUNCH(32);
+#undef QTT_GRAD_LAUNCH
+    } else {
+      grad_chain_kernel<<<nwg, GB, lds, ctx->stream>>>(net->d_pack, net->d_pack_off, net->d_bond, tv, d_goff,
+                                                       d_roff, rtot, xs, ybar, npts, gsize, Rscr, gpart,
+                                                       ctx->d_flag, (int)maxb);
+    }
     rc = qtt_hip_check(ctx, hipGetLastError(), "grad_chain_kernel");
     if (rc) return rc;
   } else {

@@ -38,6 +38,13 @@ struct TopoView {
   int ncoord;
 };
 
+// Per-site descriptor of a packed chain (qtt_net::d_desc): bonds left/right,
+// the coordinate whose digit selects the slice (< 0: no digit) and the bit
+// position of that digit in the coordinate's digit word.
+struct SiteDesc {
+  int chil, chir, coord, shift;
+};
+
 struct HostTopo {
   int nv = 0, ncoord = 0;
   std::vector<int> parent, sdim, vcoord, vdigit, ndigits, nch, choff, chs;

@@ -258,10 +258,13 @@ def test_operate_batch_and_integrate_batch(q, po, eng, rng):
 
 
 # ------------------------------------------------------------------ gradient
-def test_gradient_matches_oracle(q, po, eng, rng):
+@pytest.mark.parametrize("bond", [3, 8, 16, 32, 40])
+def test_gradient_matches_oracle(q, po, eng, rng, bond):
+    # bonds <= 32 run the register kernel (8/16/32 lanes of state), 40 the general one
     s = q.continuous_siteinds(q.named_grid((12, 1)), map_dimension=2)
-    f = q.random_itn(s, link_space=8, rng=rng, std=0.4)
+    f = q.random_itn(s, link_space=bond, rng=rng, std=0.4)
     xs = rng.uniform(0, 1, (5000, 2))
+    xs[:3] = [[0.0, 0.0], [1.0, 1.0], [0.5, 0.999999]]
     yb = rng.standard_normal(5000)
     g = q.eval_grad(f, xs, yb)
     t = f.topo
