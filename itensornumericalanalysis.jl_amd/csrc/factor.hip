@@ -300,6 +300,232 @@ __device__ void org2r(double* W, int rows, int k, int ld, const double* tau, dou
   }
 }
 
+// ---------------------------------------------------------------------------
+// register-resident Householder QR for matrices of up to 8 * RPL rows and
+// QNS * groups columns (8-lane groups).  Lane gl of a group holds rows
+// gl + 8 t (t < RPL) of each of its columns in registers; column c belongs to
+// group c % groups, slot c / groups.  Only the reflectors travel through
+// memory: reflector j is the row VT[j * 8 RPL, (j + 1) * 8 RPL), zero past
+// the last matrix row, so every step reads it with fixed offsets from one
+// base address (one contiguous 64-byte load per group and register).
+// ---------------------------------------------------------------------------
+constexpr int QNS = 2;
+
+// Lane masks built with integer arithmetic (all ones / zero), so the per-row
+// tests of the unrolled loops need no condition registers.
+__device__ __forceinline__ uint64_t lt_mask(int a, int b) {  // a < b
+  return (uint64_t)(int64_t)((a - b) >> 31);
+}
+__device__ __forceinline__ uint64_t eq_mask(int a, int b) {  // a == b
+  const int d = a - b;
+  return (uint64_t)(int64_t)(~((d | -d) >> 31));
+}
+__device__ __forceinline__ double keep(double x, uint64_t m) {
+  return __longlong_as_double((long long)((uint64_t)__double_as_longlong(x) & m));
+}
+// The lane index made opaque to the optimizer: the per-row masks are then
+// recomputed where they are used instead of being hoisted out of the column
+// loop and held (dozens of condition registers) across it.
+__device__ __forceinline__ int opaque(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
+template <int RPL>
+__device__ __forceinline__ void reg_load_v(const double* VT, int j, int gl, double (&v)[RPL]) {
+  const double* p = VT + j * (8 * RPL) + gl;
+#pragma unroll
+  for (int t = 0; t < RPL; ++t) v[t] = p[8 * t];
+}
+
+// a <- (I - tau v v^T) a for the column held in a
+template <int RPL>
+__device__ __forceinline__ void reg_apply(double (&a)[RPL], const double (&v)[RPL], double tj) {
+  double d0 = 0.0, d1 = 0.0;
+#pragma unroll
+  for (int t = 0; t + 1 < RPL; t += 2) {
+    d0 = __builtin_fma(v[t], a[t], d0);
+    d1 = __builtin_fma(v[t + 1], a[t + 1], d1);
+  }
+  const double d = sum8(d0 + d1) * tj;
+#pragma unroll
+  for (int t = 0; t < RPL; ++t) a[t] = __builtin_fma(-d, v[t], a[t]);
+}
+
+// the same with the reflector read from memory in each pass (fewer live registers)
+template <int RPL>
+__device__ __forceinline__ void reg_apply_mem(double (&a)[RPL], const double* VT, int j, int gl, double tj) {
+  const double* p = VT + j * (8 * RPL) + gl;
+  double d0 = 0.0, d1 = 0.0;
+#pragma unroll
+  for (int t = 0; t + 1 < RPL; t += 2) {
+    d0 = __builtin_fma(p[8 * t], a[t], d0);
+    d1 = __builtin_fma(p[8 * t + 8], a[t + 1], d1);
+  }
+  const double d = sum8(d0 + d1) * tj;
+#pragma unroll
+  for (int t = 0; t < RPL; ++t) a[t] = __builtin_fma(-d, p[8 * t], a[t]);
+}
+
+// Reflector of column j held in a (LAPACK dlarfg): a keeps its R part (beta at
+// row j, zeros below), the reflector goes to VT (zero above row j, one at row
+// j), tau[j] the scalar factor.
+template <int RPL>
+__device__ __forceinline__ void reg_reflector(double (&a)[RPL], int j, int gl_in, double* VT, double* tau) {
+  const int gl = opaque(gl_in);
+  double s0 = 0.0, s1 = 0.0, xj = 0.0;
+#pragma unroll
+  for (int t = 0; t + 1 < RPL; t += 2) {
+    const int r0 = gl + 8 * t, r1 = r0 + 8;
+    const double x0 = keep(a[t], lt_mask(j, r0)), x1 = keep(a[t + 1], lt_mask(j, r1));
+    s0 = __builtin_fma(x0, x0, s0);
+    s1 = __builtin_fma(x1, x1, s1);
+    xj += keep(a[t], eq_mask(r0, j)) + keep(a[t + 1], eq_mask(r1, j));
+  }
+  const double xn2 = sum8(s0 + s1);
+  const double x0 = sum8(xj);
+  double tj = 0.0, beta = x0, scal = 0.0;
+  if (xn2 > 0.0) {
+    const double nrm = sqrt(x0 * x0 + xn2);
+    beta = x0 >= 0.0 ? -nrm : nrm;
+    tj = (beta - x0) / beta;
+    scal = 1.0 / (x0 - beta);
+  }
+  double* p = VT + j * (8 * RPL) + gl;
+#pragma unroll
+  for (int t = 0; t < RPL; ++t) {
+    const int row = gl + 8 * t;
+    const uint64_t below = lt_mask(j, row), diag = eq_mask(row, j), above = lt_mask(row, j);
+    p[8 * t] = keep(a[t] * scal, below) + keep(1.0, diag);  // rows past the matrix hold a = 0
+    a[t] = keep(a[t], above) + keep(beta, diag);
+  }
+  if (gl == 0) tau[j] = tj;
+}
+
+// QR of W (rows x cols, leading dimension cols, rows <= 8 RPL): R (k x cols)
+// into Rb, the reflectors into VT, which overwrites W (k * 8 RPL doubles from
+// W).  The owner of column j + 1 updates it first and builds the
+// next reflector right away, so there is one barrier per column.
+template <int RPL>
+__device__ __forceinline__ void qr_reg(double* W, int rows, int cols, double* tau, double* Rb) {
+  const int tid = threadIdx.x, bs = blockDim.x, gl = tid & 7, gid = tid >> 3, ng = bs >> 3;
+  const int k = rows < cols ? rows : cols;
+  double a[QNS][RPL];
+  const int gll = opaque(gl);
+#pragma unroll
+  for (int s = 0; s < QNS; ++s) {
+    const int c = gid + ng * s;
+    const int cc = c < cols ? c : cols - 1;
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) {
+      const int row = gll + 8 * t;
+      const double x = W[(row < rows ? row : rows - 1) * cols + cc];  // clamped: always in the matrix
+      a[s][t] = c < cols ? keep(x, lt_mask(row, rows)) : 0.0;
+    }
+  }
+  __syncthreads();
+  double* VT = W;
+  if (gid == 0) reg_reflector<RPL>(a[0], 0, gl, VT, tau);
+  __syncthreads();
+  for (int j = 0; j < k; ++j) {
+    const double tj = tau[j];
+    if (gid + ng * (QNS - 1) > j) {
+#pragma unroll
+      for (int s = 0; s < QNS; ++s) {
+        const int c = gid + ng * s;
+        if (c > j && c < cols) {
+          if (tj != 0.0) reg_apply_mem<RPL>(a[s], VT, j, gl, tj);
+          if (c == j + 1 && c < k) reg_reflector<RPL>(a[s], c, gl, VT, tau);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  const int glr = opaque(gl);
+#pragma unroll
+  for (int s = 0; s < QNS; ++s) {
+    const int c = gid + ng * s;
+    if (c < cols) {
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        const int row = glr + 8 * t;
+        if (row < k) Rb[row * cols + c] = a[s][t];
+      }
+    }
+  }
+}
+
+// u <- H_0 ... H_jtop u
+template <int RPL>
+__device__ __forceinline__ void reg_apply_chain(double (&u)[RPL], const double* VT, const double* tau, int gl,
+                                                int jtop) {
+  for (int j = jtop; j >= 0; --j) {
+    const double tj = tau[j];
+    if (tj != 0.0) reg_apply_mem<RPL>(u, VT, j, gl, tj);
+  }
+}
+
+// Offset in the core of row gl + 8 t of the matrix (rows split as (aa, bb),
+// bb < bx, bond index 0), -1 past the last row.
+template <int RPL>
+__device__ __forceinline__ void reg_dest(int rows, int gl, int m, int bx, int (&dst)[RPL]) {
+#pragma unroll
+  for (int t = 0; t < RPL; ++t) {
+    const int row = gl + 8 * t;
+    const int aa = row / bx, bb = row - aa * bx;
+    const int valid = (int)lt_mask(row, rows);
+    dst[t] = ((aa * m * bx + bb) & valid) | ~valid;
+  }
+}
+
+template <int RPL>
+__device__ __forceinline__ void reg_scatter(const double (&u)[RPL], const int (&dst)[RPL], double* Xc, int col,
+                                            int bx) {
+  double* xc = Xc + (int64_t)col * bx;
+#pragma unroll
+  for (int t = 0; t < RPL; ++t)
+    if (dst[t] >= 0) xc[dst[t]] = u[t];
+}
+
+// x <- Q (rows x k): column c = H_0 ... H_c e_c, no barriers.
+template <int RPL>
+__device__ __forceinline__ void q_form_reg(const double* VT, int rows, int k, const double* tau, double* Xc,
+                                           int bx) {
+  const int tid = threadIdx.x, gl = tid & 7, gid = tid >> 3, ng = blockDim.x >> 3;
+  int dst[RPL];
+  reg_dest<RPL>(rows, gl, k, bx, dst);
+  for (int c = gid; c < k; c += ng) {
+    const int glo = opaque(gl);
+    double u[RPL];
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) u[t] = keep(1.0, eq_mask(glo + 8 * t, c));
+    reg_apply_chain<RPL>(u, VT, tau, gl, c);
+    reg_scatter<RPL>(u, dst, Xc, c, bx);
+  }
+}
+
+// x <- U = Q [J_perm^T; 0] (rows x m): column i starts as row perm[i] of J
+// (length n) and takes all k reflectors.
+template <int RPL>
+__device__ __forceinline__ void u_form_reg(const double* VT, int rows, int k, const double* tau, const double* Jb,
+                                           int n, const int* perm, int m, double* Xc, int bx) {
+  const int tid = threadIdx.x, gl = tid & 7, gid = tid >> 3, ng = blockDim.x >> 3;
+  int dst[RPL];
+  reg_dest<RPL>(rows, gl, m, bx, dst);
+  for (int i = gid; i < m; i += ng) {
+    const double* jr = Jb + (int64_t)perm[i] * n;
+    const int glo = opaque(gl);
+    double u[RPL];
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) {
+      const int row = glo + 8 * t;
+      u[t] = keep(jr[row < n ? row : n - 1], lt_mask(row, n));  // clamped read, zero past n
+    }
+    reg_apply_chain<RPL>(u, VT, tau, gl, k - 1);
+    reg_scatter<RPL>(u, dst, Xc, i, bx);
+  }
+}
+
 __device__ inline int rr_player(int pos, int t, int nn) {
   return pos == 0 ? 0 : 1 + (pos - 1 + t) % (nn - 1);
 }
@@ -398,6 +624,146 @@ __device__ bool jacobi_rows(double* X, int n, int b, double* J, int* flag, doubl
   return false;
 }
 
+// One-sided Jacobi as jacobi_rows<G> with G lanes per pair sized to the row
+// length (b <= 4 G): each group loads both rows of its pair with all loads in
+// flight at once (4 entries per lane), rotates them in registers and stores
+// them back, then does the same for the two rows of J (length n <= b).  A
+// round then costs about one memory latency plus the rotation instead of one
+// latency per 8 entries.  Rows are read in whole blocks from one base
+// address; entries past a row's end (fewer than 4 G, inside the workspace,
+// which is padded) are read and replaced by zeros.
+constexpr int JEPL = 4;
+
+// LDS-qualified pointers: the rounds run in a separate (not inlined) function
+// so that its registers do not add to the rest of the sweep kernel, and these
+// types keep its memory operations on the LDS instructions.
+typedef __attribute__((address_space(3))) double lds_f64;
+typedef __attribute__((address_space(3))) int lds_i32;
+
+template <int G>
+__device__ __forceinline__ double jsum(double v) {
+  v = sum8(v);
+  if constexpr (G >= 16) v += dpp_d<0x140>(v);  // row_mirror: 16-lane sums
+  if constexpr (G >= 32) v += __shfl_xor(v, 16);
+  if constexpr (G >= 64) v += __shfl_xor(v, 32);
+  return v;
+}
+
+template <int G>
+__device__ __forceinline__ void load_row(const lds_f64* r, int len, int gl, double (&x)[JEPL]) {
+  const lds_f64* p = r + gl;
+#pragma unroll
+  for (int e = 0; e < JEPL; ++e) x[e] = keep(p[G * e], lt_mask(gl + G * e, len));
+}
+
+template <int G>
+__device__ __forceinline__ void store_row(lds_f64* r, int len, int gl, const double (&x)[JEPL]) {
+#pragma unroll
+  for (int e = 0; e < JEPL; ++e) {
+    const int i = gl + G * e;
+    if (i < len) r[i] = x[e];
+  }
+}
+
+// 1/x and 1/sqrt(x) from the hardware estimates refined by two Newton steps
+// (about 1 ulp); they replace the IEEE division and square root sequences,
+// whose latency dominates a Jacobi round.
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+
+__device__ __forceinline__ double rsq_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double e = __builtin_fma(-x * y, y, 1.0);
+  y = __builtin_fma(0.5 * y, e, y);
+  e = __builtin_fma(-x * y, y, 1.0);
+  return __builtin_fma(0.5 * y, e, y);
+}
+
+__device__ __forceinline__ void rotate4(double (&u)[JEPL], double (&w)[JEPL], double cs, double sn) {
+#pragma unroll
+  for (int e = 0; e < JEPL; ++e) {
+    const double u0 = u[e], w0 = w[e];
+    u[e] = cs * u0 - sn * w0;
+    w[e] = sn * u0 + cs * w0;
+  }
+}
+
+template <int G>
+__device__ __noinline__ bool jacobi_rows_reg(lds_f64* X, int n, int b, lds_f64* J, lds_i32* flag, double* red) {
+  const int tid = threadIdx.x, bs = blockDim.x;
+  const int gid = tid / G, ng = bs / G;
+  for (int e = tid; e < n * n; e += bs) J[e] = (e / n == e % n) ? 1.0 : 0.0;
+  __syncthreads();
+  if (n < 2) return true;
+  const int nn = n + (n & 1);
+  const double tol = DBL_EPSILON * (double)(b > n ? b : n);
+  double fro = 0.0;
+  for (int64_t e = tid; e < (int64_t)n * b; e += bs) fro += X[e] * X[e];
+  const double small = block_sum(fro, red) * DBL_EPSILON * DBL_EPSILON;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    if (tid == 0) *flag = 0;
+    __syncthreads();
+    for (int t = 0; t < nn - 1; ++t) {
+      for (int pi = gid; pi < nn / 2; pi += ng) {
+        int p = rr_player(pi, t, nn), q = rr_player(nn - 1 - pi, t, nn);
+        if (p > q) {
+          const int tmp = p;
+          p = q;
+          q = tmp;
+        }
+        if (q >= n) continue;
+        const int gl = opaque(tid & (G - 1));
+        lds_f64* xp = X + p * b;
+        lds_f64* xq = X + q * b;
+        double u[JEPL], w[JEPL];
+        load_row<G>(xp, b, gl, u);
+        load_row<G>(xq, b, gl, w);
+        double a0 = 0.0, c0 = 0.0, g0 = 0.0, a1 = 0.0, c1 = 0.0, g1 = 0.0;
+#pragma unroll
+        for (int e = 0; e < JEPL; e += 2) {
+          a0 = __builtin_fma(u[e], u[e], a0);
+          c0 = __builtin_fma(w[e], w[e], c0);
+          g0 = __builtin_fma(u[e], w[e], g0);
+          a1 = __builtin_fma(u[e + 1], u[e + 1], a1);
+          c1 = __builtin_fma(w[e + 1], w[e + 1], c1);
+          g1 = __builtin_fma(u[e + 1], w[e + 1], g1);
+        }
+        const double a = jsum<G>(a0 + a1), c = jsum<G>(c0 + c1), g = jsum<G>(g0 + g1);
+        // |g| > tol sqrt(a c), compared squared (a c stays finite for core entries)
+        if (g != 0.0 && a > small && c > small && g * g > tol * tol * (a * c)) {
+          const double zeta = (c - a) * rcp_nr(2.0 * g);
+          const double q1 = __builtin_fma(zeta, zeta, 1.0);
+          const double r = q1 * rsq_nr(q1);  // sqrt(1 + zeta^2)
+          const double t0 = rcp_nr(fabs(zeta) + r);
+          const double tt = zeta >= 0.0 ? t0 : -t0;
+          const double cs = rsq_nr(__builtin_fma(tt, tt, 1.0)), sn = cs * tt;
+          rotate4(u, w, cs, sn);
+          store_row<G>(xp, b, gl, u);
+          store_row<G>(xq, b, gl, w);
+          lds_f64* jp = J + p * n;
+          lds_f64* jq = J + q * n;
+          load_row<G>(jp, n, gl, u);
+          load_row<G>(jq, n, gl, w);
+          rotate4(u, w, cs, sn);
+          store_row<G>(jp, n, gl, u);
+          store_row<G>(jq, n, gl, w);
+          if (gl == 0) *flag = 1;
+        }
+      }
+      __syncthreads();
+    }
+    const int rot = *flag;
+    __syncthreads();
+    if (!rot) return true;
+  }
+  return false;
+}
+
 template <int G>
 __device__ inline void row_norms(const double* X, int n, int b, double* sig) {
   const int gl = threadIdx.x & (G - 1), gid = threadIdx.x / G, ng = blockDim.x / G;
@@ -412,17 +778,21 @@ __device__ inline void row_norms(const double* X, int n, int b, double* sig) {
 // One move of the sweep, with the workspace `ws` either in LDS or in global
 // scratch.  Two separate instantiations (one per memory) let the compiler use
 // LDS instructions on the LDS path instead of generic flat addressing.
-template <int G>
-__device__ __forceinline__ int run_move(double* ws, const int rows, const int b, const int tr, const int bx,
+template <int G, bool REG>
+__device__ __forceinline__ int run_move(double* ws, const int rows, const int b, const int tr, const int rpl, const int bx,
                                         const int64_t Au, const int64_t nu, const int64_t Bu, double* Xc,
                                         const double* Uc, double* Ud, double* gY, double cutoff, long long maxdim,
                                         int* status, int* iflag, double* red, double* dbc) {
   const int tid = threadIdx.x, bs = blockDim.x;
   const int k = rows < b ? rows : b;
-  // workspace: W rows x b | R k x b | J k x k (SVD only) | tau, sig, perm | 2 staging rows
+  // register QR for the shapes it covers (the QR of a tall SVD matrix included)
+  const bool use_reg = REG && G == 8 && rpl != 0;  // register QR (rows <= 8 rpl) for the QR steps
+  // workspace: W rows x b (or the reflector rows) | R k x b | J k x k (SVD only) |
+  //            tau, sig, perm | 2 staging rows
   const int64_t jsz = tr ? (int64_t)k * k : 0;
+  const int64_t wsz = use_reg ? max((int64_t)rows * b, (int64_t)k * 8 * rpl) : (int64_t)rows * b;
   double* W = ws;                               // rows x b
-  double* Rb = W + (int64_t)rows * b;           // k x b
+  double* Rb = W + wsz;                         // k x b
   double* Jb = Rb + (int64_t)k * b;             // k x k
   double* tau = Jb + jsz;                       // k
   double* sig = tau + k;                        // k
@@ -439,38 +809,64 @@ __device__ __forceinline__ int run_move(double* ws, const int rows, const int b,
   const double* Rrows;
   bool use_perm = false, tall = false;
   if (!tr) {
-    geqr2<G>(W, rows, b, b, tau);
-    for (int e = tid; e < k * b; e += bs) {
-      const int i = e / b, c = e - i * b;
-      Rb[e] = c >= i ? W[(int64_t)i * b + c] : 0.0;
-    }
-    __syncthreads();
-    org2r<G>(W, rows, k, b, tau, vbuf);
     m = k;
     n = k;
     Rrows = Rb;
-    // x <- Q (bond m)
-    for (int e = tid; e < rows * m; e += bs) {
-      const int row = e / m, j = e - row * m;
-      const int aa = row / bx, bb = row - aa * bx;
-      Xc[((int64_t)aa * m + j) * bx + bb] = W[(int64_t)row * b + j];
+    if (use_reg) {
+      if (rpl == 8) qr_reg<8>(W, rows, b, tau, Rb);
+      else qr_reg<16>(W, rows, b, tau, Rb);
+      __syncthreads();
+      // x <- Q (bond m)
+      if (rpl == 8) q_form_reg<8>(W, rows, k, tau, Xc, bx);
+      else q_form_reg<16>(W, rows, k, tau, Xc, bx);
+    } else {
+      geqr2<G>(W, rows, b, b, tau);
+      for (int e = tid; e < k * b; e += bs) {
+        const int i = e / b, c = e - i * b;
+        Rb[e] = c >= i ? W[(int64_t)i * b + c] : 0.0;
+      }
+      __syncthreads();
+      org2r<G>(W, rows, k, b, tau, vbuf);
+      // x <- Q (bond m)
+      for (int e = tid; e < rows * m; e += bs) {
+        const int row = e / m, j = e - row * m;
+        const int aa = row / bx, bb = row - aa * bx;
+        Xc[((int64_t)aa * m + j) * bx + bb] = W[(int64_t)row * b + j];
+      }
     }
   } else {
     tall = rows > b;
     double* X = W;
     n = rows;
     if (tall) {
-      geqr2<G>(W, rows, b, b, tau);
-      for (int e = tid; e < b * b; e += bs) {
-        const int i = e / b, c = e - i * b;
-        Rb[e] = c >= i ? W[(int64_t)i * b + c] : 0.0;
+      if (use_reg) {  // the reflectors stay in W for U below
+        if (rpl == 8) qr_reg<8>(W, rows, b, tau, Rb);
+        else qr_reg<16>(W, rows, b, tau, Rb);
+      } else {
+        geqr2<G>(W, rows, b, b, tau);
+        for (int e = tid; e < b * b; e += bs) {
+          const int i = e / b, c = e - i * b;
+          Rb[e] = c >= i ? W[(int64_t)i * b + c] : 0.0;
+        }
+        __syncthreads();
+        org2r<G>(W, rows, b, b, tau, vbuf);
       }
       __syncthreads();
-      org2r<G>(W, rows, b, b, tau, vbuf);
       X = Rb;
       n = b;
     }
-    if (!jacobi_rows<G>(X, n, b, Jb, iflag, red)) {
+    // lanes per row pair sized so a row is at most 4 entries per lane
+    bool conv;
+    // (REG: the workspace is in LDS)
+    lds_f64* Xl = (lds_f64*)X;
+    lds_f64* Jl = (lds_f64*)Jb;
+    lds_i32* fl = (lds_i32*)iflag;
+    if (REG && G == 8 && b <= 32) conv = jacobi_rows_reg<8>(Xl, n, b, Jl, fl, red);
+    else if (REG && G == 8 && b <= 64) conv = jacobi_rows_reg<16>(Xl, n, b, Jl, fl, red);
+    else if (REG && G == 8 && b <= 128) conv = jacobi_rows_reg<32>(Xl, n, b, Jl, fl, red);
+    else if (REG && G == 8 && b <= 256) conv = jacobi_rows_reg<64>(Xl, n, b, Jl, fl, red);
+    else conv = jacobi_rows<G>(X, n, b, Jb, iflag, red);
+    if (!conv) {
       if (tid == 0) status[1] = 1;
     }
     row_norms<G>(X, n, b, sig);
@@ -513,6 +909,10 @@ __device__ __forceinline__ int run_move(double* ws, const int rows, const int b,
     Rrows = X;
     use_perm = true;
     // x <- U (bond m): U[row][i] = sum_j Q[row][j] J[perm[i]][j]  (or J[perm[i]][row])
+    if (tall && use_reg) {
+      if (rpl == 8) u_form_reg<8>(W, rows, b, tau, Jb, n, perm, m, Xc, bx);
+      else u_form_reg<16>(W, rows, b, tau, Jb, n, perm, m, Xc, bx);
+    } else {
     for (int e = tid; e < rows * m; e += bs) {
       const int row = e / m, i = e - row * m;
       const double* jr = Jb + (int64_t)perm[i] * n;
@@ -526,6 +926,7 @@ __device__ __forceinline__ int run_move(double* ws, const int rows, const int b,
       }
       const int aa = row / bx, bb = row - aa * bx;
       Xc[((int64_t)aa * m + i) * bx + bb] = s;
+    }
     }
   }
   // u <- R (x)_{au} u : Y[(aa*m + i)*Bu + bb] = sum_c R[i][c] U[(aa*nu + c)*Bu + bb].
@@ -545,7 +946,18 @@ __device__ __forceinline__ int run_move(double* ws, const int rows, const int b,
       }
       double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
       const double* up = Uc + (int64_t)aa * nuu * bu + bb;
-      for (int c = 0; c < nuu; ++c) {
+      int c = 0;
+      // eight entries of u in flight per wait (the core is read from global memory)
+      for (; c + 8 <= nuu; c += 8) {
+        double uv[8];
+#pragma unroll
+        for (int q2 = 0; q2 < 8; ++q2) uv[q2] = up[(int64_t)(c + q2) * bu];
+#pragma unroll
+        for (int q2 = 0; q2 < 8; ++q2)
+#pragma unroll
+          for (int q = 0; q < 8; ++q) acc[q] = __builtin_fma(rr[q][c + q2], uv[q2], acc[q]);
+      }
+      for (; c < nuu; ++c) {
         const double uv = up[(int64_t)c * bu];
 #pragma unroll
         for (int q = 0; q < 8; ++q) acc[q] = __builtin_fma(rr[q][c], uv, acc[q]);
@@ -576,37 +988,52 @@ __global__ void __launch_bounds__(512) truncate_kernel(const SweepArgs* __restri
   for (int v = tid; v < t.nv; v += bs) bond[v] = a.bond_g[v];
   if (tid == 0) dbc[0] = 0.0;
   __syncthreads();
-  int64_t dx[QTT_MAX_NDIM], du[QTT_MAX_NDIM];
   for (int st = 0; st < a.nsteps; ++st) {
     const int x = a.steps[3 * st], u = a.steps[3 * st + 1], tr = a.steps[3 * st + 2];
-    const int ndx = core_dims(t, bond, x, dx);
-    const int ndu = core_dims(t, bond, u, du);
-    const int ax = axis_of(t, x, u), au = axis_of(t, u, x);
-    int64_t Ax, nx, Bx, Au, nu, Bu;
-    split_dims(dx, ndx, ax, Ax, nx, Bx);
-    split_dims(du, ndu, au, Au, nu, Bu);
-    const int rows = (int)(Ax * Bx), b = (int)nx;
+    // the move's shape is the same in every lane: keep it in scalar registers
+    // (all sizes are < 2^31, checked on the host)
+    int sx[3], su[3];
+    {
+      int64_t dx[QTT_MAX_NDIM], du[QTT_MAX_NDIM];
+      const int ndx = core_dims(t, bond, x, dx);
+      const int ndu = core_dims(t, bond, u, du);
+      const int ax = axis_of(t, x, u), au = axis_of(t, u, x);
+      int64_t A0, n0, B0, A1, n1, B1;
+      split_dims(dx, ndx, ax, A0, n0, B0);
+      split_dims(du, ndu, au, A1, n1, B1);
+      sx[0] = __builtin_amdgcn_readfirstlane((int)A0);
+      sx[1] = __builtin_amdgcn_readfirstlane((int)n0);
+      sx[2] = __builtin_amdgcn_readfirstlane((int)B0);
+      su[0] = __builtin_amdgcn_readfirstlane((int)A1);
+      su[1] = __builtin_amdgcn_readfirstlane((int)n1);
+      su[2] = __builtin_amdgcn_readfirstlane((int)B1);
+    }
+    const int64_t Bx = sx[2], Au = su[0], nu = su[1], Bu = su[2];
+    const int rows = sx[0] * sx[2], b = sx[1];
     const int k = rows < b ? rows : b;
     const int64_t jsz = tr ? (int64_t)k * k : 0;
-    const int64_t need = (int64_t)rows * b + (int64_t)k * b + jsz + 4 * (int64_t)k + 2 * (int64_t)rows + 16;
+    const bool small = rows <= 256 && b <= 256;
+    // register QR: rows <= 64 or <= 128 (8 or 16 rows per lane), columns <= 2 per 8-lane group
+    const int rpl = small && b <= 2 * (bs / 8) ? (rows <= 64 ? 8 : rows <= 128 ? 16 : 0) : 0;
+    const int64_t wsz = rpl ? max((int64_t)rows * b, (int64_t)k * 8 * rpl) : (int64_t)rows * b;
+    const int64_t need = wsz + (int64_t)k * b + jsz + 4 * (int64_t)k + 2 * (int64_t)rows + 16;
     double* Xc = a.data + a.off[x];
     const double* Uc = a.data + a.off[u];
     double* Ud = a.data + a.off[u];
-    const bool small = rows <= 256 && b <= 256;
     int m;
     if (need <= a.ws_lds) {
       if (small)
-        m = run_move<8>(wsl, rows, b, tr, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<8, true>(wsl, rows, b, tr, rpl, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
                         iflag, red, dbc);
       else
-        m = run_move<64>(wsl, rows, b, tr, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<64, false>(wsl, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
                          iflag, red, dbc);
     } else {
       if (small)
-        m = run_move<8>(a.gscr, rows, b, tr, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<8, false>(a.gscr, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
                         iflag, red, dbc);
       else
-        m = run_move<64>(a.gscr, rows, b, tr, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<64, false>(a.gscr, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
                          iflag, red, dbc);
     }
     if (tid == 0) {
@@ -657,15 +1084,17 @@ int launch_truncate_batch(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double c
     }
     if (maxcap >= (int64_t(1) << 31) / 8)
       return qtt_fail(ctx, QTT_EINVAL, "truncate: a core is too large for the sweep kernel");
-    // workspace bound: rows*b <= core size, k <= b, staging rows <= core size; plus the Y buffer
-    ws[i] = 3 * maxcap + 2 * maxb * maxb + 4 * maxb + 64 + maxcap;
+    // workspace bound: W (or the reflector rows, <= max(2 core, 64 maxb)), R, J,
+    // staging rows (<= 2 core), vectors; plus the Y buffer
+    ws[i] = 4 * maxcap + 2 * maxb * maxb + 68 * maxb + 64 + maxcap;
     soff[i] = tot_steps;
     tot_doubles += ws[i];
     tot_steps += (int64_t)steps[i].size();
     maxnv = std::max(maxnv, net->t.nv);
   }
   const size_t args_bytes = (size_t)nnet * sizeof(SweepArgs);
-  const size_t bytes = (tot_doubles + nnet + 8) * sizeof(double) + args_bytes + tot_steps * sizeof(int) + 256;
+  // the tail slack covers the whole-block row reads of the Jacobi rounds (< 256 doubles past a row)
+  const size_t bytes = (tot_doubles + nnet + 8) * sizeof(double) + args_bytes + tot_steps * sizeof(int) + 4096;
   double* scr = qtt_scratch(ctx, bytes);
   if (!scr) return qtt_fail(ctx, QTT_ENOMEM, "truncate scratch");
   double* d_err = scr;
