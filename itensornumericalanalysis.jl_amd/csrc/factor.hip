@@ -24,6 +24,24 @@
 
 #include "qtt_internal.h"
 
+// Phase timestamps of the sweep for profiling builds (-DQTT_PHASE_TIMING; step
+// -1 = the step opened last, read back with qtt_debug_times); empty otherwise.
+#ifdef QTT_PHASE_TIMING
+__device__ unsigned long long qtt_phase_t[8192 * 8];
+__device__ int qtt_phase_step;
+__device__ inline void qtt_phase_mark(int step, int phase, unsigned long long v) {
+  if (step >= 0) qtt_phase_step = step;
+  if (qtt_phase_step < 8192) qtt_phase_t[qtt_phase_step * 8 + phase] = v;
+}
+extern "C" int qtt_debug_times(unsigned long long* out, int n) {
+  return (int)hipMemcpyFromSymbol(out, HIP_SYMBOL(qtt_phase_t), (size_t)n * 8 * sizeof(unsigned long long));
+}
+#define QTT_PHASE(step, phase) \
+  do { if (threadIdx.x == 0) qtt_phase_mark((step), (phase), wall_clock64()); } while (0)
+#else
+#define QTT_PHASE(step, phase) do { } while (0)
+#endif
+
 namespace {
 
 // ---------------------------------------------------------------------------
@@ -147,6 +165,30 @@ __device__ inline int axis_of(const TopoView& t, int x, int u) {
   for (int k = 0; k < t.nch[x]; ++k)
     if (ch[k] == u) return 2 + k;
   return -1;
+}
+
+// Core v viewed as (A, n, B) around its bond axis to w: dims [parent bond,
+// site, child bonds...], n = the bond to w, A / B the products before / after.
+__device__ inline void move_shape(const TopoView& t, const int* bond, int v, int w, int* out) {
+  int A = 1, n = 1, B = 1;
+  bool seen = false;
+  auto visit = [&](int d, bool axis) {
+    if (axis) {
+      n = d;
+      seen = true;
+    } else if (seen) {
+      B *= d;
+    } else {
+      A *= d;
+    }
+  };
+  visit(bond[v], t.parent[v] == w);
+  visit(t.sdim[v], false);
+  const int* ch = t.chs + t.choff[v];
+  for (int k = 0; k < t.nch[v]; ++k) visit(bond[ch[k]], ch[k] == w);
+  out[0] = A;
+  out[1] = n;
+  out[2] = B;
 }
 
 __device__ inline void split_dims(const int64_t* dims, int nd, int ax, int64_t& A, int64_t& n, int64_t& B) {
@@ -301,15 +343,14 @@ __device__ void org2r(double* W, int rows, int k, int ld, const double* tau, dou
 }
 
 // ---------------------------------------------------------------------------
-// register-resident Householder QR for matrices of up to 8 * RPL rows and
-// QNS * groups columns (8-lane groups).  Lane gl of a group holds rows
-// gl + 8 t (t < RPL) of each of its columns in registers; column c belongs to
-// group c % groups, slot c / groups.  Only the reflectors travel through
-// memory: reflector j is the row VT[j * 8 RPL, (j + 1) * 8 RPL), zero past
+// register-resident Householder QR for matrices of up to GL * RPL rows and
+// NS * groups columns (groups of GL lanes).  Lane gl of a group holds rows
+// gl + GL t (t < RPL) of each of its columns in registers; column c belongs
+// to group c % groups, slot c / groups.  Only the reflectors travel through
+// memory: reflector j is the row VT[j * GL RPL, (j + 1) * GL RPL), zero past
 // the last matrix row, so every step reads it with fixed offsets from one
-// base address (one contiguous 64-byte load per group and register).
+// base address (one contiguous load per group and register).
 // ---------------------------------------------------------------------------
-constexpr int QNS = 2;
 
 // Lane masks built with integer arithmetic (all ones / zero), so the per-row
 // tests of the unrolled loops need no condition registers.
@@ -331,15 +372,23 @@ __device__ __forceinline__ int opaque(int x) {
   return x;
 }
 
-template <int RPL>
-__device__ __forceinline__ void reg_load_v(const double* VT, int j, int gl, double (&v)[RPL]) {
-  const double* p = VT + j * (8 * RPL) + gl;
+// sum over aligned groups of GL (8 or 16) lanes
+template <int GL>
+__device__ __forceinline__ double lanesum(double v) {
+  v = sum8(v);
+  if constexpr (GL >= 16) v += dpp_d<0x140>(v);  // row_mirror: 16-lane sums
+  return v;
+}
+
+template <int GL, int RPL, typename CP>
+__device__ __forceinline__ void reg_load_v(CP VT, int j, int gl, double (&v)[RPL]) {
+  CP p = VT + j * (GL * RPL) + gl;
 #pragma unroll
-  for (int t = 0; t < RPL; ++t) v[t] = p[8 * t];
+  for (int t = 0; t < RPL; ++t) v[t] = p[GL * t];
 }
 
 // a <- (I - tau v v^T) a for the column held in a
-template <int RPL>
+template <int GL, int RPL>
 __device__ __forceinline__ void reg_apply(double (&a)[RPL], const double (&v)[RPL], double tj) {
   double d0 = 0.0, d1 = 0.0;
 #pragma unroll
@@ -347,43 +396,43 @@ __device__ __forceinline__ void reg_apply(double (&a)[RPL], const double (&v)[RP
     d0 = __builtin_fma(v[t], a[t], d0);
     d1 = __builtin_fma(v[t + 1], a[t + 1], d1);
   }
-  const double d = sum8(d0 + d1) * tj;
+  const double d = lanesum<GL>(d0 + d1) * tj;
 #pragma unroll
   for (int t = 0; t < RPL; ++t) a[t] = __builtin_fma(-d, v[t], a[t]);
 }
 
 // the same with the reflector read from memory in each pass (fewer live registers)
-template <int RPL>
-__device__ __forceinline__ void reg_apply_mem(double (&a)[RPL], const double* VT, int j, int gl, double tj) {
-  const double* p = VT + j * (8 * RPL) + gl;
+template <int GL, int RPL, typename CP>
+__device__ __forceinline__ void reg_apply_mem(double (&a)[RPL], CP VT, int j, int gl, double tj) {
+  CP p = VT + j * (GL * RPL) + gl;
   double d0 = 0.0, d1 = 0.0;
 #pragma unroll
   for (int t = 0; t + 1 < RPL; t += 2) {
-    d0 = __builtin_fma(p[8 * t], a[t], d0);
-    d1 = __builtin_fma(p[8 * t + 8], a[t + 1], d1);
+    d0 = __builtin_fma(p[GL * t], a[t], d0);
+    d1 = __builtin_fma(p[GL * t + GL], a[t + 1], d1);
   }
-  const double d = sum8(d0 + d1) * tj;
+  const double d = lanesum<GL>(d0 + d1) * tj;
 #pragma unroll
-  for (int t = 0; t < RPL; ++t) a[t] = __builtin_fma(-d, p[8 * t], a[t]);
+  for (int t = 0; t < RPL; ++t) a[t] = __builtin_fma(-d, p[GL * t], a[t]);
 }
 
 // Reflector of column j held in a (LAPACK dlarfg): a keeps its R part (beta at
 // row j, zeros below), the reflector goes to VT (zero above row j, one at row
 // j), tau[j] the scalar factor.
-template <int RPL>
-__device__ __forceinline__ void reg_reflector(double (&a)[RPL], int j, int gl_in, double* VT, double* tau) {
+template <int GL, int RPL, typename DP>
+__device__ __forceinline__ void reg_reflector(double (&a)[RPL], int j, int gl_in, DP VT, DP tau) {
   const int gl = opaque(gl_in);
   double s0 = 0.0, s1 = 0.0, xj = 0.0;
 #pragma unroll
   for (int t = 0; t + 1 < RPL; t += 2) {
-    const int r0 = gl + 8 * t, r1 = r0 + 8;
+    const int r0 = gl + GL * t, r1 = r0 + GL;
     const double x0 = keep(a[t], lt_mask(j, r0)), x1 = keep(a[t + 1], lt_mask(j, r1));
     s0 = __builtin_fma(x0, x0, s0);
     s1 = __builtin_fma(x1, x1, s1);
     xj += keep(a[t], eq_mask(r0, j)) + keep(a[t + 1], eq_mask(r1, j));
   }
-  const double xn2 = sum8(s0 + s1);
-  const double x0 = sum8(xj);
+  const double xn2 = lanesum<GL>(s0 + s1);
+  const double x0 = lanesum<GL>(xj);
   double tj = 0.0, beta = x0, scal = 0.0;
   if (xn2 > 0.0) {
     const double nrm = sqrt(x0 * x0 + xn2);
@@ -391,51 +440,51 @@ __device__ __forceinline__ void reg_reflector(double (&a)[RPL], int j, int gl_in
     tj = (beta - x0) / beta;
     scal = 1.0 / (x0 - beta);
   }
-  double* p = VT + j * (8 * RPL) + gl;
+  DP p = VT + j * (GL * RPL) + gl;
 #pragma unroll
   for (int t = 0; t < RPL; ++t) {
-    const int row = gl + 8 * t;
+    const int row = gl + GL * t;
     const uint64_t below = lt_mask(j, row), diag = eq_mask(row, j), above = lt_mask(row, j);
-    p[8 * t] = keep(a[t] * scal, below) + keep(1.0, diag);  // rows past the matrix hold a = 0
+    p[GL * t] = keep(a[t] * scal, below) + keep(1.0, diag);  // rows past the matrix hold a = 0
     a[t] = keep(a[t], above) + keep(beta, diag);
   }
   if (gl == 0) tau[j] = tj;
 }
 
-// QR of W (rows x cols, leading dimension cols, rows <= 8 RPL): R (k x cols)
-// into Rb, the reflectors into VT, which overwrites W (k * 8 RPL doubles from
-// W).  The owner of column j + 1 updates it first and builds the
-// next reflector right away, so there is one barrier per column.
-template <int RPL>
-__device__ __forceinline__ void qr_reg(double* W, int rows, int cols, double* tau, double* Rb) {
-  const int tid = threadIdx.x, bs = blockDim.x, gl = tid & 7, gid = tid >> 3, ng = bs >> 3;
+// QR of W (rows x cols, leading dimension cols, rows <= GL RPL, cols <= NS
+// groups): R (k x cols) into Rb, the reflectors into VT, which overwrites W
+// (k * GL RPL doubles from W).  The owner of column j + 1 updates it first and
+// builds the next reflector right away, so there is one barrier per column.
+template <int GL, int RPL, int NS, typename DP>
+__device__ __forceinline__ void qr_body(DP W, int rows, int cols, DP tau, DP Rb) {
+  const int tid = threadIdx.x, bs = blockDim.x, gl = tid & (GL - 1), gid = tid / GL, ng = bs / GL;
   const int k = rows < cols ? rows : cols;
-  double a[QNS][RPL];
+  double a[NS][RPL];
   const int gll = opaque(gl);
 #pragma unroll
-  for (int s = 0; s < QNS; ++s) {
+  for (int s = 0; s < NS; ++s) {
     const int c = gid + ng * s;
     const int cc = c < cols ? c : cols - 1;
 #pragma unroll
     for (int t = 0; t < RPL; ++t) {
-      const int row = gll + 8 * t;
+      const int row = gll + GL * t;
       const double x = W[(row < rows ? row : rows - 1) * cols + cc];  // clamped: always in the matrix
       a[s][t] = c < cols ? keep(x, lt_mask(row, rows)) : 0.0;
     }
   }
   __syncthreads();
-  double* VT = W;
-  if (gid == 0) reg_reflector<RPL>(a[0], 0, gl, VT, tau);
+  DP VT = W;
+  if (gid == 0) reg_reflector<GL, RPL>(a[0], 0, gl, VT, tau);
   __syncthreads();
   for (int j = 0; j < k; ++j) {
     const double tj = tau[j];
-    if (gid + ng * (QNS - 1) > j) {
+    if (gid + ng * (NS - 1) > j) {
 #pragma unroll
-      for (int s = 0; s < QNS; ++s) {
+      for (int s = 0; s < NS; ++s) {
         const int c = gid + ng * s;
         if (c > j && c < cols) {
-          if (tj != 0.0) reg_apply_mem<RPL>(a[s], VT, j, gl, tj);
-          if (c == j + 1 && c < k) reg_reflector<RPL>(a[s], c, gl, VT, tau);
+          if (tj != 0.0) reg_apply_mem<GL, RPL>(a[s], VT, j, gl, tj);
+          if (c == j + 1 && c < k) reg_reflector<GL, RPL>(a[s], c, gl, VT, tau);
         }
       }
     }
@@ -443,12 +492,12 @@ __device__ __forceinline__ void qr_reg(double* W, int rows, int cols, double* ta
   }
   const int glr = opaque(gl);
 #pragma unroll
-  for (int s = 0; s < QNS; ++s) {
+  for (int s = 0; s < NS; ++s) {
     const int c = gid + ng * s;
     if (c < cols) {
 #pragma unroll
       for (int t = 0; t < RPL; ++t) {
-        const int row = glr + 8 * t;
+        const int row = glr + GL * t;
         if (row < k) Rb[row * cols + c] = a[s][t];
       }
     }
@@ -456,22 +505,21 @@ __device__ __forceinline__ void qr_reg(double* W, int rows, int cols, double* ta
 }
 
 // u <- H_0 ... H_jtop u
-template <int RPL>
-__device__ __forceinline__ void reg_apply_chain(double (&u)[RPL], const double* VT, const double* tau, int gl,
-                                                int jtop) {
+template <int GL, int RPL, typename CP>
+__device__ __forceinline__ void reg_apply_chain(double (&u)[RPL], CP VT, CP tau, int gl, int jtop) {
   for (int j = jtop; j >= 0; --j) {
     const double tj = tau[j];
-    if (tj != 0.0) reg_apply_mem<RPL>(u, VT, j, gl, tj);
+    if (tj != 0.0) reg_apply_mem<GL, RPL>(u, VT, j, gl, tj);
   }
 }
 
-// Offset in the core of row gl + 8 t of the matrix (rows split as (aa, bb),
+// Offset in the core of row gl + GL t of the matrix (rows split as (aa, bb),
 // bb < bx, bond index 0), -1 past the last row.
-template <int RPL>
+template <int GL, int RPL>
 __device__ __forceinline__ void reg_dest(int rows, int gl, int m, int bx, int (&dst)[RPL]) {
 #pragma unroll
   for (int t = 0; t < RPL; ++t) {
-    const int row = gl + 8 * t;
+    const int row = gl + GL * t;
     const int aa = row / bx, bb = row - aa * bx;
     const int valid = (int)lt_mask(row, rows);
     dst[t] = ((aa * m * bx + bb) & valid) | ~valid;
@@ -488,46 +536,70 @@ __device__ __forceinline__ void reg_scatter(const double (&u)[RPL], const int (&
 }
 
 // x <- Q (rows x k): column c = H_0 ... H_c e_c, no barriers.
-template <int RPL>
-__device__ __forceinline__ void q_form_reg(const double* VT, int rows, int k, const double* tau, double* Xc,
-                                           int bx) {
-  const int tid = threadIdx.x, gl = tid & 7, gid = tid >> 3, ng = blockDim.x >> 3;
+template <int GL, int RPL, typename CP>
+__device__ __forceinline__ void q_form_body(CP VT, int rows, int k, CP tau, double* Xc, int bx) {
+  const int tid = threadIdx.x, gl = tid & (GL - 1), gid = tid / GL, ng = blockDim.x / GL;
   int dst[RPL];
-  reg_dest<RPL>(rows, gl, k, bx, dst);
+  reg_dest<GL, RPL>(rows, gl, k, bx, dst);
   for (int c = gid; c < k; c += ng) {
     const int glo = opaque(gl);
     double u[RPL];
 #pragma unroll
-    for (int t = 0; t < RPL; ++t) u[t] = keep(1.0, eq_mask(glo + 8 * t, c));
-    reg_apply_chain<RPL>(u, VT, tau, gl, c);
+    for (int t = 0; t < RPL; ++t) u[t] = keep(1.0, eq_mask(glo + GL * t, c));
+    reg_apply_chain<GL, RPL>(u, VT, tau, gl, c);
     reg_scatter<RPL>(u, dst, Xc, c, bx);
   }
 }
 
 // x <- U = Q [J_perm^T; 0] (rows x m): column i starts as row perm[i] of J
 // (length n) and takes all k reflectors.
-template <int RPL>
-__device__ __forceinline__ void u_form_reg(const double* VT, int rows, int k, const double* tau, const double* Jb,
-                                           int n, const int* perm, int m, double* Xc, int bx) {
-  const int tid = threadIdx.x, gl = tid & 7, gid = tid >> 3, ng = blockDim.x >> 3;
+template <int GL, int RPL, typename CP, typename CIP>
+__device__ __forceinline__ void u_form_body(CP VT, int rows, int k, CP tau, CP Jb, int n, CIP perm, int m,
+                                            double* Xc, int bx) {
+  const int tid = threadIdx.x, gl = tid & (GL - 1), gid = tid / GL, ng = blockDim.x / GL;
   int dst[RPL];
-  reg_dest<RPL>(rows, gl, m, bx, dst);
+  reg_dest<GL, RPL>(rows, gl, m, bx, dst);
   for (int i = gid; i < m; i += ng) {
-    const double* jr = Jb + (int64_t)perm[i] * n;
+    CP jr = Jb + perm[i] * n;
     const int glo = opaque(gl);
     double u[RPL];
 #pragma unroll
     for (int t = 0; t < RPL; ++t) {
-      const int row = glo + 8 * t;
+      const int row = glo + GL * t;
       u[t] = keep(jr[row < n ? row : n - 1], lt_mask(row, n));  // clamped read, zero past n
     }
-    reg_apply_chain<RPL>(u, VT, tau, gl, k - 1);
+    reg_apply_chain<GL, RPL>(u, VT, tau, gl, k - 1);
     reg_scatter<RPL>(u, dst, Xc, i, bx);
   }
 }
 
+// The register QR pieces run as separate (not inlined) functions on an LDS
+// workspace: their registers then do not add to the sweep kernel's, and the
+// LDS-qualified pointers keep the memory operations on LDS instructions.
+typedef __attribute__((address_space(3))) double lds_f64;
+typedef __attribute__((address_space(3))) int lds_i32;
+
+template <int GL, int RPL, int NS>
+__device__ __noinline__ void qr_reg(lds_f64* W, int rows, int cols, lds_f64* tau, lds_f64* Rb) {
+  qr_body<GL, RPL, NS, lds_f64*>(W, rows, cols, tau, Rb);
+}
+
+template <int GL, int RPL>
+__device__ __noinline__ void q_form_reg(const lds_f64* VT, int rows, int k, const lds_f64* tau, double* Xc, int bx) {
+  q_form_body<GL, RPL, const lds_f64*>(VT, rows, k, tau, Xc, bx);
+}
+
+template <int GL, int RPL>
+__device__ __noinline__ void u_form_reg(const lds_f64* VT, int rows, int k, const lds_f64* tau, const lds_f64* Jb,
+                                        int n, const lds_i32* perm, int m, double* Xc, int bx) {
+  u_form_body<GL, RPL, const lds_f64*, const lds_i32*>(VT, rows, k, tau, Jb, n, perm, m, Xc, bx);
+}
+
 __device__ inline int rr_player(int pos, int t, int nn) {
-  return pos == 0 ? 0 : 1 + (pos - 1 + t) % (nn - 1);
+  // pos < nn and t < nn - 1, so one conditional subtraction is the modulo
+  int x = pos - 1 + t;
+  x = x >= nn - 1 ? x - (nn - 1) : x;
+  return pos == 0 ? 0 : 1 + x;
 }
 
 // One-sided Jacobi on the rows of X (n x b, ld b), rotations accumulated in J
@@ -634,11 +706,8 @@ __device__ bool jacobi_rows(double* X, int n, int b, double* J, int* flag, doubl
 // which is padded) are read and replaced by zeros.
 constexpr int JEPL = 4;
 
-// LDS-qualified pointers: the rounds run in a separate (not inlined) function
-// so that its registers do not add to the rest of the sweep kernel, and these
-// types keep its memory operations on the LDS instructions.
-typedef __attribute__((address_space(3))) double lds_f64;
-typedef __attribute__((address_space(3))) int lds_i32;
+// The rounds run in a separate (not inlined) function on LDS-qualified
+// pointers, like the register QR above.
 
 template <int G>
 __device__ __forceinline__ double jsum(double v) {
@@ -720,9 +789,13 @@ __device__ __noinline__ bool jacobi_rows_reg(lds_f64* X, int n, int b, lds_f64* 
         const int gl = opaque(tid & (G - 1));
         lds_f64* xp = X + p * b;
         lds_f64* xq = X + q * b;
-        double u[JEPL], w[JEPL];
+        lds_f64* jp = J + p * n;
+        lds_f64* jq = J + q * n;
+        double u[JEPL], w[JEPL], ju[JEPL], jw[JEPL];
         load_row<G>(xp, b, gl, u);
         load_row<G>(xq, b, gl, w);
+        load_row<G>(jp, n, gl, ju);  // read with X so both rows arrive in one wait
+        load_row<G>(jq, n, gl, jw);
         double a0 = 0.0, c0 = 0.0, g0 = 0.0, a1 = 0.0, c1 = 0.0, g1 = 0.0;
 #pragma unroll
         for (int e = 0; e < JEPL; e += 2) {
@@ -743,15 +816,11 @@ __device__ __noinline__ bool jacobi_rows_reg(lds_f64* X, int n, int b, lds_f64* 
           const double tt = zeta >= 0.0 ? t0 : -t0;
           const double cs = rsq_nr(__builtin_fma(tt, tt, 1.0)), sn = cs * tt;
           rotate4(u, w, cs, sn);
+          rotate4(ju, jw, cs, sn);
           store_row<G>(xp, b, gl, u);
           store_row<G>(xq, b, gl, w);
-          lds_f64* jp = J + p * n;
-          lds_f64* jq = J + q * n;
-          load_row<G>(jp, n, gl, u);
-          load_row<G>(jq, n, gl, w);
-          rotate4(u, w, cs, sn);
-          store_row<G>(jp, n, gl, u);
-          store_row<G>(jq, n, gl, w);
+          store_row<G>(jp, n, gl, ju);
+          store_row<G>(jq, n, gl, jw);
           if (gl == 0) *flag = 1;
         }
       }
@@ -775,6 +844,61 @@ __device__ inline void row_norms(const double* X, int n, int b, double* sig) {
   }
 }
 
+// Mode product Y[(aa*m + i)*Bu + bb] = sum_c R[row_i][c] U[(aa*nu + c)*Bu + bb]
+// (row_i = perm[i] or i).  One task = one (aa, bb) column and 8 output rows:
+// u is read once per 8 rows, eight of its entries in flight per wait.
+template <typename RP, typename IP>
+__device__ __forceinline__ void mode_product_body(RP Rrows, IP perm, bool use_perm, int m, int b,
+                                                  const double* __restrict__ Uc, double* __restrict__ gY, int Au,
+                                                  int nu, int Bu) {
+  const int tid = threadIdx.x, bs = blockDim.x;
+  const int ncol = Au * Bu;
+  const int nib = (m + 7) >> 3;
+  for (int task = tid; task < ncol * nib; task += bs) {
+    const int j = task % ncol, ib = task / ncol;
+    const int aa = j / Bu, bb = j - aa * Bu;
+    RP rr[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int i = min(ib * 8 + q, m - 1);
+      rr[q] = Rrows + (use_perm ? perm[i] : i) * b;
+    }
+    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const double* up = Uc + (int64_t)aa * nu * Bu + bb;
+    int c = 0;
+    for (; c + 8 <= nu; c += 8) {
+      double uv[8];
+#pragma unroll
+      for (int q2 = 0; q2 < 8; ++q2) uv[q2] = up[(int64_t)(c + q2) * Bu];
+#pragma unroll
+      for (int q2 = 0; q2 < 8; ++q2)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc[q] = __builtin_fma(rr[q][c + q2], uv[q2], acc[q]);
+    }
+    for (; c < nu; ++c) {
+      const double uv = up[(int64_t)c * Bu];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) acc[q] = __builtin_fma(rr[q][c], uv, acc[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int i = ib * 8 + q;
+      if (i < m) gY[((int64_t)aa * m + i) * Bu + bb] = acc[q];
+    }
+  }
+}
+
+// LDS-resident R
+__device__ __forceinline__ void mode_product_lds(const lds_f64* Rrows, const lds_i32* perm, bool use_perm, int m, int b,
+                                              const double* Uc, double* gY, int Au, int nu, int Bu) {
+  mode_product_body(Rrows, perm, use_perm, m, b, Uc, gY, Au, nu, Bu);
+}
+
+__device__ __forceinline__ void mode_product(const double* Rrows, const int* perm, bool use_perm, int m, int b,
+                                             const double* Uc, double* gY, int Au, int nu, int Bu) {
+  mode_product_body(Rrows, perm, use_perm, m, b, Uc, gY, Au, nu, Bu);
+}
+
 // One move of the sweep, with the workspace `ws` either in LDS or in global
 // scratch.  Two separate instantiations (one per memory) let the compiler use
 // LDS instructions on the LDS path instead of generic flat addressing.
@@ -786,7 +910,10 @@ __device__ __forceinline__ int run_move(double* ws, const int rows, const int b,
   const int tid = threadIdx.x, bs = blockDim.x;
   const int k = rows < b ? rows : b;
   // register QR for the shapes it covers (the QR of a tall SVD matrix included)
-  const bool use_reg = REG && G == 8 && rpl != 0;  // register QR (rows <= 8 rpl) for the QR steps
+  // register QR for the QR steps: rpl = rows per lane bound / 8 (8: rows <= 64,
+  // 16: rows <= 128, both 8-lane groups with 2 column slots; 24: rows <= 192,
+  // 16-lane groups with 3 column slots)
+  const bool use_reg = REG && G == 8 && rpl != 0;
   // workspace: W rows x b (or the reflector rows) | R k x b | J k x k (SVD only) |
   //            tau, sig, perm | 2 staging rows
   const int64_t jsz = tr ? (int64_t)k * k : 0;
@@ -805,6 +932,7 @@ __device__ __forceinline__ int run_move(double* ws, const int rows, const int b,
     W[e] = Xc[((int64_t)aa * b + i) * bx + bb];
   }
   __syncthreads();
+  QTT_PHASE(-1, 1);  // gathered
   int m, n;
   const double* Rrows;
   bool use_perm = false, tall = false;
@@ -813,12 +941,16 @@ __device__ __forceinline__ int run_move(double* ws, const int rows, const int b,
     n = k;
     Rrows = Rb;
     if (use_reg) {
-      if (rpl == 8) qr_reg<8>(W, rows, b, tau, Rb);
-      else qr_reg<16>(W, rows, b, tau, Rb);
+      lds_f64 *Wl = (lds_f64*)W, *taul = (lds_f64*)tau, *Rl = (lds_f64*)Rb;
+      if (rpl == 8) qr_reg<8, 8, 2>(Wl, rows, b, taul, Rl);
+      else if (rpl == 16) qr_reg<8, 16, 2>(Wl, rows, b, taul, Rl);
+      else qr_reg<16, 12, 3>(Wl, rows, b, taul, Rl);
       __syncthreads();
+      QTT_PHASE(-1, 2);  // factored
       // x <- Q (bond m)
-      if (rpl == 8) q_form_reg<8>(W, rows, k, tau, Xc, bx);
-      else q_form_reg<16>(W, rows, k, tau, Xc, bx);
+      if (rpl == 8) q_form_reg<8, 8>(Wl, rows, k, taul, Xc, bx);
+      else if (rpl == 16) q_form_reg<8, 16>(Wl, rows, k, taul, Xc, bx);
+      else q_form_reg<16, 12>(Wl, rows, k, taul, Xc, bx);
     } else {
       geqr2<G>(W, rows, b, b, tau);
       for (int e = tid; e < k * b; e += bs) {
@@ -826,6 +958,7 @@ __device__ __forceinline__ int run_move(double* ws, const int rows, const int b,
         Rb[e] = c >= i ? W[(int64_t)i * b + c] : 0.0;
       }
       __syncthreads();
+      QTT_PHASE(-1, 2);  // factored
       org2r<G>(W, rows, k, b, tau, vbuf);
       // x <- Q (bond m)
       for (int e = tid; e < rows * m; e += bs) {
@@ -840,8 +973,10 @@ __device__ __forceinline__ int run_move(double* ws, const int rows, const int b,
     n = rows;
     if (tall) {
       if (use_reg) {  // the reflectors stay in W for U below
-        if (rpl == 8) qr_reg<8>(W, rows, b, tau, Rb);
-        else qr_reg<16>(W, rows, b, tau, Rb);
+        lds_f64 *Wl = (lds_f64*)W, *taul = (lds_f64*)tau, *Rl = (lds_f64*)Rb;
+        if (rpl == 8) qr_reg<8, 8, 2>(Wl, rows, b, taul, Rl);
+        else if (rpl == 16) qr_reg<8, 16, 2>(Wl, rows, b, taul, Rl);
+        else qr_reg<16, 12, 3>(Wl, rows, b, taul, Rl);
       } else {
         geqr2<G>(W, rows, b, b, tau);
         for (int e = tid; e < b * b; e += bs) {
@@ -855,6 +990,7 @@ __device__ __forceinline__ int run_move(double* ws, const int rows, const int b,
       X = Rb;
       n = b;
     }
+    QTT_PHASE(-1, 2);  // factored (tall) / gathered (wide)
     // lanes per row pair sized so a row is at most 4 entries per lane
     bool conv;
     // (REG: the workspace is in LDS)
@@ -910,8 +1046,11 @@ __device__ __forceinline__ int run_move(double* ws, const int rows, const int b,
     use_perm = true;
     // x <- U (bond m): U[row][i] = sum_j Q[row][j] J[perm[i]][j]  (or J[perm[i]][row])
     if (tall && use_reg) {
-      if (rpl == 8) u_form_reg<8>(W, rows, b, tau, Jb, n, perm, m, Xc, bx);
-      else u_form_reg<16>(W, rows, b, tau, Jb, n, perm, m, Xc, bx);
+      const lds_f64 *Wl = (const lds_f64*)W, *taul = (const lds_f64*)tau, *Jl = (const lds_f64*)Jb;
+      const lds_i32* pl = (const lds_i32*)perm;
+      if (rpl == 8) u_form_reg<8, 8>(Wl, rows, b, taul, Jl, n, pl, m, Xc, bx);
+      else if (rpl == 16) u_form_reg<8, 16>(Wl, rows, b, taul, Jl, n, pl, m, Xc, bx);
+      else u_form_reg<16, 12>(Wl, rows, b, taul, Jl, n, pl, m, Xc, bx);
     } else {
     for (int e = tid; e < rows * m; e += bs) {
       const int row = e / m, i = e - row * m;
@@ -929,48 +1068,17 @@ __device__ __forceinline__ int run_move(double* ws, const int rows, const int b,
     }
     }
   }
-  // u <- R (x)_{au} u : Y[(aa*m + i)*Bu + bb] = sum_c R[i][c] U[(aa*nu + c)*Bu + bb].
-  // One task = one (aa, bb) column and 8 output rows: u is read once per 8 rows.
+  __syncthreads();
+  QTT_PHASE(-1, 3);  // x written
+  // u <- R (x)_{au} u : Y[(aa*m + i)*Bu + bb] = sum_c R[i][c] U[(aa*nu + c)*Bu + bb]
   const int ytot = (int)(Au * m * Bu);
-  {
-    const int ncol = (int)(Au * Bu), bu = (int)Bu, nuu = (int)nu;
-    const int nib = (m + 7) >> 3;
-    for (int task = tid; task < ncol * nib; task += bs) {
-      const int j = task % ncol, ib = task / ncol;
-      const int aa = j / bu, bb = j - aa * bu;
-      const double* rr[8];
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const int i = min(ib * 8 + q, m - 1);
-        rr[q] = Rrows + (int64_t)(use_perm ? perm[i] : i) * b;
-      }
-      double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-      const double* up = Uc + (int64_t)aa * nuu * bu + bb;
-      int c = 0;
-      // eight entries of u in flight per wait (the core is read from global memory)
-      for (; c + 8 <= nuu; c += 8) {
-        double uv[8];
-#pragma unroll
-        for (int q2 = 0; q2 < 8; ++q2) uv[q2] = up[(int64_t)(c + q2) * bu];
-#pragma unroll
-        for (int q2 = 0; q2 < 8; ++q2)
-#pragma unroll
-          for (int q = 0; q < 8; ++q) acc[q] = __builtin_fma(rr[q][c + q2], uv[q2], acc[q]);
-      }
-      for (; c < nuu; ++c) {
-        const double uv = up[(int64_t)c * bu];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) acc[q] = __builtin_fma(rr[q][c], uv, acc[q]);
-      }
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const int i = ib * 8 + q;
-        if (i < m) gY[((int64_t)aa * m + i) * bu + bb] = acc[q];
-      }
-    }
-  }
+  if (REG)
+    mode_product_lds((const lds_f64*)Rrows, (const lds_i32*)perm, use_perm, m, b, Uc, gY, (int)Au, (int)nu, (int)Bu);
+  else
+    mode_product(Rrows, perm, use_perm, m, b, Uc, gY, (int)Au, (int)nu, (int)Bu);
   __syncthreads();
   for (int e = tid; e < ytot; e += bs) Ud[e] = gY[e];
+  QTT_PHASE(-1, 4);  // u written
   return m;
 }
 
@@ -990,33 +1098,38 @@ __global__ void __launch_bounds__(512) truncate_kernel(const SweepArgs* __restri
   __syncthreads();
   for (int st = 0; st < a.nsteps; ++st) {
     const int x = a.steps[3 * st], u = a.steps[3 * st + 1], tr = a.steps[3 * st + 2];
-    // the move's shape is the same in every lane: keep it in scalar registers
-    // (all sizes are < 2^31, checked on the host)
+    QTT_PHASE(st, 0);
+    // the move's shape, computed once and kept in scalar registers (all sizes
+    // are < 2^31, checked on the host)
+    if (tid == 0) {
+      move_shape(t, bond, x, u, iflag + 4);
+      move_shape(t, bond, u, x, iflag + 7);
+    }
+    __syncthreads();
     int sx[3], su[3];
-    {
-      int64_t dx[QTT_MAX_NDIM], du[QTT_MAX_NDIM];
-      const int ndx = core_dims(t, bond, x, dx);
-      const int ndu = core_dims(t, bond, u, du);
-      const int ax = axis_of(t, x, u), au = axis_of(t, u, x);
-      int64_t A0, n0, B0, A1, n1, B1;
-      split_dims(dx, ndx, ax, A0, n0, B0);
-      split_dims(du, ndu, au, A1, n1, B1);
-      sx[0] = __builtin_amdgcn_readfirstlane((int)A0);
-      sx[1] = __builtin_amdgcn_readfirstlane((int)n0);
-      sx[2] = __builtin_amdgcn_readfirstlane((int)B0);
-      su[0] = __builtin_amdgcn_readfirstlane((int)A1);
-      su[1] = __builtin_amdgcn_readfirstlane((int)n1);
-      su[2] = __builtin_amdgcn_readfirstlane((int)B1);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      sx[i] = __builtin_amdgcn_readfirstlane(iflag[4 + i]);
+      su[i] = __builtin_amdgcn_readfirstlane(iflag[7 + i]);
     }
     const int64_t Bx = sx[2], Au = su[0], nu = su[1], Bu = su[2];
     const int rows = sx[0] * sx[2], b = sx[1];
     const int k = rows < b ? rows : b;
     const int64_t jsz = tr ? (int64_t)k * k : 0;
     const bool small = rows <= 256 && b <= 256;
-    // register QR: rows <= 64 or <= 128 (8 or 16 rows per lane), columns <= 2 per 8-lane group
-    const int rpl = small && b <= 2 * (bs / 8) ? (rows <= 64 ? 8 : rows <= 128 ? 16 : 0) : 0;
-    const int64_t wsz = rpl ? max((int64_t)rows * b, (int64_t)k * 8 * rpl) : (int64_t)rows * b;
-    const int64_t need = wsz + (int64_t)k * b + jsz + 4 * (int64_t)k + 2 * (int64_t)rows + 16;
+    // register QR: rows <= 64 or <= 128 with columns <= 2 per 8-lane group, or
+    // rows <= 192 with columns <= 3 per 16-lane group (the reflector rows take
+    // k * 8 rpl doubles)
+    int rpl = !small ? 0
+                    : rows <= 64 && b <= 2 * (bs / 8) ? 8
+                    : rows <= 128 && b <= 2 * (bs / 8) ? 16
+                    : rows <= 192 && b <= 3 * (bs / 16) ? 24 : 0;
+    const int64_t base = (int64_t)k * b + jsz + 4 * (int64_t)k + 2 * (int64_t)rows + 16;
+    int64_t need = max((int64_t)rows * b, rpl ? (int64_t)k * 8 * rpl : 0) + base;
+    if (rpl && need > a.ws_lds) {  // the reflector rows do not fit: plain LDS path
+      rpl = 0;
+      need = (int64_t)rows * b + base;
+    }
     double* Xc = a.data + a.off[x];
     const double* Uc = a.data + a.off[u];
     double* Ud = a.data + a.off[u];
@@ -1036,6 +1149,7 @@ __global__ void __launch_bounds__(512) truncate_kernel(const SweepArgs* __restri
         m = run_move<64, false>(a.gscr, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
                          iflag, red, dbc);
     }
+    QTT_PHASE(st, 5);
     if (tid == 0) {
       if (t.parent[x] == u) bond[x] = m; else bond[u] = m;
     }
