@@ -38,8 +38,24 @@ extern "C" int qtt_debug_times(unsigned long long* out, int n) {
 }
 #define QTT_PHASE(step, phase) \
   do { if (threadIdx.x == 0) qtt_phase_mark((step), (phase), wall_clock64()); } while (0)
+// cycles of lane 0 per segment of a Jacobi round, plus round / sweep counts
+__device__ unsigned long long qtt_jac_t[8];
+extern "C" int qtt_debug_jacobi(unsigned long long* out) {
+  return (int)hipMemcpyFromSymbol(out, HIP_SYMBOL(qtt_jac_t), 8 * sizeof(unsigned long long));
+}
+#define QTT_JT_INIT unsigned long long jt_last_ = clock64()
+#define QTT_JT(i)                                                   \
+  do {                                                              \
+    const unsigned long long c_ = clock64();                        \
+    if (threadIdx.x == 0) qtt_jac_t[i] += c_ - jt_last_;            \
+    jt_last_ = c_;                                                  \
+  } while (0)
+#define QTT_JC(i) do { if (threadIdx.x == 0) qtt_jac_t[i] += 1; } while (0)
 #else
 #define QTT_PHASE(step, phase) do { } while (0)
+#define QTT_JT_INIT do { } while (0)
+#define QTT_JT(i) do { } while (0)
+#define QTT_JC(i) do { } while (0)
 #endif
 
 namespace {
@@ -763,21 +779,29 @@ __device__ __forceinline__ void rotate4(double (&u)[JEPL], double (&w)[JEPL], do
 }
 
 template <int G>
-__device__ __noinline__ bool jacobi_rows_reg(lds_f64* X, int n, int b, lds_f64* J, lds_i32* flag, double* red) {
+__device__ __noinline__ bool jacobi_rows_reg(lds_f64* X, int ldx, int n, int b, lds_f64* J, int ldj, lds_i32* flag,
+                                             double* red) {
   const int tid = threadIdx.x, bs = blockDim.x;
   const int gid = tid / G, ng = bs / G;
-  for (int e = tid; e < n * n; e += bs) J[e] = (e / n == e % n) ? 1.0 : 0.0;
+  for (int e = tid; e < n * n; e += bs) J[(e / n) * ldj + e % n] = (e / n == e % n) ? 1.0 : 0.0;
   __syncthreads();
   if (n < 2) return true;
   const int nn = n + (n & 1);
   const double tol = DBL_EPSILON * (double)(b > n ? b : n);
   double fro = 0.0;
-  for (int64_t e = tid; e < (int64_t)n * b; e += bs) fro += X[e] * X[e];
+  for (int e = tid; e < n * b; e += bs) {
+    const double x = X[(e / b) * ldx + e % b];
+    fro += x * x;
+  }
   const double small = block_sum(fro, red) * DBL_EPSILON * DBL_EPSILON;
+  QTT_JT_INIT;
   for (int sweep = 0; sweep < 100; ++sweep) {
     if (tid == 0) *flag = 0;
     __syncthreads();
+    QTT_JC(6);
     for (int t = 0; t < nn - 1; ++t) {
+      QTT_JT(4);
+      QTT_JC(7);
       for (int pi = gid; pi < nn / 2; pi += ng) {
         int p = rr_player(pi, t, nn), q = rr_player(nn - 1 - pi, t, nn);
         if (p > q) {
@@ -787,10 +811,10 @@ __device__ __noinline__ bool jacobi_rows_reg(lds_f64* X, int n, int b, lds_f64* 
         }
         if (q >= n) continue;
         const int gl = opaque(tid & (G - 1));
-        lds_f64* xp = X + p * b;
-        lds_f64* xq = X + q * b;
-        lds_f64* jp = J + p * n;
-        lds_f64* jq = J + q * n;
+        lds_f64* xp = X + p * ldx;
+        lds_f64* xq = X + q * ldx;
+        lds_f64* jp = J + p * ldj;
+        lds_f64* jq = J + q * ldj;
         double u[JEPL], w[JEPL], ju[JEPL], jw[JEPL];
         load_row<G>(xp, b, gl, u);
         load_row<G>(xq, b, gl, w);
@@ -807,6 +831,7 @@ __device__ __noinline__ bool jacobi_rows_reg(lds_f64* X, int n, int b, lds_f64* 
           g1 = __builtin_fma(u[e + 1], w[e + 1], g1);
         }
         const double a = jsum<G>(a0 + a1), c = jsum<G>(c0 + c1), g = jsum<G>(g0 + g1);
+        QTT_JT(0);
         // |g| > tol sqrt(a c), compared squared (a c stays finite for core entries)
         if (g != 0.0 && a > small && c > small && g * g > tol * tol * (a * c)) {
           const double zeta = (c - a) * rcp_nr(2.0 * g);
@@ -815,6 +840,7 @@ __device__ __noinline__ bool jacobi_rows_reg(lds_f64* X, int n, int b, lds_f64* 
           const double t0 = rcp_nr(fabs(zeta) + r);
           const double tt = zeta >= 0.0 ? t0 : -t0;
           const double cs = rsq_nr(__builtin_fma(tt, tt, 1.0)), sn = cs * tt;
+          QTT_JT(1);
           rotate4(u, w, cs, sn);
           rotate4(ju, jw, cs, sn);
           store_row<G>(xp, b, gl, u);
@@ -822,9 +848,12 @@ __device__ __noinline__ bool jacobi_rows_reg(lds_f64* X, int n, int b, lds_f64* 
           store_row<G>(jp, n, gl, ju);
           store_row<G>(jq, n, gl, jw);
           if (gl == 0) *flag = 1;
+          QTT_JT(2);
         }
       }
+      QTT_JT(5);
       __syncthreads();
+      QTT_JT(3);
     }
     const int rot = *flag;
     __syncthreads();
@@ -903,7 +932,8 @@ __device__ __forceinline__ void mode_product(const double* Rrows, const int* per
 // scratch.  Two separate instantiations (one per memory) let the compiler use
 // LDS instructions on the LDS path instead of generic flat addressing.
 template <int G, bool REG>
-__device__ __forceinline__ int run_move(double* ws, const int rows, const int b, const int tr, const int rpl, const int bx,
+__device__ __forceinline__ int run_move(double* ws, double* jpad, const int rows, const int b, const int tr, const int rpl,
+                                        const int bx,
                                         const int64_t Au, const int64_t nu, const int64_t Bu, double* Xc,
                                         const double* Uc, double* Ud, double* gY, double cutoff, long long maxdim,
                                         int* status, int* iflag, double* red, double* dbc) {
@@ -993,15 +1023,34 @@ __device__ __forceinline__ int run_move(double* ws, const int rows, const int b,
     QTT_PHASE(-1, 2);  // factored (tall) / gathered (wide)
     // lanes per row pair sized so a row is at most 4 entries per lane
     bool conv;
-    // (REG: the workspace is in LDS)
-    lds_f64* Xl = (lds_f64*)X;
-    lds_f64* Jl = (lds_f64*)Jb;
-    lds_i32* fl = (lds_i32*)iflag;
-    if (REG && G == 8 && b <= 32) conv = jacobi_rows_reg<8>(Xl, n, b, Jl, fl, red);
-    else if (REG && G == 8 && b <= 64) conv = jacobi_rows_reg<16>(Xl, n, b, Jl, fl, red);
-    else if (REG && G == 8 && b <= 128) conv = jacobi_rows_reg<32>(Xl, n, b, Jl, fl, red);
-    else if (REG && G == 8 && b <= 256) conv = jacobi_rows_reg<64>(Xl, n, b, Jl, fl, red);
-    else conv = jacobi_rows<G>(X, n, b, Jb, iflag, red);
+    if (REG && G == 8 && b <= 256) {
+      // (REG: the workspace is in LDS.)  With room for it, the rounds run on a
+      // copy whose row strides are odd numbers of doubles, so the rows that
+      // the groups of a wavefront read together fall in different banks.
+      lds_f64 *Xl = (lds_f64*)X, *Jl = (lds_f64*)Jb;
+      int ldx = b, ldj = n;
+      if (jpad) {
+        ldx = b | 1;
+        ldj = n | 1;
+        Xl = (lds_f64*)jpad;
+        Jl = Xl + n * ldx;
+        for (int e = tid; e < n * b; e += bs) Xl[(e / b) * ldx + e % b] = X[e];
+        __syncthreads();
+      }
+      lds_i32* fl = (lds_i32*)iflag;
+      if (b <= 32) conv = jacobi_rows_reg<8>(Xl, ldx, n, b, Jl, ldj, fl, red);
+      else if (b <= 64) conv = jacobi_rows_reg<16>(Xl, ldx, n, b, Jl, ldj, fl, red);
+      else if (b <= 128) conv = jacobi_rows_reg<32>(Xl, ldx, n, b, Jl, ldj, fl, red);
+      else conv = jacobi_rows_reg<64>(Xl, ldx, n, b, Jl, ldj, fl, red);
+      if (jpad) {
+        __syncthreads();
+        for (int e = tid; e < n * b; e += bs) X[e] = Xl[(e / b) * ldx + e % b];
+        for (int e = tid; e < n * n; e += bs) Jb[e] = Jl[(e / n) * ldj + e % n];
+        __syncthreads();
+      }
+    } else {
+      conv = jacobi_rows<G>(X, n, b, Jb, iflag, red);
+    }
     if (!conv) {
       if (tid == 0) status[1] = 1;
     }
@@ -1130,23 +1179,29 @@ __global__ void __launch_bounds__(512) truncate_kernel(const SweepArgs* __restri
       rpl = 0;
       need = (int64_t)rows * b + base;
     }
+    // padded copy for the Jacobi rounds of an SVD move, when it fits after the workspace
+    double* jpad = nullptr;
+    if (tr) {
+      const int nj = rows > b ? b : rows;
+      if (need + (int64_t)nj * ((b | 1) + (nj | 1)) + 16 <= a.ws_lds) jpad = wsl + need;
+    }
     double* Xc = a.data + a.off[x];
     const double* Uc = a.data + a.off[u];
     double* Ud = a.data + a.off[u];
     int m;
     if (need <= a.ws_lds) {
       if (small)
-        m = run_move<8, true>(wsl, rows, b, tr, rpl, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<8, true>(wsl, jpad, rows, b, tr, rpl, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
                         iflag, red, dbc);
       else
-        m = run_move<64, false>(wsl, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<64, false>(wsl, nullptr, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
                          iflag, red, dbc);
     } else {
       if (small)
-        m = run_move<8, false>(a.gscr, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<8, false>(a.gscr, nullptr, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
                         iflag, red, dbc);
       else
-        m = run_move<64, false>(a.gscr, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<64, false>(a.gscr, nullptr, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
                          iflag, red, dbc);
     }
     QTT_PHASE(st, 5);
