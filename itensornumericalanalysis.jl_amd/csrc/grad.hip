@@ -259,12 +259,16 @@ grad_chain_reg_kernel(const double* __restrict__ pack, const int64_t* __restrict
                       const int64_t* __restrict__ roff, int64_t rtot, const double* __restrict__ xs,
                       const double* __restrict__ ybar, int64_t npts, int64_t gsize, double* __restrict__ Rscr,
                       double* __restrict__ gpart, int* __restrict__ flag) {
-  constexpr int LS = GT + 1;  // odd row stride: the rows read together sit in different banks
+  // The tile's points are staged half at a time (HP = 128 points, two
+  // wavefronts), which halves the LDS footprint so that two workgroups share a
+  // compute unit.
+  constexpr int HP = GT / 2;
+  constexpr int LS = HP + 1;  // odd row stride: the rows read together sit in different banks
   constexpr int BL = RedBlock<CHI>::BL, BR = RedBlock<CHI>::BR, NRB = RedBlock<CHI>::NRB;
   extern __shared__ double lds[];
-  double* Ls = lds;                                    // [CHI][LS]  L_k per point
-  double* Rs = lds + CHI * LS;                         // [CHI][LS]  ybar R_k per point
-  uint64_t* bits = (uint64_t*)(lds + 2 * CHI * LS);    // [0, 4): digit ballots, [4, 8): active ballots
+  double* Ls = lds;                                    // [CHI][LS]  L_k per point of the half
+  double* Rs = lds + CHI * LS;                         // [CHI][LS]  ybar R_k per point of the half
+  uint64_t* bits = (uint64_t*)(lds + 2 * CHI * LS);    // [0, 2): digit ballots, [2, 4): active ballots
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int s_w = wv & 1;
   const int lb = lane / NRB, rb = lane % NRB;
@@ -274,9 +278,9 @@ grad_chain_reg_kernel(const double* __restrict__ pack, const int64_t* __restrict
   double* R = Rscr + (size_t)blockIdx.x * rtot * GT;
   double* G = gpart + (size_t)blockIdx.x * gsize;
   for (int64_t e = tid; e < gsize; e += GT) G[e] = 0.0;
-  if (tid < CHI) {  // column GT is a zero point, paired with the last point of an odd count
-    Ls[tid * LS + GT] = 0.0;
-    Rs[tid * LS + GT] = 0.0;
+  if (tid < CHI) {  // column HP is a zero point, paired with the last point of an odd count
+    Ls[tid * LS + HP] = 0.0;
+    Rs[tid * LS + HP] = 0.0;
   }
   const int64_t ntiles = (npts + GT - 1) / GT;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -312,70 +316,78 @@ grad_chain_reg_kernel(const double* __restrict__ pack, const int64_t* __restrict
       const SiteDesc sd = desc[k];
       const uint64_t dmask = site_digit_mask(sd, kw);
       const double* Rk = R + roff[k] * GT + tid;
+      double rv[CHI];
 #pragma unroll
-      for (int i = 0; i < CHI; ++i) {
-        Ls[i * LS + tid] = L[i];
-        Rs[i * LS + tid] = i < sd.chir ? yb * Rk[(int64_t)i * GT] : 0.0;
-      }
+      for (int i = 0; i < CHI; ++i) rv[i] = i < sd.chir ? yb * Rk[(int64_t)i * GT] : 0.0;
       const uint64_t db = __ballot(dmask != 0 && active);
       const uint64_t ab = __ballot(active);
-      if (lane == 0) {
-        bits[wv] = db;
-        bits[4 + wv] = ab;
-      }
-      __syncthreads();
       const int sk = sdim[k];
-      if (s_w < sk) {
-        double acc[BL][BR];
+      double acc[BL][BR];
+#pragma unroll
+      for (int i = 0; i < BL; ++i)
+#pragma unroll
+        for (int q = 0; q < BR; ++q) acc[i][q] = 0.0;
+      for (int h = 0; h < 2; ++h) {
+        if ((wv >> 1) == h) {  // this half's two wavefronts stage their points
+          const int col = tid & (HP - 1);
+#pragma unroll
+          for (int i = 0; i < CHI; ++i) {
+            Ls[i * LS + col] = L[i];
+            Rs[i * LS + col] = rv[i];
+          }
+          if (lane == 0) {
+            bits[wv & 1] = db;
+            bits[2 + (wv & 1)] = ab;
+          }
+        }
+        __syncthreads();
+        if (s_w < sk) {
+          for (int c = 0; c < HP / 64; ++c) {
+            uint64_t m = bits[2 + c];
+            if (sk == 2) m = s_w ? (m & bits[c]) : (m & ~bits[c]);
+            m = uniform64(m);
+            while (m) {
+              const int ja = c * 64 + __builtin_ctzll(m);
+              m &= m - 1;
+              int jb = HP;
+              if (m) {
+                jb = c * 64 + __builtin_ctzll(m);
+                m &= m - 1;
+              }
+              double la[BL], lq[BL], ra[BR], rq[BR];
+#pragma unroll
+              for (int i = 0; i < BL; ++i) {
+                la[i] = Ls[(lrow0 + i) * LS + ja];
+                lq[i] = Ls[(lrow0 + i) * LS + jb];
+              }
+#pragma unroll
+              for (int q = 0; q < BR; ++q) {
+                ra[q] = Rs[(rcol0 + q) * LS + ja];
+                rq[q] = Rs[(rcol0 + q) * LS + jb];
+              }
+#pragma unroll
+              for (int i = 0; i < BL; ++i)
+#pragma unroll
+                for (int q = 0; q < BR; ++q) {
+                  acc[i][q] = __builtin_fma(la[i], ra[q], acc[i][q]);
+                  acc[i][q] = __builtin_fma(lq[i], rq[q], acc[i][q]);
+                }
+            }
+          }
+        }
+        __syncthreads();
+      }
+      if (s_w < sk && lane_on) {
+        double* Gk = G + goff[k];
 #pragma unroll
         for (int i = 0; i < BL; ++i)
 #pragma unroll
-          for (int q = 0; q < BR; ++q) acc[i][q] = 0.0;
-        for (int c = 0; c < GT / 64; ++c) {
-          uint64_t m = bits[4 + c];
-          if (sk == 2) m = s_w ? (m & bits[c]) : (m & ~bits[c]);
-          m = uniform64(m);
-          while (m) {
-            const int ja = c * 64 + __builtin_ctzll(m);
-            m &= m - 1;
-            int jb = GT;
-            if (m) {
-              jb = c * 64 + __builtin_ctzll(m);
-              m &= m - 1;
-            }
-            double la[BL], lq[BL], ra[BR], rq[BR];
-#pragma unroll
-            for (int i = 0; i < BL; ++i) {
-              la[i] = Ls[(lrow0 + i) * LS + ja];
-              lq[i] = Ls[(lrow0 + i) * LS + jb];
-            }
-#pragma unroll
-            for (int q = 0; q < BR; ++q) {
-              ra[q] = Rs[(rcol0 + q) * LS + ja];
-              rq[q] = Rs[(rcol0 + q) * LS + jb];
-            }
-#pragma unroll
-            for (int i = 0; i < BL; ++i)
-#pragma unroll
-              for (int q = 0; q < BR; ++q) {
-                acc[i][q] = __builtin_fma(la[i], ra[q], acc[i][q]);
-                acc[i][q] = __builtin_fma(lq[i], rq[q], acc[i][q]);
-              }
+          for (int q = 0; q < BR; ++q) {
+            const int l = lrow0 + i, r = rcol0 + q;
+            if (l < sd.chil && r < sd.chir) Gk[((int64_t)l * sk + s_w) * sd.chir + r] += acc[i][q];
           }
-        }
-        if (lane_on) {
-          double* Gk = G + goff[k];
-#pragma unroll
-          for (int i = 0; i < BL; ++i)
-#pragma unroll
-            for (int q = 0; q < BR; ++q) {
-              const int l = lrow0 + i, r = rcol0 + q;
-              if (l < sd.chil && r < sd.chir) Gk[((int64_t)l * sk + s_w) * sd.chir + r] += acc[i][q];
-            }
-        }
       }
       if (k + 1 < nv) step_to_child<CHI>(pack + poff[k], sd, dmask, L);
-      __syncthreads();
     }
   }
 }
@@ -420,7 +432,7 @@ int launch_grad(qtt_ctx* ctx, const qtt_net* cnet, const double* xs, const doubl
   const int chi = maxb <= 8 ? 8 : maxb <= 16 ? 16 : 32;
   const bool reg = binary && maxb <= 32;
   const int tilep = reg ? GT : GB;
-  const size_t lds = reg ? ((size_t)2 * chi * (GT + 1) + 8) * sizeof(double)
+  const size_t lds = reg ? ((size_t)2 * chi * (GT / 2 + 1) + 4) * sizeof(double)
                          : (size_t)3 * maxb * GB * sizeof(double) + GB * sizeof(int);
   if (lds > 160 * 1024) return qtt_fail(ctx, QTT_EINVAL, "gradient: bond dimension too large");
   const int64_t ntiles = (npts + tilep - 1) / tilep;
