@@ -29,6 +29,21 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 HBM_PEAK_GBS = 8000.0
 
 
+def measured_traffic(npts):
+    """HBM bytes per evaluate launch at npts points, from the committed counter
+    summary (FETCH_SIZE + WRITE_SIZE, each scaled by the same counters on a copy
+    of known size); None when absent or taken at another size."""
+    path = os.path.join(ROOT, "profiles", "eval_laplacian_2d_hbm_traffic.json")
+    try:
+        with open(path) as fh:
+            rec = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    if int(rec.get("npts", -1)) != npts:
+        return None
+    return float(rec["bytes_per_launch"])
+
+
 def chain_eval_flops(bonds):
     """Useful flops per point of the chain reduction: 2 sum_k chi_k chi_{k+1}."""
     b = list(bonds) + [1]
@@ -43,6 +58,8 @@ def main():
     ap.add_argument("--npts", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-others", action="store_true", help="skip the 1D and gradient workloads")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="first run one device-to-device copy of MIB MiB (a reference for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); the engine has no "
@@ -60,7 +77,15 @@ def main():
     ctx = q.engine_context()
     ctx.set_stream(stream.cuda_stream)
 
-    w = workloads.config2(npts=args.npts)
+    if args.copy_calib:
+        a = torch.empty(args.copy_calib << 17, dtype=torch.float64, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+
+    w = workloads.laplacian_2d(npts=args.npts)
     # setup (not timed): lossless compression of f, then one operate to get the result
     fc = q.truncate(w.f, cutoff=1e-16, maxdim=w.extra["compress_maxdim"])
     g, terr = q.operate(w.op, fc, cutoff=w.cutoff, maxdim=w.maxdim, return_error=True)
@@ -113,6 +138,9 @@ def main():
     kms = k32 if use_split else k64
     achieved = flops_pt * args.npts / (k64 * 1e-3) / 1e12  # float64 kernel
     bytes_pt = 2 * 8 + 8
+    chi = next(c for c in (4, 8, 16, 32, 1 << 30) if max(bonds) <= c)
+    kname = f"eval_chain_reg_kernel<{chi}>" if chi <= 32 else "eval_chain_f64_kernel"
+    traffic = measured_traffic(args.npts)
 
     # parity of the timed result against the CPU oracle on a sample of points
     from oracle import pyoracle as po
@@ -144,7 +172,7 @@ def main():
                          f"operate+truncate (1 thread) {cpu_op_ms:.1f} ms",
                "operate_truncate_ms": cpu_op_ms}
 
-    others = {} if args.no_others else other_configs(q, po, ctx, stream, args)
+    others = {} if args.no_others else other_workloads(q, po, ctx, stream, args)
 
     rec = {
         "metric": "points/s + operate+truncate ms; mixed vs FP64 GPU at <=1e-10 rel err",
@@ -159,7 +187,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32-split" if use_split else "f64",
         "data": "synthetic (seeded numpy PCG64; no reference data)",
-        "config": {"workload": "config2: 2D Laplacian, 2x20 interleaved digits on a 40-site chain, "
+        "config": {"workload": "2D Laplacian: 2x20 interleaved digits on a 40-site chain, "
                                "operate+truncate cutoff 1e-10 maxdim 128, evaluate 1e6 points",
                    "npts": args.npts, "result_bonds_max": max(bonds), "parallelism": "single GPU"},
         "operate_truncate_ms": op_ms,
@@ -170,11 +198,11 @@ def main():
         "trunc_err": terr,
         "flops_per_point": flops_pt,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "eval_chain_f64_kernel", "kernel_ms": k64,
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "kernel": kname, "kernel_ms": k64,
                      "hbm_gbs": bytes_pt * args.npts / (k64 * 1e-3) / 1e9},
         "cpu_baseline": cpu,
-        "other_configs": others,
+        "other_workloads": others,
     }
     print(json.dumps(rec))
 
@@ -189,13 +217,13 @@ def _time_gpu(fn, reps, torch, stream):
     return (time.perf_counter() - t0) / reps
 
 
-def other_configs(q, po, ctx, stream, args):
+def other_workloads(q, po, ctx, stream, args):
     """The 1D chain workload and the gradient workload, timed in the same run."""
     import torch
     from itensornumericalanalysis_jl_amd import workloads
     out = {}
     # 1D: 12 digits, D2 Dirichlet, cutoff 1e-12, 1e5 points, integral
-    w = workloads.config1()
+    w = workloads.sine_1d()
     t = w.f.topo
     g = q.operate(w.op, w.f, cutoff=w.cutoff)
     xd = torch.tensor(w.xs, device="cuda")
@@ -208,12 +236,12 @@ def other_configs(q, po, ctx, stream, args):
     iref = po.integrate(t.parent, t.sdim, gref, 0.5)
     yr, _ = po.evaluate(t.parent, t.sdim, gref, t.vcoord, t.vdigit, t.Lc, w.xs[:20000])
     yy = y.cpu().numpy()[:20000]
-    out["config1"] = {"points_per_s": len(w.xs) / ev, "operate_truncate_ms": op * 1e3,
+    out["sine_1d"] = {"points_per_s": len(w.xs) / ev, "operate_truncate_ms": op * 1e3,
                       "integral": integ, "integral_rel_err_vs_oracle": abs(integ - iref) / abs(iref),
                       "eval_rel_err_vs_oracle": float(np.linalg.norm(yy - yr) / np.linalg.norm(yr)),
                       "result_bonds": [int(b) for b in g.bonds()]}
     # gradient: 2 x 14 digits, bond 32, 2^20 points, MSE loss
-    w4 = workloads.config4()
+    w4 = workloads.fit_2d()
     f4 = w4.f
     xs = torch.tensor(w4.xs, device="cuda")
     yv = torch.empty(len(w4.xs), dtype=torch.float64, device="cuda")
@@ -239,7 +267,7 @@ def other_configs(q, po, ctx, stream, args):
                        ybar[:ns].cpu().numpy())
     num = sum(float(np.sum((a - b) ** 2)) for a, b in zip(sub, ref)) ** 0.5
     den = sum(float(np.sum(b ** 2)) for b in ref) ** 0.5
-    out["config4"] = {"gradient_ms": gt * 1e3, "points": len(w4.xs),
+    out["fit_2d"] = {"gradient_ms": gt * 1e3, "points": len(w4.xs),
                       "forward_flops_per_point": flops_fwd,
                       "gradient_tflops_3x_forward": 3 * flops_fwd * len(w4.xs) / gt / 1e12,
                       "grad_rel_err_vs_oracle_2^14_subset": num / den}

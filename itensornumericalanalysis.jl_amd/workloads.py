@@ -1,18 +1,18 @@
 """Synthetic workloads (seeded, numpy PCG64) for the five benchmark shapes.
 
-  config1  1D, 12 digits on a chain; f = sin(2 pi x) + 0.5 cos(6 pi x) + exp(-x)
+  sine_1d         1D, 12 digits on a chain; f = sin(2 pi x) + 0.5 cos(6 pi x) + exp(-x)
            (bond 5); Dirichlet second derivative, cutoff 1e-12; 1e5 points, seed 0.
-  config2  2D, 20 digits per dimension interleaved on a 40-site chain;
+  laplacian_2d    2D, 20 digits per dimension interleaved on a 40-site chain;
            f = sum_k a_k sin(w_k x) exp(-b_k y), k = 1..8, a ~ N(0,1),
            w ~ U(1,40), b ~ U(0,3) (seed 1), bond 16; Laplacian (Dirichlet),
            cutoff 1e-10, maxdim 128; 1e6 points drawn after the coefficients.
-  config3  3D comb tree (3 teeth of 16 digits, tooth i = dimension i, most
+  comb_3d         3D comb tree (3 teeth of 16 digits, tooth i = dimension i, most
            significant digit on the backbone), dense N(0,1) cores normalised,
            bond 128, seed 2; first derivative in x; recompression to maxdim 128.
-  config4  2D, 14 digits per dimension interleaved on a 28-site chain, cores
+  fit_2d          2D, 14 digits per dimension interleaved on a 28-site chain, cores
            N(0, 0.1^2), bond 32, seed 3; target
            g = exp(-((x-.3)^2 + (y-.6)^2)/.02) + .2 sin(12 pi x y); 2^20 points.
-  config5  512 chains, digits ~ U{8..24}, bonds ~ U{4..96}, N(0,1) cores, seed 4;
+  fredholm_batch  512 chains, digits ~ U{8..24}, bonds ~ U{4..96}, N(0,1) cores, seed 4;
            kernel operator exp(-|x-y|); cutoff 1e-10; 4096 points each.
 Julia's random streams cannot be reproduced here, so the draws are this
 module's own (documented order: coefficients, then points).
@@ -43,19 +43,19 @@ class Workload:
     extra: dict = field(default_factory=dict)
 
 
-def config1(npts: int = 100_000, seed: int = 0) -> Workload:
+def sine_1d(npts: int = 100_000, seed: int = 0) -> Workload:
     s = continuous_siteinds(named_grid((12, 1)))
     f = sin_itn(s, k=2 * math.pi) + cos_itn(s, k=6 * math.pi, c=0.5) + exp_itn(s, k=-1.0)
     op = second_derivative_operator(s)
     xs = np.random.default_rng(seed).uniform(0.0, 1.0, (npts, 1))
-    return Workload("config1: 1D L=12 chain, D2 Dirichlet, cutoff 1e-12", s, f, op, xs, 1e-12, 0)
+    return Workload("sine_1d: 1D L=12 chain, D2 Dirichlet, cutoff 1e-12", s, f, op, xs, 1e-12, 0)
 
 
-def config1_function(x):
+def sine_1d_function(x):
     return np.sin(2 * np.pi * x) + 0.5 * np.cos(6 * np.pi * x) + np.exp(-x)
 
 
-def config2(npts: int = 1_000_000, seed: int = 1) -> Workload:
+def laplacian_2d(npts: int = 1_000_000, seed: int = 1) -> Workload:
     s = continuous_siteinds(named_grid((40, 1)), map_dimension=2)
     rng = np.random.default_rng(seed)
     a = rng.standard_normal(8)
@@ -67,36 +67,36 @@ def config2(npts: int = 1_000_000, seed: int = 1) -> Workload:
         f = term if f is None else f + term
     xs = rng.uniform(0.0, 1.0, (npts, 2))
     op = laplacian_operator(s)
-    return Workload("config2: 2D 2x20 bits interleaved chain, Laplacian, cutoff 1e-10 maxdim 128",
+    return Workload("laplacian_2d: 2D 2x20 bits interleaved chain, Laplacian, cutoff 1e-10 maxdim 128",
                     s, f, op, xs, 1e-10, 128, {"a": a, "w": w, "b": b, "compress_maxdim": 64})
 
 
-def config2_function(xs, a, w, b):
+def laplacian_2d_function(xs, a, w, b):
     x, y = xs[:, 0], xs[:, 1]
     return sum(a[k] * np.sin(w[k] * x) * np.exp(-b[k] * y) for k in range(len(a)))
 
 
-def config3(seed: int = 2, bond: int = 128, teeth: int = 3, digits: int = 16) -> Workload:
+def comb_3d(seed: int = 2, bond: int = 128, teeth: int = 3, digits: int = 16) -> Workload:
     g = named_comb_tree((teeth, digits))
     s = continuous_siteinds(g, [[(i, j) for j in range(1, digits + 1)] for i in range(1, teeth + 1)])
     f = random_itn(s, link_space=bond, rng=np.random.default_rng(seed), normalize_cores=True)
     op = first_derivative_operator(s, dim=1)
-    return Workload(f"config3: comb {teeth}x{digits}, bond {bond}, d/dx, maxdim {bond}", s, f, op, None,
+    return Workload(f"comb_3d: comb {teeth}x{digits}, bond {bond}, d/dx, maxdim {bond}", s, f, op, None,
                     0.0, bond)
 
 
-def config4(npts: int = 1 << 20, seed: int = 3, bond: int = 32) -> Workload:
+def fit_2d(npts: int = 1 << 20, seed: int = 3, bond: int = 32) -> Workload:
     s = continuous_siteinds(named_grid((28, 1)), map_dimension=2)
     rng = np.random.default_rng(seed)
     f = random_itn(s, link_space=bond, rng=rng, std=0.1)
     xs = rng.uniform(0.0, 1.0, (npts, 2))
     x, y = xs[:, 0], xs[:, 1]
     target = np.exp(-((x - 0.3) ** 2 + (y - 0.6) ** 2) / 0.02) + 0.2 * np.sin(12 * np.pi * x * y)
-    return Workload("config4: 2D 2x14 bits chain, bond 32, MSE gradient", s, f, None, xs, 0.0, 0,
+    return Workload("fit_2d: 2D 2x14 bits chain, bond 32, MSE gradient", s, f, None, xs, 0.0, 0,
                     {"target": target})
 
 
-def config5(nfun: int = 512, npts: int = 4096, seed: int = 4):
+def fredholm_batch(nfun: int = 512, npts: int = 4096, seed: int = 4):
     """List of (Workload) for the batch of independent chains."""
     rng = np.random.default_rng(seed)
     Ls = rng.integers(8, 25, nfun)
@@ -106,5 +106,5 @@ def config5(nfun: int = 512, npts: int = 4096, seed: int = 4):
         s = continuous_siteinds(named_grid((int(Ls[i]), 1)))
         f = random_itn(s, link_space=int(chis[i]), rng=rng)
         xs = rng.uniform(0.0, 1.0, (npts, 1))
-        out.append(Workload(f"config5[{i}]", s, f, fredholm_operator(s), xs, 1e-10, 0))
+        out.append(Workload(f"fredholm_batch[{i}]", s, f, fredholm_operator(s), xs, 1e-10, 0))
     return out

@@ -110,7 +110,7 @@ def test_operator_bonds_minimal(q):
 
 def test_workloads_shapes(q):
     from itensornumericalanalysis_jl_amd import workloads
-    c1 = workloads.config1(npts=100)
+    c1 = workloads.sine_1d(npts=100)
     assert c1.f.topo.nv == 12 and c1.f.maxlinkdim() == 5 and c1.xs.shape == (100, 1)
-    c2 = workloads.config2(npts=10)
+    c2 = workloads.laplacian_2d(npts=10)
     assert c2.f.topo.nv == 40 and c2.f.maxlinkdim() == 16 and c2.xs.shape == (10, 2)

@@ -33,26 +33,26 @@ def _oracle_apply_truncate(po, f, op, cutoff, maxdim):
 
 
 # ------------------------------------------------------------------ evaluate
-def test_eval_config1(q, po, eng):
+def test_eval_sine_1d(q, po, eng):
     from itensornumericalanalysis_jl_amd import workloads
-    w = workloads.config1(npts=20000)
+    w = workloads.sine_1d(npts=20000)
     y = q.evaluate(w.f, w.xs)
     ref = oracle_eval(po, w.f, w.xs)
     assert rel_l2(y, ref) < TOL
     assert np.max(np.abs(y - ref)) < TOL * np.max(np.abs(ref))
     # and the closed form on the grid
     xg = grid_of(w.xs[:, 0], 12)
-    assert np.allclose(y, workloads.config1_function(xg), rtol=1e-12, atol=1e-12)
+    assert np.allclose(y, workloads.sine_1d_function(xg), rtol=1e-12, atol=1e-12)
 
 
-def test_eval_config2(q, po, eng):
+def test_eval_laplacian_2d(q, po, eng):
     from itensornumericalanalysis_jl_amd import workloads
-    w = workloads.config2(npts=20000)
+    w = workloads.laplacian_2d(npts=20000)
     y = q.evaluate(w.f, w.xs)
     ref = oracle_eval(po, w.f, w.xs)
     assert rel_l2(y, ref) < TOL
     xg = grid_of(w.xs, 20)
-    closed = workloads.config2_function(xg, w.extra["a"], w.extra["w"], w.extra["b"])
+    closed = workloads.laplacian_2d_function(xg, w.extra["a"], w.extra["w"], w.extra["b"])
     assert rel_l2(y, closed) < 1e-12
 
 
@@ -110,7 +110,7 @@ def test_eval_device_tensors_deterministic(q, po, eng, rng):
 @pytest.mark.parametrize("prec,tol", [(1, 1e-10), (2, 1e-5)])
 def test_eval_float32_paths(q, po, eng, prec, tol):
     from itensornumericalanalysis_jl_amd import workloads
-    w = workloads.config2(npts=20000)
+    w = workloads.laplacian_2d(npts=20000)
     y = q.evaluate(w.f, w.xs, precision=prec)
     ref = oracle_eval(po, w.f, w.xs)
     err = rel_l2(y, ref)
@@ -121,7 +121,7 @@ def test_eval_float32_paths(q, po, eng, prec, tol):
 # ------------------------------------------------------------------ integrate / inner
 def test_integrate_chain_and_tree(q, po, eng, rng):
     from itensornumericalanalysis_jl_amd import workloads
-    w = workloads.config1(npts=10)
+    w = workloads.sine_1d(npts=10)
     t = w.f.topo
     for take_sum, wt in ((False, 0.5), (True, 1.0)):
         got = q.integrate(w.f, take_sum=take_sum)
@@ -179,9 +179,9 @@ def test_apply_exact_matches_oracle(q, po, eng, rng):
     assert list(g.bonds()) == [c.shape[0] for c in C]
 
 
-def test_operate_truncate_config1(q, po, eng, rng):
+def test_operate_truncate_sine_1d(q, po, eng, rng):
     from itensornumericalanalysis_jl_amd import workloads
-    w = workloads.config1(npts=10)
+    w = workloads.sine_1d(npts=10)
     g, err = q.operate(w.op, w.f, cutoff=w.cutoff, return_error=True)
     ref, rerr = _oracle_apply_truncate(po, w.f, w.op, w.cutoff, 0)
     assert list(g.bonds()) == [c.shape[0] for c in ref]
@@ -190,15 +190,15 @@ def test_operate_truncate_config1(q, po, eng, rng):
     # telescoping identity of the Dirichlet second derivative: exact for the
     # untruncated product, to the truncation error (cutoff 1e-12) after it
     h = 2.0 ** -12
-    fx = workloads.config1_function
+    fx = workloads.sine_1d_function
     ident = -(fx(1 - h) + fx(0.0)) / h
     assert q.integrate(q.operate(w.op, w.f, truncate=False)) == pytest.approx(ident, rel=1e-11)
     assert q.integrate(g) == pytest.approx(ident, rel=1e-6)
 
 
-def test_operate_truncate_config2(q, po, eng, rng):
+def test_operate_truncate_laplacian_2d(q, po, eng, rng):
     from itensornumericalanalysis_jl_amd import workloads
-    w = workloads.config2(npts=10)
+    w = workloads.laplacian_2d(npts=10)
     fc = q.truncate(w.f, cutoff=1e-16, maxdim=64)
     t = w.f.topo
     fref, _ = po.truncate(t.parent, t.sdim, w.f.cores, 1e-16, 64)
@@ -226,7 +226,7 @@ def test_truncate_random(q, po, eng, rng, graph):
 # ------------------------------------------------------------------ comb tree (config-3 shape, reduced)
 def test_comb_operate_and_integrate(q, po, eng, rng):
     from itensornumericalanalysis_jl_amd import workloads
-    w = workloads.config3(bond=8, digits=6)
+    w = workloads.comb_3d(bond=8, digits=6)
     f = w.f
     t = f.topo
     assert q.integrate(f) == pytest.approx(po.integrate(t.parent, t.sdim, f.cores, 0.5), rel=TOL)
@@ -241,7 +241,7 @@ def test_comb_operate_and_integrate(q, po, eng, rng):
 # ------------------------------------------------------------------ batches (config-5 shape, reduced)
 def test_operate_batch_and_integrate_batch(q, po, eng, rng):
     from itensornumericalanalysis_jl_amd import workloads
-    ws = workloads.config5(nfun=12, npts=64, seed=5)
+    ws = workloads.fredholm_batch(nfun=12, npts=64, seed=5)
     for w in ws:  # keep the oracle fast: cap the bond
         if w.f.maxlinkdim() > 24:
             w.f = q.random_itn(w.s, link_space=24, rng=rng)
