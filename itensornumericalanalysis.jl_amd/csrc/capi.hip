@@ -2,6 +2,7 @@
 // entry points declared in include/qtt.h.  Host-side C++; the arithmetic runs
 // in the kernels of eval.hip, sweep.hip, factor.hip and grad.hip.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <new>
@@ -244,6 +245,8 @@ int qtt_ctx_create(int device, qtt_ctx** out) {
   c->stream = c->own_stream;
   const char* ek = getenv("QTT_EVAL_KERNEL");
   c->force_lds_eval = ek && std::strcmp(ek, "lds") == 0;
+  const char* ppl = std::getenv("QTT_EVAL_PPL");  // 1 or 2 (default: chosen by bond size)
+  c->eval_ppl = ppl ? std::atoi(ppl) : 0;
   *out = c;
   return QTT_OK;
 }

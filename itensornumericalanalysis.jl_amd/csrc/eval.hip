@@ -186,6 +186,94 @@ eval_chain_reg_kernel(const double* __restrict__ pack, const int64_t* __restrict
   if (bad) flag[0] = 1;
 }
 
+// Two points per lane (p and p + blockDim.x of the block's span): every core
+// entry loaded through the scalar path feeds four FMAs instead of two, which
+// halves the scalar loads, the address arithmetic and the loop overhead per
+// point.  Same arithmetic per point as eval_chain_reg_kernel.
+template <int CHI>
+__global__ void __launch_bounds__(256)
+eval_chain_reg2_kernel(const double* __restrict__ pack, const int64_t* __restrict__ poff,
+                       const SiteDesc* __restrict__ desc, int nv, int nc, const int* __restrict__ ndigits,
+                       const double* __restrict__ xs, int64_t npts, double* __restrict__ out,
+                       int* __restrict__ flag) {
+  const int64_t p0 = (int64_t)blockIdx.x * (2 * blockDim.x) + threadIdx.x;
+  const int64_t p1 = p0 + blockDim.x;
+  const bool act0 = p0 < npts, act1 = p1 < npts;
+  bool bad0 = false, bad1 = false;
+  uint64_t kw0[QTT_MAX_COORD] = {0, 0, 0, 0}, kw1[QTT_MAX_COORD] = {0, 0, 0, 0};
+#pragma unroll
+  for (int c = 0; c < QTT_MAX_COORD; ++c) {
+    if (c < nc) {
+      kw0[c] = digit_word(act0 ? xs[p0 * nc + c] : 0.0, ndigits[c], bad0);
+      kw1[c] = digit_word(act1 ? xs[p1 * nc + c] : 0.0, ndigits[c], bad1);
+    }
+  }
+  double v0[CHI], v1[CHI], w0[CHI], w1[CHI];
+#pragma unroll
+  for (int i = 0; i < CHI; ++i) {
+    v0[i] = 0.0;
+    v1[i] = 0.0;
+  }
+  v0[0] = 1.0;
+  v1[0] = 1.0;
+  for (int k = nv - 1; k >= 0; --k) {
+    const SiteDesc sd = desc[k];
+    uint64_t m0 = 0, m1 = 0;
+    if (sd.coord >= 0) {
+      const int c = sd.coord;
+      const uint64_t k0 = c == 0 ? kw0[0] : c == 1 ? kw0[1] : c == 2 ? kw0[2] : kw0[3];
+      const uint64_t k1 = c == 0 ? kw1[0] : c == 1 ? kw1[1] : c == 2 ? kw1[2] : kw1[3];
+      m0 = 0 - ((k0 >> sd.shift) & 1u);
+      m1 = 0 - ((k1 >> sd.shift) & 1u);
+    }
+    const double* __restrict__ P = pack + poff[k];
+    const int lpad = (sd.chil + 3) & ~3;
+#pragma unroll
+    for (int l0 = 0; l0 < CHI; l0 += 4) {
+      if (l0 < sd.chil) {
+        double a0[4] = {0.0, 0.0, 0.0, 0.0}, a1[4] = {0.0, 0.0, 0.0, 0.0};
+        double b0[4] = {0.0, 0.0, 0.0, 0.0}, b1[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int r0 = 0; r0 < CHI; r0 += 4) {
+          if (r0 < sd.chir) {
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) {
+              const double* __restrict__ q = P + ((r0 + rr) * lpad + l0) * 2;
+              const double x0 = v0[r0 + rr], x1 = v1[r0 + rr];
+#pragma unroll
+              for (int i = 0; i < 4; ++i) {
+                a0[i] = __builtin_fma(q[2 * i], x0, a0[i]);
+                a1[i] = __builtin_fma(q[2 * i + 1], x0, a1[i]);
+                b0[i] = __builtin_fma(q[2 * i], x1, b0[i]);
+                b1[i] = __builtin_fma(q[2 * i + 1], x1, b1[i]);
+              }
+            }
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          w0[l0 + i] = bitsel(a0[i], a1[i], m0);
+          w1[l0 + i] = bitsel(b0[i], b1[i], m1);
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          w0[l0 + i] = 0.0;
+          w1[l0 + i] = 0.0;
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < CHI; ++i) {
+      v0[i] = w0[i];
+      v1[i] = w1[i];
+    }
+  }
+  if (act0) out[p0] = bad0 ? __builtin_nan("") : v0[0];
+  if (act1) out[p1] = bad1 ? __builtin_nan("") : v1[0];
+  if (bad0 || bad1) flag[0] = 1;
+}
+
 // float32 variants of the chain kernel.  SPLIT keeps each operand as a
 // (hi, lo) float pair and accumulates in float-float (Dekker/Knuth error-free
 // transformations), PLAIN is ordinary float32.
@@ -446,9 +534,23 @@ int launch_eval(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t npts, doub
     if (rc) return rc;
     if (precision == QTT_F64 && maxb <= 32 && !ctx->force_lds_eval) {
       const int bsr = 256;
+      const SiteDesc* dd = (const SiteDesc*)net->d_desc;
+      // two points per lane for bonds <= 16 (register room), one above
+      const int ppl = ctx->eval_ppl ? ctx->eval_ppl : (maxb <= 16 ? 2 : 1);
+      if (ppl == 2 && maxb <= 16) {
+        const int64_t nb2 = (npts + 2 * bsr - 1) / (2 * bsr);
+        if (nb2 > INT32_MAX) return qtt_fail(ctx, QTT_EINVAL, "evaluate: too many points");
+#define QTT_REG2_LAUNCH(C)                                                                         \
+  eval_chain_reg2_kernel<C><<<dim3((unsigned)nb2), bsr, 0, ctx->stream>>>(                         \
+      net->d_pack, net->d_pack_off, dd, t.nv, t.ncoord, tv.ndigits, xs, npts, out, ctx->d_flag)
+        if (maxb <= 4) QTT_REG2_LAUNCH(4);
+        else if (maxb <= 8) QTT_REG2_LAUNCH(8);
+        else QTT_REG2_LAUNCH(16);
+#undef QTT_REG2_LAUNCH
+        return qtt_hip_check(ctx, hipGetLastError(), "eval_chain_reg2_kernel");
+      }
       const int64_t nbr = (npts + bsr - 1) / bsr;
       if (nbr > INT32_MAX) return qtt_fail(ctx, QTT_EINVAL, "evaluate: too many points");
-      const SiteDesc* dd = (const SiteDesc*)net->d_desc;
 #define QTT_REG_LAUNCH(C)                                                                          \
   eval_chain_reg_kernel<C><<<dim3((unsigned)nbr), bsr, 0, ctx->stream>>>(                          \
       net->d_pack, net->d_pack_off, dd, t.nv, t.ncoord, tv.ndigits, xs, npts, out, ctx->d_flag)

@@ -22,6 +22,7 @@ struct qtt_ctx {
   size_t scratch_bytes = 0;
   int cus = 256;
   bool force_lds_eval = false;  // QTT_EVAL_KERNEL=lds selects the LDS-vector chain kernel
+  int eval_ppl = 0;             // points per lane of the register chain kernel (0: by bond size)
 };
 
 // Topology view passed by value to kernels (device pointers).
