@@ -535,8 +535,9 @@ int launch_eval(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t npts, doub
     if (precision == QTT_F64 && maxb <= 32 && !ctx->force_lds_eval) {
       const int bsr = 256;
       const SiteDesc* dd = (const SiteDesc*)net->d_desc;
-      // two points per lane for bonds <= 16 (register room), one above
-      const int ppl = ctx->eval_ppl ? ctx->eval_ppl : (maxb <= 16 ? 2 : 1);
+      // two points per lane for bonds <= 16 (register room) when there are
+      // enough points to fill the GPU at half the workgroup count, one otherwise
+      const int ppl = ctx->eval_ppl ? ctx->eval_ppl : (maxb <= 16 && npts >= (int64_t(1) << 19) ? 2 : 1);
       if (ppl == 2 && maxb <= 16) {
         const int64_t nb2 = (npts + 2 * bsr - 1) / (2 * bsr);
         if (nb2 > INT32_MAX) return qtt_fail(ctx, QTT_EINVAL, "evaluate: too many points");
