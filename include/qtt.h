@@ -151,6 +151,13 @@ int qtt_apply_batch(qtt_ctx* ctx, int n, const qtt_mpo* const* ops, const qtt_ne
                     double cutoff, int64_t maxdim, int precision, qtt_net** outs, double* errs);
 /* n integrals in one launch (one workgroup per network); out[i] on the host. */
 int qtt_integrate_batch(qtt_ctx* ctx, int n, const qtt_net* const* nets, double weight, double* out);
+/* evaluate(nets[i], xs[i]) for a batch of functions, each at its own points:
+ * xs[i] device [npts[i]][ncoord_i] float64, out[i] device [npts[i]].  Float64
+ * chains with bonds <= 32 run in one launch per bond class; other networks are
+ * evaluated one launch each.  Returns once the launches are queued (their
+ * argument tables copied); the values are ready after qtt_sync. */
+int qtt_eval_batch(qtt_ctx* ctx, int n, const qtt_net* const* nets, const double* const* xs,
+                   const int64_t* npts, double* const* out, int precision);
 
 #ifdef __cplusplus
 }

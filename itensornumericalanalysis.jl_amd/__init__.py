@@ -16,8 +16,8 @@ from .operators import (add_operators, backward_shift_op, first_derivative_opera
                         forward_shift_op, fourth_derivative_operator, fredholm_operator,
                         identity_operator, laplacian_operator, second_derivative_operator, stencil,
                         third_derivative_operator)
-from .api import (Precision, engine_context, eval_grad, evaluate, inner, integrate, integrate_batch,
-                  on_stream, operate, operate_batch, truncate)
+from .api import (Precision, engine_context, eval_grad, evaluate, evaluate_batch, inner, integrate,
+                  integrate_batch, on_stream, operate, operate_batch, truncate)
 
 real_continuous_siteinds = continuous_siteinds
 
@@ -30,5 +30,5 @@ __all__ = [
     "second_derivative_operator", "third_derivative_operator", "fourth_derivative_operator",
     "laplacian_operator", "identity_operator", "fredholm_operator", "add_operators", "evaluate",
     "integrate", "inner", "operate", "truncate", "eval_grad", "Precision", "engine_context",
-    "operate_batch", "integrate_batch",
+    "operate_batch", "integrate_batch", "evaluate_batch",
 ]

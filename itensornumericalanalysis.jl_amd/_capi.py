@@ -20,7 +20,7 @@ EXPORTS = [
     "qtt_sync", "qtt_net_create", "qtt_net_bonds", "qtt_net_read", "qtt_net_copy",
     "qtt_net_destroy", "qtt_mpo_create", "qtt_mpo_destroy", "qtt_eval", "qtt_eval_host",
     "qtt_integrate", "qtt_inner", "qtt_apply", "qtt_truncate", "qtt_eval_grad",
-    "qtt_eval_grad_host", "qtt_net_size", "qtt_apply_batch", "qtt_integrate_batch",
+    "qtt_eval_grad_host", "qtt_net_size", "qtt_apply_batch", "qtt_integrate_batch", "qtt_eval_batch",
 ]
 
 
@@ -79,6 +79,7 @@ def lib():
         L.qtt_apply_batch.argtypes = [vp, C.c_int, C.POINTER(vp), C.POINTER(vp), C.c_double, C.c_int64, C.c_int,
                                       C.POINTER(vp), dp]
         L.qtt_integrate_batch.argtypes = [vp, C.c_int, C.POINTER(vp), C.c_double, dp]
+        L.qtt_eval_batch.argtypes = [vp, C.c_int, C.POINTER(vp), C.POINTER(vp), ip, C.POINTER(vp), C.c_int]
         _lib = L
     return _lib
 

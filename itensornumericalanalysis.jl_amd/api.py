@@ -209,6 +209,31 @@ def operate_batch(ops, fs, cutoff=None, maxdim=None, precision: int = Precision.
     return res, errs
 
 
+def evaluate_batch(fs, xs_list, precision: int = Precision.F64, outs=None):
+    """evaluate(fs[i], xs_list[i]) for a batch of functions; each xs_list[i] is a
+    contiguous float64 CUDA tensor [npts_i, ndims_i].  Returns the list of CUDA
+    result tensors (ready once the engine's stream is synchronized)."""
+    import torch
+    n = len(fs)
+    if len(xs_list) != n:
+        raise ValueError("one point array per function")
+    dns = [device_network(f) for f in fs]
+    for f, x in zip(fs, xs_list):
+        if not (isinstance(x, torch.Tensor) and x.is_cuda and x.dtype == torch.float64 and x.dim() == 2
+                and x.is_contiguous() and x.shape[1] == f.dimension()):
+            raise ValueError("each point array must be a contiguous float64 CUDA tensor [npts, ndims]")
+    if outs is None:
+        outs = [torch.empty(x.shape[0], dtype=torch.float64, device=x.device) for x in xs_list]
+    ctx = engine_context()
+    arr_f = (C.c_void_p * n)(*[d.handle.value for d in dns])
+    arr_x = (C.c_void_p * n)(*[x.data_ptr() for x in xs_list])
+    arr_o = (C.c_void_p * n)(*[o.data_ptr() for o in outs])
+    npts = np.array([x.shape[0] for x in xs_list], dtype=np.int64)
+    ctx.check(_capi.lib().qtt_eval_batch(ctx.handle, n, arr_f, arr_x, npts.ctypes.data_as(C.POINTER(C.c_int64)),
+                                         arr_o, int(precision)))
+    return outs
+
+
 def integrate_batch(fs, take_sum: bool = False):
     """Integrals of a batch of functions in one launch."""
     n = len(fs)

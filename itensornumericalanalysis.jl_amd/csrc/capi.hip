@@ -586,6 +586,18 @@ int qtt_integrate_batch(qtt_ctx* ctx, int n, const qtt_net* const* nets, double 
   return launch_integrate_batch(ctx, n, nets, weight, out);
 }
 
+int qtt_eval_batch(qtt_ctx* ctx, int n, const qtt_net* const* nets, const double* const* xs,
+                   const int64_t* npts, double* const* out, int precision) {
+  if (!ctx || n < 0 || (n && (!nets || !xs || !npts || !out))) return qtt_fail(ctx, QTT_EINVAL, "null argument");
+  int rc = check_precision(ctx, precision);
+  if (rc) return rc;
+  for (int i = 0; i < n; i++) {
+    if (!nets[i] || npts[i] < 0 || (npts[i] && (!xs[i] || !out[i])))
+      return qtt_fail(ctx, QTT_EINVAL, "bad batch entry");
+  }
+  return launch_eval_batch(ctx, n, const_cast<qtt_net* const*>(nets), xs, npts, out, precision);
+}
+
 int qtt_truncate(qtt_ctx* ctx, qtt_net* net, double cutoff, int64_t maxdim, double* trunc_err) {
   if (!ctx || !net) return qtt_fail(ctx, QTT_EINVAL, "null argument");
   if (maxdim < 0) return qtt_fail(ctx, QTT_EINVAL, "maxdim < 0");

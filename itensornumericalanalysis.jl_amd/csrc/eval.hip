@@ -126,12 +126,11 @@ eval_chain_f64_kernel(const double* __restrict__ pack, const int64_t* __restrict
 // back for each group of 4 rows before their 32 FMAs.
 
 template <int CHI>
-__global__ void __launch_bounds__(256)
-eval_chain_reg_kernel(const double* __restrict__ pack, const int64_t* __restrict__ poff,
-                      const SiteDesc* __restrict__ desc, int nv, int nc, const int* __restrict__ ndigits,
-                      const double* __restrict__ xs, int64_t npts, double* __restrict__ out,
-                      int* __restrict__ flag) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void eval_chain_reg_body(const double* __restrict__ pack, const int64_t* __restrict__ poff,
+                                                    const SiteDesc* __restrict__ desc, int nv, int nc,
+                                                    const int* __restrict__ ndigits, const double* __restrict__ xs,
+                                                    int64_t npts, double* __restrict__ out, int* __restrict__ flag,
+                                                    int64_t p) {
   const bool active = p < npts;
   bool bad = false;
   uint64_t kw[QTT_MAX_COORD] = {0, 0, 0, 0};
@@ -184,6 +183,38 @@ eval_chain_reg_kernel(const double* __restrict__ pack, const int64_t* __restrict
   }
   if (active) out[p] = bad ? __builtin_nan("") : v[0];
   if (bad) flag[0] = 1;
+}
+
+template <int CHI>
+__global__ void __launch_bounds__(256)
+eval_chain_reg_kernel(const double* __restrict__ pack, const int64_t* __restrict__ poff,
+                      const SiteDesc* __restrict__ desc, int nv, int nc, const int* __restrict__ ndigits,
+                      const double* __restrict__ xs, int64_t npts, double* __restrict__ out,
+                      int* __restrict__ flag) {
+  eval_chain_reg_body<CHI>(pack, poff, desc, nv, nc, ndigits, xs, npts, out, flag,
+                           (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// A batch of chains in one launch: workgroup b evaluates 256 points of the
+// item item_of[b] (its first workgroup is items[i].block0).
+struct EvalItem {
+  const double* pack;
+  const int64_t* poff;
+  const SiteDesc* desc;
+  const int* ndigits;
+  const double* xs;
+  double* out;
+  int64_t npts;
+  int64_t block0;
+  int nv, nc;
+};
+
+template <int CHI>
+__global__ void __launch_bounds__(256)
+eval_chain_batch_kernel(const EvalItem* __restrict__ items, const int* __restrict__ item_of, int* __restrict__ flag) {
+  const EvalItem& it = items[item_of[blockIdx.x]];
+  eval_chain_reg_body<CHI>(it.pack, it.poff, it.desc, it.nv, it.nc, it.ndigits, it.xs, it.npts, it.out, flag,
+                           (int64_t)(blockIdx.x - it.block0) * blockDim.x + threadIdx.x);
 }
 
 // Two points per lane (p and p + blockDim.x of the block's span): every core
@@ -451,8 +482,11 @@ eval_tree_kernel(const double* __restrict__ data, const int64_t* __restrict__ of
 static int ensure_pack(qtt_ctx* ctx, qtt_net* net) {
   if (net->pack_valid) return QTT_OK;
   const int nv = net->t.nv;
-  std::vector<int64_t> poff(nv);
-  std::vector<int> desc(4 * nv);
+  // host tables kept on the network: they outlive the asynchronous copies below
+  std::vector<int64_t>& poff = net->h_pack_off;
+  std::vector<int>& desc = net->h_desc;
+  poff.assign(nv, 0);
+  desc.assign(4 * nv, 0);
   int64_t o = 0;
   for (int k = 0; k < nv; k++) {
     int64_t chil = net->bond[k];
@@ -497,9 +531,6 @@ static int ensure_pack(qtt_ctx* ctx, qtt_net* net) {
   pack_chain_kernel<<<grid, 256, 0, ctx->stream>>>(net->d_data, net->d_off, net->d_bond, tv.sdim,
                                                    net->d_pack_off, nv, net->d_pack);
   rc = qtt_hip_check(ctx, hipGetLastError(), "pack_chain_kernel");
-  if (rc) return rc;
-  // the host copy of poff must outlive the async copy
-  rc = qtt_hip_check(ctx, hipStreamSynchronize(ctx->stream), "pack sync");
   if (rc) return rc;
   net->pack_valid = true;
   net->pack32_valid = false;
@@ -615,4 +646,68 @@ int launch_eval(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t npts, doub
   rc = qtt_hip_check(ctx, hipGetLastError(), "eval_tree_kernel");
   if (rc) return rc;
   return qtt_hip_check(ctx, hipStreamSynchronize(ctx->stream), "eval_tree sync");
+}
+
+// f_i at its own points for a batch of networks.  Float64 chains with bonds
+// <= 32 run in one launch per bond class (4, 8, 16, 32); the others take the
+// single-network launcher one by one.  Blocks until the launches are queued
+// and their argument tables copied.
+int launch_eval_batch(qtt_ctx* ctx, int n, qtt_net* const* nets, const double* const* xs, const int64_t* npts,
+                      double* const* out, int precision) {
+  std::vector<EvalItem> items[4];
+  std::vector<int> item_of[4];
+  for (int i = 0; i < n; i++) {
+    qtt_net* net = nets[i];
+    if (npts[i] == 0) continue;
+    const HostTopo& t = net->t;
+    int64_t maxb = 1;
+    for (int v = 0; v < t.nv; v++) maxb = std::max(maxb, net->bond[v]);
+    if (precision != QTT_F64 || !t.chain || t.ncoord > QTT_MAX_COORD || maxb > 32 || ctx->force_lds_eval) {
+      int rc = launch_eval(ctx, net, xs[i], npts[i], out[i], precision);
+      if (rc) return rc;
+      continue;
+    }
+    int rc = ensure_pack(ctx, net);
+    if (rc) return rc;
+    const int cls = maxb <= 4 ? 0 : maxb <= 8 ? 1 : maxb <= 16 ? 2 : 3;
+    EvalItem it;
+    it.pack = net->d_pack;
+    it.poff = net->d_pack_off;
+    it.desc = (const SiteDesc*)net->d_desc;
+    it.ndigits = net->view().ndigits;
+    it.xs = xs[i];
+    it.out = out[i];
+    it.npts = npts[i];
+    it.block0 = (int64_t)item_of[cls].size();
+    it.nv = t.nv;
+    it.nc = t.ncoord;
+    const int64_t nb = (npts[i] + 255) / 256;
+    if (it.block0 + nb > INT32_MAX) return qtt_fail(ctx, QTT_EINVAL, "evaluate batch: too many points");
+    item_of[cls].insert(item_of[cls].end(), (size_t)nb, (int)items[cls].size());
+    items[cls].push_back(it);
+  }
+  size_t bytes = 256;
+  for (int c = 0; c < 4; c++)
+    bytes += items[c].size() * sizeof(EvalItem) + item_of[c].size() * sizeof(int) + 512;
+  char* scr = (char*)qtt_scratch(ctx, bytes);
+  if (!scr) return qtt_fail(ctx, QTT_ENOMEM, "evaluate batch scratch");
+  for (int c = 0; c < 4; c++) {
+    if (items[c].empty()) continue;
+    EvalItem* d_items = (EvalItem*)scr;
+    scr += (items[c].size() * sizeof(EvalItem) + 255) & ~size_t(255);
+    int* d_map = (int*)scr;
+    scr += (item_of[c].size() * sizeof(int) + 255) & ~size_t(255);
+    int rc = qtt_hip_check(ctx, hipMemcpyAsync(d_items, items[c].data(), items[c].size() * sizeof(EvalItem), hipMemcpyHostToDevice, ctx->stream), "batch items");
+    if (!rc) rc = qtt_hip_check(ctx, hipMemcpyAsync(d_map, item_of[c].data(), item_of[c].size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream), "batch map");
+    if (rc) return rc;
+    const dim3 grid((unsigned)item_of[c].size());
+    if (c == 0) eval_chain_batch_kernel<4><<<grid, 256, 0, ctx->stream>>>(d_items, d_map, ctx->d_flag);
+    else if (c == 1) eval_chain_batch_kernel<8><<<grid, 256, 0, ctx->stream>>>(d_items, d_map, ctx->d_flag);
+    else if (c == 2) eval_chain_batch_kernel<16><<<grid, 256, 0, ctx->stream>>>(d_items, d_map, ctx->d_flag);
+    else eval_chain_batch_kernel<32><<<grid, 256, 0, ctx->stream>>>(d_items, d_map, ctx->d_flag);
+    rc = qtt_hip_check(ctx, hipGetLastError(), "eval_chain_batch_kernel");
+    if (rc) return rc;
+  }
+  // the host tables must outlive their asynchronous copies
+  return qtt_hip_check(ctx, hipStreamSynchronize(ctx->stream), "evaluate batch sync");
 }

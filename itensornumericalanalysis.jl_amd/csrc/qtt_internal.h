@@ -68,6 +68,8 @@ struct qtt_net {
   int64_t* d_pack_off = nullptr;
   int64_t pack_elems = 0;
   int* d_desc = nullptr;  // per site {chil, chir, coord, shift} for the register kernel
+  std::vector<int64_t> h_pack_off;  // host sources of the pack offsets / descriptors
+  std::vector<int> h_desc;
   bool pack_valid = false;
   // float32 copies for the mixed-precision evaluate
   float* d_pack32 = nullptr;
@@ -95,6 +97,8 @@ double* qtt_scratch(qtt_ctx* ctx, size_t bytes);
 // kernel launchers (eval.hip, sweep.hip, factor.hip, grad.hip)
 int launch_eval(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t npts, double* out,
                 int precision);
+int launch_eval_batch(qtt_ctx* ctx, int n, qtt_net* const* nets, const double* const* xs, const int64_t* npts,
+                      double* const* out, int precision);
 int launch_integrate(qtt_ctx* ctx, const qtt_net* net, double weight, double* out_host);
 int launch_inner(qtt_ctx* ctx, const qtt_net* a, const qtt_net* b, double* out_host);
 int launch_integrate_batch(qtt_ctx* ctx, int n, const qtt_net* const* nets, double weight, double* out_host);

@@ -64,6 +64,18 @@ def test_eval_random_chain_bonds(q, po, eng, rng, bond):
     assert rel_l2(q.evaluate(f, xs), oracle_eval(po, f, xs)) < TOL
 
 
+@pytest.mark.parametrize("bond", [5, 16])
+def test_eval_many_points(q, po, eng, rng, bond):
+    # >= 2^19 points with bonds <= 16 take the two-points-per-lane kernel; a ragged count
+    s = q.continuous_siteinds(q.named_grid((16, 1)), map_dimension=2)
+    f = q.random_itn(s, link_space=bond, rng=rng, std=1.0 / math.sqrt(bond))
+    xs = rng.uniform(0, 1, ((1 << 19) + 123, 2))
+    y = q.evaluate(f, xs)
+    ref = oracle_eval(po, f, xs)
+    assert rel_l2(y, ref) < TOL
+    assert np.array_equal(y[:4096], q.evaluate(f, xs[:4096]))  # same arithmetic as one point per lane
+
+
 def test_eval_tree(q, po, eng, rng):
     s = q.continuous_siteinds(q.named_comb_tree((3, 5)), map_dimension=2)
     f = q.random_itn(s, link_space=4, rng=rng, std=0.5)
@@ -255,6 +267,22 @@ def test_operate_batch_and_integrate_batch(q, po, eng, rng):
         assert _compare_networks(q, po, g, gref, w.f, rng, npts=512) < 1e-11
         assert integ == pytest.approx(po.integrate(t.parent, t.sdim, gref, 0.5), rel=1e-11)
         assert e == pytest.approx(rerr, rel=1e-6, abs=1e-18)
+
+
+def test_evaluate_batch(q, po, eng, rng):
+    # uneven digit counts and bonds (all four register classes, one LDS-kernel network), ragged point counts
+    import torch
+    fs, xs = [], []
+    for i, (L, bond) in enumerate([(8, 3), (12, 7), (20, 16), (16, 29), (10, 40), (14, 1), (9, 12)]):
+        s = q.continuous_siteinds(q.named_grid((L, 1)))
+        fs.append(q.random_itn(s, link_space=bond, rng=rng, std=1.0 / math.sqrt(bond)))
+        xs.append(rng.uniform(0, 1, (100 + 777 * i, 1)))
+    outs = q.evaluate_batch(fs, [torch.tensor(x, device="cuda") for x in xs])
+    eng.sync()
+    for f, x, y in zip(fs, xs, outs):
+        yb = y.cpu().numpy()
+        assert rel_l2(yb, oracle_eval(po, f, x)) < TOL
+        assert np.array_equal(yb, q.evaluate(f, x))  # same arithmetic as the single-network launch
 
 
 # ------------------------------------------------------------------ gradient
