@@ -281,6 +281,38 @@ def other_workloads(q, po, ctx, stream, args):
                       "forward_flops_per_point": flops_fwd,
                       "gradient_tflops_3x_forward": 3 * flops_fwd * len(w4.xs) / gt / 1e12,
                       "grad_rel_err_vs_oracle_2^14_subset": num / den}
+    # batch: 512 chains of uneven digit counts and bonds, Fredholm kernel operator
+    wsb = workloads.fredholm_batch()
+    opsb, fsb = [w.op for w in wsb], [w.f for w in wsb]
+    gsb, _ = q.operate_batch(opsb, fsb, cutoff=1e-10)  # warm (uploads, scratch)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gsb, errsb = q.operate_batch(opsb, fsb, cutoff=1e-10)
+    torch.cuda.synchronize()
+    t_op = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    intsb = q.integrate_batch(gsb)
+    t_int = time.perf_counter() - t0
+    xsb = [torch.tensor(w.xs, device="cuda") for w in wsb]
+    ysb = q.evaluate_batch(gsb, xsb)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ysb = q.evaluate_batch(gsb, xsb, outs=ysb)
+    torch.cuda.synchronize()
+    t_ev = time.perf_counter() - t0
+    errs_b = []
+    for i in (1, 2):  # parity on two small members against the oracle
+        w = wsb[i]
+        t = w.f.topo
+        C = po.contract(t.parent, t.sdim, w.op.cores, w.f.cores)
+        gref, _ = po.truncate(t.parent, t.sdim, C, 1e-10)
+        yr, _ = po.evaluate(t.parent, t.sdim, gref, t.vcoord, t.vdigit, t.Lc, w.xs)
+        errs_b.append(float(np.abs(ysb[i].cpu().numpy() - yr).max() / np.abs(yr).max()))
+    out["fredholm_batch"] = {"functions": len(wsb), "operate_truncate_ms": t_op * 1e3,
+                             "integrate_ms": t_int * 1e3, "evaluate_ms": t_ev * 1e3,
+                             "points_each": int(len(wsb[0].xs)),
+                             "result_bond_max": int(max(max(g.bonds()) for g in gsb)),
+                             "eval_normrel_err_vs_oracle_2_members": errs_b}
     return out
 
 
