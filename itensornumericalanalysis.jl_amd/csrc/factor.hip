@@ -220,8 +220,8 @@ __device__ inline void split_dims(const int64_t* dims, int nd, int ax, int64_t& 
 // The strided loops below run 4 elements per iteration (loads first, then the
 // arithmetic) so several LDS/L2 loads are in flight per wait; the summation
 // order is fixed, so results stay deterministic.
-template <int G>
-__device__ inline void grp_reflector(double* W, int rows, int ld, int j, double* tau) {
+template <int G, typename P = double*>
+__device__ inline void grp_reflector(P W, int rows, int ld, int j, P tau) {
   const int gl = threadIdx.x & (G - 1);
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
   int i = j + 1 + gl;
@@ -247,10 +247,10 @@ __device__ inline void grp_reflector(double* W, int rows, int ld, int j, double*
     const double scal = 1.0 / (x0 - beta);
     i = j + 1 + gl;
     for (; i + 3 * G < rows; i += 4 * G) {
-      double* p0 = W + (int64_t)i * ld + j;
-      double* p1 = W + (int64_t)(i + G) * ld + j;
-      double* p2 = W + (int64_t)(i + 2 * G) * ld + j;
-      double* p3 = W + (int64_t)(i + 3 * G) * ld + j;
+      P p0 = W + (int64_t)i * ld + j;
+      P p1 = W + (int64_t)(i + G) * ld + j;
+      P p2 = W + (int64_t)(i + 2 * G) * ld + j;
+      P p3 = W + (int64_t)(i + 3 * G) * ld + j;
       const double y0 = *p0, y1 = *p1, y2 = *p2, y3 = *p3;
       *p0 = y0 * scal;
       *p1 = y1 * scal;
@@ -266,9 +266,8 @@ __device__ inline void grp_reflector(double* W, int rows, int ld, int j, double*
 }
 
 // apply H = I - tau v v^T (v[0] = 1 implicit, v[i] = vcol[i * vstride] for i > j) to column col
-template <int G>
-__device__ inline void grp_apply(double* W, int rows, int ld, int j, int col, const double* vcol,
-                                 int vstride, double tj) {
+template <int G, typename P = double*, typename CP = const double*>
+__device__ inline void grp_apply(P W, int rows, int ld, int j, int col, CP vcol, int vstride, double tj) {
   const int gl = threadIdx.x & (G - 1);
   double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
   int i = j + 1 + gl;
@@ -288,10 +287,10 @@ __device__ inline void grp_apply(double* W, int rows, int ld, int j, int col, co
   for (; i + 3 * G < rows; i += 4 * G) {
     const double v0 = vcol[(int64_t)i * vstride], v1 = vcol[(int64_t)(i + G) * vstride];
     const double v2 = vcol[(int64_t)(i + 2 * G) * vstride], v3 = vcol[(int64_t)(i + 3 * G) * vstride];
-    double* p0 = W + (int64_t)i * ld + col;
-    double* p1 = W + (int64_t)(i + G) * ld + col;
-    double* p2 = W + (int64_t)(i + 2 * G) * ld + col;
-    double* p3 = W + (int64_t)(i + 3 * G) * ld + col;
+    P p0 = W + (int64_t)i * ld + col;
+    P p1 = W + (int64_t)(i + G) * ld + col;
+    P p2 = W + (int64_t)(i + 2 * G) * ld + col;
+    P p3 = W + (int64_t)(i + 3 * G) * ld + col;
     const double w0 = *p0, w1 = *p1, w2 = *p2, w3 = *p3;
     *p0 = __builtin_fma(-d, v0, w0);
     *p1 = __builtin_fma(-d, v1, w1);
@@ -310,8 +309,8 @@ __device__ inline int first_owned(int start, int gid, int ng) {
 // dimension ld; reflectors below the diagonal, R on and above it, tau[k].
 // Column c belongs to group c % ng; the owner of column j+1 updates it first
 // and builds the next reflector right away (look-ahead): one barrier per column.
-template <int G>
-__device__ void geqr2(double* W, int rows, int cols, int ld, double* tau) {
+template <int G, typename P = double*>
+__device__ void geqr2(P W, int rows, int cols, int ld, P tau) {
   const int gid = threadIdx.x / G, ng = blockDim.x / G;
   const int k = rows < cols ? rows : cols;
   if (k <= 0) return;
@@ -862,6 +861,174 @@ __device__ __noinline__ bool jacobi_rows_reg(lds_f64* X, int ldx, int n, int b, 
   return false;
 }
 
+// ---------------------------------------------------------------------------
+// Blocked Householder QR for matrices in global memory (the move's workspace
+// does not fit LDS): panels of PNB columns are factored in LDS, and the rest
+// of the matrix is updated once per panel in the compact WY form
+// H_0 ... H_{nb-1} = I - V T V^T (LAPACK dlarft / dlarfb), one lane per column
+// with the panel's reflectors broadcast from LDS.  Q (or U of a tall SVD move)
+// is formed the same way, panels in reverse order.
+// ---------------------------------------------------------------------------
+constexpr int PNB = 16;
+
+// T (nb x nb upper triangular, ld PNB) of the reflectors V (unit lower
+// trapezoidal, mr x nb, ld PNB, reflector entries below the diagonal of the
+// panel) and their scalars tau.  Y is nb x nb scratch.
+__device__ void build_T(const lds_f64* V, int mr, int nb, const lds_f64* tau, lds_f64* T, lds_f64* Y) {
+  const int tid = threadIdx.x, bs = blockDim.x;
+  for (int e = tid; e < nb * nb; e += bs) {  // Y[j][i] = v_j . v_i, j < i
+    const int j = e / nb, i = e - j * nb;
+    if (j < i) {
+      double acc = V[i * PNB + j];  // v_i is zero above row i and one at row i
+      for (int r = i + 1; r < mr; ++r) acc = __builtin_fma(V[r * PNB + j], V[r * PNB + i], acc);
+      Y[j * PNB + i] = acc;
+    }
+  }
+  __syncthreads();
+  if (tid < nb) {  // T[:, i] = -tau_i T[:, 0:i] Y[0:i, i]; lane r owns row r
+    const int r = tid;
+    for (int i = 0; i < nb; ++i) {
+      double t = 0.0;
+      if (i == r) {
+        t = tau[i];
+      } else if (i > r) {
+        double acc = 0.0;
+        for (int j = r; j < i; ++j) acc = __builtin_fma(T[r * PNB + j], Y[j * PNB + i], acc);
+        t = -tau[i] * acc;
+      }
+      T[r * PNB + i] = t;
+    }
+  }
+  __syncthreads();
+}
+
+// Column c (one lane) of a matrix M, rows from the panel's first row on:
+// M[:, c] <- (I - V op(T) V^T) M[:, c], op(T) = T^T (trans) or T.  at(r)
+// is the element in panel row r (r < mr).  V rows past the panel's first nb
+// are all reflector entries; the first nb rows hold the unit diagonal.
+template <typename At>
+__device__ __forceinline__ void wy_column(At at, int mr, int nb, const lds_f64* V, const lds_f64* T, bool trans) {
+  double w[PNB];
+#pragma unroll
+  for (int j = 0; j < PNB; ++j) w[j] = 0.0;
+  const int top = mr < nb ? mr : nb;
+  for (int r = 0; r < top; ++r) {
+    const double a = at(r);
+#pragma unroll
+    for (int j = 0; j < PNB; ++j) {
+      const double v = j < r ? V[r * PNB + j] : (j == r ? 1.0 : 0.0);
+      if (j < nb) w[j] = __builtin_fma(v, a, w[j]);
+    }
+  }
+  for (int r = nb; r < mr; ++r) {
+    const double a = at(r);
+#pragma unroll
+    for (int j = 0; j < PNB; ++j)
+      if (j < nb) w[j] = __builtin_fma(V[r * PNB + j], a, w[j]);
+  }
+  double u[PNB];
+#pragma unroll
+  for (int i = 0; i < PNB; ++i) {
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < PNB; ++j) {
+      if (i < nb && j < nb) {
+        const double t = trans ? (j <= i ? T[j * PNB + i] : 0.0) : (j >= i ? T[i * PNB + j] : 0.0);
+        acc = __builtin_fma(t, w[j], acc);
+      }
+    }
+    u[i] = acc;
+  }
+  for (int r = 0; r < top; ++r) {
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < PNB; ++j) {
+      const double v = j < r ? V[r * PNB + j] : (j == r ? 1.0 : 0.0);
+      if (j < nb) acc = __builtin_fma(v, u[j], acc);
+    }
+    at(r) -= acc;
+  }
+  for (int r = nb; r < mr; ++r) {
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < PNB; ++j)
+      if (j < nb) acc = __builtin_fma(V[r * PNB + j], u[j], acc);
+    at(r) -= acc;
+  }
+}
+
+// QR of W (global, rows x cols, leading dimension ld): R on and above the
+// diagonal, reflectors below, tau[k].  Pl: rows x PNB LDS panel, Tl, Yl:
+// PNB x PNB, tl: PNB scalars (all LDS).
+__device__ __noinline__ void qr_panel(double* W, int rows, int cols, int ld, double* tau, lds_f64* Pl, lds_f64* Tl,
+                                      lds_f64* Yl, lds_f64* tl) {
+  const int tid = threadIdx.x, bs = blockDim.x;
+  const int k = rows < cols ? rows : cols;
+  for (int p0 = 0; p0 < k; p0 += PNB) {
+    const int nb = k - p0 < PNB ? k - p0 : PNB, mr = rows - p0;
+    for (int e = tid; e < mr * PNB; e += bs) {
+      const int r = e / PNB, c = e - r * PNB;
+      Pl[e] = c < nb ? W[(int64_t)(p0 + r) * ld + p0 + c] : 0.0;
+    }
+    __syncthreads();
+    geqr2<8, lds_f64*>(Pl, mr, nb, PNB, tl);
+    __syncthreads();
+    for (int e = tid; e < mr * nb; e += bs) {
+      const int r = e / nb, c = e - r * nb;
+      W[(int64_t)(p0 + r) * ld + p0 + c] = Pl[r * PNB + c];
+    }
+    for (int e = tid; e < nb; e += bs) tau[p0 + e] = tl[e];
+    if (p0 + nb < cols) {
+      build_T(Pl, mr, nb, tl, Tl, Yl);
+      for (int c = p0 + nb + tid; c < cols; c += bs) {
+        double* col = W + (int64_t)p0 * ld + c;
+        wy_column([=](int r) -> double& { return col[(int64_t)r * ld]; }, mr, nb, Pl, Tl, true);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// x <- Q (rows x k) of the reflectors in W (global, ld) written straight into
+// the core Xc (rows split as (aa, bb), bb < bx, bond axis of size k); or, with
+// Jb, x <- U = Q [J_perm^T; 0] (rows x m, row i of J of length n).
+__device__ __noinline__ void q_form_panel(const double* W, int rows, int k, int ld, const double* tau, double* Xc,
+                                          int bx, const double* Jb, int n, const int* perm, int m, lds_f64* Pl,
+                                          lds_f64* Tl, lds_f64* Yl, lds_f64* tl) {
+  const int tid = threadIdx.x, bs = blockDim.x;
+  const int ncol = Jb ? m : k;
+  for (int e = tid; e < rows * ncol; e += bs) {
+    const int r = e / ncol, c = e - r * ncol;
+    const int aa = r / bx, bb = r - aa * bx;
+    double v;
+    if (Jb) v = r < n ? Jb[(int64_t)perm[c] * n + r] : 0.0;
+    else v = r == c ? 1.0 : 0.0;
+    Xc[((int64_t)aa * ncol + c) * bx + bb] = v;
+  }
+  __syncthreads();
+  for (int p0 = ((k - 1) / PNB) * PNB; p0 >= 0; p0 -= PNB) {
+    const int nb = k - p0 < PNB ? k - p0 : PNB, mr = rows - p0;
+    for (int e = tid; e < mr * PNB; e += bs) {
+      const int r = e / PNB, c = e - r * PNB;
+      Pl[e] = c < nb ? W[(int64_t)(p0 + r) * ld + p0 + c] : 0.0;
+    }
+    for (int e = tid; e < nb; e += bs) tl[e] = tau[p0 + e];
+    __syncthreads();
+    build_T(Pl, mr, nb, tl, Tl, Yl);
+    // Q: only columns >= p0 differ from the identity yet; U: every column
+    const int c0 = Jb ? 0 : p0;
+    for (int c = c0 + tid; c < ncol; c += bs) {
+      wy_column(
+          [=](int r) -> double& {
+            const int rr = p0 + r, aa = rr / bx;
+            return Xc[((int64_t)aa * ncol + c) * bx + (rr - aa * bx)];
+          },
+          mr, nb, Pl, Tl, false);
+    }
+    __syncthreads();
+  }
+}
+
 template <int G>
 __device__ inline void row_norms(const double* X, int n, int b, double* sig) {
   const int gl = threadIdx.x & (G - 1), gid = threadIdx.x / G, ng = blockDim.x / G;
@@ -932,8 +1099,8 @@ __device__ __forceinline__ void mode_product(const double* Rrows, const int* per
 // scratch.  Two separate instantiations (one per memory) let the compiler use
 // LDS instructions on the LDS path instead of generic flat addressing.
 template <int G, bool REG>
-__device__ __forceinline__ int run_move(double* ws, double* jpad, const int rows, const int b, const int tr, const int rpl,
-                                        const int bx,
+__device__ __forceinline__ int run_move(double* ws, double* jpad, lds_f64* pnl, const int rows, const int b, const int tr,
+                                        const int rpl, const int bx,
                                         const int64_t Au, const int64_t nu, const int64_t Bu, double* Xc,
                                         const double* Uc, double* Ud, double* gY, double cutoff, long long maxdim,
                                         int* status, int* iflag, double* red, double* dbc) {
@@ -981,6 +1148,16 @@ __device__ __forceinline__ int run_move(double* ws, double* jpad, const int rows
       if (rpl == 8) q_form_reg<8, 8>(Wl, rows, k, taul, Xc, bx);
       else if (rpl == 16) q_form_reg<8, 16>(Wl, rows, k, taul, Xc, bx);
       else q_form_reg<16, 12>(Wl, rows, k, taul, Xc, bx);
+    } else if (pnl) {  // workspace in global memory: blocked QR, panels in LDS
+      lds_f64 *Pl = pnl, *Tl = pnl + rows * PNB, *Yl = Tl + PNB * PNB, *tl = Yl + PNB * PNB;
+      qr_panel(W, rows, b, b, tau, Pl, Tl, Yl, tl);
+      for (int e = tid; e < k * b; e += bs) {
+        const int i = e / b, c = e - i * b;
+        Rb[e] = c >= i ? W[(int64_t)i * b + c] : 0.0;
+      }
+      __syncthreads();
+      QTT_PHASE(-1, 2);  // factored
+      q_form_panel(W, rows, k, b, tau, Xc, bx, nullptr, 0, nullptr, 0, Pl, Tl, Yl, tl);  // x <- Q (bond m)
     } else {
       geqr2<G>(W, rows, b, b, tau);
       for (int e = tid; e < k * b; e += bs) {
@@ -1007,6 +1184,13 @@ __device__ __forceinline__ int run_move(double* ws, double* jpad, const int rows
         if (rpl == 8) qr_reg<8, 8, 2>(Wl, rows, b, taul, Rl);
         else if (rpl == 16) qr_reg<8, 16, 2>(Wl, rows, b, taul, Rl);
         else qr_reg<16, 12, 3>(Wl, rows, b, taul, Rl);
+      } else if (pnl) {  // blocked QR; the reflectors stay in W for U below
+        lds_f64 *Pl = pnl, *Tl = pnl + rows * PNB, *Yl = Tl + PNB * PNB, *tl = Yl + PNB * PNB;
+        qr_panel(W, rows, b, b, tau, Pl, Tl, Yl, tl);
+        for (int e = tid; e < b * b; e += bs) {
+          const int i = e / b, c = e - i * b;
+          Rb[e] = c >= i ? W[(int64_t)i * b + c] : 0.0;
+        }
       } else {
         geqr2<G>(W, rows, b, b, tau);
         for (int e = tid; e < b * b; e += bs) {
@@ -1094,7 +1278,10 @@ __device__ __forceinline__ int run_move(double* ws, double* jpad, const int rows
     Rrows = X;
     use_perm = true;
     // x <- U (bond m): U[row][i] = sum_j Q[row][j] J[perm[i]][j]  (or J[perm[i]][row])
-    if (tall && use_reg) {
+    if (tall && pnl) {
+      lds_f64 *Pl = pnl, *Tl = pnl + rows * PNB, *Yl = Tl + PNB * PNB, *tl = Yl + PNB * PNB;
+      q_form_panel(W, rows, b, b, tau, Xc, bx, Jb, n, perm, m, Pl, Tl, Yl, tl);
+    } else if (tall && use_reg) {
       const lds_f64 *Wl = (const lds_f64*)W, *taul = (const lds_f64*)tau, *Jl = (const lds_f64*)Jb;
       const lds_i32* pl = (const lds_i32*)perm;
       if (rpl == 8) u_form_reg<8, 8>(Wl, rows, b, taul, Jl, n, pl, m, Xc, bx);
@@ -1179,6 +1366,9 @@ __global__ void __launch_bounds__(512) truncate_kernel(const SweepArgs* __restri
       rpl = 0;
       need = (int64_t)rows * b + base;
     }
+    // global workspace: LDS holds the QR panels (rows x PNB, T, Y, tau)
+    lds_f64* pnl = nullptr;
+    if (need > a.ws_lds && (int64_t)rows * PNB + 2 * PNB * PNB + PNB <= a.ws_lds) pnl = (lds_f64*)wsl;
     // padded copy for the Jacobi rounds of an SVD move, when it fits after the workspace
     double* jpad = nullptr;
     if (tr) {
@@ -1191,17 +1381,17 @@ __global__ void __launch_bounds__(512) truncate_kernel(const SweepArgs* __restri
     int m;
     if (need <= a.ws_lds) {
       if (small)
-        m = run_move<8, true>(wsl, jpad, rows, b, tr, rpl, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<8, true>(wsl, jpad, nullptr, rows, b, tr, rpl, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
                         iflag, red, dbc);
       else
-        m = run_move<64, false>(wsl, nullptr, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<64, false>(wsl, nullptr, nullptr, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
                          iflag, red, dbc);
     } else {
       if (small)
-        m = run_move<8, false>(a.gscr, nullptr, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<8, false>(a.gscr, nullptr, pnl, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
                         iflag, red, dbc);
       else
-        m = run_move<64, false>(a.gscr, nullptr, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<64, false>(a.gscr, nullptr, pnl, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
                          iflag, red, dbc);
     }
     QTT_PHASE(st, 5);
