@@ -51,7 +51,23 @@ extern "C" int qtt_debug_jacobi(unsigned long long* out) {
     jt_last_ = c_;                                                  \
   } while (0)
 #define QTT_JC(i) do { if (threadIdx.x == 0) qtt_jac_t[i] += 1; } while (0)
+// cycles of thread 0 per segment of a blocked-QR panel
+__device__ unsigned long long qtt_pnl_t[8];
+extern "C" int qtt_debug_panel(unsigned long long* out) {
+  return (int)hipMemcpyFromSymbol(out, HIP_SYMBOL(qtt_pnl_t), 8 * sizeof(unsigned long long));
+}
+#define QTT_PT_INIT unsigned long long pt_last_ = clock64()
+#define QTT_PANEL_COUNT (qtt_pnl_t[7] += 1)
+#define QTT_PT(i)                                                   \
+  do {                                                              \
+    const unsigned long long c_ = clock64();                        \
+    if (threadIdx.x == 0) qtt_pnl_t[i] += c_ - pt_last_;            \
+    pt_last_ = c_;                                                  \
+  } while (0)
 #else
+#define QTT_PT_INIT do { } while (0)
+#define QTT_PT(i) do { } while (0)
+#define QTT_PANEL_COUNT (void)0
 #define QTT_PHASE(step, phase) do { } while (0)
 #define QTT_JT_INIT do { } while (0)
 #define QTT_JT(i) do { } while (0)
@@ -379,6 +395,11 @@ __device__ __forceinline__ uint64_t eq_mask(int a, int b) {  // a == b
 __device__ __forceinline__ double keep(double x, uint64_t m) {
   return __longlong_as_double((long long)((uint64_t)__double_as_longlong(x) & m));
 }
+// The argument of a non-inlined function is uniform across the wavefront but
+// arrives in a vector register; reading it into a scalar register makes the
+// loops and tests that depend on it scalar (no per-lane masks).
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
 // The lane index made opaque to the optimizer: the per-row masks are then
 // recomputed where they are used instead of being hoisted out of the column
 // loop and held (dozens of condition registers) across it.
@@ -596,18 +617,19 @@ typedef __attribute__((address_space(3))) int lds_i32;
 
 template <int GL, int RPL, int NS>
 __device__ __noinline__ void qr_reg(lds_f64* W, int rows, int cols, lds_f64* tau, lds_f64* Rb) {
-  qr_body<GL, RPL, NS, lds_f64*>(W, rows, cols, tau, Rb);
+  qr_body<GL, RPL, NS, lds_f64*>(W, uni(rows), uni(cols), tau, Rb);
 }
 
 template <int GL, int RPL>
 __device__ __noinline__ void q_form_reg(const lds_f64* VT, int rows, int k, const lds_f64* tau, double* Xc, int bx) {
-  q_form_body<GL, RPL, const lds_f64*>(VT, rows, k, tau, Xc, bx);
+  q_form_body<GL, RPL, const lds_f64*>(VT, uni(rows), uni(k), tau, Xc, uni(bx));
 }
 
 template <int GL, int RPL>
 __device__ __noinline__ void u_form_reg(const lds_f64* VT, int rows, int k, const lds_f64* tau, const lds_f64* Jb,
                                         int n, const lds_i32* perm, int m, double* Xc, int bx) {
-  u_form_body<GL, RPL, const lds_f64*, const lds_i32*>(VT, rows, k, tau, Jb, n, perm, m, Xc, bx);
+  u_form_body<GL, RPL, const lds_f64*, const lds_i32*>(VT, uni(rows), uni(k), tau, Jb, uni(n), perm, uni(m), Xc,
+                                                       uni(bx));
 }
 
 __device__ inline int rr_player(int pos, int t, int nn) {
@@ -778,8 +800,9 @@ __device__ __forceinline__ void rotate4(double (&u)[JEPL], double (&w)[JEPL], do
 }
 
 template <int G>
-__device__ __noinline__ bool jacobi_rows_reg(lds_f64* X, int ldx, int n, int b, lds_f64* J, int ldj, lds_i32* flag,
-                                             double* red) {
+__device__ __noinline__ bool jacobi_rows_reg(lds_f64* X, int ldx_in, int n_in, int b_in, lds_f64* J, int ldj_in,
+                                             lds_i32* flag, double* red) {
+  const int ldx = uni(ldx_in), n = uni(n_in), b = uni(b_in), ldj = uni(ldj_in);
   const int tid = threadIdx.x, bs = blockDim.x;
   const int gid = tid / G, ng = bs / G;
   for (int e = tid; e < n * n; e += bs) J[(e / n) * ldj + e % n] = (e / n == e % n) ? 1.0 : 0.0;
@@ -870,18 +893,24 @@ __device__ __noinline__ bool jacobi_rows_reg(lds_f64* X, int ldx, int n, int b, 
 // is formed the same way, panels in reverse order.
 // ---------------------------------------------------------------------------
 constexpr int PNB = 16;
+constexpr int PLD = PNB + 1;  // row stride of the panel that is read by columns: odd, so no bank conflicts
 
 // T (nb x nb upper triangular, ld PNB) of the reflectors V (unit lower
-// trapezoidal, mr x nb, ld PNB, reflector entries below the diagonal of the
+// trapezoidal, mr x nb, ld PLD, reflector entries below the diagonal of the
 // panel) and their scalars tau.  Y is nb x nb scratch.
 __device__ void build_T(const lds_f64* V, int mr, int nb, const lds_f64* tau, lds_f64* T, lds_f64* Y) {
   const int tid = threadIdx.x, bs = blockDim.x;
-  for (int e = tid; e < nb * nb; e += bs) {  // Y[j][i] = v_j . v_i, j < i
+  nb = PNB;  // padded: the extra reflectors have tau = 0, so their rows and columns of T are 0
+  // Y[j][i] = v_j . v_i for j < i (rows past the panel: none); one wavefront
+  // per pair, the rows split over its lanes
+  const int lane = tid & 63, wv = tid >> 6, nw = bs >> 6;
+  for (int e = wv; e < nb * nb; e += nw) {
     const int j = e / nb, i = e - j * nb;
-    if (j < i) {
-      double acc = V[i * PNB + j];  // v_i is zero above row i and one at row i
-      for (int r = i + 1; r < mr; ++r) acc = __builtin_fma(V[r * PNB + j], V[r * PNB + i], acc);
-      Y[j * PNB + i] = acc;
+    if (j < i && i < mr) {
+      double acc = 0.0;
+      for (int r = i + 1 + lane; r < mr; r += 64) acc = __builtin_fma(V[r * PLD + j], V[r * PLD + i], acc);
+      acc = sum64(acc) + V[i * PLD + j];  // v_i is zero above row i and one at row i
+      if (lane == 0) Y[j * PNB + i] = acc;
     }
   }
   __syncthreads();
@@ -891,7 +920,7 @@ __device__ void build_T(const lds_f64* V, int mr, int nb, const lds_f64* tau, ld
       double t = 0.0;
       if (i == r) {
         t = tau[i];
-      } else if (i > r) {
+      } else if (i > r && tau[i] != 0.0) {  // padding reflectors (tau 0) keep zero columns
         double acc = 0.0;
         for (int j = r; j < i; ++j) acc = __builtin_fma(T[r * PNB + j], Y[j * PNB + i], acc);
         t = -tau[i] * acc;
@@ -903,99 +932,177 @@ __device__ void build_T(const lds_f64* V, int mr, int nb, const lds_f64* tau, ld
 }
 
 // Column c (one lane) of a matrix M, rows from the panel's first row on:
-// M[:, c] <- (I - V op(T) V^T) M[:, c], op(T) = T^T (trans) or T.  at(r)
-// is the element in panel row r (r < mr).  V rows past the panel's first nb
-// are all reflector entries; the first nb rows hold the unit diagonal.
+// M[:, c] <- (I - V op(T) V^T) M[:, c], op(T) = T^T (trans) or T.  at(r) is the
+// element in panel row r (r < mr).  Panels are padded to PNB reflectors with
+// zero rows and columns in T, so the shapes are compile-time; Vt holds the
+// panel's first PNB rows of V (unit lower triangular), V the rest.
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) f64x2 lds_f64x2;
+
+// row r of a PNB-wide LDS matrix, two entries per 16-byte load
+__device__ __forceinline__ void vrow(const lds_f64* V, int r, double (&v)[PNB]) {
+  const lds_f64x2* p = (const lds_f64x2*)(V + r * PNB);
+#pragma unroll
+  for (int j = 0; j < PNB / 2; ++j) {
+    const f64x2 x = p[j];
+    v[2 * j] = x.x;
+    v[2 * j + 1] = x.y;
+  }
+}
+
 template <typename At>
-__device__ __forceinline__ void wy_column(At at, int mr, int nb, const lds_f64* V, const lds_f64* T, bool trans) {
+__device__ __forceinline__ void wy_column(At at, int mr, const lds_f64* V, const lds_f64* Vt, const lds_f64* T,
+                                          bool trans) {
   double w[PNB];
 #pragma unroll
   for (int j = 0; j < PNB; ++j) w[j] = 0.0;
-  const int top = mr < nb ? mr : nb;
+  const int top = mr < PNB ? mr : PNB;
   for (int r = 0; r < top; ++r) {
     const double a = at(r);
+    double v[PNB];
+    vrow(Vt, r, v);
 #pragma unroll
-    for (int j = 0; j < PNB; ++j) {
-      const double v = j < r ? V[r * PNB + j] : (j == r ? 1.0 : 0.0);
-      if (j < nb) w[j] = __builtin_fma(v, a, w[j]);
+    for (int j = 0; j < PNB; ++j) w[j] = __builtin_fma(v[j], a, w[j]);
+  }
+  // the column lives in global memory: eight of its entries in flight per wait
+  int r = top;
+  for (; r + 8 <= mr; r += 8) {
+    double a[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) a[q] = at(r + q);
+#pragma unroll 2
+    for (int q = 0; q < 8; ++q) {
+      double v[PNB];
+      vrow(V, r + q, v);
+#pragma unroll
+      for (int j = 0; j < PNB; ++j) w[j] = __builtin_fma(v[j], a[q], w[j]);
     }
   }
-  for (int r = nb; r < mr; ++r) {
+  for (; r < mr; ++r) {
     const double a = at(r);
+    double v[PNB];
+    vrow(V, r, v);
 #pragma unroll
-    for (int j = 0; j < PNB; ++j)
-      if (j < nb) w[j] = __builtin_fma(V[r * PNB + j], a, w[j]);
+    for (int j = 0; j < PNB; ++j) w[j] = __builtin_fma(v[j], a, w[j]);
   }
   double u[PNB];
 #pragma unroll
   for (int i = 0; i < PNB; ++i) {
     double acc = 0.0;
+    if (trans) {
 #pragma unroll
-    for (int j = 0; j < PNB; ++j) {
-      if (i < nb && j < nb) {
-        const double t = trans ? (j <= i ? T[j * PNB + i] : 0.0) : (j >= i ? T[i * PNB + j] : 0.0);
-        acc = __builtin_fma(t, w[j], acc);
-      }
+      for (int j = 0; j <= i; ++j) acc = __builtin_fma(T[j * PNB + i], w[j], acc);
+    } else {
+#pragma unroll
+      for (int j = i; j < PNB; ++j) acc = __builtin_fma(T[i * PNB + j], w[j], acc);
     }
     u[i] = acc;
   }
-  for (int r = 0; r < top; ++r) {
+  for (r = 0; r < top; ++r) {
+    double v[PNB];
+    vrow(Vt, r, v);
     double acc = 0.0;
 #pragma unroll
-    for (int j = 0; j < PNB; ++j) {
-      const double v = j < r ? V[r * PNB + j] : (j == r ? 1.0 : 0.0);
-      if (j < nb) acc = __builtin_fma(v, u[j], acc);
+    for (int j = 0; j < PNB; ++j) acc = __builtin_fma(v[j], u[j], acc);
+    at(r) -= acc;
+  }
+  r = top;
+  for (; r + 8 <= mr; r += 8) {
+    double a[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) a[q] = at(r + q);
+#pragma unroll 2
+    for (int q = 0; q < 8; ++q) {
+      double v[PNB];
+      vrow(V, r + q, v);
+      double acc0 = 0.0, acc1 = 0.0;
+#pragma unroll
+      for (int j = 0; j < PNB; j += 2) {
+        acc0 = __builtin_fma(v[j], u[j], acc0);
+        acc1 = __builtin_fma(v[j + 1], u[j + 1], acc1);
+      }
+      at(r + q) = a[q] - (acc0 + acc1);
     }
-    at(r) -= acc;
   }
-  for (int r = nb; r < mr; ++r) {
+  for (; r < mr; ++r) {
+    double v[PNB];
+    vrow(V, r, v);
     double acc = 0.0;
 #pragma unroll
-    for (int j = 0; j < PNB; ++j)
-      if (j < nb) acc = __builtin_fma(V[r * PNB + j], u[j], acc);
+    for (int j = 0; j < PNB; ++j) acc = __builtin_fma(v[j], u[j], acc);
     at(r) -= acc;
   }
+}
+
+// After a panel's reflectors are in Pp (mr x PNB, ld PLD, columns past nb
+// zero) and their scalars in tl: zero the scalars of the padding columns,
+// copy the reflectors to Pl (ld PNB: rows aligned for 16-byte loads) and build
+// Vt, the first PNB rows of V with the unit diagonal and zeros above it.
+__device__ __forceinline__ void panel_pad(const lds_f64* Pp, int mr, int nb, lds_f64* tl, lds_f64* Pl, lds_f64* Vt) {
+  const int tid = threadIdx.x, bs = blockDim.x;
+  for (int e = tid; e < PNB * PNB; e += bs) {
+    const int r = e / PNB, j = e - r * PNB;
+    Vt[e] = r >= mr ? 0.0 : j < r ? Pp[r * PLD + j] : (j == r ? 1.0 : 0.0);
+  }
+  for (int e = tid; e < mr * PNB; e += bs) {
+    const int r = e / PNB, j = e - r * PNB;
+    Pl[e] = Pp[r * PLD + j];
+  }
+  for (int e = nb + tid; e < PNB; e += bs) tl[e] = 0.0;
+  __syncthreads();
 }
 
 // QR of W (global, rows x cols, leading dimension ld): R on and above the
 // diagonal, reflectors below, tau[k].  Pl: rows x PNB LDS panel, Tl, Yl:
 // PNB x PNB, tl: PNB scalars (all LDS).
-__device__ __noinline__ void qr_panel(double* W, int rows, int cols, int ld, double* tau, lds_f64* Pl, lds_f64* Tl,
-                                      lds_f64* Yl, lds_f64* tl) {
+__device__ __noinline__ void qr_panel(double* W, int rows_in, int cols_in, int ld_in, double* tau, lds_f64* Pp,
+                                      lds_f64* Pl, lds_f64* Tl, lds_f64* Yl, lds_f64* tl, lds_f64* Vt) {
   const int tid = threadIdx.x, bs = blockDim.x;
+  const int rows = uni(rows_in), cols = uni(cols_in), ld = uni(ld_in);
   const int k = rows < cols ? rows : cols;
+  QTT_PT_INIT;
   for (int p0 = 0; p0 < k; p0 += PNB) {
     const int nb = k - p0 < PNB ? k - p0 : PNB, mr = rows - p0;
     for (int e = tid; e < mr * PNB; e += bs) {
       const int r = e / PNB, c = e - r * PNB;
-      Pl[e] = c < nb ? W[(int64_t)(p0 + r) * ld + p0 + c] : 0.0;
+      Pp[r * PLD + c] = c < nb ? W[(int64_t)(p0 + r) * ld + p0 + c] : 0.0;
     }
     __syncthreads();
-    geqr2<8, lds_f64*>(Pl, mr, nb, PNB, tl);
+    QTT_PT(0);
+    geqr2<64, lds_f64*>(Pp, mr, nb, PLD, tl);  // a wavefront per column: few rows per lane
     __syncthreads();
+    QTT_PT(1);
     for (int e = tid; e < mr * nb; e += bs) {
       const int r = e / nb, c = e - r * nb;
-      W[(int64_t)(p0 + r) * ld + p0 + c] = Pl[r * PNB + c];
+      W[(int64_t)(p0 + r) * ld + p0 + c] = Pp[r * PLD + c];
     }
     for (int e = tid; e < nb; e += bs) tau[p0 + e] = tl[e];
+    QTT_PT(2);
     if (p0 + nb < cols) {
-      build_T(Pl, mr, nb, tl, Tl, Yl);
+      panel_pad(Pp, mr, nb, tl, Pl, Vt);
+      build_T(Pp, mr, nb, tl, Tl, Yl);
+      QTT_PT(3);
       for (int c = p0 + nb + tid; c < cols; c += bs) {
         double* col = W + (int64_t)p0 * ld + c;
-        wy_column([=](int r) -> double& { return col[(int64_t)r * ld]; }, mr, nb, Pl, Tl, true);
+        wy_column([=](int r) -> double& { return col[(int64_t)r * ld]; }, mr, Pl, Vt, Tl, true);
       }
+      QTT_PT(4);
     }
     __syncthreads();
+    QTT_PT(5);
+    if (threadIdx.x == 0) QTT_PANEL_COUNT;
   }
 }
 
 // x <- Q (rows x k) of the reflectors in W (global, ld) written straight into
 // the core Xc (rows split as (aa, bb), bb < bx, bond axis of size k); or, with
 // Jb, x <- U = Q [J_perm^T; 0] (rows x m, row i of J of length n).
-__device__ __noinline__ void q_form_panel(const double* W, int rows, int k, int ld, const double* tau, double* Xc,
-                                          int bx, const double* Jb, int n, const int* perm, int m, lds_f64* Pl,
-                                          lds_f64* Tl, lds_f64* Yl, lds_f64* tl) {
+__device__ __noinline__ void q_form_panel(const double* W, int rows_in, int k_in, int ld_in, const double* tau,
+                                          double* Xc, int bx_in, const double* Jb, int n_in, const int* perm, int m_in,
+                                          lds_f64* Pp, lds_f64* Pl, lds_f64* Tl, lds_f64* Yl, lds_f64* tl,
+                                          lds_f64* Vt) {
   const int tid = threadIdx.x, bs = blockDim.x;
+  const int rows = uni(rows_in), k = uni(k_in), ld = uni(ld_in), bx = uni(bx_in), n = uni(n_in), m = uni(m_in);
   const int ncol = Jb ? m : k;
   for (int e = tid; e < rows * ncol; e += bs) {
     const int r = e / ncol, c = e - r * ncol;
@@ -1010,11 +1117,12 @@ __device__ __noinline__ void q_form_panel(const double* W, int rows, int k, int 
     const int nb = k - p0 < PNB ? k - p0 : PNB, mr = rows - p0;
     for (int e = tid; e < mr * PNB; e += bs) {
       const int r = e / PNB, c = e - r * PNB;
-      Pl[e] = c < nb ? W[(int64_t)(p0 + r) * ld + p0 + c] : 0.0;
+      Pp[r * PLD + c] = c < nb ? W[(int64_t)(p0 + r) * ld + p0 + c] : 0.0;
     }
     for (int e = tid; e < nb; e += bs) tl[e] = tau[p0 + e];
     __syncthreads();
-    build_T(Pl, mr, nb, tl, Tl, Yl);
+    panel_pad(Pp, mr, nb, tl, Pl, Vt);
+    build_T(Pp, mr, nb, tl, Tl, Yl);
     // Q: only columns >= p0 differ from the identity yet; U: every column
     const int c0 = Jb ? 0 : p0;
     for (int c = c0 + tid; c < ncol; c += bs) {
@@ -1023,7 +1131,7 @@ __device__ __noinline__ void q_form_panel(const double* W, int rows, int k, int 
             const int rr = p0 + r, aa = rr / bx;
             return Xc[((int64_t)aa * ncol + c) * bx + (rr - aa * bx)];
           },
-          mr, nb, Pl, Tl, false);
+          mr, Pl, Vt, Tl, false);
     }
     __syncthreads();
   }
@@ -1149,15 +1257,16 @@ __device__ __forceinline__ int run_move(double* ws, double* jpad, lds_f64* pnl, 
       else if (rpl == 16) q_form_reg<8, 16>(Wl, rows, k, taul, Xc, bx);
       else q_form_reg<16, 12>(Wl, rows, k, taul, Xc, bx);
     } else if (pnl) {  // workspace in global memory: blocked QR, panels in LDS
-      lds_f64 *Pl = pnl, *Tl = pnl + rows * PNB, *Yl = Tl + PNB * PNB, *tl = Yl + PNB * PNB;
-      qr_panel(W, rows, b, b, tau, Pl, Tl, Yl, tl);
+      lds_f64 *Pp = pnl, *Pl = Pp + ((rows * PLD + 1) & ~1), *Tl = Pl + rows * PNB, *Yl = Tl + PNB * PNB, *tl = Yl + PNB * PNB,
+              *Vt = tl + PNB;
+      qr_panel(W, rows, b, b, tau, Pp, Pl, Tl, Yl, tl, Vt);
       for (int e = tid; e < k * b; e += bs) {
         const int i = e / b, c = e - i * b;
         Rb[e] = c >= i ? W[(int64_t)i * b + c] : 0.0;
       }
       __syncthreads();
       QTT_PHASE(-1, 2);  // factored
-      q_form_panel(W, rows, k, b, tau, Xc, bx, nullptr, 0, nullptr, 0, Pl, Tl, Yl, tl);  // x <- Q (bond m)
+      q_form_panel(W, rows, k, b, tau, Xc, bx, nullptr, 0, nullptr, 0, Pp, Pl, Tl, Yl, tl, Vt);  // x <- Q (bond m)
     } else {
       geqr2<G>(W, rows, b, b, tau);
       for (int e = tid; e < k * b; e += bs) {
@@ -1185,8 +1294,9 @@ __device__ __forceinline__ int run_move(double* ws, double* jpad, lds_f64* pnl, 
         else if (rpl == 16) qr_reg<8, 16, 2>(Wl, rows, b, taul, Rl);
         else qr_reg<16, 12, 3>(Wl, rows, b, taul, Rl);
       } else if (pnl) {  // blocked QR; the reflectors stay in W for U below
-        lds_f64 *Pl = pnl, *Tl = pnl + rows * PNB, *Yl = Tl + PNB * PNB, *tl = Yl + PNB * PNB;
-        qr_panel(W, rows, b, b, tau, Pl, Tl, Yl, tl);
+        lds_f64 *Pp = pnl, *Pl = Pp + ((rows * PLD + 1) & ~1), *Tl = Pl + rows * PNB, *Yl = Tl + PNB * PNB, *tl = Yl + PNB * PNB,
+              *Vt = tl + PNB;
+        qr_panel(W, rows, b, b, tau, Pp, Pl, Tl, Yl, tl, Vt);
         for (int e = tid; e < b * b; e += bs) {
           const int i = e / b, c = e - i * b;
           Rb[e] = c >= i ? W[(int64_t)i * b + c] : 0.0;
@@ -1279,8 +1389,9 @@ __device__ __forceinline__ int run_move(double* ws, double* jpad, lds_f64* pnl, 
     use_perm = true;
     // x <- U (bond m): U[row][i] = sum_j Q[row][j] J[perm[i]][j]  (or J[perm[i]][row])
     if (tall && pnl) {
-      lds_f64 *Pl = pnl, *Tl = pnl + rows * PNB, *Yl = Tl + PNB * PNB, *tl = Yl + PNB * PNB;
-      q_form_panel(W, rows, b, b, tau, Xc, bx, Jb, n, perm, m, Pl, Tl, Yl, tl);
+      lds_f64 *Pp = pnl, *Pl = Pp + ((rows * PLD + 1) & ~1), *Tl = Pl + rows * PNB, *Yl = Tl + PNB * PNB, *tl = Yl + PNB * PNB,
+              *Vt = tl + PNB;
+      q_form_panel(W, rows, b, b, tau, Xc, bx, Jb, n, perm, m, Pp, Pl, Tl, Yl, tl, Vt);
     } else if (tall && use_reg) {
       const lds_f64 *Wl = (const lds_f64*)W, *taul = (const lds_f64*)tau, *Jl = (const lds_f64*)Jb;
       const lds_i32* pl = (const lds_i32*)perm;
@@ -1368,7 +1479,8 @@ __global__ void __launch_bounds__(512) truncate_kernel(const SweepArgs* __restri
     }
     // global workspace: LDS holds the QR panels (rows x PNB, T, Y, tau)
     lds_f64* pnl = nullptr;
-    if (need > a.ws_lds && (int64_t)rows * PNB + 2 * PNB * PNB + PNB <= a.ws_lds) pnl = (lds_f64*)wsl;
+    if (need > a.ws_lds && (int64_t)rows * (PNB + PLD) + 3 * PNB * PNB + PNB + 8 <= a.ws_lds)
+      pnl = (lds_f64*)(lds + ((64 + ((t.nv + 1) >> 1) + 1) & ~1));  // 16-byte aligned for the row loads
     // padded copy for the Jacobi rounds of an SVD move, when it fits after the workspace
     double* jpad = nullptr;
     if (tr) {
