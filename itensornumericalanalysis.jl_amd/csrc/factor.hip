@@ -964,14 +964,15 @@ __device__ __forceinline__ void wy_column(At at, int mr, const lds_f64* V, const
 #pragma unroll
     for (int j = 0; j < PNB; ++j) w[j] = __builtin_fma(v[j], a, w[j]);
   }
-  // the column lives in global memory: eight of its entries in flight per wait
+  // the column lives in global memory: CB of its entries in flight per wait
+  constexpr int CB = 8;
   int r = top;
-  for (; r + 8 <= mr; r += 8) {
-    double a[8];
+  for (; r + CB <= mr; r += CB) {
+    double a[CB];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) a[q] = at(r + q);
+    for (int q = 0; q < CB; ++q) a[q] = at(r + q);
 #pragma unroll 2
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < CB; ++q) {
       double v[PNB];
       vrow(V, r + q, v);
 #pragma unroll
@@ -1007,12 +1008,12 @@ __device__ __forceinline__ void wy_column(At at, int mr, const lds_f64* V, const
     at(r) -= acc;
   }
   r = top;
-  for (; r + 8 <= mr; r += 8) {
-    double a[8];
+  for (; r + CB <= mr; r += CB) {
+    double a[CB];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) a[q] = at(r + q);
+    for (int q = 0; q < CB; ++q) a[q] = at(r + q);
 #pragma unroll 2
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < CB; ++q) {
       double v[PNB];
       vrow(V, r + q, v);
       double acc0 = 0.0, acc1 = 0.0;
@@ -1203,12 +1204,73 @@ __device__ __forceinline__ void mode_product(const double* Rrows, const int* per
   mode_product_body(Rrows, perm, use_perm, m, b, Uc, gY, Au, nu, Bu);
 }
 
+// Mode product with R in global memory (the global-workspace path): R is
+// staged in LDS RB rows at a time, transposed (Rs[c][q], RB doubles per c), so
+// each step reads RB entries of R with 16-byte broadcast loads; one thread per
+// (aa, bb) column keeps RB sums and reads its column of u once per block.
+constexpr int RB = 32;
+
+__device__ __noinline__ void mode_product_staged(const double* Rrows, const int* perm, bool use_perm, int m_in,
+                                                 int b_in, const double* Uc, double* gY, int Au_in, int nu_in,
+                                                 int Bu_in, lds_f64* Rs) {
+  const int tid = threadIdx.x, bs = blockDim.x;
+  const int m = uni(m_in), b = uni(b_in), Au = uni(Au_in), nu = uni(nu_in), Bu = uni(Bu_in);
+  const int ncol = Au * Bu;
+  for (int i0 = 0; i0 < m; i0 += RB) {
+    const int rb = m - i0 < RB ? m - i0 : RB;
+    for (int e = tid; e < RB * nu; e += bs) {
+      const int q = e / nu, c = e - q * nu;
+      double v = 0.0;
+      if (q < rb) v = Rrows[(int64_t)(use_perm ? perm[i0 + q] : i0 + q) * b + c];
+      Rs[c * RB + q] = v;
+    }
+    __syncthreads();
+    for (int j = tid; j < ncol; j += bs) {
+      const int aa = j / Bu, bb = j - aa * Bu;
+      const double* up = Uc + (int64_t)aa * nu * Bu + bb;
+      double acc[RB];
+#pragma unroll
+      for (int q = 0; q < RB; ++q) acc[q] = 0.0;
+      int c = 0;
+      for (; c + 4 <= nu; c += 4) {
+        double uv[4];
+#pragma unroll
+        for (int q2 = 0; q2 < 4; ++q2) uv[q2] = up[(int64_t)(c + q2) * Bu];
+#pragma unroll
+        for (int q2 = 0; q2 < 4; ++q2) {
+          const lds_f64x2* rp = (const lds_f64x2*)(Rs + (c + q2) * RB);
+#pragma unroll
+          for (int q = 0; q < RB / 2; ++q) {
+            const f64x2 r2 = rp[q];
+            acc[2 * q] = __builtin_fma(r2.x, uv[q2], acc[2 * q]);
+            acc[2 * q + 1] = __builtin_fma(r2.y, uv[q2], acc[2 * q + 1]);
+          }
+        }
+      }
+      for (; c < nu; ++c) {
+        const double u = up[(int64_t)c * Bu];
+        const lds_f64x2* rp = (const lds_f64x2*)(Rs + c * RB);
+#pragma unroll
+        for (int q = 0; q < RB / 2; ++q) {
+          const f64x2 r2 = rp[q];
+          acc[2 * q] = __builtin_fma(r2.x, u, acc[2 * q]);
+          acc[2 * q + 1] = __builtin_fma(r2.y, u, acc[2 * q + 1]);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < RB; ++q)
+        if (q < rb) gY[((int64_t)aa * m + i0 + q) * Bu + bb] = acc[q];
+    }
+    __syncthreads();
+  }
+}
+
 // One move of the sweep, with the workspace `ws` either in LDS or in global
 // scratch.  Two separate instantiations (one per memory) let the compiler use
 // LDS instructions on the LDS path instead of generic flat addressing.
 template <int G, bool REG>
-__device__ __forceinline__ int run_move(double* ws, double* jpad, lds_f64* pnl, const int rows, const int b, const int tr,
-                                        const int rpl, const int bx,
+__device__ __forceinline__ int run_move(double* ws, double* jpad, lds_f64* pnl, const int pnl_cap, const int rows,
+                                        const int b, const int tr, const int rpl, const int bx,
                                         const int64_t Au, const int64_t nu, const int64_t Bu, double* Xc,
                                         const double* Uc, double* Ud, double* gY, double cutoff, long long maxdim,
                                         int* status, int* iflag, double* red, double* dbc) {
@@ -1421,6 +1483,8 @@ __device__ __forceinline__ int run_move(double* ws, double* jpad, lds_f64* pnl, 
   const int ytot = (int)(Au * m * Bu);
   if (REG)
     mode_product_lds((const lds_f64*)Rrows, (const lds_i32*)perm, use_perm, m, b, Uc, gY, (int)Au, (int)nu, (int)Bu);
+  else if (pnl && (int64_t)nu * RB <= pnl_cap)  // global workspace: stage R in LDS
+    mode_product_staged(Rrows, perm, use_perm, m, b, Uc, gY, (int)Au, (int)nu, (int)Bu, pnl);
   else
     mode_product(Rrows, perm, use_perm, m, b, Uc, gY, (int)Au, (int)nu, (int)Bu);
   __syncthreads();
@@ -1479,8 +1543,12 @@ __global__ void __launch_bounds__(512) truncate_kernel(const SweepArgs* __restri
     }
     // global workspace: LDS holds the QR panels (rows x PNB, T, Y, tau)
     lds_f64* pnl = nullptr;
-    if (need > a.ws_lds && (int64_t)rows * (PNB + PLD) + 3 * PNB * PNB + PNB + 8 <= a.ws_lds)
-      pnl = (lds_f64*)(lds + ((64 + ((t.nv + 1) >> 1) + 1) & ~1));  // 16-byte aligned for the row loads
+    int pnl_cap = 0;  // doubles of LDS at pnl
+    if (need > a.ws_lds && (int64_t)rows * (PNB + PLD) + 3 * PNB * PNB + PNB + 8 <= a.ws_lds) {
+      const int off = (64 + ((t.nv + 1) >> 1) + 1) & ~1;  // 16-byte aligned for the row loads
+      pnl = (lds_f64*)(lds + off);
+      pnl_cap = QTT_LDS_DOUBLES - off;
+    }
     // padded copy for the Jacobi rounds of an SVD move, when it fits after the workspace
     double* jpad = nullptr;
     if (tr) {
@@ -1493,17 +1561,17 @@ __global__ void __launch_bounds__(512) truncate_kernel(const SweepArgs* __restri
     int m;
     if (need <= a.ws_lds) {
       if (small)
-        m = run_move<8, true>(wsl, jpad, nullptr, rows, b, tr, rpl, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<8, true>(wsl, jpad, nullptr, 0, rows, b, tr, rpl, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
                         iflag, red, dbc);
       else
-        m = run_move<64, false>(wsl, nullptr, nullptr, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<64, false>(wsl, nullptr, nullptr, 0, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
                          iflag, red, dbc);
     } else {
       if (small)
-        m = run_move<8, false>(a.gscr, nullptr, pnl, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<8, false>(a.gscr, nullptr, pnl, pnl_cap, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
                         iflag, red, dbc);
       else
-        m = run_move<64, false>(a.gscr, nullptr, pnl, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<64, false>(a.gscr, nullptr, pnl, pnl_cap, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
                          iflag, red, dbc);
     }
     QTT_PHASE(st, 5);
