@@ -114,7 +114,9 @@ def evaluate(f: ITensorNetworkFunction, xs, dims=None, precision: int = Precisio
 def _sigma(f: ITensorNetworkFunction, take_sum: bool):
     """The uniform product state used by integrate(op, f) (src/integration.jl:30)."""
     c = 1.0 if take_sum else 1.0 / f.s.base
-    cores = [np.full(f.topo.core_shape(v, np.ones(f.topo.nv, dtype=np.int64)), c)
+    # a vertex without a digit has no site index to sum over
+    cores = [np.full(f.topo.core_shape(v, np.ones(f.topo.nv, dtype=np.int64)),
+                     c if f.topo.sdim[v] > 1 else 1.0)
              for v in range(f.topo.nv)]
     return ITensorNetworkFunction(f.s, cores, f.topo)
 

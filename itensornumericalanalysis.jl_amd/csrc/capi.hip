@@ -256,6 +256,7 @@ void qtt_ctx_destroy(qtt_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   if (ctx->d_scratch) (void)hipFree(ctx->d_scratch);
+  if (ctx->d_sync) (void)hipFree(ctx->d_sync);
   if (ctx->d_flag) (void)hipFree(ctx->d_flag);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
   delete ctx;

@@ -38,6 +38,9 @@ extern "C" int qtt_debug_times(unsigned long long* out, int n) {
 }
 #define QTT_PHASE(step, phase) \
   do { if (threadIdx.x == 0) qtt_phase_mark((step), (phase), wall_clock64()); } while (0)
+// the move's shape in the two spare slots of its record
+#define QTT_SHAPE(step, rows, b, tr) \
+  do { if (threadIdx.x == 0 && (step) < 8192) { qtt_phase_t[(step) * 8 + 6] = (rows); qtt_phase_t[(step) * 8 + 7] = (b) | ((tr) << 20); } } while (0)
 // cycles of lane 0 per segment of a Jacobi round, plus round / sweep counts
 __device__ unsigned long long qtt_jac_t[8];
 extern "C" int qtt_debug_jacobi(unsigned long long* out) {
@@ -69,6 +72,7 @@ extern "C" int qtt_debug_panel(unsigned long long* out) {
 #define QTT_PT(i) do { } while (0)
 #define QTT_PANEL_COUNT (void)0
 #define QTT_PHASE(step, phase) do { } while (0)
+#define QTT_SHAPE(step, rows, b, tr) do { } while (0)
 #define QTT_JT_INIT do { } while (0)
 #define QTT_JT(i) do { } while (0)
 #define QTT_JC(i) do { } while (0)
@@ -1526,6 +1530,7 @@ __global__ void __launch_bounds__(512) truncate_kernel(const SweepArgs* __restri
     const int64_t Bx = sx[2], Au = su[0], nu = su[1], Bu = su[2];
     const int rows = sx[0] * sx[2], b = sx[1];
     const int k = rows < b ? rows : b;
+    QTT_SHAPE(st, rows, b, tr);
     const int64_t jsz = tr ? (int64_t)k * k : 0;
     const bool small = rows <= 256 && b <= 256;
     // register QR: rows <= 64 or <= 128 with columns <= 2 per 8-lane group, or

@@ -20,6 +20,8 @@ struct qtt_ctx {
   int* d_flag = nullptr;  // [0] = invalid coordinate seen, [1] = SVD non-convergence
   double* d_scratch = nullptr;
   size_t scratch_bytes = 0;
+  int* d_sync = nullptr;  // counters of the dataflow kernels, zero between launches
+  int sync_cap = 0;
   int cus = 256;
   bool force_lds_eval = false;  // QTT_EVAL_KERNEL=lds selects the LDS-vector chain kernel
   int eval_ppl = 0;             // points per lane of the register chain kernel (0: by bond size)

@@ -8,6 +8,9 @@
 // products (one child at a time) followed by a row-by-row dot product.
 #include "qtt_internal.h"
 
+#include <algorithm>
+#include <cstring>
+
 namespace {
 
 __device__ inline double wave_sum(double v) {
@@ -17,7 +20,8 @@ __device__ inline double wave_sum(double v) {
 }
 
 // Post-order contraction of all cores with the site weight, one workgroup.
-// Wave per output row p, lanes over the flattened child index j.
+// Wave per output row p, lanes over the flattened child index j.  The weight
+// belongs to a site index, so a vertex without a digit (sdim 1) takes none.
 __device__ void integrate_body(const double* __restrict__ data, const int64_t* __restrict__ off,
                                const int* __restrict__ bond, const TopoView& t, double weight,
                                const int64_t* __restrict__ voff, double* vec, double* out);
@@ -69,11 +73,248 @@ __device__ void integrate_body(const double* __restrict__ data, const int64_t* _
         acc += f;
       }
       acc = wave_sum(acc);
-      if (lane == 0) vec[voff[v] + pp] = weight * acc;
+      if (lane == 0) vec[voff[v] + pp] = (sd > 1 ? weight : 1.0) * acc;
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) out[0] = vec[voff[0]];
+}
+
+// ---------------------------------------------------------------------------
+// Dataflow integrate over many workgroups (networks of many MB: the 3D comb
+// with bond 128 holds 45 MB of cores).
+//
+// A unit is one row segment of a vertex's core folded over its site:
+// F[j] = sum_s A_v[p][s][j] for j in [ci*FLOW_CL, (ci+1)*FLOW_CL) of the
+// flattened child index.  Its partial sum sum_j F[j] prod_k vec_k[c_k(j)] goes
+// to part[poff_v + p*nchunk_v + ci]; vec_v[p] is the sum of row p's partials
+// times the site weight.  A task is up to FLOW_UPT consecutive units of one
+// vertex, run by one workgroup of four wavefronts (two units each).
+//
+// Every task first loads and folds its core segment into registers, then
+// waits until the child vertices' unit counters are complete, so the HBM
+// stream of all cores overlaps the dependency chain from the leaves to the
+// root.  Tasks are handed out by an atomic ticket in the order of the task
+// table, where every vertex comes after its children; a task therefore only
+// waits on tasks that were handed out earlier to running workgroups, and the
+// waits always end.  A wait is also bounded by FLOW_SPIN_LIMIT polls (it sets
+// an error flag and carries on), so every wave exits in any case.  The
+// workgroup that completes the root zeroes the counters for the next launch.
+// ---------------------------------------------------------------------------
+// Per-task timestamps for profiling builds (-DQTT_PHASE_TIMING): ticket taken,
+// segments loaded, children ready, published; read back with qtt_debug_flow.
+#ifdef QTT_PHASE_TIMING
+__device__ unsigned long long qtt_flow_t[8192 * 8];
+extern "C" int qtt_debug_flow(unsigned long long* out, int n) {
+  return (int)hipMemcpyFromSymbol(out, HIP_SYMBOL(qtt_flow_t), (size_t)n * 8 * sizeof(unsigned long long));
+}
+#define QTT_FLOW_T(task, i) \
+  do { if (threadIdx.x == 0 && (task) < 8192) qtt_flow_t[(task) * 8 + (i)] = wall_clock64(); } while (0)
+#else
+#define QTT_FLOW_T(task, i) do { } while (0)
+#endif
+
+constexpr int FLOW_EPL = 16;                  // elements per lane per unit
+constexpr int FLOW_CL = 64 * FLOW_EPL;        // unit length (flattened child index)
+constexpr int FLOW_UPW = 2;                   // units per wavefront
+constexpr int FLOW_UPT = 4 * FLOW_UPW;        // units per task
+constexpr int FLOW_SPIN_LIMIT = 1 << 22;      // polls of a child counter (about 0.2 s)
+
+struct FlowTask {
+  int v, u0, nu, pad;
+};
+
+struct FlowArgs {
+  const double* data;
+  const int64_t* off;
+  const int* bond;
+  TopoView t;
+  const FlowTask* tasks;
+  const int64_t* vinfo;  // per vertex: poff, pc, nchunk, units
+  double* part;
+  int* sync;  // [0] ticket, [2] wait limit hit, [4 + v] units done of vertex v
+  double* out;  // [0] integral, [1] wait-limit flag
+  double weight;
+  int ntasks;
+};
+
+__global__ void __launch_bounds__(256) integrate_flow_kernel(FlowArgs a) {
+  extern __shared__ double s_ch[];  // child vectors, one after the other
+  __shared__ int s_i[2];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  if (tid == 0) s_i[0] = __hip_atomic_fetch_add(a.sync, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  const int ti = __builtin_amdgcn_readfirstlane(s_i[0]);
+  if (ti >= a.ntasks) return;  // the grid has exactly ntasks workgroups
+  QTT_FLOW_T(ti, 0);
+  const FlowTask tk = a.tasks[ti];
+  const int v = tk.v;
+  const int64_t poff = a.vinfo[4 * v];
+  const int pc = (int)a.vinfo[4 * v + 1], nchunk = (int)a.vinfo[4 * v + 2];
+  const int sd = a.t.sdim[v];
+  const double* core = a.data + a.off[v];
+  // 1. this task's core segments, folded over the site, into registers
+  double F[FLOW_UPW][FLOW_EPL];
+#pragma unroll
+  for (int i = 0; i < FLOW_UPW; ++i) {
+    const int u = tk.u0 + wid + 4 * i;
+    const bool on = u < tk.u0 + tk.nu;
+    const int p = on ? u / nchunk : 0, ci = on ? u - p * nchunk : 0;
+    const double* row = core + (int64_t)p * sd * pc;
+    const int j0 = ci * FLOW_CL + lane;
+    if (sd == 2) {
+#pragma unroll
+      for (int e = 0; e < FLOW_EPL; ++e) {
+        const int j = j0 + 64 * e;
+        F[i][e] = (on && j < pc) ? row[j] + row[(int64_t)pc + j] : 0.0;
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < FLOW_EPL; ++e) {
+        const int j = j0 + 64 * e;
+        double f = 0.0;
+        if (on && j < pc)
+          for (int s = 0; s < sd; ++s) f += row[(int64_t)s * pc + j];
+        F[i][e] = f;
+      }
+    }
+  }
+  // 2. wait for the children
+  const int m = a.t.nch[v];
+  const int* ch = a.t.chs + a.t.choff[v];
+#ifdef QTT_PHASE_TIMING
+  {  // the loads have landed
+    double z = 0.0;
+#pragma unroll
+    for (int i = 0; i < FLOW_UPW; ++i)
+#pragma unroll
+      for (int e = 0; e < FLOW_EPL; ++e) z += F[i][e];
+    if (z == 12345.678) a.out[1] = z;
+    QTT_FLOW_T(ti, 1);
+  }
+#endif
+  if (tid == 0) {
+    for (int k = 0; k < m; ++k) {
+      const int c = ch[k];
+      const int need = (int)a.vinfo[4 * c + 3];
+      int spins = 0, nap = 0;
+      while (__hip_atomic_load(a.sync + 4 + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+        // back off (64 to 1024 cycles between polls) to keep the polling of
+        // hundreds of waiting workgroups off the memory system
+        if (nap < 4) __builtin_amdgcn_s_sleep(1);
+        else if (nap < 12) __builtin_amdgcn_s_sleep(4);
+        else __builtin_amdgcn_s_sleep(16);
+        ++nap;
+        if (++spins > FLOW_SPIN_LIMIT) {
+          __hip_atomic_store(a.sync + 2, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  QTT_FLOW_T(ti, 2);
+  // 3. the child vectors into LDS: vec_c[x] = w_c * sum of row x's partials
+  int lo = 0;
+  for (int k = 0; k < m; ++k) {
+    const int c = ch[k], bc = a.bond[c];
+    const int64_t pof = a.vinfo[4 * c];
+    const int nck = (int)a.vinfo[4 * c + 2];
+    const double wc = a.t.sdim[c] > 1 ? a.weight : 1.0;
+    for (int x = tid; x < bc; x += 256) {
+      double s = 0.0;
+      for (int q = 0; q < nck; ++q) s += a.part[pof + (int64_t)x * nck + q];
+      s_ch[lo + x] = wc * s;
+    }
+    lo += bc;
+  }
+  __syncthreads();
+  QTT_FLOW_T(ti, 4);
+  // 4. contract the folded segments with the child vectors
+  const int b1 = m > 0 ? a.bond[ch[m - 1]] : 1;  // fastest child index
+  const int b0 = m > 1 ? a.bond[ch[m - 2]] : 1;
+  const int lo1 = lo - b1, lo0 = lo1 - b0;
+#pragma unroll
+  for (int i = 0; i < FLOW_UPW; ++i) {
+    const int u = tk.u0 + wid + 4 * i;
+    const bool on = u < tk.u0 + tk.nu;
+    const int p = on ? u / nchunk : 0, ci = on ? u - p * nchunk : 0;
+    (void)p;
+    const int j0 = ci * FLOW_CL + lane;
+    double acc = 0.0;
+    if (on) {
+      if (m == 0) {
+#pragma unroll
+        for (int e = 0; e < FLOW_EPL; ++e) acc += F[i][e];
+      } else if (m == 1) {
+#pragma unroll
+        for (int e = 0; e < FLOW_EPL; ++e) {
+          const int j = j0 + 64 * e;
+          if (j < pc) acc += F[i][e] * s_ch[lo1 + j];
+        }
+      } else if (m == 2) {
+        int c0 = j0 / b1, c1 = j0 - c0 * b1;
+#pragma unroll
+        for (int e = 0; e < FLOW_EPL; ++e) {
+          const int j = j0 + 64 * e;
+          if (j < pc) acc += F[i][e] * s_ch[lo0 + c0] * s_ch[lo1 + c1];
+          c1 += 64;  // advance (c0, c1) by 64 without a division
+          while (c1 >= b1) {
+            c1 -= b1;
+            ++c0;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < FLOW_EPL; ++e) {
+          const int j = j0 + 64 * e;
+          if (j < pc) {
+            double w = 1.0;
+            int r = j, o = lo;
+            for (int k = m - 1; k >= 0; --k) {
+              const int bk = a.bond[ch[k]];
+              o -= bk;
+              const int q = r / bk;
+              w *= s_ch[o + (r - q * bk)];
+              r = q;
+            }
+            acc += F[i][e] * w;
+          }
+        }
+      }
+    }
+    acc = wave_sum(acc);
+    if (on && lane == 0) a.part[poff + u] = acc;
+  }
+  // 5. publish: the barrier orders every wave's partials before thread 0's
+  // release, then the units are counted
+  QTT_FLOW_T(ti, 5);
+  __syncthreads();
+  QTT_FLOW_T(ti, 6);
+  if (tid == 0) {
+    const int prev = __hip_atomic_fetch_add(a.sync + 4 + v, tk.nu, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    s_i[1] = v == 0 && prev + tk.nu == (int)a.vinfo[3];
+    if (s_i[1]) {  // the root (bond 1) is complete
+      double s = 0.0;
+      for (int q = 0; q < nchunk; ++q) s += a.part[poff + q];
+      a.out[0] = (sd > 1 ? a.weight : 1.0) * s;
+      a.out[1] = (double)__hip_atomic_load(a.sync + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    QTT_FLOW_T(ti, 3);
+  }
+  __syncthreads();
+  // Every task counts its units after its own waits, so once the root is
+  // complete no workgroup waits any more (every ticket is taken, every wait
+  // has ended): zero the counters for the next launch.  (After a wait that hit
+  // its limit the host zeroes them instead.)
+  if (s_i[1]) {
+    for (int x = tid; x < a.t.nv; x += 256) a.sync[4 + x] = 0;
+    if (tid == 0) {
+      a.sync[0] = 0;
+      a.sync[2] = 0;
+    }
+  }
 }
 
 // Two-layer environment sweep for <a|b>, one workgroup.
@@ -138,8 +379,103 @@ inner_tree_kernel(const double* __restrict__ A, const int64_t* __restrict__ aoff
 
 }  // namespace
 
+// Counters of the dataflow kernels: zeroed once when allocated; every launch
+// leaves them zero again.
+static int* sync_counters(qtt_ctx* ctx, int n) {
+  if (n <= ctx->sync_cap) return ctx->d_sync;
+  if (ctx->d_sync) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(ctx->d_sync);
+    ctx->d_sync = nullptr;
+    ctx->sync_cap = 0;
+  }
+  const int cap = std::max(n, 4096);
+  if (hipMalloc(&ctx->d_sync, cap * sizeof(int)) != hipSuccess ||
+      hipMemsetAsync(ctx->d_sync, 0, cap * sizeof(int), ctx->stream) != hipSuccess) {
+    (void)hipGetLastError();
+    if (ctx->d_sync) (void)hipFree(ctx->d_sync);
+    ctx->d_sync = nullptr;
+    return nullptr;
+  }
+  ctx->sync_cap = cap;
+  return ctx->d_sync;
+}
+
+// Networks from this many core elements up use the dataflow kernel.
+constexpr int64_t FLOW_MIN_ELEMS = int64_t(1) << 15;
+
+static int launch_integrate_flow(qtt_ctx* ctx, const qtt_net* net, double weight, double* out_host, bool* used) {
+  *used = false;
+  const HostTopo& t = net->t;
+  std::vector<int64_t> info(4 * (size_t)t.nv);
+  std::vector<FlowTask> tasks;
+  int64_t ptot = 0, maxch = 0;
+  for (int v = t.nv - 1; v >= 0; --v) {  // children (larger ids) before parents
+    int64_t pc = 1, chs = 0;
+    for (int k = 0; k < t.nch[v]; k++) {
+      const int64_t b = net->bond[t.chs[t.choff[v] + k]];
+      pc *= b;
+      chs += b;
+    }
+    const int64_t nchunk = (pc + FLOW_CL - 1) / FLOW_CL;
+    const int64_t units = net->bond[v] * nchunk;
+    if (pc >= (int64_t(1) << 30) || units >= (int64_t(1) << 30)) return QTT_OK;
+    maxch = std::max(maxch, chs);
+    info[4 * v] = ptot;
+    info[4 * v + 1] = pc;
+    info[4 * v + 2] = nchunk;
+    info[4 * v + 3] = units;
+    ptot += units;
+    for (int64_t u0 = 0; u0 < units; u0 += FLOW_UPT)
+      tasks.push_back({v, (int)u0, (int)std::min<int64_t>(FLOW_UPT, units - u0), 0});
+  }
+  if (maxch * sizeof(double) > 64 * 1024 || tasks.size() >= (size_t(1) << 30)) return QTT_OK;
+  int* sync = sync_counters(ctx, t.nv + 4);
+  if (!sync) return qtt_fail(ctx, QTT_ENOMEM, "integrate counters");
+  const size_t tab = info.size() * sizeof(int64_t) + tasks.size() * sizeof(FlowTask);
+  double* scr = qtt_scratch(ctx, (2 + ptot) * sizeof(double) + tab + 64);
+  if (!scr) return qtt_fail(ctx, QTT_ENOMEM, "integrate scratch");
+  *used = true;
+  std::vector<char> h(tab);
+  std::memcpy(h.data(), info.data(), info.size() * sizeof(int64_t));
+  std::memcpy(h.data() + info.size() * sizeof(int64_t), tasks.data(), tasks.size() * sizeof(FlowTask));
+  FlowArgs a;
+  a.data = net->d_data;
+  a.off = net->d_off;
+  a.bond = net->d_bond;
+  a.t = net->view();
+  a.out = scr;
+  a.part = scr + 2;
+  a.vinfo = (const int64_t*)(a.part + ptot);
+  a.tasks = (const FlowTask*)(a.vinfo + info.size());
+  a.sync = sync;
+  a.weight = weight;
+  a.ntasks = (int)tasks.size();
+  int rc = qtt_hip_check(ctx, hipMemcpyAsync((void*)a.vinfo, h.data(), tab, hipMemcpyHostToDevice, ctx->stream), "integrate tables");
+  if (rc) return rc;
+  integrate_flow_kernel<<<a.ntasks, 256, maxch * sizeof(double), ctx->stream>>>(a);
+  rc = qtt_hip_check(ctx, hipGetLastError(), "integrate_flow_kernel");
+  if (rc) return rc;
+  double r[2];
+  rc = qtt_hip_check(ctx, hipMemcpyAsync(r, scr, sizeof(r), hipMemcpyDeviceToHost, ctx->stream), "integral");
+  if (!rc) rc = qtt_hip_check(ctx, hipStreamSynchronize(ctx->stream), "integrate sync");
+  if (rc) return rc;
+  if (r[1] != 0.0) {
+    (void)hipMemsetAsync(sync, 0, (t.nv + 4) * sizeof(int), ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream);
+    return qtt_fail(ctx, QTT_EHIP, "integrate: a dependency wait exceeded its limit");
+  }
+  *out_host = r[0];
+  return QTT_OK;
+}
+
 int launch_integrate(qtt_ctx* ctx, const qtt_net* net, double weight, double* out_host) {
   const HostTopo& t = net->t;
+  if (net->total >= FLOW_MIN_ELEMS) {
+    bool used = false;
+    const int rc = launch_integrate_flow(ctx, net, weight, out_host, &used);
+    if (rc || used) return rc;
+  }
   std::vector<int64_t> voff(t.nv);
   int64_t tot = 0;
   for (int v = 0; v < t.nv; v++) {

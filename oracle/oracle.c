@@ -240,11 +240,13 @@ int or_integrate(int nv, const int* parent, const int* sdim, const long* bond,
     const int* ch = tree_ch(&t, v);
     long pc = 1;
     for (int k = 0; k < m; k++) pc *= bond[ch[k]];
+    /* the weight is per site index: a vertex without a digit has none */
+    const double wv = sdim[v] > 1 ? weight : 1.0;
     for (long p = 0; p < bond[v]; p++) {
       for (long j = 0; j < pc; j++) {
         double s = 0.0;
         for (int si = 0; si < sdim[v]; si++) s += cores[v][(p * sdim[v] + si) * pc + j];
-        fold[j] = weight * s;
+        fold[j] = wv * s;
       }
       vec[voff[v] + p] = contract_children(fold, m, ch, bond, vec, voff);
     }

@@ -263,6 +263,40 @@ def test_comb_operate_and_integrate(q, po, eng, rng):
     assert q.integrate(g) == pytest.approx(po.integrate(t.parent, t.sdim, gref, 0.5), rel=1e-11)
 
 
+@pytest.mark.parametrize("case", ["comb", "star", "no_digit"])
+def test_integrate_dataflow(q, po, eng, rng, case):
+    # networks of >= 2^15 core elements take the many-workgroup dataflow kernel
+    if case == "comb":
+        s = q.continuous_siteinds(q.named_comb_tree((3, 6)))
+        bond = 24
+    elif case == "star":  # a digit-less centre with four arms: a vertex with four children
+        verts = ["c"] + [(a, j) for a in range(1, 5) for j in range(1, 6)]
+        edges = [("c", (a, 1)) for a in range(1, 5)] + [((a, j), (a, j + 1)) for a in range(1, 5)
+                                                         for j in range(1, 5)]
+        s = q.continuous_siteinds(q.NamedGraph(verts, edges), [[(a, j) for j in range(1, 6)] for a in range(1, 5)])
+        bond = 14
+    else:  # the third tooth carries no digits
+        s = q.continuous_siteinds(q.named_comb_tree((3, 6)), [[(1, j) for j in range(1, 7)],
+                                                              [(2, j) for j in range(1, 7)]])
+        bond = 24
+    f = q.random_itn(s, link_space=bond, rng=rng, std=1.0 / math.sqrt(bond))
+    assert sum(c.size for c in f.cores) >= 1 << 15
+    t = f.topo
+    for take_sum, wgt in ((False, 0.5), (True, 1.0)):
+        ref = po.integrate(t.parent, t.sdim, f.cores, wgt)
+        assert q.integrate(f, take_sum=take_sum) == pytest.approx(ref, rel=TOL)
+
+
+def test_comb_full_size_integrate(q, po, eng):
+    # 3 x 16 digits, bond 128: 45 MB of cores
+    from itensornumericalanalysis_jl_amd import workloads
+    w = workloads.comb_3d()
+    t = w.f.topo
+    ref = po.integrate(t.parent, t.sdim, w.f.cores, 0.5)
+    for _ in range(3):  # repeated launches: the counters are left zero by each launch
+        assert q.integrate(w.f) == pytest.approx(ref, rel=TOL)
+
+
 # ------------------------------------------------------------------ batches (config-5 shape, reduced)
 def test_operate_batch_and_integrate_batch(q, po, eng, rng):
     from itensornumericalanalysis_jl_amd import workloads

@@ -447,3 +447,15 @@ def test_reference_known_answers_file():
     assert {c["source"].split(":")[0] for c in cases} >= {
         "test/test_indexmaps.jl", "test/test_integration.jl", "test/test_realitensorfunction.jl",
         "test/test_operators.jl"}
+
+
+def test_integrate_vertices_without_digits(q, po, rng):
+    # a vertex that carries no digit has no site index to sum over, so it adds
+    # no 1/2 factor: the integral is the mean over the 2^L grid of the digits
+    g = q.named_comb_tree((3, 3))
+    s = q.continuous_siteinds(g, [[(1, j) for j in range(1, 4)], [(2, j) for j in range(1, 4)]])
+    f = q.random_itn(s, link_space=2, rng=rng)
+    assert list(f.topo.sdim).count(1) == 3
+    Tf = dense.dense_tensor(f.topo.parent, f.cores)
+    assert math.isclose(po.integrate(f.topo.parent, f.topo.sdim, f.cores, 0.5), Tf.mean(), rel_tol=1e-12)
+    assert math.isclose(po.integrate(f.topo.parent, f.topo.sdim, f.cores, 1.0), float(Tf.sum()), rel_tol=1e-12)
