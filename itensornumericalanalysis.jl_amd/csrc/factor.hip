@@ -943,18 +943,25 @@ __device__ void build_T(const lds_f64* V, int mr, int nb, const lds_f64* tau, ld
 typedef double f64x2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(3))) f64x2 lds_f64x2;
 
-// row r of a PNB-wide LDS matrix, two entries per 16-byte load
+// row r of an LDS matrix of row stride VLD: two entries per 16-byte load when
+// the rows are aligned (VLD = PNB), else one per 8-byte load (VLD = PLD)
+template <int VLD = PNB>
 __device__ __forceinline__ void vrow(const lds_f64* V, int r, double (&v)[PNB]) {
-  const lds_f64x2* p = (const lds_f64x2*)(V + r * PNB);
+  if constexpr (VLD == PNB) {
+    const lds_f64x2* p = (const lds_f64x2*)(V + r * PNB);
 #pragma unroll
-  for (int j = 0; j < PNB / 2; ++j) {
-    const f64x2 x = p[j];
-    v[2 * j] = x.x;
-    v[2 * j + 1] = x.y;
+    for (int j = 0; j < PNB / 2; ++j) {
+      const f64x2 x = p[j];
+      v[2 * j] = x.x;
+      v[2 * j + 1] = x.y;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < PNB; ++j) v[j] = V[r * VLD + j];
   }
 }
 
-template <typename At>
+template <int VLD, typename At>
 __device__ __forceinline__ void wy_column(At at, int mr, const lds_f64* V, const lds_f64* Vt, const lds_f64* T,
                                           bool trans) {
   double w[PNB];
@@ -978,7 +985,7 @@ __device__ __forceinline__ void wy_column(At at, int mr, const lds_f64* V, const
 #pragma unroll 2
     for (int q = 0; q < CB; ++q) {
       double v[PNB];
-      vrow(V, r + q, v);
+      vrow<VLD>(V, r + q, v);
 #pragma unroll
       for (int j = 0; j < PNB; ++j) w[j] = __builtin_fma(v[j], a[q], w[j]);
     }
@@ -986,7 +993,7 @@ __device__ __forceinline__ void wy_column(At at, int mr, const lds_f64* V, const
   for (; r < mr; ++r) {
     const double a = at(r);
     double v[PNB];
-    vrow(V, r, v);
+    vrow<VLD>(V, r, v);
 #pragma unroll
     for (int j = 0; j < PNB; ++j) w[j] = __builtin_fma(v[j], a, w[j]);
   }
@@ -1019,7 +1026,7 @@ __device__ __forceinline__ void wy_column(At at, int mr, const lds_f64* V, const
 #pragma unroll 2
     for (int q = 0; q < CB; ++q) {
       double v[PNB];
-      vrow(V, r + q, v);
+      vrow<VLD>(V, r + q, v);
       double acc0 = 0.0, acc1 = 0.0;
 #pragma unroll
       for (int j = 0; j < PNB; j += 2) {
@@ -1031,7 +1038,7 @@ __device__ __forceinline__ void wy_column(At at, int mr, const lds_f64* V, const
   }
   for (; r < mr; ++r) {
     double v[PNB];
-    vrow(V, r, v);
+    vrow<VLD>(V, r, v);
     double acc = 0.0;
 #pragma unroll
     for (int j = 0; j < PNB; ++j) acc = __builtin_fma(v[j], u[j], acc);
@@ -1043,23 +1050,28 @@ __device__ __forceinline__ void wy_column(At at, int mr, const lds_f64* V, const
 // zero) and their scalars in tl: zero the scalars of the padding columns,
 // copy the reflectors to Pl (ld PNB: rows aligned for 16-byte loads) and build
 // Vt, the first PNB rows of V with the unit diagonal and zeros above it.
+template <int VLD>
 __device__ __forceinline__ void panel_pad(const lds_f64* Pp, int mr, int nb, lds_f64* tl, lds_f64* Pl, lds_f64* Vt) {
   const int tid = threadIdx.x, bs = blockDim.x;
   for (int e = tid; e < PNB * PNB; e += bs) {
     const int r = e / PNB, j = e - r * PNB;
     Vt[e] = r >= mr ? 0.0 : j < r ? Pp[r * PLD + j] : (j == r ? 1.0 : 0.0);
   }
-  for (int e = tid; e < mr * PNB; e += bs) {
-    const int r = e / PNB, j = e - r * PNB;
-    Pl[e] = Pp[r * PLD + j];
+  if constexpr (VLD == PNB) {  // else the updates read Pp itself (Pl == Pp)
+    for (int e = tid; e < mr * PNB; e += bs) {
+      const int r = e / PNB, j = e - r * PNB;
+      Pl[e] = Pp[r * PLD + j];
+    }
   }
   for (int e = nb + tid; e < PNB; e += bs) tl[e] = 0.0;
   __syncthreads();
 }
 
 // QR of W (global, rows x cols, leading dimension ld): R on and above the
-// diagonal, reflectors below, tau[k].  Pl: rows x PNB LDS panel, Tl, Yl:
-// PNB x PNB, tl: PNB scalars (all LDS).
+// diagonal, reflectors below, tau[k].  Pp: rows x PLD LDS panel; Pl: its
+// aligned rows x PNB copy (VLD = PNB) or Pp itself (VLD = PLD, for panels too
+// tall for two copies); Tl, Yl: PNB x PNB, tl: PNB scalars (all LDS).
+template <int VLD>
 __device__ __noinline__ void qr_panel(double* W, int rows_in, int cols_in, int ld_in, double* tau, lds_f64* Pp,
                                       lds_f64* Pl, lds_f64* Tl, lds_f64* Yl, lds_f64* tl, lds_f64* Vt) {
   const int tid = threadIdx.x, bs = blockDim.x;
@@ -1084,12 +1096,12 @@ __device__ __noinline__ void qr_panel(double* W, int rows_in, int cols_in, int l
     for (int e = tid; e < nb; e += bs) tau[p0 + e] = tl[e];
     QTT_PT(2);
     if (p0 + nb < cols) {
-      panel_pad(Pp, mr, nb, tl, Pl, Vt);
+      panel_pad<VLD>(Pp, mr, nb, tl, Pl, Vt);
       build_T(Pp, mr, nb, tl, Tl, Yl);
       QTT_PT(3);
       for (int c = p0 + nb + tid; c < cols; c += bs) {
         double* col = W + (int64_t)p0 * ld + c;
-        wy_column([=](int r) -> double& { return col[(int64_t)r * ld]; }, mr, Pl, Vt, Tl, true);
+        wy_column<VLD>([=](int r) -> double& { return col[(int64_t)r * ld]; }, mr, Pl, Vt, Tl, true);
       }
       QTT_PT(4);
     }
@@ -1101,7 +1113,9 @@ __device__ __noinline__ void qr_panel(double* W, int rows_in, int cols_in, int l
 
 // x <- Q (rows x k) of the reflectors in W (global, ld) written straight into
 // the core Xc (rows split as (aa, bb), bb < bx, bond axis of size k); or, with
-// Jb, x <- U = Q [J_perm^T; 0] (rows x m, row i of J of length n).
+// Jb, x <- U = Q [J_perm^T; 0] (rows x m, row i of J of length n).  Panels as
+// in qr_panel.
+template <int VLD>
 __device__ __noinline__ void q_form_panel(const double* W, int rows_in, int k_in, int ld_in, const double* tau,
                                           double* Xc, int bx_in, const double* Jb, int n_in, const int* perm, int m_in,
                                           lds_f64* Pp, lds_f64* Pl, lds_f64* Tl, lds_f64* Yl, lds_f64* tl,
@@ -1126,12 +1140,12 @@ __device__ __noinline__ void q_form_panel(const double* W, int rows_in, int k_in
     }
     for (int e = tid; e < nb; e += bs) tl[e] = tau[p0 + e];
     __syncthreads();
-    panel_pad(Pp, mr, nb, tl, Pl, Vt);
+    panel_pad<VLD>(Pp, mr, nb, tl, Pl, Vt);
     build_T(Pp, mr, nb, tl, Tl, Yl);
     // Q: only columns >= p0 differ from the identity yet; U: every column
     const int c0 = Jb ? 0 : p0;
     for (int c = c0 + tid; c < ncol; c += bs) {
-      wy_column(
+      wy_column<VLD>(
           [=](int r) -> double& {
             const int rr = p0 + r, aa = rr / bx;
             return Xc[((int64_t)aa * ncol + c) * bx + (rr - aa * bx)];
@@ -1140,6 +1154,301 @@ __device__ __noinline__ void q_form_panel(const double* W, int rows_in, int k_in
     }
     __syncthreads();
   }
+}
+
+// ---------------------------------------------------------------------------
+// Blocked Householder QR for matrices too tall for an LDS panel.
+//
+// The panel (PNB columns) is factored in place in global memory, one pass over
+// its rows per column: the pass applies reflector j and, in the same read,
+// sums what reflector j+1 needs (the norm of column j+1, its dot products with
+// the later columns, and with the earlier reflectors for T).  The rows are
+// split over all threads and the sums reduced once per column.  The trailing
+// columns, and later the Q (or U) being formed, are updated in the WY form
+// with the rows split over the wavefronts and two columns per lane; each
+// wavefront stages 64 rows of V at a time in its own LDS region and reads
+// them back as broadcasts.  T of every panel is kept in global memory for the
+// Q formation.
+//
+// LDS (tall_lds_doubles): per-wave regions 8 x 2048 | W final 16 x 128 |
+// T 16 x 16 | sums 16 | reduction 8 x 16.
+// ---------------------------------------------------------------------------
+constexpr int TALL_REGION = 2048;
+constexpr int TALL_LDS = 8 * TALL_REGION + 16 * 128 + PNB * PNB + 16 + 8 * 16;
+
+// the 16 values of acc summed over the workgroup into out (all threads see them)
+__device__ __forceinline__ void tall_reduce16(double (&acc)[PNB], lds_f64* buf, lds_f64* out) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+  for (int i = 0; i < PNB; ++i) {
+    const double t = sum64(acc[i]);
+    if (lane == 0) buf[wv * PNB + i] = t;
+  }
+  __syncthreads();
+  if (tid < PNB) {
+    double t = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += buf[w * PNB + tid];
+    out[tid] = t;
+  }
+  __syncthreads();
+}
+
+// Factor the panel P = W[p0:, p0:p0+nb] (global, ld): R entries on and above
+// its diagonal, reflectors below (unit diagonal implied), tau[p0 + j], and the
+// panel's T into Tl (LDS, PNB x PNB; padding rows and columns zero).
+__device__ __noinline__ void tall_panel(double* W, int rows_in, int ld_in, int p0_in, int nb_in, double* tau,
+                                        lds_f64* Tl, lds_f64* S, lds_f64* buf) {
+  const int tid = threadIdx.x, bs = blockDim.x;
+  const int rows = uni(rows_in), ld = uni(ld_in), p0 = uni(p0_in), nb = uni(nb_in);
+  double* P = W + (int64_t)p0 * ld + p0;
+  const int mr = rows - p0;
+  for (int e = tid; e < PNB * PNB; e += bs) Tl[e] = 0.0;
+  // sums for column 0 over rows 1..: slot 0 its norm^2, slot c its dot with column c
+  {
+    double acc[PNB];
+#pragma unroll
+    for (int i = 0; i < PNB; ++i) acc[i] = 0.0;
+    for (int r = 1 + tid; r < mr; r += bs) {
+      const double* row = P + (int64_t)r * ld;
+      const double a0 = row[0];
+#pragma unroll
+      for (int c = 0; c < PNB; ++c)
+        if (c < nb) acc[c] = __builtin_fma(a0, row[c], acc[c]);
+    }
+    tall_reduce16(acc, buf, S);
+  }
+  for (int j = 0; j < nb; ++j) {
+    // slot j: norm^2 of column j below row j; slot c > j: its dot with column c;
+    // slot i < j: its dot with reflector i (all over the rows below row j)
+    const double* rj = P + (int64_t)j * ld;
+    const double alpha = rj[j], sg = S[j];
+    double tj = 0.0, beta = alpha, scale = 0.0;
+    if (sg != 0.0) {
+      beta = -copysign(sqrt(alpha * alpha + sg), alpha);
+      tj = (beta - alpha) / beta;
+      scale = 1.0 / (alpha - beta);
+    }
+    double d[PNB];  // tau_j * (v_j . a_c) for c > j
+#pragma unroll
+    for (int c = 0; c < PNB; ++c) d[c] = (c > j && c < nb) ? tj * (rj[c] + scale * S[c]) : 0.0;
+    // T[:, j] = -tau_j T[:, 0:j] y, y_i = v_i . v_j = v_i[j] + scale * slot i
+    if (tid < j) {
+      double acc = 0.0;
+      for (int l = tid; l < j; ++l) acc = __builtin_fma(Tl[tid * PNB + l], rj[l] + scale * S[l], acc);
+      Tl[tid * PNB + j] = -tj * acc;
+    }
+    __syncthreads();  // row j and the sums are read: both are rewritten below
+    if (tid == 0) {
+      Tl[j * PNB + j] = tj;
+      tau[p0 + j] = tj;
+      P[(int64_t)j * ld + j] = beta;
+    }
+    double acc[PNB];
+#pragma unroll
+    for (int i = 0; i < PNB; ++i) acc[i] = 0.0;
+    for (int r = j + 1 + tid; r < mr; r += bs) {
+      double* row = P + (int64_t)r * ld;
+      double x[PNB];
+#pragma unroll
+      for (int c = 0; c < PNB; ++c) x[c] = c < nb ? row[c] : 0.0;
+      double v = 0.0;
+#pragma unroll
+      for (int c = 0; c < PNB; ++c)
+        if (c == j) v = scale * x[c];
+      row[j] = v;
+#pragma unroll
+      for (int c = 0; c < PNB; ++c)
+        if (c > j && c < nb) {
+          x[c] = __builtin_fma(-d[c], v, x[c]);
+          row[c] = x[c];
+        }
+      if (r >= j + 2) {
+        double xn = 0.0;
+#pragma unroll
+        for (int c = 0; c < PNB; ++c)
+          if (c == j + 1) xn = x[c];
+#pragma unroll
+        for (int c = 0; c < PNB; ++c) {
+          // slot c: c > j+1 -> column c, c == j+1 -> norm^2, c < j -> reflector c, c == j -> v_j
+          const double o = c == j ? v : x[c];
+          if (c < nb) acc[c] = __builtin_fma(xn, o, acc[c]);
+        }
+      }
+    }
+    // row j of R right of the diagonal: a_c[j] - tau_j (v_j . a_c) (v_j[j] = 1)
+#pragma unroll
+    for (int c = 0; c < PNB; ++c)
+      if (tid == c && c > j && c < nb) P[(int64_t)j * ld + c] = rj[c] - d[c];
+    tall_reduce16(acc, buf, S);
+  }
+}
+
+// M[0:mr, c0:c1] <- (I - V op(T) V^T) M[0:mr, c0:c1], op(T) = T^T (trans) or T.
+// V = the panel's reflectors (global, ldv; nb columns, unit diagonal implied,
+// zero above), M global with leading dimension ldm.
+__device__ __noinline__ void tall_wy(double* M, int ldm_in, int mr_in, int c0_in, int c1_in, const double* V,
+                                     int ldv_in, int nb_in, const lds_f64* Tl, bool trans, lds_f64* L) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
+  const int ldm = uni(ldm_in), mr = uni(mr_in), c0 = uni(c0_in), c1 = uni(c1_in), ldv = uni(ldv_in),
+            nb = uni(nb_in);
+  lds_f64* reg = L + wv * TALL_REGION;  // this wavefront's region
+  lds_f64* Wf = L + 8 * TALL_REGION;    // 16 x 128: V^T M, then U = op(T) V^T M in region 0
+  const int chunk = (mr + nwv - 1) / nwv;
+  const int r0 = min(mr, wv * chunk), r1 = min(mr, r0 + chunk);
+  // stage V rows rb .. rb+63 of this wavefront (row = lane) with the implied structure
+  auto stage = [&](int rb) {
+    const int r = rb + lane;
+#pragma unroll
+    for (int j = 0; j < PNB; ++j) {
+      double v = 0.0;
+      if (r < r1 && j < nb) v = r > j ? V[(int64_t)r * ldv + j] : (r == j ? 1.0 : 0.0);
+      reg[lane * PNB + j] = v;
+    }
+    __builtin_amdgcn_wave_barrier();  // the wavefront's own LDS rows: in order, no workgroup barrier needed
+  };
+  for (int cb = c0; cb < c1; cb += 128) {
+    const int ca = cb + lane, cc = cb + 64 + lane;
+    const bool ona = ca < c1, onc = cc < c1;
+    {  // W = V^T M over this wavefront's rows, reduced over the wavefronts into Wf
+      double wa[PNB], wc[PNB];
+#pragma unroll
+      for (int j = 0; j < PNB; ++j) wa[j] = wc[j] = 0.0;
+      for (int rb = r0; rb < r1; rb += 64) {
+        stage(rb);
+        const int nr = min(64, r1 - rb);
+        int q = 0;
+        for (; q + 4 <= nr; q += 4) {  // four rows of M in flight per wait
+          double ma[4], mc[4];
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const int64_t ro = (int64_t)(rb + q + t) * ldm;
+            ma[t] = ona ? M[ro + ca] : 0.0;
+            mc[t] = onc ? M[ro + cc] : 0.0;
+          }
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            double v[PNB];
+            vrow(reg, q + t, v);
+#pragma unroll
+            for (int j = 0; j < PNB; ++j) {
+              wa[j] = __builtin_fma(v[j], ma[t], wa[j]);
+              wc[j] = __builtin_fma(v[j], mc[t], wc[j]);
+            }
+          }
+        }
+        for (; q < nr; ++q) {
+          const int64_t ro = (int64_t)(rb + q) * ldm;
+          const double ma = ona ? M[ro + ca] : 0.0, mc = onc ? M[ro + cc] : 0.0;
+          double v[PNB];
+          vrow(reg, q, v);
+#pragma unroll
+          for (int j = 0; j < PNB; ++j) {
+            wa[j] = __builtin_fma(v[j], ma, wa[j]);
+            wc[j] = __builtin_fma(v[j], mc, wc[j]);
+          }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < PNB; ++j) {
+        reg[j * 128 + lane] = wa[j];
+        reg[j * 128 + 64 + lane] = wc[j];
+      }
+    }
+    __syncthreads();
+    for (int e = tid; e < PNB * 128; e += blockDim.x) {
+      double t = 0.0;
+      for (int w = 0; w < nwv; ++w) t += L[w * TALL_REGION + e];
+      Wf[e] = t;
+    }
+    __syncthreads();
+    // U[i][col] = sum_j op(T)[i][j] W[j][col] into region 0
+    for (int e = tid; e < PNB * 128; e += blockDim.x) {
+      const int i = e >> 7, col = e & 127;
+      double t = 0.0;
+#pragma unroll
+      for (int j = 0; j < PNB; ++j) t = __builtin_fma(trans ? Tl[j * PNB + i] : Tl[i * PNB + j], Wf[j * 128 + col], t);
+      L[e] = t;
+    }
+    __syncthreads();
+    double ua[PNB], uc[PNB];
+#pragma unroll
+    for (int i = 0; i < PNB; ++i) {
+      ua[i] = L[i * 128 + lane];
+      uc[i] = L[i * 128 + 64 + lane];
+    }
+    __syncthreads();  // region 0 is staging space again below
+    // M -= V U over this wavefront's rows
+    for (int rb = r0; rb < r1; rb += 64) {
+      stage(rb);
+      const int nr = min(64, r1 - rb);
+      for (int q = 0; q < nr; ++q) {
+        const int64_t ro = (int64_t)(rb + q) * ldm;
+        double v[PNB];
+        vrow(reg, q, v);
+        double sa = 0.0, sc = 0.0;
+#pragma unroll
+        for (int j = 0; j < PNB; ++j) {
+          sa = __builtin_fma(v[j], ua[j], sa);
+          sc = __builtin_fma(v[j], uc[j], sc);
+        }
+        if (ona) M[ro + ca] -= sa;
+        if (onc) M[ro + cc] -= sc;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// QR of W (global rows x cols, ld) for the tall case: as qr_panel, plus every
+// panel's T in Tg (PNB x PNB each).
+__device__ __noinline__ void qr_tall(double* W, int rows_in, int cols_in, int ld_in, double* tau, double* Tg,
+                                     lds_f64* L) {
+  const int tid = threadIdx.x, bs = blockDim.x;
+  const int rows = uni(rows_in), cols = uni(cols_in), ld = uni(ld_in);
+  const int k = rows < cols ? rows : cols;
+  lds_f64* Tl = L + 8 * TALL_REGION + 16 * 128;
+  lds_f64* S = Tl + PNB * PNB;
+  lds_f64* buf = S + 16;
+  for (int p0 = 0; p0 < k; p0 += PNB) {
+    const int nb = k - p0 < PNB ? k - p0 : PNB;
+    tall_panel(W, rows, ld, p0, nb, tau, Tl, S, buf);
+    for (int e = tid; e < PNB * PNB; e += bs) Tg[(p0 / PNB) * PNB * PNB + e] = Tl[e];
+    if (p0 + nb < cols)
+      tall_wy(W + (int64_t)p0 * ld, ld, rows - p0, p0 + nb, cols, W + (int64_t)p0 * ld + p0, ld, nb, Tl, true, L);
+    __syncthreads();
+  }
+}
+
+// x <- Q (rows x k) of qr_tall's reflectors, or with Jb x <- U = Q [J_perm^T; 0]
+// (rows x m), formed row-major in Mq (global) and then written into the core Xc
+// (rows split as (aa, bb), bb < bx).
+__device__ __noinline__ void q_form_tall(const double* W, int rows_in, int k_in, int ld_in, const double* Tg,
+                                         double* Mq, double* Xc, int bx_in, const double* Jb, int n_in,
+                                         const int* perm, int m_in, lds_f64* L) {
+  const int tid = threadIdx.x, bs = blockDim.x;
+  const int rows = uni(rows_in), k = uni(k_in), ld = uni(ld_in), bx = uni(bx_in), n = uni(n_in), m = uni(m_in);
+  const int ncol = Jb ? m : k;
+  lds_f64* Tl = L + 8 * TALL_REGION + 16 * 128;
+  for (int64_t e = tid; e < (int64_t)rows * ncol; e += bs) {
+    const int r = (int)(e / ncol), c = (int)(e - (int64_t)r * ncol);
+    Mq[e] = Jb ? (r < n ? Jb[(int64_t)perm[c] * n + r] : 0.0) : (r == c ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  for (int p0 = ((k - 1) / PNB) * PNB; p0 >= 0; p0 -= PNB) {
+    const int nb = k - p0 < PNB ? k - p0 : PNB;
+    for (int e = tid; e < PNB * PNB; e += bs) Tl[e] = Tg[(p0 / PNB) * PNB * PNB + e];
+    __syncthreads();
+    // Q: only columns >= p0 differ from the identity yet; U: every column
+    tall_wy(Mq + (int64_t)p0 * ncol, ncol, rows - p0, Jb ? 0 : p0, ncol, W + (int64_t)p0 * ld + p0, ld, nb, Tl,
+            false, L);
+  }
+  for (int64_t e = tid; e < (int64_t)rows * ncol; e += bs) {
+    const int bb = (int)(e % bx);
+    const int64_t t2 = e / bx;
+    const int c = (int)(t2 % ncol), aa = (int)(t2 / ncol);
+    Xc[e] = Mq[(int64_t)(aa * bx + bb) * ncol + c];
+  }
+  __syncthreads();
 }
 
 template <int G>
@@ -1273,7 +1582,8 @@ __device__ __noinline__ void mode_product_staged(const double* Rrows, const int*
 // scratch.  Two separate instantiations (one per memory) let the compiler use
 // LDS instructions on the LDS path instead of generic flat addressing.
 template <int G, bool REG>
-__device__ __forceinline__ int run_move(double* ws, double* jpad, lds_f64* pnl, const int pnl_cap, const int rows,
+__device__ __forceinline__ int run_move(double* ws, double* jpad, lds_f64* pnl, const int pnl_cap, const int pmode,
+                                        const int rows,
                                         const int b, const int tr, const int rpl, const int bx,
                                         const int64_t Au, const int64_t nu, const int64_t Bu, double* Xc,
                                         const double* Uc, double* Ud, double* gY, double cutoff, long long maxdim,
@@ -1322,17 +1632,32 @@ __device__ __forceinline__ int run_move(double* ws, double* jpad, lds_f64* pnl, 
       if (rpl == 8) q_form_reg<8, 8>(Wl, rows, k, taul, Xc, bx);
       else if (rpl == 16) q_form_reg<8, 16>(Wl, rows, k, taul, Xc, bx);
       else q_form_reg<16, 12>(Wl, rows, k, taul, Xc, bx);
-    } else if (pnl) {  // workspace in global memory: blocked QR, panels in LDS
-      lds_f64 *Pp = pnl, *Pl = Pp + ((rows * PLD + 1) & ~1), *Tl = Pl + rows * PNB, *Yl = Tl + PNB * PNB, *tl = Yl + PNB * PNB,
-              *Vt = tl + PNB;
-      qr_panel(W, rows, b, b, tau, Pp, Pl, Tl, Yl, tl, Vt);
+    } else if (pmode == 3) {  // too tall for an LDS panel: panels in global memory
+      double* Tg = vbuf;
+      double* Mq = Tg + (((int64_t)k + PNB) * PNB);
+      qr_tall(W, rows, b, b, tau, Tg, pnl);
       for (int e = tid; e < k * b; e += bs) {
         const int i = e / b, c = e - i * b;
         Rb[e] = c >= i ? W[(int64_t)i * b + c] : 0.0;
       }
       __syncthreads();
       QTT_PHASE(-1, 2);  // factored
-      q_form_panel(W, rows, k, b, tau, Xc, bx, nullptr, 0, nullptr, 0, Pp, Pl, Tl, Yl, tl, Vt);  // x <- Q (bond m)
+      q_form_tall(W, rows, k, b, Tg, Mq, Xc, bx, nullptr, 0, nullptr, 0, pnl);  // x <- Q (bond m)
+    } else if (pnl) {  // workspace in global memory: blocked QR, panels in LDS
+      lds_f64 *Pp = pnl, *Pl = pmode == 1 ? Pp + ((rows * PLD + 1) & ~1) : Pp,
+              *Tl = pmode == 1 ? Pl + rows * PNB : Pp + ((rows * PLD + 1) & ~1), *Yl = Tl + PNB * PNB,
+              *tl = Yl + PNB * PNB, *Vt = tl + PNB;
+      if (pmode == 1) qr_panel<PNB>(W, rows, b, b, tau, Pp, Pl, Tl, Yl, tl, Vt);
+      else qr_panel<PLD>(W, rows, b, b, tau, Pp, Pl, Tl, Yl, tl, Vt);
+      for (int e = tid; e < k * b; e += bs) {
+        const int i = e / b, c = e - i * b;
+        Rb[e] = c >= i ? W[(int64_t)i * b + c] : 0.0;
+      }
+      __syncthreads();
+      QTT_PHASE(-1, 2);  // factored
+      // x <- Q (bond m)
+      if (pmode == 1) q_form_panel<PNB>(W, rows, k, b, tau, Xc, bx, nullptr, 0, nullptr, 0, Pp, Pl, Tl, Yl, tl, Vt);
+      else q_form_panel<PLD>(W, rows, k, b, tau, Xc, bx, nullptr, 0, nullptr, 0, Pp, Pl, Tl, Yl, tl, Vt);
     } else {
       geqr2<G>(W, rows, b, b, tau);
       for (int e = tid; e < k * b; e += bs) {
@@ -1359,10 +1684,18 @@ __device__ __forceinline__ int run_move(double* ws, double* jpad, lds_f64* pnl, 
         if (rpl == 8) qr_reg<8, 8, 2>(Wl, rows, b, taul, Rl);
         else if (rpl == 16) qr_reg<8, 16, 2>(Wl, rows, b, taul, Rl);
         else qr_reg<16, 12, 3>(Wl, rows, b, taul, Rl);
+      } else if (pmode == 3) {  // tall QR in global memory; the reflectors stay in W for U below
+        qr_tall(W, rows, b, b, tau, vbuf, pnl);
+        for (int e = tid; e < b * b; e += bs) {
+          const int i = e / b, c = e - i * b;
+          Rb[e] = c >= i ? W[(int64_t)i * b + c] : 0.0;
+        }
       } else if (pnl) {  // blocked QR; the reflectors stay in W for U below
-        lds_f64 *Pp = pnl, *Pl = Pp + ((rows * PLD + 1) & ~1), *Tl = Pl + rows * PNB, *Yl = Tl + PNB * PNB, *tl = Yl + PNB * PNB,
-              *Vt = tl + PNB;
-        qr_panel(W, rows, b, b, tau, Pp, Pl, Tl, Yl, tl, Vt);
+        lds_f64 *Pp = pnl, *Pl = pmode == 1 ? Pp + ((rows * PLD + 1) & ~1) : Pp,
+                *Tl = pmode == 1 ? Pl + rows * PNB : Pp + ((rows * PLD + 1) & ~1), *Yl = Tl + PNB * PNB,
+                *tl = Yl + PNB * PNB, *Vt = tl + PNB;
+        if (pmode == 1) qr_panel<PNB>(W, rows, b, b, tau, Pp, Pl, Tl, Yl, tl, Vt);
+        else qr_panel<PLD>(W, rows, b, b, tau, Pp, Pl, Tl, Yl, tl, Vt);
         for (int e = tid; e < b * b; e += bs) {
           const int i = e / b, c = e - i * b;
           Rb[e] = c >= i ? W[(int64_t)i * b + c] : 0.0;
@@ -1454,10 +1787,14 @@ __device__ __forceinline__ int run_move(double* ws, double* jpad, lds_f64* pnl, 
     Rrows = X;
     use_perm = true;
     // x <- U (bond m): U[row][i] = sum_j Q[row][j] J[perm[i]][j]  (or J[perm[i]][row])
-    if (tall && pnl) {
-      lds_f64 *Pp = pnl, *Pl = Pp + ((rows * PLD + 1) & ~1), *Tl = Pl + rows * PNB, *Yl = Tl + PNB * PNB, *tl = Yl + PNB * PNB,
-              *Vt = tl + PNB;
-      q_form_panel(W, rows, b, b, tau, Xc, bx, Jb, n, perm, m, Pp, Pl, Tl, Yl, tl, Vt);
+    if (tall && pmode == 3) {
+      q_form_tall(W, rows, b, b, vbuf, vbuf + (((int64_t)b + PNB) * PNB), Xc, bx, Jb, n, perm, m, pnl);
+    } else if (tall && pnl) {
+      lds_f64 *Pp = pnl, *Pl = pmode == 1 ? Pp + ((rows * PLD + 1) & ~1) : Pp,
+              *Tl = pmode == 1 ? Pl + rows * PNB : Pp + ((rows * PLD + 1) & ~1), *Yl = Tl + PNB * PNB,
+              *tl = Yl + PNB * PNB, *Vt = tl + PNB;
+      if (pmode == 1) q_form_panel<PNB>(W, rows, b, b, tau, Xc, bx, Jb, n, perm, m, Pp, Pl, Tl, Yl, tl, Vt);
+      else q_form_panel<PLD>(W, rows, b, b, tau, Xc, bx, Jb, n, perm, m, Pp, Pl, Tl, Yl, tl, Vt);
     } else if (tall && use_reg) {
       const lds_f64 *Wl = (const lds_f64*)W, *taul = (const lds_f64*)tau, *Jl = (const lds_f64*)Jb;
       const lds_i32* pl = (const lds_i32*)perm;
@@ -1546,10 +1883,18 @@ __global__ void __launch_bounds__(512) truncate_kernel(const SweepArgs* __restri
       rpl = 0;
       need = (int64_t)rows * b + base;
     }
-    // global workspace: LDS holds the QR panels (rows x PNB, T, Y, tau)
+    // global workspace: LDS holds the QR panels (rows x PLD and its aligned
+    // rows x PNB copy when both fit (pmode 1), else the first alone (pmode 2);
+    // T, Y, tau), or for taller matrices the tall QR's staging (pmode 3)
     lds_f64* pnl = nullptr;
     int pnl_cap = 0;  // doubles of LDS at pnl
-    if (need > a.ws_lds && (int64_t)rows * (PNB + PLD) + 3 * PNB * PNB + PNB + 8 <= a.ws_lds) {
+    int pmode = 0;
+    if (need > a.ws_lds) {
+      const int64_t fixed = 3 * PNB * PNB + PNB + 8;
+      pmode = (int64_t)rows * (PNB + PLD) + fixed <= a.ws_lds ? 1 : (int64_t)rows * PLD + fixed <= a.ws_lds ? 2 : 0;
+      if (!pmode && b <= 4 * rows && TALL_LDS + 64 <= a.ws_lds) pmode = 3;  // tall: panels in global memory
+    }
+    if (pmode) {
       const int off = (64 + ((t.nv + 1) >> 1) + 1) & ~1;  // 16-byte aligned for the row loads
       pnl = (lds_f64*)(lds + off);
       pnl_cap = QTT_LDS_DOUBLES - off;
@@ -1566,17 +1911,17 @@ __global__ void __launch_bounds__(512) truncate_kernel(const SweepArgs* __restri
     int m;
     if (need <= a.ws_lds) {
       if (small)
-        m = run_move<8, true>(wsl, jpad, nullptr, 0, rows, b, tr, rpl, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<8, true>(wsl, jpad, nullptr, 0, 0, rows, b, tr, rpl, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
                         iflag, red, dbc);
       else
-        m = run_move<64, false>(wsl, nullptr, nullptr, 0, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<64, false>(wsl, nullptr, nullptr, 0, 0, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
                          iflag, red, dbc);
     } else {
       if (small)
-        m = run_move<8, false>(a.gscr, nullptr, pnl, pnl_cap, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<8, false>(a.gscr, nullptr, pnl, pnl_cap, pmode, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
                         iflag, red, dbc);
       else
-        m = run_move<64, false>(a.gscr, nullptr, pnl, pnl_cap, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<64, false>(a.gscr, nullptr, pnl, pnl_cap, pmode, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
                          iflag, red, dbc);
     }
     QTT_PHASE(st, 5);

@@ -263,6 +263,45 @@ def test_comb_operate_and_integrate(q, po, eng, rng):
     assert q.integrate(g) == pytest.approx(po.integrate(t.parent, t.sdim, gref, 0.5), rel=1e-11)
 
 
+def _star_network(q, rng):
+    """A vertex x with three arms of six digits: arm a ends at the root, x's
+    bonds are 24 (to a), 20 and 25, so moves at x have 1000-1200 rows: too tall
+    for the two-copy LDS panel."""
+    verts = [("a", j) for j in range(6, 0, -1)] + ["x"] + [(arm, j) for arm in "bc" for j in range(1, 7)]
+    edges = ([(("a", j), ("a", j - 1)) for j in range(6, 1, -1)] + [(("a", 1), "x"), ("x", ("b", 1)), ("x", ("c", 1))]
+             + [((arm, j), (arm, j + 1)) for arm in "bc" for j in range(1, 6)])
+    s = q.continuous_siteinds(q.NamedGraph(verts, edges), [verts])
+    topo = q.Topology(s)
+    to_parent = {("a", 6): 1, ("a", 5): 2, ("a", 4): 4, ("a", 3): 8, ("a", 2): 16, ("a", 1): 24, "x": 24}
+    for arm, top in (("b", 20), ("c", 25)):
+        for j in range(1, 7):
+            to_parent[(arm, j)] = min(top, 2 ** (7 - j))
+    bond = np.array([to_parent[v] for v in topo.names], dtype=np.int64)
+    cores = [rng.standard_normal(topo.core_shape(v, bond)) / math.sqrt(24) for v in range(topo.nv)]
+    return q.ITensorNetworkFunction(s, cores, topo)
+
+
+@pytest.mark.parametrize("case", ["star", "comb"])
+def test_truncate_global_workspace(q, po, eng, rng, case):
+    # moves whose matrices do not fit LDS: at the star's centre (1000-1200 rows: one LDS
+    # panel copy, or panels in global memory) and at the comb's backbone (3200 x 40)
+    if case == "star":
+        f = _star_network(q, rng)
+        g, err = q.truncate(f, cutoff=1e-10, return_error=True)
+        ref, rerr = po.truncate(f.topo.parent, f.topo.sdim, f.cores, 1e-10)
+    else:
+        from itensornumericalanalysis_jl_amd import workloads
+        w = workloads.comb_3d(bond=40)
+        f = w.f
+        t = f.topo
+        g, err = q.operate(w.op, f, cutoff=w.cutoff, maxdim=w.maxdim, return_error=True)
+        C = po.contract(t.parent, t.sdim, w.op.cores, f.cores)
+        ref, rerr = po.truncate(t.parent, t.sdim, C, w.cutoff, w.maxdim)
+    assert list(g.bonds()) == [c.shape[0] for c in ref]
+    assert _compare_networks(q, po, g, ref, f, rng) < 1e-11
+    assert err == pytest.approx(rerr, rel=1e-8, abs=1e-20)
+
+
 @pytest.mark.parametrize("case", ["comb", "star", "no_digit"])
 def test_integrate_dataflow(q, po, eng, rng, case):
     # networks of >= 2^15 core elements take the many-workgroup dataflow kernel
