@@ -759,17 +759,18 @@ __device__ __forceinline__ double jsum(double v) {
   return v;
 }
 
-template <int G>
-__device__ __forceinline__ void load_row(const lds_f64* r, int len, int gl, double (&x)[JEPL]) {
-  const lds_f64* p = r + gl;
+// F: the element type, LDS (lds_f64) or global (double) memory; JE entries per lane
+template <int G, int JE, typename F>
+__device__ __forceinline__ void load_row(const F* r, int len, int gl, double (&x)[JE]) {
+  const F* p = r + gl;
 #pragma unroll
-  for (int e = 0; e < JEPL; ++e) x[e] = keep(p[G * e], lt_mask(gl + G * e, len));
+  for (int e = 0; e < JE; ++e) x[e] = keep(p[G * e], lt_mask(gl + G * e, len));
 }
 
-template <int G>
-__device__ __forceinline__ void store_row(lds_f64* r, int len, int gl, const double (&x)[JEPL]) {
+template <int G, int JE, typename F>
+__device__ __forceinline__ void store_row(F* r, int len, int gl, const double (&x)[JE]) {
 #pragma unroll
-  for (int e = 0; e < JEPL; ++e) {
+  for (int e = 0; e < JE; ++e) {
     const int i = gl + G * e;
     if (i < len) r[i] = x[e];
   }
@@ -794,17 +795,20 @@ __device__ __forceinline__ double rsq_nr(double x) {
   return __builtin_fma(0.5 * y, e, y);
 }
 
-__device__ __forceinline__ void rotate4(double (&u)[JEPL], double (&w)[JEPL], double cs, double sn) {
+template <int JE>
+__device__ __forceinline__ void rotate4(double (&u)[JE], double (&w)[JE], double cs, double sn) {
 #pragma unroll
-  for (int e = 0; e < JEPL; ++e) {
+  for (int e = 0; e < JE; ++e) {
     const double u0 = u[e], w0 = w[e];
     u[e] = cs * u0 - sn * w0;
     w[e] = sn * u0 + cs * w0;
   }
 }
 
-template <int G>
-__device__ __noinline__ bool jacobi_rows_reg(lds_f64* X, int ldx_in, int n_in, int b_in, lds_f64* J, int ldj_in,
+// (JE and F generalise it to longer rows and to matrices in global memory,
+// where a row's tail past its end lies inside the padded workspace too.)
+template <int G, int JE = JEPL, typename F = lds_f64>
+__device__ __noinline__ bool jacobi_rows_reg(F* X, int ldx_in, int n_in, int b_in, F* J, int ldj_in,
                                              lds_i32* flag, double* red) {
   const int ldx = uni(ldx_in), n = uni(n_in), b = uni(b_in), ldj = uni(ldj_in);
   const int tid = threadIdx.x, bs = blockDim.x;
@@ -837,18 +841,18 @@ __device__ __noinline__ bool jacobi_rows_reg(lds_f64* X, int ldx_in, int n_in, i
         }
         if (q >= n) continue;
         const int gl = opaque(tid & (G - 1));
-        lds_f64* xp = X + p * ldx;
-        lds_f64* xq = X + q * ldx;
-        lds_f64* jp = J + p * ldj;
-        lds_f64* jq = J + q * ldj;
-        double u[JEPL], w[JEPL], ju[JEPL], jw[JEPL];
-        load_row<G>(xp, b, gl, u);
-        load_row<G>(xq, b, gl, w);
-        load_row<G>(jp, n, gl, ju);  // read with X so both rows arrive in one wait
-        load_row<G>(jq, n, gl, jw);
+        F* xp = X + (int64_t)p * ldx;
+        F* xq = X + (int64_t)q * ldx;
+        F* jp = J + (int64_t)p * ldj;
+        F* jq = J + (int64_t)q * ldj;
+        double u[JE], w[JE], ju[JE], jw[JE];
+        load_row<G, JE>(xp, b, gl, u);
+        load_row<G, JE>(xq, b, gl, w);
+        load_row<G, JE>(jp, n, gl, ju);  // read with X so both rows arrive in one wait
+        load_row<G, JE>(jq, n, gl, jw);
         double a0 = 0.0, c0 = 0.0, g0 = 0.0, a1 = 0.0, c1 = 0.0, g1 = 0.0;
 #pragma unroll
-        for (int e = 0; e < JEPL; e += 2) {
+        for (int e = 0; e < JE; e += 2) {
           a0 = __builtin_fma(u[e], u[e], a0);
           c0 = __builtin_fma(w[e], w[e], c0);
           g0 = __builtin_fma(u[e], w[e], g0);
@@ -869,10 +873,10 @@ __device__ __noinline__ bool jacobi_rows_reg(lds_f64* X, int ldx_in, int n_in, i
           QTT_JT(1);
           rotate4(u, w, cs, sn);
           rotate4(ju, jw, cs, sn);
-          store_row<G>(xp, b, gl, u);
-          store_row<G>(xq, b, gl, w);
-          store_row<G>(jp, n, gl, ju);
-          store_row<G>(jq, n, gl, jw);
+          store_row<G, JE>(xp, b, gl, u);
+          store_row<G, JE>(xq, b, gl, w);
+          store_row<G, JE>(jp, n, gl, ju);
+          store_row<G, JE>(jq, n, gl, jw);
           if (gl == 0) *flag = 1;
           QTT_JT(2);
         }
@@ -1165,25 +1169,25 @@ __device__ __noinline__ void q_form_panel(const double* W, int rows_in, int k_in
 // the later columns, and with the earlier reflectors for T).  The rows are
 // split over all threads and the sums reduced once per column.  The trailing
 // columns, and later the Q (or U) being formed, are updated in the WY form
-// with the rows split over the wavefronts and two columns per lane; each
+// with the rows split over the wavefronts and one column per lane; each
 // wavefront stages 64 rows of V at a time in its own LDS region and reads
 // them back as broadcasts.  T of every panel is kept in global memory for the
 // Q formation.
 //
-// LDS (tall_lds_doubles): per-wave regions 8 x 2048 | W final 16 x 128 |
+// LDS (TALL_LDS doubles): per-wave regions 8 x 2048 | W final 16 x 128 |
 // T 16 x 16 | sums 16 | reduction 8 x 16.
 // ---------------------------------------------------------------------------
 constexpr int TALL_REGION = 2048;
+constexpr int TALL_MF = 16;  // rows of M in flight per wait in the WY updates (divides 64)
 constexpr int TALL_LDS = 8 * TALL_REGION + 16 * 128 + PNB * PNB + 16 + 8 * 16;
 
-// the 16 values of acc summed over the workgroup into out (all threads see them)
-__device__ __forceinline__ void tall_reduce16(double (&acc)[PNB], lds_f64* buf, lds_f64* out) {
+// Sum the slot value of every lane over the lanes of its column (lane & 15)
+// and over the workgroup into out[0..15] (all threads see them).
+__device__ __forceinline__ void tall_reduce16(double acc, lds_f64* buf, lds_f64* out) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-#pragma unroll
-  for (int i = 0; i < PNB; ++i) {
-    const double t = sum64(acc[i]);
-    if (lane == 0) buf[wv * PNB + i] = t;
-  }
+  acc += __shfl_xor(acc, 16);
+  acc += __shfl_xor(acc, 32);
+  if (lane < PNB) buf[wv * PNB + lane] = acc;
   __syncthreads();
   if (tid < PNB) {
     double t = 0.0;
@@ -1195,25 +1199,36 @@ __device__ __forceinline__ void tall_reduce16(double (&acc)[PNB], lds_f64* buf, 
 
 // Factor the panel P = W[p0:, p0:p0+nb] (global, ld): R entries on and above
 // its diagonal, reflectors below (unit diagonal implied), tau[p0 + j], and the
-// panel's T into Tl (LDS, PNB x PNB; padding rows and columns zero).
+// panel's T into Tl (LDS, PNB x PNB; padding rows and columns zero).  Each
+// wavefront reads four panel rows per load, a 16-lane group per row and a lane
+// per column, so the loads are coalesced and each lane keeps the sum of its
+// own column's slot; the group's lanes also read columns j and j+1 of their
+// row (one broadcast load each) instead of exchanging them.
 __device__ __noinline__ void tall_panel(double* W, int rows_in, int ld_in, int p0_in, int nb_in, double* tau,
                                         lds_f64* Tl, lds_f64* S, lds_f64* buf) {
-  const int tid = threadIdx.x, bs = blockDim.x;
+  const int tid = threadIdx.x, bs = blockDim.x, lane = tid & 63, wv = tid >> 6;
   const int rows = uni(rows_in), ld = uni(ld_in), p0 = uni(p0_in), nb = uni(nb_in);
   double* P = W + (int64_t)p0 * ld + p0;
   const int mr = rows - p0;
+  const int c = lane & 15;                     // this lane's column
+  const int rstep = (bs >> 6) * 4;             // rows per workgroup step
+  const int rsub = wv * 4 + (lane >> 4);       // this lane's row within a step
+  const bool cin = c < nb;
+  constexpr int U = 8;                         // steps in flight per wait
   for (int e = tid; e < PNB * PNB; e += bs) Tl[e] = 0.0;
-  // sums for column 0 over rows 1..: slot 0 its norm^2, slot c its dot with column c
-  {
-    double acc[PNB];
+  {  // sums for column 0 over rows 1..: slot 0 its norm^2, slot c its dot with column c
+    double acc = 0.0;
+    for (int rb = 1; rb < mr; rb += U * rstep) {
+      double x[U], x0[U];  // this lane's column and column 0 (the same load for the whole group)
 #pragma unroll
-    for (int i = 0; i < PNB; ++i) acc[i] = 0.0;
-    for (int r = 1 + tid; r < mr; r += bs) {
-      const double* row = P + (int64_t)r * ld;
-      const double a0 = row[0];
+      for (int u = 0; u < U; ++u) {
+        const int r = rb + u * rstep + rsub;
+        const bool in = r < mr;
+        x[u] = (in && cin) ? P[(int64_t)r * ld + c] : 0.0;
+        x0[u] = in ? P[(int64_t)r * ld] : 0.0;
+      }
 #pragma unroll
-      for (int c = 0; c < PNB; ++c)
-        if (c < nb) acc[c] = __builtin_fma(a0, row[c], acc[c]);
+      for (int u = 0; u < U; ++u) acc = __builtin_fma(x0[u], x[u], acc);
     }
     tall_reduce16(acc, buf, S);
   }
@@ -1228,9 +1243,12 @@ __device__ __noinline__ void tall_panel(double* W, int rows_in, int ld_in, int p
       tj = (beta - alpha) / beta;
       scale = 1.0 / (alpha - beta);
     }
-    double d[PNB];  // tau_j * (v_j . a_c) for c > j
-#pragma unroll
-    for (int c = 0; c < PNB; ++c) d[c] = (c > j && c < nb) ? tj * (rj[c] + scale * S[c]) : 0.0;
+    // this lane's column: tau_j (v_j . a_c) for c > j; and for column j+1,
+    // needed by every lane (row j is rewritten below, so both are read now)
+    const double dc = (c > j && cin) ? tj * (rj[c] + scale * S[c]) : 0.0;
+    const double rjc = cin ? rj[c] : 0.0;
+    const bool nxt = j + 1 < nb;
+    const double dn = nxt ? tj * (rj[j + 1] + scale * S[j + 1]) : 0.0;
     // T[:, j] = -tau_j T[:, 0:j] y, y_i = v_i . v_j = v_i[j] + scale * slot i
     if (tid < j) {
       double acc = 0.0;
@@ -1243,156 +1261,141 @@ __device__ __noinline__ void tall_panel(double* W, int rows_in, int ld_in, int p
       tau[p0 + j] = tj;
       P[(int64_t)j * ld + j] = beta;
     }
-    double acc[PNB];
+    // row j of R right of the diagonal: a_c[j] - tau_j (v_j . a_c) (v_j[j] = 1)
+    if (tid < PNB && c > j && cin) P[(int64_t)j * ld + c] = rjc - dc;
+    double acc = 0.0;
+    for (int rb = j + 1; rb < mr; rb += U * rstep) {
+      // this lane's column, and columns j and j+1 of the same row (one load
+      // for the whole group each): no cross-lane traffic in the pass
+      double x[U], xj[U], xj1[U];
 #pragma unroll
-    for (int i = 0; i < PNB; ++i) acc[i] = 0.0;
-    for (int r = j + 1 + tid; r < mr; r += bs) {
-      double* row = P + (int64_t)r * ld;
-      double x[PNB];
+      for (int u = 0; u < U; ++u) {
+        const int r = rb + u * rstep + rsub;
+        const bool in = r < mr;
+        const double* row = P + (int64_t)r * ld;
+        x[u] = (in && cin) ? row[c] : 0.0;
+        xj[u] = in ? row[j] : 0.0;
+        xj1[u] = (in && nxt) ? row[j + 1] : 0.0;
+      }
 #pragma unroll
-      for (int c = 0; c < PNB; ++c) x[c] = c < nb ? row[c] : 0.0;
-      double v = 0.0;
-#pragma unroll
-      for (int c = 0; c < PNB; ++c)
-        if (c == j) v = scale * x[c];
-      row[j] = v;
-#pragma unroll
-      for (int c = 0; c < PNB; ++c)
-        if (c > j && c < nb) {
-          x[c] = __builtin_fma(-d[c], v, x[c]);
-          row[c] = x[c];
-        }
-      if (r >= j + 2) {
-        double xn = 0.0;
-#pragma unroll
-        for (int c = 0; c < PNB; ++c)
-          if (c == j + 1) xn = x[c];
-#pragma unroll
-        for (int c = 0; c < PNB; ++c) {
-          // slot c: c > j+1 -> column c, c == j+1 -> norm^2, c < j -> reflector c, c == j -> v_j
-          const double o = c == j ? v : x[c];
-          if (c < nb) acc[c] = __builtin_fma(xn, o, acc[c]);
-        }
+      for (int u = 0; u < U; ++u) {
+        const int r = rb + u * rstep + rsub;
+        const double v = scale * xj[u];  // reflector j at row r
+        double y = x[u];
+        if (c > j) y = __builtin_fma(-dc, v, y);
+        if (r < mr && cin && c >= j) P[(int64_t)r * ld + c] = c == j ? v : y;
+        // sums for column j+1 over rows below j+1: slot c > j+1 column c,
+        // c == j+1 its norm^2, c == j reflector j, c < j reflector c
+        const double xn = __builtin_fma(-dn, v, xj1[u]);
+        const double o = c == j ? v : y;
+        if (r >= j + 2 && r < mr && nxt) acc = __builtin_fma(xn, o, acc);
       }
     }
-    // row j of R right of the diagonal: a_c[j] - tau_j (v_j . a_c) (v_j[j] = 1)
-#pragma unroll
-    for (int c = 0; c < PNB; ++c)
-      if (tid == c && c > j && c < nb) P[(int64_t)j * ld + c] = rj[c] - d[c];
-    tall_reduce16(acc, buf, S);
+    if (j + 1 < nb) tall_reduce16(acc, buf, S);
   }
+  __syncthreads();
 }
 
 // M[0:mr, c0:c1] <- (I - V op(T) V^T) M[0:mr, c0:c1], op(T) = T^T (trans) or T.
 // V = the panel's reflectors (global, ldv; nb columns, unit diagonal implied,
-// zero above), M global with leading dimension ldm.
+// zero above), M global with leading dimension ldm.  Both products are FP64
+// MFMA tiles (v_mfma_f64_16x16x4f64: lane l holds A[l%16][l/16] and
+// B[l/16][l%16], and D[l/16 + 4e][l%16] in element e), so the operands come
+// from coalesced loads of 16 consecutive doubles per 16-lane group, not from
+// LDS broadcasts.
+//   W = V^T M: per (16-column tile, row chunk) unit, A = V^T, B = M, 4 rows a step;
+//   U = op(T) W (16 x 16 per tile, in LDS);
+//   M -= V U: per (tile, 16-row block), C = M, A = V, B = -U, 4 reflectors a step.
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ double vref(const double* V, int ldv, int r, int j, int mr, int nb) {
+  return (j < nb && r < mr) ? (r > j ? V[(int64_t)r * ldv + j] : (r == j ? 1.0 : 0.0)) : 0.0;
+}
+
 __device__ __noinline__ void tall_wy(double* M, int ldm_in, int mr_in, int c0_in, int c1_in, const double* V,
                                      int ldv_in, int nb_in, const lds_f64* Tl, bool trans, lds_f64* L) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
   const int ldm = uni(ldm_in), mr = uni(mr_in), c0 = uni(c0_in), c1 = uni(c1_in), ldv = uni(ldv_in),
             nb = uni(nb_in);
-  lds_f64* reg = L + wv * TALL_REGION;  // this wavefront's region
-  lds_f64* Wf = L + 8 * TALL_REGION;    // 16 x 128: V^T M, then U = op(T) V^T M in region 0
-  const int chunk = (mr + nwv - 1) / nwv;
-  const int r0 = min(mr, wv * chunk), r1 = min(mr, r0 + chunk);
-  // stage V rows rb .. rb+63 of this wavefront (row = lane) with the implied structure
-  auto stage = [&](int rb) {
-    const int r = rb + lane;
+  const int li = lane & 15, lk = lane >> 4;
+  lds_f64* Pw = L;                  // partial W tiles: [unit][e * 64 + lane]
+  lds_f64* Wt = L + 32 * 256;       // W tiles [tile][j][c]
+  lds_f64* Ut = Wt + 8 * 256;       // U tiles [tile][j][c]
+  for (int cg = c0; cg < c1; cg += 128) {
+    const int nct = (min(128, c1 - cg) + 15) >> 4;        // 16-column tiles in this group
+    const int nch = max(1, min(32 / nct, (mr + 63) / 64));  // row chunks per tile
+    const int nunit = nct * nch, chunk = (((mr + nch - 1) / nch) + 3) & ~3;
+    // W = V^T M
+    for (int u = wv; u < nunit; u += nwv) {
+      const int t = u / nch, h = u - t * nch;
+      const int col = cg + t * 16 + li;
+      const bool con = col < c1;
+      const int rs = h * chunk, re = min(mr, rs + chunk);
+      f64x4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+      int r = rs;
+      for (; r + 16 <= re; r += 16) {  // 16 rows (four steps) in flight per wait
+        double a[4], b[4];
 #pragma unroll
-    for (int j = 0; j < PNB; ++j) {
-      double v = 0.0;
-      if (r < r1 && j < nb) v = r > j ? V[(int64_t)r * ldv + j] : (r == j ? 1.0 : 0.0);
-      reg[lane * PNB + j] = v;
-    }
-    __builtin_amdgcn_wave_barrier();  // the wavefront's own LDS rows: in order, no workgroup barrier needed
-  };
-  for (int cb = c0; cb < c1; cb += 128) {
-    const int ca = cb + lane, cc = cb + 64 + lane;
-    const bool ona = ca < c1, onc = cc < c1;
-    {  // W = V^T M over this wavefront's rows, reduced over the wavefronts into Wf
-      double wa[PNB], wc[PNB];
-#pragma unroll
-      for (int j = 0; j < PNB; ++j) wa[j] = wc[j] = 0.0;
-      for (int rb = r0; rb < r1; rb += 64) {
-        stage(rb);
-        const int nr = min(64, r1 - rb);
-        int q = 0;
-        for (; q + 4 <= nr; q += 4) {  // four rows of M in flight per wait
-          double ma[4], mc[4];
-#pragma unroll
-          for (int t = 0; t < 4; ++t) {
-            const int64_t ro = (int64_t)(rb + q + t) * ldm;
-            ma[t] = ona ? M[ro + ca] : 0.0;
-            mc[t] = onc ? M[ro + cc] : 0.0;
-          }
-#pragma unroll
-          for (int t = 0; t < 4; ++t) {
-            double v[PNB];
-            vrow(reg, q + t, v);
-#pragma unroll
-            for (int j = 0; j < PNB; ++j) {
-              wa[j] = __builtin_fma(v[j], ma[t], wa[j]);
-              wc[j] = __builtin_fma(v[j], mc[t], wc[j]);
-            }
-          }
+        for (int q = 0; q < 4; ++q) {
+          const int rr = r + 4 * q + lk;
+          a[q] = vref(V, ldv, rr, li, mr, nb);
+          b[q] = con ? M[(int64_t)rr * ldm + col] : 0.0;
         }
-        for (; q < nr; ++q) {
-          const int64_t ro = (int64_t)(rb + q) * ldm;
-          const double ma = ona ? M[ro + ca] : 0.0, mc = onc ? M[ro + cc] : 0.0;
-          double v[PNB];
-          vrow(reg, q, v);
-#pragma unroll
-          for (int j = 0; j < PNB; ++j) {
-            wa[j] = __builtin_fma(v[j], ma, wa[j]);
-            wc[j] = __builtin_fma(v[j], mc, wc[j]);
-          }
-        }
+        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0], b[0], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1], b[1], acc1, 0, 0, 0);
+        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[2], b[2], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[3], b[3], acc1, 0, 0, 0);
+      }
+      for (; r < re; r += 4) {
+        const int rr = r + lk;
+        const bool rin = rr < re;
+        const double a = rin ? vref(V, ldv, rr, li, mr, nb) : 0.0;
+        const double b = (rin && con) ? M[(int64_t)rr * ldm + col] : 0.0;
+        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc0, 0, 0, 0);
       }
 #pragma unroll
-      for (int j = 0; j < PNB; ++j) {
-        reg[j * 128 + lane] = wa[j];
-        reg[j * 128 + 64 + lane] = wc[j];
+      for (int e = 0; e < 4; ++e) Pw[u * 256 + e * 64 + lane] = acc0[e] + acc1[e];
+    }
+    __syncthreads();
+    // reduce the row chunks: W[t][j][c] with j = lane/16 + 4e, c = lane%16
+    for (int x = tid; x < nct * 256; x += blockDim.x) {
+      const int t = x >> 8, y = x & 255;
+      const int e = y >> 6, ln = y & 63;
+      double sum = 0.0;
+      for (int h = 0; h < nch; ++h) sum += Pw[(t * nch + h) * 256 + y];
+      Wt[t * 256 + ((ln >> 4) + 4 * e) * 16 + (ln & 15)] = sum;
+    }
+    __syncthreads();
+    // U = op(T) W, stored negated for the update
+    for (int x = tid; x < nct * 256; x += blockDim.x) {
+      const int t = x >> 8, i = (x >> 4) & 15, c = x & 15;
+      double sum = 0.0;
+#pragma unroll
+      for (int j = 0; j < PNB; ++j) sum = __builtin_fma(trans ? Tl[j * PNB + i] : Tl[i * PNB + j], Wt[t * 256 + j * 16 + c], sum);
+      Ut[t * 256 + i * 16 + c] = -sum;
+    }
+    __syncthreads();
+    // M -= V U, one (tile, 16-row block) at a time
+    const int nrb = (mr + 15) >> 4;
+    for (int x = wv; x < nct * nrb; x += nwv) {
+      const int t = x / nrb, rb = (x - t * nrb) * 16;
+      const int col = cg + t * 16 + li;
+      const bool con = col < c1;
+      double bu[4], a[4];
+      f64x4 cm;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        bu[q] = Ut[t * 256 + (4 * q + lk) * 16 + li];        // B = -U[4q + k][c]
+        a[q] = vref(V, ldv, rb + li, 4 * q + lk, mr, nb);     // A = V[row][4q + k]
+        const int rr = rb + lk + 4 * q;
+        cm[q] = (con && rr < mr) ? M[(int64_t)rr * ldm + col] : 0.0;
       }
-    }
-    __syncthreads();
-    for (int e = tid; e < PNB * 128; e += blockDim.x) {
-      double t = 0.0;
-      for (int w = 0; w < nwv; ++w) t += L[w * TALL_REGION + e];
-      Wf[e] = t;
-    }
-    __syncthreads();
-    // U[i][col] = sum_j op(T)[i][j] W[j][col] into region 0
-    for (int e = tid; e < PNB * 128; e += blockDim.x) {
-      const int i = e >> 7, col = e & 127;
-      double t = 0.0;
 #pragma unroll
-      for (int j = 0; j < PNB; ++j) t = __builtin_fma(trans ? Tl[j * PNB + i] : Tl[i * PNB + j], Wf[j * 128 + col], t);
-      L[e] = t;
-    }
-    __syncthreads();
-    double ua[PNB], uc[PNB];
+      for (int q = 0; q < 4; ++q) cm = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], bu[q], cm, 0, 0, 0);
 #pragma unroll
-    for (int i = 0; i < PNB; ++i) {
-      ua[i] = L[i * 128 + lane];
-      uc[i] = L[i * 128 + 64 + lane];
-    }
-    __syncthreads();  // region 0 is staging space again below
-    // M -= V U over this wavefront's rows
-    for (int rb = r0; rb < r1; rb += 64) {
-      stage(rb);
-      const int nr = min(64, r1 - rb);
-      for (int q = 0; q < nr; ++q) {
-        const int64_t ro = (int64_t)(rb + q) * ldm;
-        double v[PNB];
-        vrow(reg, q, v);
-        double sa = 0.0, sc = 0.0;
-#pragma unroll
-        for (int j = 0; j < PNB; ++j) {
-          sa = __builtin_fma(v[j], ua[j], sa);
-          sc = __builtin_fma(v[j], uc[j], sc);
-        }
-        if (ona) M[ro + ca] -= sa;
-        if (onc) M[ro + cc] -= sc;
+      for (int e = 0; e < 4; ++e) {
+        const int rr = rb + lk + 4 * e;
+        if (con && rr < mr) M[(int64_t)rr * ldm + col] = cm[e];
       }
     }
     __syncthreads();
@@ -1582,8 +1585,8 @@ __device__ __noinline__ void mode_product_staged(const double* Rrows, const int*
 // scratch.  Two separate instantiations (one per memory) let the compiler use
 // LDS instructions on the LDS path instead of generic flat addressing.
 template <int G, bool REG>
-__device__ __forceinline__ int run_move(double* ws, double* jpad, lds_f64* pnl, const int pnl_cap, const int pmode,
-                                        const int rows,
+__device__ __forceinline__ int run_move(double* ws, const bool gws, double* jpad, lds_f64* pnl, const int pnl_cap,
+                                        const int pmode, const int rows,
                                         const int b, const int tr, const int rpl, const int bx,
                                         const int64_t Au, const int64_t nu, const int64_t Bu, double* Xc,
                                         const double* Uc, double* Ud, double* gY, double cutoff, long long maxdim,
@@ -1741,8 +1744,19 @@ __device__ __forceinline__ int run_move(double* ws, double* jpad, lds_f64* pnl, 
         for (int e = tid; e < n * n; e += bs) Jb[e] = Jl[(e / n) * ldj + e % n];
         __syncthreads();
       }
+    } else if (gws && b <= 512) {
+      // workspace in global memory: both rows of a pair (and of J) in
+      // registers, 32 lanes per pair (two pairs per wavefront; 64 for rows
+      // over 384), all loads of a pair in flight at once
+      lds_i32* fl = (lds_i32*)iflag;
+      if (b <= 128) conv = jacobi_rows_reg<32, 4, double>(X, b, n, b, Jb, n, fl, red);
+      else if (b <= 256) conv = jacobi_rows_reg<32, 8, double>(X, b, n, b, Jb, n, fl, red);
+      else if (b <= 384) conv = jacobi_rows_reg<32, 12, double>(X, b, n, b, Jb, n, fl, red);
+      else conv = jacobi_rows_reg<64, 8, double>(X, b, n, b, Jb, n, fl, red);
     } else {
-      conv = jacobi_rows<G>(X, n, b, Jb, iflag, red);
+      // eight lanes per pair: every group of a wavefront has a pair of its
+      // own, so the rows of 64 pairs are read at once
+      conv = jacobi_rows<8>(X, n, b, Jb, iflag, red);
     }
     if (!conv) {
       if (tid == 0) status[1] = 1;
@@ -1911,17 +1925,17 @@ __global__ void __launch_bounds__(512) truncate_kernel(const SweepArgs* __restri
     int m;
     if (need <= a.ws_lds) {
       if (small)
-        m = run_move<8, true>(wsl, jpad, nullptr, 0, 0, rows, b, tr, rpl, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<8, true>(wsl, false, jpad, nullptr, 0, 0, rows, b, tr, rpl, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
                         iflag, red, dbc);
       else
-        m = run_move<64, false>(wsl, nullptr, nullptr, 0, 0, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<64, false>(wsl, false, nullptr, nullptr, 0, 0, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
                          iflag, red, dbc);
     } else {
       if (small)
-        m = run_move<8, false>(a.gscr, nullptr, pnl, pnl_cap, pmode, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<8, false>(a.gscr, true, nullptr, pnl, pnl_cap, pmode, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
                         iflag, red, dbc);
       else
-        m = run_move<64, false>(a.gscr, nullptr, pnl, pnl_cap, pmode, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<64, false>(a.gscr, true, nullptr, pnl, pnl_cap, pmode, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
                          iflag, red, dbc);
     }
     QTT_PHASE(st, 5);
