@@ -228,7 +228,7 @@ def _time_gpu(fn, reps, torch, stream):
 
 
 def other_workloads(q, po, ctx, stream, args):
-    """The 1D chain workload and the gradient workload, timed in the same run."""
+    """The 1D chain, gradient, 3D comb and batch workloads, timed in the same run."""
     import torch
     from itensornumericalanalysis_jl_amd import workloads
     out = {}
@@ -281,6 +281,36 @@ def other_workloads(q, po, ctx, stream, args):
                       "forward_flops_per_point": flops_fwd,
                       "gradient_tflops_3x_forward": 3 * flops_fwd * len(w4.xs) / gt / 1e12,
                       "grad_rel_err_vs_oracle_2^14_subset": num / den}
+    # 3D comb: 3 x 16 digits, dense random cores of bond 128 (45 MB), d/dx then
+    # truncation to maxdim 128, and the full integral (dataflow kernel)
+    w3 = workloads.comb_3d()
+    t3 = w3.f.topo
+    core_bytes = 8 * sum(int(c.size) for c in w3.f.cores)
+    i3 = q.integrate(w3.f)  # uploads the cores
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        i3 = q.integrate(w3.f)
+    t_int3 = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    i3ref = po.integrate(t3.parent, t3.sdim, w3.f.cores, 0.5)
+    t_int3_cpu = time.perf_counter() - t0
+    g3, e3 = q.operate(w3.op, w3.f, cutoff=w3.cutoff, maxdim=w3.maxdim, return_error=True)
+    torch.cuda.synchronize()
+    del g3
+    t0 = time.perf_counter()
+    g3, e3 = q.operate(w3.op, w3.f, cutoff=w3.cutoff, maxdim=w3.maxdim, return_error=True)
+    torch.cuda.synchronize()
+    t_op3 = time.perf_counter() - t0
+    out["comb_3d"] = {"integrate_ms": t_int3 * 1e3, "core_bytes": core_bytes,
+                      "integrate_gbs": core_bytes / t_int3 / 1e9,
+                      "integrate_rel_err_vs_oracle": abs(i3 - i3ref) / abs(i3ref),
+                      "cpu_integrate_ms_1_thread": t_int3_cpu * 1e3,
+                      "operate_truncate_ms": t_op3 * 1e3, "trunc_err": e3,
+                      "result_bond_max": int(max(g3.bonds())),
+                      "note": "integrate_ms is a synchronous call (host plan + launch + copy back); "
+                              "operate parity at full size is not rerun here (the oracle takes ~75 s on one core)"}
+    del g3
     # batch: 512 chains of uneven digit counts and bonds, Fredholm kernel operator
     wsb = workloads.fredholm_batch()
     opsb, fsb = [w.op for w in wsb], [w.f for w in wsb]

@@ -939,226 +939,8 @@ __device__ void build_T(const lds_f64* V, int mr, int nb, const lds_f64* tau, ld
   __syncthreads();
 }
 
-// Column c (one lane) of a matrix M, rows from the panel's first row on:
-// M[:, c] <- (I - V op(T) V^T) M[:, c], op(T) = T^T (trans) or T.  at(r) is the
-// element in panel row r (r < mr).  Panels are padded to PNB reflectors with
-// zero rows and columns in T, so the shapes are compile-time; Vt holds the
-// panel's first PNB rows of V (unit lower triangular), V the rest.
 typedef double f64x2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(3))) f64x2 lds_f64x2;
-
-// row r of an LDS matrix of row stride VLD: two entries per 16-byte load when
-// the rows are aligned (VLD = PNB), else one per 8-byte load (VLD = PLD)
-template <int VLD = PNB>
-__device__ __forceinline__ void vrow(const lds_f64* V, int r, double (&v)[PNB]) {
-  if constexpr (VLD == PNB) {
-    const lds_f64x2* p = (const lds_f64x2*)(V + r * PNB);
-#pragma unroll
-    for (int j = 0; j < PNB / 2; ++j) {
-      const f64x2 x = p[j];
-      v[2 * j] = x.x;
-      v[2 * j + 1] = x.y;
-    }
-  } else {
-#pragma unroll
-    for (int j = 0; j < PNB; ++j) v[j] = V[r * VLD + j];
-  }
-}
-
-template <int VLD, typename At>
-__device__ __forceinline__ void wy_column(At at, int mr, const lds_f64* V, const lds_f64* Vt, const lds_f64* T,
-                                          bool trans) {
-  double w[PNB];
-#pragma unroll
-  for (int j = 0; j < PNB; ++j) w[j] = 0.0;
-  const int top = mr < PNB ? mr : PNB;
-  for (int r = 0; r < top; ++r) {
-    const double a = at(r);
-    double v[PNB];
-    vrow(Vt, r, v);
-#pragma unroll
-    for (int j = 0; j < PNB; ++j) w[j] = __builtin_fma(v[j], a, w[j]);
-  }
-  // the column lives in global memory: CB of its entries in flight per wait
-  constexpr int CB = 8;
-  int r = top;
-  for (; r + CB <= mr; r += CB) {
-    double a[CB];
-#pragma unroll
-    for (int q = 0; q < CB; ++q) a[q] = at(r + q);
-#pragma unroll 2
-    for (int q = 0; q < CB; ++q) {
-      double v[PNB];
-      vrow<VLD>(V, r + q, v);
-#pragma unroll
-      for (int j = 0; j < PNB; ++j) w[j] = __builtin_fma(v[j], a[q], w[j]);
-    }
-  }
-  for (; r < mr; ++r) {
-    const double a = at(r);
-    double v[PNB];
-    vrow<VLD>(V, r, v);
-#pragma unroll
-    for (int j = 0; j < PNB; ++j) w[j] = __builtin_fma(v[j], a, w[j]);
-  }
-  double u[PNB];
-#pragma unroll
-  for (int i = 0; i < PNB; ++i) {
-    double acc = 0.0;
-    if (trans) {
-#pragma unroll
-      for (int j = 0; j <= i; ++j) acc = __builtin_fma(T[j * PNB + i], w[j], acc);
-    } else {
-#pragma unroll
-      for (int j = i; j < PNB; ++j) acc = __builtin_fma(T[i * PNB + j], w[j], acc);
-    }
-    u[i] = acc;
-  }
-  for (r = 0; r < top; ++r) {
-    double v[PNB];
-    vrow(Vt, r, v);
-    double acc = 0.0;
-#pragma unroll
-    for (int j = 0; j < PNB; ++j) acc = __builtin_fma(v[j], u[j], acc);
-    at(r) -= acc;
-  }
-  r = top;
-  for (; r + CB <= mr; r += CB) {
-    double a[CB];
-#pragma unroll
-    for (int q = 0; q < CB; ++q) a[q] = at(r + q);
-#pragma unroll 2
-    for (int q = 0; q < CB; ++q) {
-      double v[PNB];
-      vrow<VLD>(V, r + q, v);
-      double acc0 = 0.0, acc1 = 0.0;
-#pragma unroll
-      for (int j = 0; j < PNB; j += 2) {
-        acc0 = __builtin_fma(v[j], u[j], acc0);
-        acc1 = __builtin_fma(v[j + 1], u[j + 1], acc1);
-      }
-      at(r + q) = a[q] - (acc0 + acc1);
-    }
-  }
-  for (; r < mr; ++r) {
-    double v[PNB];
-    vrow<VLD>(V, r, v);
-    double acc = 0.0;
-#pragma unroll
-    for (int j = 0; j < PNB; ++j) acc = __builtin_fma(v[j], u[j], acc);
-    at(r) -= acc;
-  }
-}
-
-// After a panel's reflectors are in Pp (mr x PNB, ld PLD, columns past nb
-// zero) and their scalars in tl: zero the scalars of the padding columns,
-// copy the reflectors to Pl (ld PNB: rows aligned for 16-byte loads) and build
-// Vt, the first PNB rows of V with the unit diagonal and zeros above it.
-template <int VLD>
-__device__ __forceinline__ void panel_pad(const lds_f64* Pp, int mr, int nb, lds_f64* tl, lds_f64* Pl, lds_f64* Vt) {
-  const int tid = threadIdx.x, bs = blockDim.x;
-  for (int e = tid; e < PNB * PNB; e += bs) {
-    const int r = e / PNB, j = e - r * PNB;
-    Vt[e] = r >= mr ? 0.0 : j < r ? Pp[r * PLD + j] : (j == r ? 1.0 : 0.0);
-  }
-  if constexpr (VLD == PNB) {  // else the updates read Pp itself (Pl == Pp)
-    for (int e = tid; e < mr * PNB; e += bs) {
-      const int r = e / PNB, j = e - r * PNB;
-      Pl[e] = Pp[r * PLD + j];
-    }
-  }
-  for (int e = nb + tid; e < PNB; e += bs) tl[e] = 0.0;
-  __syncthreads();
-}
-
-// QR of W (global, rows x cols, leading dimension ld): R on and above the
-// diagonal, reflectors below, tau[k].  Pp: rows x PLD LDS panel; Pl: its
-// aligned rows x PNB copy (VLD = PNB) or Pp itself (VLD = PLD, for panels too
-// tall for two copies); Tl, Yl: PNB x PNB, tl: PNB scalars (all LDS).
-template <int VLD>
-__device__ __noinline__ void qr_panel(double* W, int rows_in, int cols_in, int ld_in, double* tau, lds_f64* Pp,
-                                      lds_f64* Pl, lds_f64* Tl, lds_f64* Yl, lds_f64* tl, lds_f64* Vt) {
-  const int tid = threadIdx.x, bs = blockDim.x;
-  const int rows = uni(rows_in), cols = uni(cols_in), ld = uni(ld_in);
-  const int k = rows < cols ? rows : cols;
-  QTT_PT_INIT;
-  for (int p0 = 0; p0 < k; p0 += PNB) {
-    const int nb = k - p0 < PNB ? k - p0 : PNB, mr = rows - p0;
-    for (int e = tid; e < mr * PNB; e += bs) {
-      const int r = e / PNB, c = e - r * PNB;
-      Pp[r * PLD + c] = c < nb ? W[(int64_t)(p0 + r) * ld + p0 + c] : 0.0;
-    }
-    __syncthreads();
-    QTT_PT(0);
-    geqr2<64, lds_f64*>(Pp, mr, nb, PLD, tl);  // a wavefront per column: few rows per lane
-    __syncthreads();
-    QTT_PT(1);
-    for (int e = tid; e < mr * nb; e += bs) {
-      const int r = e / nb, c = e - r * nb;
-      W[(int64_t)(p0 + r) * ld + p0 + c] = Pp[r * PLD + c];
-    }
-    for (int e = tid; e < nb; e += bs) tau[p0 + e] = tl[e];
-    QTT_PT(2);
-    if (p0 + nb < cols) {
-      panel_pad<VLD>(Pp, mr, nb, tl, Pl, Vt);
-      build_T(Pp, mr, nb, tl, Tl, Yl);
-      QTT_PT(3);
-      for (int c = p0 + nb + tid; c < cols; c += bs) {
-        double* col = W + (int64_t)p0 * ld + c;
-        wy_column<VLD>([=](int r) -> double& { return col[(int64_t)r * ld]; }, mr, Pl, Vt, Tl, true);
-      }
-      QTT_PT(4);
-    }
-    __syncthreads();
-    QTT_PT(5);
-    if (threadIdx.x == 0) QTT_PANEL_COUNT;
-  }
-}
-
-// x <- Q (rows x k) of the reflectors in W (global, ld) written straight into
-// the core Xc (rows split as (aa, bb), bb < bx, bond axis of size k); or, with
-// Jb, x <- U = Q [J_perm^T; 0] (rows x m, row i of J of length n).  Panels as
-// in qr_panel.
-template <int VLD>
-__device__ __noinline__ void q_form_panel(const double* W, int rows_in, int k_in, int ld_in, const double* tau,
-                                          double* Xc, int bx_in, const double* Jb, int n_in, const int* perm, int m_in,
-                                          lds_f64* Pp, lds_f64* Pl, lds_f64* Tl, lds_f64* Yl, lds_f64* tl,
-                                          lds_f64* Vt) {
-  const int tid = threadIdx.x, bs = blockDim.x;
-  const int rows = uni(rows_in), k = uni(k_in), ld = uni(ld_in), bx = uni(bx_in), n = uni(n_in), m = uni(m_in);
-  const int ncol = Jb ? m : k;
-  for (int e = tid; e < rows * ncol; e += bs) {
-    const int r = e / ncol, c = e - r * ncol;
-    const int aa = r / bx, bb = r - aa * bx;
-    double v;
-    if (Jb) v = r < n ? Jb[(int64_t)perm[c] * n + r] : 0.0;
-    else v = r == c ? 1.0 : 0.0;
-    Xc[((int64_t)aa * ncol + c) * bx + bb] = v;
-  }
-  __syncthreads();
-  for (int p0 = ((k - 1) / PNB) * PNB; p0 >= 0; p0 -= PNB) {
-    const int nb = k - p0 < PNB ? k - p0 : PNB, mr = rows - p0;
-    for (int e = tid; e < mr * PNB; e += bs) {
-      const int r = e / PNB, c = e - r * PNB;
-      Pp[r * PLD + c] = c < nb ? W[(int64_t)(p0 + r) * ld + p0 + c] : 0.0;
-    }
-    for (int e = tid; e < nb; e += bs) tl[e] = tau[p0 + e];
-    __syncthreads();
-    panel_pad<VLD>(Pp, mr, nb, tl, Pl, Vt);
-    build_T(Pp, mr, nb, tl, Tl, Yl);
-    // Q: only columns >= p0 differ from the identity yet; U: every column
-    const int c0 = Jb ? 0 : p0;
-    for (int c = c0 + tid; c < ncol; c += bs) {
-      wy_column<VLD>(
-          [=](int r) -> double& {
-            const int rr = p0 + r, aa = rr / bx;
-            return Xc[((int64_t)aa * ncol + c) * bx + (rr - aa * bx)];
-          },
-          mr, Pl, Vt, Tl, false);
-    }
-    __syncthreads();
-  }
-}
 
 // ---------------------------------------------------------------------------
 // Blocked Householder QR for matrices too tall for an LDS panel.
@@ -1454,6 +1236,52 @@ __device__ __noinline__ void q_form_tall(const double* W, int rows_in, int k_in,
   __syncthreads();
 }
 
+// QR of W (global rows x cols, ld) whose panels fit LDS (rows * PLD <= 16384):
+// each panel is factored in LDS (a wavefront per column), its reflectors and
+// T written back (T of every panel to Tg, for q_form_tall), and the trailing
+// columns updated by tall_wy with the reflectors read from W.  L: the tall
+// layout (TALL_LDS doubles); the panel lives in its first part, which tall_wy
+// reuses once the panel is written back.
+__device__ __noinline__ void qr_panel(double* W, int rows_in, int cols_in, int ld_in, double* tau, double* Tg,
+                                      lds_f64* L) {
+  const int tid = threadIdx.x, bs = blockDim.x;
+  const int rows = uni(rows_in), cols = uni(cols_in), ld = uni(ld_in);
+  const int k = rows < cols ? rows : cols;
+  lds_f64* Pp = L;                             // mr x PLD
+  lds_f64* Yl = L + 8 * TALL_REGION;           // PNB x PNB
+  lds_f64* tl = Yl + PNB * PNB;                // PNB
+  lds_f64* Tl = L + 8 * TALL_REGION + 16 * 128;
+  QTT_PT_INIT;
+  for (int p0 = 0; p0 < k; p0 += PNB) {
+    const int nb = k - p0 < PNB ? k - p0 : PNB, mr = rows - p0;
+    for (int e = tid; e < mr * PNB; e += bs) {
+      const int r = e / PNB, c = e - r * PNB;
+      Pp[r * PLD + c] = c < nb ? W[(int64_t)(p0 + r) * ld + p0 + c] : 0.0;
+    }
+    __syncthreads();
+    QTT_PT(0);
+    geqr2<64, lds_f64*>(Pp, mr, nb, PLD, tl);  // a wavefront per column: few rows per lane
+    __syncthreads();
+    QTT_PT(1);
+    for (int e = tid; e < mr * nb; e += bs) {
+      const int r = e / nb, c = e - r * nb;
+      W[(int64_t)(p0 + r) * ld + p0 + c] = Pp[r * PLD + c];
+    }
+    for (int e = tid; e < nb; e += bs) tau[p0 + e] = tl[e];
+    for (int e = nb + tid; e < PNB; e += bs) tl[e] = 0.0;  // padding reflectors
+    __syncthreads();
+    QTT_PT(2);
+    build_T(Pp, mr, nb, tl, Tl, Yl);
+    for (int e = tid; e < PNB * PNB; e += bs) Tg[(p0 / PNB) * PNB * PNB + e] = Tl[e];
+    QTT_PT(3);
+    if (p0 + nb < cols)
+      tall_wy(W + (int64_t)p0 * ld, ld, mr, p0 + nb, cols, W + (int64_t)p0 * ld + p0, ld, nb, Tl, true, L);
+    __syncthreads();
+    QTT_PT(4);
+    if (threadIdx.x == 0) QTT_PANEL_COUNT;
+  }
+}
+
 template <int G>
 __device__ inline void row_norms(const double* X, int n, int b, double* sig) {
   const int gl = threadIdx.x & (G - 1), gid = threadIdx.x / G, ng = blockDim.x / G;
@@ -1635,10 +1463,11 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, double* jpad
       if (rpl == 8) q_form_reg<8, 8>(Wl, rows, k, taul, Xc, bx);
       else if (rpl == 16) q_form_reg<8, 16>(Wl, rows, k, taul, Xc, bx);
       else q_form_reg<16, 12>(Wl, rows, k, taul, Xc, bx);
-    } else if (pmode == 3) {  // too tall for an LDS panel: panels in global memory
+    } else if (pmode) {  // workspace in global memory: blocked QR, WY updates as MFMA tiles
       double* Tg = vbuf;
       double* Mq = Tg + (((int64_t)k + PNB) * PNB);
-      qr_tall(W, rows, b, b, tau, Tg, pnl);
+      if (pmode == 3) qr_tall(W, rows, b, b, tau, Tg, pnl);  // panels in global memory
+      else qr_panel(W, rows, b, b, tau, Tg, pnl);            // panels in LDS
       for (int e = tid; e < k * b; e += bs) {
         const int i = e / b, c = e - i * b;
         Rb[e] = c >= i ? W[(int64_t)i * b + c] : 0.0;
@@ -1646,21 +1475,6 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, double* jpad
       __syncthreads();
       QTT_PHASE(-1, 2);  // factored
       q_form_tall(W, rows, k, b, Tg, Mq, Xc, bx, nullptr, 0, nullptr, 0, pnl);  // x <- Q (bond m)
-    } else if (pnl) {  // workspace in global memory: blocked QR, panels in LDS
-      lds_f64 *Pp = pnl, *Pl = pmode == 1 ? Pp + ((rows * PLD + 1) & ~1) : Pp,
-              *Tl = pmode == 1 ? Pl + rows * PNB : Pp + ((rows * PLD + 1) & ~1), *Yl = Tl + PNB * PNB,
-              *tl = Yl + PNB * PNB, *Vt = tl + PNB;
-      if (pmode == 1) qr_panel<PNB>(W, rows, b, b, tau, Pp, Pl, Tl, Yl, tl, Vt);
-      else qr_panel<PLD>(W, rows, b, b, tau, Pp, Pl, Tl, Yl, tl, Vt);
-      for (int e = tid; e < k * b; e += bs) {
-        const int i = e / b, c = e - i * b;
-        Rb[e] = c >= i ? W[(int64_t)i * b + c] : 0.0;
-      }
-      __syncthreads();
-      QTT_PHASE(-1, 2);  // factored
-      // x <- Q (bond m)
-      if (pmode == 1) q_form_panel<PNB>(W, rows, k, b, tau, Xc, bx, nullptr, 0, nullptr, 0, Pp, Pl, Tl, Yl, tl, Vt);
-      else q_form_panel<PLD>(W, rows, k, b, tau, Xc, bx, nullptr, 0, nullptr, 0, Pp, Pl, Tl, Yl, tl, Vt);
     } else {
       geqr2<G>(W, rows, b, b, tau);
       for (int e = tid; e < k * b; e += bs) {
@@ -1687,18 +1501,9 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, double* jpad
         if (rpl == 8) qr_reg<8, 8, 2>(Wl, rows, b, taul, Rl);
         else if (rpl == 16) qr_reg<8, 16, 2>(Wl, rows, b, taul, Rl);
         else qr_reg<16, 12, 3>(Wl, rows, b, taul, Rl);
-      } else if (pmode == 3) {  // tall QR in global memory; the reflectors stay in W for U below
-        qr_tall(W, rows, b, b, tau, vbuf, pnl);
-        for (int e = tid; e < b * b; e += bs) {
-          const int i = e / b, c = e - i * b;
-          Rb[e] = c >= i ? W[(int64_t)i * b + c] : 0.0;
-        }
-      } else if (pnl) {  // blocked QR; the reflectors stay in W for U below
-        lds_f64 *Pp = pnl, *Pl = pmode == 1 ? Pp + ((rows * PLD + 1) & ~1) : Pp,
-                *Tl = pmode == 1 ? Pl + rows * PNB : Pp + ((rows * PLD + 1) & ~1), *Yl = Tl + PNB * PNB,
-                *tl = Yl + PNB * PNB, *Vt = tl + PNB;
-        if (pmode == 1) qr_panel<PNB>(W, rows, b, b, tau, Pp, Pl, Tl, Yl, tl, Vt);
-        else qr_panel<PLD>(W, rows, b, b, tau, Pp, Pl, Tl, Yl, tl, Vt);
+      } else if (pmode) {  // blocked QR; the reflectors stay in W for U below
+        if (pmode == 3) qr_tall(W, rows, b, b, tau, vbuf, pnl);
+        else qr_panel(W, rows, b, b, tau, vbuf, pnl);
         for (int e = tid; e < b * b; e += bs) {
           const int i = e / b, c = e - i * b;
           Rb[e] = c >= i ? W[(int64_t)i * b + c] : 0.0;
@@ -1801,14 +1606,8 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, double* jpad
     Rrows = X;
     use_perm = true;
     // x <- U (bond m): U[row][i] = sum_j Q[row][j] J[perm[i]][j]  (or J[perm[i]][row])
-    if (tall && pmode == 3) {
+    if (tall && pmode) {
       q_form_tall(W, rows, b, b, vbuf, vbuf + (((int64_t)b + PNB) * PNB), Xc, bx, Jb, n, perm, m, pnl);
-    } else if (tall && pnl) {
-      lds_f64 *Pp = pnl, *Pl = pmode == 1 ? Pp + ((rows * PLD + 1) & ~1) : Pp,
-              *Tl = pmode == 1 ? Pl + rows * PNB : Pp + ((rows * PLD + 1) & ~1), *Yl = Tl + PNB * PNB,
-              *tl = Yl + PNB * PNB, *Vt = tl + PNB;
-      if (pmode == 1) q_form_panel<PNB>(W, rows, b, b, tau, Xc, bx, Jb, n, perm, m, Pp, Pl, Tl, Yl, tl, Vt);
-      else q_form_panel<PLD>(W, rows, b, b, tau, Xc, bx, Jb, n, perm, m, Pp, Pl, Tl, Yl, tl, Vt);
     } else if (tall && use_reg) {
       const lds_f64 *Wl = (const lds_f64*)W, *taul = (const lds_f64*)tau, *Jl = (const lds_f64*)Jb;
       const lds_i32* pl = (const lds_i32*)perm;
@@ -1897,17 +1696,13 @@ __global__ void __launch_bounds__(512) truncate_kernel(const SweepArgs* __restri
       rpl = 0;
       need = (int64_t)rows * b + base;
     }
-    // global workspace: LDS holds the QR panels (rows x PLD and its aligned
-    // rows x PNB copy when both fit (pmode 1), else the first alone (pmode 2);
-    // T, Y, tau), or for taller matrices the tall QR's staging (pmode 3)
+    // global workspace: the blocked QR's LDS (TALL_LDS doubles) holds the
+    // panel when rows * PLD fit its first part (pmode 1), else the panels stay
+    // in global memory (pmode 3)
     lds_f64* pnl = nullptr;
     int pnl_cap = 0;  // doubles of LDS at pnl
     int pmode = 0;
-    if (need > a.ws_lds) {
-      const int64_t fixed = 3 * PNB * PNB + PNB + 8;
-      pmode = (int64_t)rows * (PNB + PLD) + fixed <= a.ws_lds ? 1 : (int64_t)rows * PLD + fixed <= a.ws_lds ? 2 : 0;
-      if (!pmode && b <= 4 * rows && TALL_LDS + 64 <= a.ws_lds) pmode = 3;  // tall: panels in global memory
-    }
+    if (need > a.ws_lds && TALL_LDS + 64 <= a.ws_lds) pmode = (int64_t)rows * PLD <= 8 * TALL_REGION ? 1 : 3;
     if (pmode) {
       const int off = (64 + ((t.nv + 1) >> 1) + 1) & ~1;  // 16-byte aligned for the row loads
       pnl = (lds_f64*)(lds + off);
