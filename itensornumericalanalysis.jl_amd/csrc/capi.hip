@@ -107,6 +107,22 @@ static bool same_topo(const HostTopo& a, const HostTopo& b) {
          a.vdigit == b.vdigit;
 }
 
+// the packed int arrays of a topology, as topo_view reads them
+static std::vector<int> topo_ints(const HostTopo& t) {
+  int nv = t.nv;
+  size_t n = size_t(7) * nv + std::max(t.ncoord, 1);
+  std::vector<int> h(n, 0);
+  std::copy(t.parent.begin(), t.parent.end(), h.begin() + 0 * nv);
+  std::copy(t.sdim.begin(), t.sdim.end(), h.begin() + 1 * nv);
+  std::copy(t.vcoord.begin(), t.vcoord.end(), h.begin() + 2 * nv);
+  std::copy(t.vdigit.begin(), t.vdigit.end(), h.begin() + 3 * nv);
+  std::copy(t.nch.begin(), t.nch.end(), h.begin() + 4 * nv);
+  std::copy(t.choff.begin(), t.choff.end(), h.begin() + 5 * nv);
+  std::copy(t.chs.begin(), t.chs.begin() + std::max(0, nv - 1), h.begin() + 6 * nv);
+  std::copy(t.ndigits.begin(), t.ndigits.end(), h.begin() + 7 * nv);
+  return h;
+}
+
 static int upload_topo(qtt_ctx* ctx, const HostTopo& t, int** d_topo) {
   int nv = t.nv;
   size_t n = size_t(7) * nv + std::max(t.ncoord, 1);
@@ -124,7 +140,10 @@ static int upload_topo(qtt_ctx* ctx, const HostTopo& t, int** d_topo) {
   return QTT_OK;
 }
 
-static TopoView make_view(const int* d, const HostTopo& t) {
+TopoView topo_view(const int* d, const HostTopo& t);
+static TopoView make_view(const int* d, const HostTopo& t) { return topo_view(d, t); }
+
+TopoView topo_view(const int* d, const HostTopo& t) {
   TopoView v;
   int nv = t.nv;
   v.parent = d;
@@ -385,10 +404,12 @@ int qtt_net_copy(qtt_ctx* ctx, const qtt_net* net, qtt_net** out) {
 void qtt_net_destroy(qtt_net* n) {
   if (!n) return;
   if (n->ctx) (void)hipStreamSynchronize(n->ctx->stream);
-  if (n->d_data) (void)hipFree(n->d_data);
-  if (n->d_off) (void)hipFree(n->d_off);
-  if (n->d_topo) (void)hipFree(n->d_topo);
-  if (n->d_bond) (void)hipFree(n->d_bond);
+  if (!n->pool) {  // pooled networks release their share of the batch allocation below
+    if (n->d_data) (void)hipFree(n->d_data);
+    if (n->d_off) (void)hipFree(n->d_off);
+    if (n->d_topo) (void)hipFree(n->d_topo);
+    if (n->d_bond) (void)hipFree(n->d_bond);
+  }
   if (n->d_pack) (void)hipFree(n->d_pack);
   if (n->d_pack_off) (void)hipFree(n->d_pack_off);
   if (n->d_pack32) (void)hipFree(n->d_pack32);
@@ -556,17 +577,74 @@ int qtt_apply_batch(qtt_ctx* ctx, int n, const qtt_mpo* const* ops, const qtt_ne
   auto cleanup = [&]() {
     for (auto* m : made) qtt_net_destroy(m);
   };
-  for (int i = 0; i < n && !rc; i++) {
-    const HostTopo& t = nets[i]->t;
-    std::vector<int64_t> b(t.nv), cap(t.nv);
-    for (int v = 0; v < t.nv; v++) b[v] = ops[i]->bond[v] * nets[i]->bond[v];
-    for (int v = 0; v < t.nv; v++) {
-      int64_t s = b[v] * t.sdim[v];
-      for (int k = 0; k < t.nch[v]; k++) s *= b[t.chs[t.choff[v] + k]];
-      cap[v] = s;
+  // All outputs in two allocations (cores; offsets + topology + bonds) with
+  // one upload of the metadata, and the exact contractions in one launch.
+  if (n > 0) {
+    std::vector<int64_t> dbase(n), mbase(n);
+    int64_t dtot = 0, mtot = 0;  // doubles of cores; bytes of metadata
+    for (int i = 0; i < n && !rc; i++) {
+      const HostTopo& t = nets[i]->t;
+      qtt_net* m = new (std::nothrow) qtt_net;
+      if (!m) {
+        rc = qtt_fail(ctx, QTT_ENOMEM, "host allocation");
+        break;
+      }
+      made[i] = m;
+      m->ctx = ctx;
+      m->t = t;
+      m->bond.resize(t.nv);
+      m->cap.resize(t.nv);
+      m->off.resize(t.nv);
+      for (int v = 0; v < t.nv; v++) m->bond[v] = ops[i]->bond[v] * nets[i]->bond[v];
+      int64_t o = 0;
+      for (int v = 0; v < t.nv; v++) {
+        int64_t sz = m->bond[v] * t.sdim[v];
+        for (int k = 0; k < t.nch[v]; k++) sz *= m->bond[t.chs[t.choff[v] + k]];
+        m->cap[v] = sz;
+        m->off[v] = o;
+        o += (sz + 1) & ~int64_t(1);  // 16-byte aligned cores
+      }
+      m->total = o;
+      dbase[i] = dtot;
+      dtot += std::max<int64_t>(o, 2);
+      mbase[i] = mtot;
+      const size_t nti = size_t(7) * t.nv + std::max(t.ncoord, 1);
+      mtot += (int64_t)((t.nv * sizeof(int64_t) + nti * sizeof(int) + t.nv * sizeof(int) + 15) & ~size_t(15));
     }
-    rc = alloc_net(ctx, t, b, cap, &made[i]);
-    if (!rc) rc = launch_contract(ctx, ops[i], nets[i], made[i]);
+    double* dpool = nullptr;
+    char* mpool = nullptr;
+    if (!rc) rc = qtt_hip_check(ctx, hipMalloc(&dpool, dtot * sizeof(double)), "hipMalloc batch cores");
+    if (!rc) rc = qtt_hip_check(ctx, hipMalloc(&mpool, std::max<int64_t>(mtot, 16)), "hipMalloc batch metadata");
+    if (rc) {
+      if (dpool) (void)hipFree(dpool);
+      cleanup();
+      return rc;
+    }
+    std::shared_ptr<void> pool(dpool, [mpool](void* p) {
+      (void)hipFree(p);
+      (void)hipFree(mpool);
+    });
+    std::vector<char> meta(mtot);
+    for (int i = 0; i < n; i++) {
+      qtt_net* m = made[i];
+      const HostTopo& t = m->t;
+      char* h = meta.data() + mbase[i];
+      char* d = mpool + mbase[i];
+      const std::vector<int> ti = topo_ints(t);
+      std::vector<int> b32(t.nv);
+      for (int v = 0; v < t.nv; v++) b32[v] = (int)m->bond[v];
+      std::memcpy(h, m->off.data(), t.nv * sizeof(int64_t));
+      std::memcpy(h + t.nv * sizeof(int64_t), ti.data(), ti.size() * sizeof(int));
+      std::memcpy(h + t.nv * sizeof(int64_t) + ti.size() * sizeof(int), b32.data(), t.nv * sizeof(int));
+      m->pool = pool;
+      m->d_data = dpool + dbase[i];
+      m->d_off = (int64_t*)d;
+      m->d_topo = (int*)(d + t.nv * sizeof(int64_t));
+      m->d_bond = (int*)(d + t.nv * sizeof(int64_t) + ti.size() * sizeof(int));
+    }
+    rc = qtt_hip_check(ctx, hipMemcpyAsync(mpool, meta.data(), mtot, hipMemcpyHostToDevice, ctx->stream), "batch metadata");
+    if (!rc) rc = launch_contract_batch(ctx, n, ops, nets, made.data());
+    if (!rc) rc = qtt_hip_check(ctx, hipStreamSynchronize(ctx->stream), "batch contract sync");  // meta outlives the copy
   }
   if (!rc && cutoff >= 0.0)
     rc = launch_truncate_batch(ctx, made.data(), n, cutoff, maxdim > 0 ? maxdim : (int64_t(1) << 40), errs);

@@ -130,41 +130,81 @@ __device__ inline double block_sum(double v, double* red) {
 // ---------------------------------------------------------------------------
 // exact operator application
 // ---------------------------------------------------------------------------
+// element e (< total) of the contracted network: vertex by binary search over
+// cstart, then the flattened indices of the output core
+__device__ __forceinline__ void contract_elem(const double* __restrict__ W, const int64_t* __restrict__ woff,
+                                              const int* __restrict__ wbond, const double* __restrict__ A,
+                                              const int64_t* __restrict__ aoff, const int* __restrict__ abond,
+                                              const TopoView& t, double* __restrict__ C,
+                                              const int64_t* __restrict__ coff, const int64_t* __restrict__ cstart,
+                                              int64_t e) {
+  int lo = 0, hi = t.nv;  // find v with cstart[v] <= e < cstart[v+1]
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (cstart[mid] <= e) lo = mid; else hi = mid;
+  }
+  const int v = lo;
+  int64_t local = e - cstart[v];
+  const int m = t.nch[v];
+  const int* ch = t.chs + t.choff[v];
+  int64_t jw = 0, ja = 0, sw = 1, sa = 1;
+  for (int k = m - 1; k >= 0; --k) {
+    const int64_t bw = wbond[ch[k]], ba = abond[ch[k]];
+    const int64_t idx = local % (bw * ba);
+    local /= bw * ba;
+    jw += (idx / ba) * sw;
+    ja += (idx % ba) * sa;
+    sw *= bw;
+    sa *= ba;
+  }
+  const int d = t.sdim[v];
+  const int so = (int)(local % d);
+  local /= d;
+  const int64_t ap = abond[v];
+  const int64_t pw = local / ap, pa = local % ap;
+  double s = 0.0;
+  for (int si = 0; si < d; ++si)
+    s += W[woff[v] + ((pw * d + si) * d + so) * sw + jw] * A[aoff[v] + (pa * d + si) * sa + ja];
+  C[coff[v] + (e - cstart[v])] = s;
+}
+
 __global__ void __launch_bounds__(256)
 contract_kernel(const double* __restrict__ W, const int64_t* __restrict__ woff, const int* __restrict__ wbond,
                 const double* __restrict__ A, const int64_t* __restrict__ aoff, const int* __restrict__ abond,
                 TopoView t, double* __restrict__ C, const int64_t* __restrict__ coff,
                 const int64_t* __restrict__ cstart, int64_t total) {
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x)
+    contract_elem(W, woff, wbond, A, aoff, abond, t, C, coff, cstart, e);
+}
+
+// A batch of contractions in one launch: the elements of all networks form one
+// index space, item i owning [estart[i], estart[i+1]).
+struct ContractItem {
+  const double* W;
+  const int64_t* woff;
+  const int* wbond;
+  const double* A;
+  const int64_t* aoff;
+  const int* abond;
+  TopoView t;
+  double* C;
+  const int64_t* coff;
+  const int64_t* cstart;
+};
+
+__global__ void __launch_bounds__(256)
+contract_batch_kernel(const ContractItem* __restrict__ items, const int64_t* __restrict__ estart, int n,
+                      int64_t total) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x) {
-    int lo = 0, hi = t.nv;  // find v with cstart[v] <= e < cstart[v+1]
+    int lo = 0, hi = n;  // find i with estart[i] <= e < estart[i+1]
     while (hi - lo > 1) {
       int mid = (lo + hi) >> 1;
-      if (cstart[mid] <= e) lo = mid; else hi = mid;
+      if (estart[mid] <= e) lo = mid; else hi = mid;
     }
-    const int v = lo;
-    int64_t local = e - cstart[v];
-    const int m = t.nch[v];
-    const int* ch = t.chs + t.choff[v];
-    int64_t jw = 0, ja = 0, sw = 1, sa = 1;
-    for (int k = m - 1; k >= 0; --k) {
-      const int64_t bw = wbond[ch[k]], ba = abond[ch[k]];
-      const int64_t idx = local % (bw * ba);
-      local /= bw * ba;
-      jw += (idx / ba) * sw;
-      ja += (idx % ba) * sa;
-      sw *= bw;
-      sa *= ba;
-    }
-    const int d = t.sdim[v];
-    const int so = (int)(local % d);
-    local /= d;
-    const int64_t ap = abond[v];
-    const int64_t pw = local / ap, pa = local % ap;
-    double s = 0.0;
-    for (int si = 0; si < d; ++si)
-      s += W[woff[v] + ((pw * d + si) * d + so) * sw + jw] * A[aoff[v] + (pa * d + si) * sa + ja];
-    C[coff[v] + (e - cstart[v])] = s;
+    const ContractItem& it = items[lo];
+    contract_elem(it.W, it.woff, it.wbond, it.A, it.aoff, it.abond, it.t, it.C, it.coff, it.cstart, e - estart[lo]);
   }
 }
 
@@ -1762,6 +1802,49 @@ int launch_contract(qtt_ctx* ctx, const qtt_mpo* op, const qtt_net* net, qtt_net
   rc = qtt_hip_check(ctx, hipGetLastError(), "contract_kernel");
   if (rc) return rc;
   return qtt_hip_check(ctx, hipStreamSynchronize(ctx->stream), "contract sync");
+}
+
+int launch_contract_batch(qtt_ctx* ctx, int n, const qtt_mpo* const* ops, const qtt_net* const* nets,
+                          qtt_net* const* outs) {
+  if (n <= 0) return QTT_OK;
+  // scratch: items | estart (n + 1) | cstart of every network (nv + 1 each)
+  std::vector<int64_t> estart(n + 1, 0), cbase(n);
+  int64_t ctot = 0;
+  for (int i = 0; i < n; i++) {
+    estart[i + 1] = estart[i];
+    for (int v = 0; v < outs[i]->t.nv; v++) estart[i + 1] += outs[i]->cap[v];
+    cbase[i] = ctot;
+    ctot += outs[i]->t.nv + 1;
+  }
+  const size_t ib = ((size_t)n * sizeof(ContractItem) + 15) & ~size_t(15);
+  const size_t bytes = ib + (size_t)(n + 1 + ctot) * sizeof(int64_t);
+  char* scr = (char*)qtt_scratch(ctx, bytes + 64);
+  if (!scr) return qtt_fail(ctx, QTT_ENOMEM, "contract scratch");
+  std::vector<char> h(bytes);
+  ContractItem* hi = (ContractItem*)h.data();
+  int64_t* hes = (int64_t*)(h.data() + ib);
+  int64_t* hcs = hes + n + 1;
+  const int64_t* d_es = (const int64_t*)(scr + ib);
+  const int64_t* d_cs = d_es + n + 1;
+  std::copy(estart.begin(), estart.end(), hes);
+  for (int i = 0; i < n; i++) {
+    const qtt_net* o = outs[i];
+    int64_t* cs = hcs + cbase[i];
+    cs[0] = 0;
+    for (int v = 0; v < o->t.nv; v++) cs[v + 1] = cs[v] + o->cap[v];
+    hi[i] = {ops[i]->d_data, ops[i]->d_off, ops[i]->d_bond, nets[i]->d_data, nets[i]->d_off, nets[i]->d_bond,
+             nets[i]->view(), o->d_data, o->d_off, d_cs + cbase[i]};
+  }
+  int rc = qtt_hip_check(ctx, hipMemcpyAsync(scr, h.data(), bytes, hipMemcpyHostToDevice, ctx->stream), "contract items");
+  if (rc) return rc;
+  const int64_t total = estart[n];
+  const int64_t nb = std::min<int64_t>((total + 255) / 256, 65536);
+  contract_batch_kernel<<<dim3((unsigned)std::max<int64_t>(nb, 1)), 256, 0, ctx->stream>>>(
+      (const ContractItem*)scr, d_es, n, total);
+  rc = qtt_hip_check(ctx, hipGetLastError(), "contract_batch_kernel");
+  if (rc) return rc;
+  // the host tables must outlive the copy
+  return qtt_hip_check(ctx, hipStreamSynchronize(ctx->stream), "contract batch sync");
 }
 
 // Truncate a batch of networks, one workgroup each, in one launch.

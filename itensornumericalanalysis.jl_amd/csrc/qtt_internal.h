@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -76,6 +77,10 @@ struct qtt_net {
   // float32 copies for the mixed-precision evaluate
   float* d_pack32 = nullptr;
   bool pack32_valid = false;
+  // Networks made by one apply batch share two device allocations (cores;
+  // offsets, topology and bonds): d_data, d_off, d_topo and d_bond point into
+  // them, and the last network destroyed frees them.  Null: own allocations.
+  std::shared_ptr<void> pool;
   TopoView view() const;
   int64_t core_size(int v) const;
 };
@@ -93,6 +98,7 @@ struct qtt_mpo {
 
 // helpers implemented in capi.hip
 int qtt_fail(qtt_ctx* ctx, int code, const std::string& msg);
+TopoView topo_view(const int* d_topo, const HostTopo& t);
 int qtt_hip_check(qtt_ctx* ctx, hipError_t e, const char* what);
 double* qtt_scratch(qtt_ctx* ctx, size_t bytes);
 
@@ -105,6 +111,8 @@ int launch_integrate(qtt_ctx* ctx, const qtt_net* net, double weight, double* ou
 int launch_inner(qtt_ctx* ctx, const qtt_net* a, const qtt_net* b, double* out_host);
 int launch_integrate_batch(qtt_ctx* ctx, int n, const qtt_net* const* nets, double weight, double* out_host);
 int launch_contract(qtt_ctx* ctx, const qtt_mpo* op, const qtt_net* net, qtt_net* out);
+int launch_contract_batch(qtt_ctx* ctx, int n, const qtt_mpo* const* ops, const qtt_net* const* nets,
+                          qtt_net* const* outs);
 int launch_truncate(qtt_ctx* ctx, qtt_net* net, double cutoff, int64_t maxdim, double* err);
 int launch_truncate_batch(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double cutoff, int64_t maxdim,
                           double* errs);
