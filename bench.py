@@ -57,7 +57,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--npts", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--no-others", action="store_true", help="skip the 1D and gradient workloads")
+    ap.add_argument("--no-others", action="store_true", help="skip the 1D, gradient, comb and batch workloads")
+    ap.add_argument("--no-tune", action="store_true", help="skip the evaluate launch-shape autotuner")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="first run one device-to-device copy of MIB MiB (a reference for HBM counters)")
     args = ap.parse_args()
@@ -113,6 +114,9 @@ def main():
         kms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
         return wall, kms
 
+    # launch shape of the float64 evaluate kernel from the autotuner (times each
+    # shape on these points; later launches of this bond class and size use it)
+    tuned = None if args.no_tune else q.tune_evaluate(g, xs_d, out=out64)
     wall64, k64 = timed_eval(0, out64)
     wall32, k32 = timed_eval(1, out32)
     y64 = out64.cpu().numpy()
@@ -139,8 +143,11 @@ def main():
     achieved = flops_pt * args.npts / (k64 * 1e-3) / 1e12  # float64 kernel
     bytes_pt = 2 * 8 + 8
     chi = next(c for c in (4, 8, 16, 32, 1 << 30) if max(bonds) <= c)
-    # the launcher's choice: two points per lane up to bond 16, one up to 32, LDS vectors above
-    kname = (f"eval_chain_reg2_kernel<{chi}>" if chi <= 16 else
+    # the launcher's choice: the tuned shape, else two points per lane up to bond
+    # 16 (from 2^19 points), one up to 32, LDS vectors above
+    ppl = tuned["points_per_lane"] if tuned and tuned["points_per_lane"] else (
+        2 if chi <= 16 and args.npts >= (1 << 19) else 1)
+    kname = (f"eval_chain_reg2_kernel<{chi}>" if chi <= 16 and ppl == 2 else
              f"eval_chain_reg_kernel<{chi}>" if chi <= 32 else "eval_chain_f64_kernel")
     traffic = measured_traffic(args.npts)
 
@@ -209,7 +216,7 @@ def main():
         "flops_per_point": flops_pt,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
-                     "kernel": kname, "kernel_ms": k64,
+                     "kernel": kname, "kernel_ms": k64, "launch_shape": tuned,
                      "hbm_gbs": bytes_pt * args.npts / (k64 * 1e-3) / 1e9},
         "cpu_baseline": cpu,
         "other_workloads": others,

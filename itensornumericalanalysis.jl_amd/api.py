@@ -111,6 +111,27 @@ def evaluate(f: ITensorNetworkFunction, xs, dims=None, precision: int = Precisio
     return float(res[0]) if single else res
 
 
+def tune_evaluate(f: ITensorNetworkFunction, xs, out=None):
+    """Time the float64 register chain kernel's launch shapes on these device
+    points (a contiguous float64 CUDA tensor [npts, ndims]) and keep the fastest
+    in the engine context for later evaluate calls of the same bond class and
+    point-count octave.  Returns {"points_per_lane", "workgroup", "ms"}
+    (zeros when the network does not take that kernel)."""
+    import torch
+    dn = device_network(f)
+    ctx = dn.ctx
+    if not (isinstance(xs, torch.Tensor) and xs.is_cuda and xs.dtype == torch.float64 and xs.dim() == 2
+            and xs.is_contiguous() and xs.shape[1] == f.dimension()):
+        raise ValueError("device points must be a contiguous float64 CUDA tensor [npts, ndims]")
+    if out is None:
+        out = torch.empty(xs.shape[0], dtype=torch.float64, device=xs.device)
+    cfg = (C.c_int * 2)()
+    ms = C.c_double(0.0)
+    ctx.check(_capi.lib().qtt_tune_eval(ctx.handle, dn.handle, C.c_void_p(xs.data_ptr()), xs.shape[0],
+                                        C.c_void_p(out.data_ptr()), cfg, C.byref(ms)))
+    return {"points_per_lane": cfg[0], "workgroup": cfg[1], "ms": ms.value}
+
+
 def _sigma(f: ITensorNetworkFunction, take_sum: bool):
     """The uniform product state used by integrate(op, f) (src/integration.jl:30)."""
     c = 1.0 if take_sum else 1.0 / f.s.base

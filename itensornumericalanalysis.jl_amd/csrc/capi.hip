@@ -677,6 +677,13 @@ int qtt_eval_batch(qtt_ctx* ctx, int n, const qtt_net* const* nets, const double
   return launch_eval_batch(ctx, n, const_cast<qtt_net* const*>(nets), xs, npts, out, precision);
 }
 
+int qtt_tune_eval(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t npts, double* out, int* cfg,
+                  double* ms) {
+  if (!ctx || !net || !cfg || !ms || npts < 0 || (npts && (!xs || !out)))
+    return qtt_fail(ctx, QTT_EINVAL, "null argument");
+  return tune_eval(ctx, net, xs, npts, out, cfg, ms);
+}
+
 int qtt_truncate(qtt_ctx* ctx, qtt_net* net, double cutoff, int64_t maxdim, double* trunc_err) {
   if (!ctx || !net) return qtt_fail(ctx, QTT_EINVAL, "null argument");
   if (maxdim < 0) return qtt_fail(ctx, QTT_EINVAL, "maxdim < 0");

@@ -554,6 +554,93 @@ static int ensure_pack32(qtt_ctx* ctx, qtt_net* net) {
   return QTT_OK;
 }
 
+// Key of a tuned launch shape: the register kernel's bond class and the octave
+// of the point count.
+static uint32_t eval_tune_key(int64_t maxb, int64_t npts) {
+  const uint32_t cls = maxb <= 4 ? 0 : maxb <= 8 ? 1 : maxb <= 16 ? 2 : 3;
+  uint32_t oct = 0;
+  while (oct < 62 && (int64_t(1) << (oct + 1)) <= npts) ++oct;
+  return cls | (oct << 4);
+}
+
+// The register chain kernel (float64 chains, bonds <= 32) with ppl points per
+// lane (2 only for bonds <= 16) and wg threads per workgroup (64..256).
+static int launch_eval_reg(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t npts, double* out, int64_t maxb,
+                           int ppl, int wg) {
+  const HostTopo& t = net->t;
+  TopoView tv = net->view();
+  const int bsr = wg;
+  const SiteDesc* dd = (const SiteDesc*)net->d_desc;
+  if (ppl == 2 && maxb <= 16) {
+    const int64_t nb2 = (npts + 2 * bsr - 1) / (2 * bsr);
+    if (nb2 > INT32_MAX) return qtt_fail(ctx, QTT_EINVAL, "evaluate: too many points");
+#define QTT_REG2_LAUNCH(C)                                                                         \
+  eval_chain_reg2_kernel<C><<<dim3((unsigned)nb2), bsr, 0, ctx->stream>>>(                         \
+      net->d_pack, net->d_pack_off, dd, t.nv, t.ncoord, tv.ndigits, xs, npts, out, ctx->d_flag)
+    if (maxb <= 4) QTT_REG2_LAUNCH(4);
+    else if (maxb <= 8) QTT_REG2_LAUNCH(8);
+    else QTT_REG2_LAUNCH(16);
+#undef QTT_REG2_LAUNCH
+    return qtt_hip_check(ctx, hipGetLastError(), "eval_chain_reg2_kernel");
+  }
+  const int64_t nbr = (npts + bsr - 1) / bsr;
+  if (nbr > INT32_MAX) return qtt_fail(ctx, QTT_EINVAL, "evaluate: too many points");
+#define QTT_REG_LAUNCH(C)                                                                          \
+  eval_chain_reg_kernel<C><<<dim3((unsigned)nbr), bsr, 0, ctx->stream>>>(                          \
+      net->d_pack, net->d_pack_off, dd, t.nv, t.ncoord, tv.ndigits, xs, npts, out, ctx->d_flag)
+  if (maxb <= 4) QTT_REG_LAUNCH(4);
+  else if (maxb <= 8) QTT_REG_LAUNCH(8);
+  else if (maxb <= 16) QTT_REG_LAUNCH(16);
+  else QTT_REG_LAUNCH(32);
+#undef QTT_REG_LAUNCH
+  return qtt_hip_check(ctx, hipGetLastError(), "eval_chain_reg_kernel");
+}
+
+// Time every launch shape of the register chain kernel on these points (one
+// warm-up and five timed launches each, HIP events on the context's stream) and
+// keep the fastest for this bond class and point-count octave.  cfg receives
+// {points per lane, workgroup size} ({0, 0}: the network does not take the
+// register kernel), ms the time of one launch.
+int tune_eval(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t npts, double* out, int* cfg, double* ms) {
+  cfg[0] = cfg[1] = 0;
+  *ms = 0.0;
+  const HostTopo& t = net->t;
+  int64_t maxb = 1;
+  for (int v = 0; v < t.nv; v++) maxb = std::max(maxb, net->bond[v]);
+  if (!t.chain || t.ncoord > QTT_MAX_COORD || maxb > 32 || npts <= 0) return QTT_OK;
+  int rc = ensure_pack(ctx, net);
+  if (rc) return rc;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  rc = qtt_hip_check(ctx, hipEventCreate(&e0), "tune event");
+  if (!rc) rc = qtt_hip_check(ctx, hipEventCreate(&e1), "tune event");
+  float best = 0.0f;
+  int bp = 0, bw = 0;
+  for (int ppl = 1; ppl <= (maxb <= 16 ? 2 : 1) && !rc; ++ppl) {
+    for (int wg = 64; wg <= 256 && !rc; wg *= 2) {
+      rc = launch_eval_reg(ctx, net, xs, npts, out, maxb, ppl, wg);
+      if (!rc) rc = qtt_hip_check(ctx, hipEventRecord(e0, ctx->stream), "tune event");
+      for (int r = 0; r < 5 && !rc; ++r) rc = launch_eval_reg(ctx, net, xs, npts, out, maxb, ppl, wg);
+      if (!rc) rc = qtt_hip_check(ctx, hipEventRecord(e1, ctx->stream), "tune event");
+      if (!rc) rc = qtt_hip_check(ctx, hipEventSynchronize(e1), "tune sync");
+      float f = 0.0f;
+      if (!rc) rc = qtt_hip_check(ctx, hipEventElapsedTime(&f, e0, e1), "tune time");
+      if (!rc && (bp == 0 || f < best)) {
+        best = f;
+        bp = ppl;
+        bw = wg;
+      }
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  ctx->eval_tuned[eval_tune_key(maxb, npts)] = {bp, bw};
+  cfg[0] = bp;
+  cfg[1] = bw;
+  *ms = best / 5.0;
+  return QTT_OK;
+}
+
 int launch_eval(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t npts, double* out,
                 int precision) {
   const HostTopo& t = net->t;
@@ -564,34 +651,17 @@ int launch_eval(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t npts, doub
     int rc = precision == QTT_F64 ? ensure_pack(ctx, net) : ensure_pack32(ctx, net);
     if (rc) return rc;
     if (precision == QTT_F64 && maxb <= 32 && !ctx->force_lds_eval) {
-      const int bsr = 256;
-      const SiteDesc* dd = (const SiteDesc*)net->d_desc;
-      // two points per lane for bonds <= 16 (register room) when there are
-      // enough points to fill the GPU at half the workgroup count, one otherwise
-      const int ppl = ctx->eval_ppl ? ctx->eval_ppl : (maxb <= 16 && npts >= (int64_t(1) << 19) ? 2 : 1);
-      if (ppl == 2 && maxb <= 16) {
-        const int64_t nb2 = (npts + 2 * bsr - 1) / (2 * bsr);
-        if (nb2 > INT32_MAX) return qtt_fail(ctx, QTT_EINVAL, "evaluate: too many points");
-#define QTT_REG2_LAUNCH(C)                                                                         \
-  eval_chain_reg2_kernel<C><<<dim3((unsigned)nb2), bsr, 0, ctx->stream>>>(                         \
-      net->d_pack, net->d_pack_off, dd, t.nv, t.ncoord, tv.ndigits, xs, npts, out, ctx->d_flag)
-        if (maxb <= 4) QTT_REG2_LAUNCH(4);
-        else if (maxb <= 8) QTT_REG2_LAUNCH(8);
-        else QTT_REG2_LAUNCH(16);
-#undef QTT_REG2_LAUNCH
-        return qtt_hip_check(ctx, hipGetLastError(), "eval_chain_reg2_kernel");
+      // tuned shape if qtt_tune_eval measured one for this bond class and point
+      // count; else two points per lane for bonds <= 16 (register room) when
+      // there are enough points to fill the GPU at half the workgroup count
+      int ppl = maxb <= 16 && npts >= (int64_t(1) << 19) ? 2 : 1, wg = 256;
+      const auto it = ctx->eval_tuned.find(eval_tune_key(maxb, npts));
+      if (it != ctx->eval_tuned.end()) {
+        ppl = it->second.ppl;
+        wg = it->second.wg;
       }
-      const int64_t nbr = (npts + bsr - 1) / bsr;
-      if (nbr > INT32_MAX) return qtt_fail(ctx, QTT_EINVAL, "evaluate: too many points");
-#define QTT_REG_LAUNCH(C)                                                                          \
-  eval_chain_reg_kernel<C><<<dim3((unsigned)nbr), bsr, 0, ctx->stream>>>(                          \
-      net->d_pack, net->d_pack_off, dd, t.nv, t.ncoord, tv.ndigits, xs, npts, out, ctx->d_flag)
-      if (maxb <= 4) QTT_REG_LAUNCH(4);
-      else if (maxb <= 8) QTT_REG_LAUNCH(8);
-      else if (maxb <= 16) QTT_REG_LAUNCH(16);
-      else QTT_REG_LAUNCH(32);
-#undef QTT_REG_LAUNCH
-      return qtt_hip_check(ctx, hipGetLastError(), "eval_chain_reg_kernel");
+      if (ctx->eval_ppl) ppl = ctx->eval_ppl;
+      return launch_eval_reg(ctx, net, xs, npts, out, maxb, ppl, wg);
     }
     // LDS: two vectors of maxb entries per lane (double, or a float pair)
     const size_t per_lane = 2 * size_t(maxb) * sizeof(double);

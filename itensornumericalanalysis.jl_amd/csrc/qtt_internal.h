@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -26,6 +27,13 @@ struct qtt_ctx {
   int cus = 256;
   bool force_lds_eval = false;  // QTT_EVAL_KERNEL=lds selects the LDS-vector chain kernel
   int eval_ppl = 0;             // points per lane of the register chain kernel (0: by bond size)
+  // launch shapes of the register chain kernel picked by qtt_tune_eval, keyed
+  // by eval_tune_key (bond class, point-count octave); absent keys use the
+  // built-in rule
+  struct EvalShape {
+    int ppl, wg;
+  };
+  std::map<uint32_t, EvalShape> eval_tuned;
 };
 
 // Topology view passed by value to kernels (device pointers).
@@ -105,6 +113,7 @@ double* qtt_scratch(qtt_ctx* ctx, size_t bytes);
 // kernel launchers (eval.hip, sweep.hip, factor.hip, grad.hip)
 int launch_eval(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t npts, double* out,
                 int precision);
+int tune_eval(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t npts, double* out, int* cfg, double* ms);
 int launch_eval_batch(qtt_ctx* ctx, int n, qtt_net* const* nets, const double* const* xs, const int64_t* npts,
                       double* const* out, int precision);
 int launch_integrate(qtt_ctx* ctx, const qtt_net* net, double weight, double* out_host);

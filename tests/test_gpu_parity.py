@@ -355,6 +355,27 @@ def test_operate_batch_and_integrate_batch(q, po, eng, rng):
         assert e == pytest.approx(rerr, rel=1e-6, abs=1e-18)
 
 
+def test_tune_evaluate(q, po, eng, rng):
+    # the autotuner times the register kernel's launch shapes and later calls use the fastest;
+    # every shape runs the same arithmetic per point, so the values do not change
+    import torch
+    s = q.continuous_siteinds(q.named_grid((16, 1)), map_dimension=2)
+    f = q.random_itn(s, link_space=12, rng=rng, std=1.0 / math.sqrt(12))
+    xs = rng.uniform(0, 1, (200003, 2))
+    xd = torch.tensor(xs, device="cuda")
+    before = q.evaluate(f, xd).cpu().numpy()
+    cfg = q.tune_evaluate(f, xd)
+    assert cfg["points_per_lane"] in (1, 2) and cfg["workgroup"] in (64, 128, 256) and cfg["ms"] > 0
+    after = q.evaluate(f, xd).cpu().numpy()
+    assert np.array_equal(before, after)
+    assert rel_l2(after[:5000], oracle_eval(po, f, xs[:5000])) < TOL
+    # a tree does not take the register kernel: nothing to tune
+    st = q.continuous_siteinds(q.named_comb_tree((2, 4)))
+    ft = q.random_itn(st, link_space=3, rng=rng)
+    xt = torch.tensor(rng.uniform(0, 1, (100, ft.dimension())), device="cuda")
+    assert q.tune_evaluate(ft, xt)["workgroup"] == 0
+
+
 def test_evaluate_batch(q, po, eng, rng):
     # uneven digit counts and bonds (all four register classes, one LDS-kernel network), ragged point counts
     import torch
