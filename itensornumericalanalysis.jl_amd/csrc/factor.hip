@@ -225,7 +225,36 @@ struct SweepArgs {
   int ws_lds;    // doubles of LDS workspace
   double* err_out;
   int* status;   // [1] <- non-convergence
+  struct CoopJob* job;  // jobs for helper workgroups (single-network launches), or null
+  int nwg;              // workgroups taking part in a job (1: no helpers)
 };
+
+// A job posted by workgroup 0 of a single-network launch to the helper
+// workgroups that follow it in the grid (a cooperative launch, so all are
+// resident).  Workgroup 0 writes the job, then releases seq; every workgroup
+// processes its share and counts itself in done; workgroup 0 waits until done
+// reaches nwg per job before it posts the next one, so no workgroup reads a
+// job while it changes.  quit ends the helpers.  Zeroed by the host before
+// the launch.  Every wait is bounded: a helper that waits too long for a job
+// exits, and workgroup 0 sets err (reported by the host) if a job does not
+// complete in time.
+struct CoopJob {
+  int seq;
+  int quit;
+  int rot;   // a rotation happened in the current Jacobi sweep
+  int err;   // a wait exceeded its limit
+  unsigned int done;
+  int pad;
+  // the job: one round t of a one-sided Jacobi sweep on the rows of X (n x b,
+  // ld ldx) with the rotations kept in J (n x n, ld ldj), both in global memory
+  double* X;
+  double* J;
+  int n, b, ldx, ldj, t, nn, e;  // e: entries per lane of the row loads (2, 4, 6 or 8)
+  double tol, small;
+};
+
+constexpr int COOP_SPIN_MASTER = 1 << 22;  // polls of workgroup 0 for a job's completion
+constexpr int COOP_SPIN_HELPER = 1 << 24;  // polls of a helper for the next job (long moves in between)
 
 __device__ inline int core_dims(const TopoView& t, const int* bond, int v, int64_t* dims) {
   const int m = t.nch[v];
@@ -1449,12 +1478,173 @@ __device__ __noinline__ void mode_product_staged(const double* Rrows, const int*
   }
 }
 
+// One round of a job on this workgroup: pair pi = wg + nwg * slot goes to
+// wavefront slot (64 lanes per pair, E entries per lane of a row of X and of
+// J), as in jacobi_rows_reg.  Returns whether any of its pairs rotated.
+template <int E>
+__device__ __noinline__ bool coop_pairs(double* X, double* J, int n_in, int b_in, int ldx_in, int ldj_in, int t_in,
+                                        int nn_in, double tol, double small, int wg_in, int nwg_in) {
+  const int n = uni(n_in), b = uni(b_in), ldx = uni(ldx_in), ldj = uni(ldj_in), t = uni(t_in), nn = uni(nn_in),
+            wg = uni(wg_in), nwg = uni(nwg_in);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+  bool any = false;
+  for (int pi = wg + nwg * wv; pi < nn / 2; pi += nwg * nwv) {
+    int p = rr_player(pi, t, nn), q = rr_player(nn - 1 - pi, t, nn);
+    if (p > q) {
+      const int tmp = p;
+      p = q;
+      q = tmp;
+    }
+    if (q >= n) continue;
+    double* xp = X + (int64_t)p * ldx;
+    double* xq = X + (int64_t)q * ldx;
+    double* jp = J + (int64_t)p * ldj;
+    double* jq = J + (int64_t)q * ldj;
+    double u[E], w[E], ju[E], jw[E];
+    const int gl = opaque(lane);
+    load_row<64, E>(xp, b, gl, u);
+    load_row<64, E>(xq, b, gl, w);
+    load_row<64, E>(jp, n, gl, ju);
+    load_row<64, E>(jq, n, gl, jw);
+    double a0 = 0.0, c0 = 0.0, g0 = 0.0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      a0 = __builtin_fma(u[e], u[e], a0);
+      c0 = __builtin_fma(w[e], w[e], c0);
+      g0 = __builtin_fma(u[e], w[e], g0);
+    }
+    const double a = jsum<64>(a0), c = jsum<64>(c0), g = jsum<64>(g0);
+    if (g != 0.0 && a > small && c > small && g * g > tol * tol * (a * c)) {
+      const double zeta = (c - a) * rcp_nr(2.0 * g);
+      const double q1 = __builtin_fma(zeta, zeta, 1.0);
+      const double r = q1 * rsq_nr(q1);
+      const double t0 = rcp_nr(fabs(zeta) + r);
+      const double tt = zeta >= 0.0 ? t0 : -t0;
+      const double cs = rsq_nr(__builtin_fma(tt, tt, 1.0)), sn = cs * tt;
+      rotate4<E>(u, w, cs, sn);
+      rotate4<E>(ju, jw, cs, sn);
+      store_row<64, E>(xp, b, gl, u);
+      store_row<64, E>(xq, b, gl, w);
+      store_row<64, E>(jp, n, gl, ju);
+      store_row<64, E>(jq, n, gl, jw);
+      any = true;
+    }
+  }
+  return any;
+}
+
+// This workgroup's share of the posted job, then its completion count.
+__device__ void coop_work(CoopJob* job, int wg, int nwg, lds_i32* sflag) {
+  const int tid = threadIdx.x;
+  const int e = job->e;
+  bool r;
+  if (e <= 2) r = coop_pairs<2>(job->X, job->J, job->n, job->b, job->ldx, job->ldj, job->t, job->nn, job->tol, job->small, wg, nwg);
+  else if (e <= 4) r = coop_pairs<4>(job->X, job->J, job->n, job->b, job->ldx, job->ldj, job->t, job->nn, job->tol, job->small, wg, nwg);
+  else if (e <= 6) r = coop_pairs<6>(job->X, job->J, job->n, job->b, job->ldx, job->ldj, job->t, job->nn, job->tol, job->small, wg, nwg);
+  else r = coop_pairs<8>(job->X, job->J, job->n, job->b, job->ldx, job->ldj, job->t, job->nn, job->tol, job->small, wg, nwg);
+  if (r && (tid & 63) == 0) __hip_atomic_store(&job->rot, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();  // every wavefront's rows are written before thread 0 releases them
+  if (tid == 0) __hip_atomic_fetch_add(&job->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  (void)sflag;
+}
+
+// Helper workgroups: take every posted job until workgroup 0 quits.
+__device__ void coop_helper(CoopJob* job, int wg, int nwg, lds_i32* sflag) {
+  const int tid = threadIdx.x;
+  for (int expected = 1;; ++expected) {
+    if (tid == 0) {
+      int spins = 0, st = 0;
+      for (;;) {
+        if (__hip_atomic_load(&job->quit, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) { st = 1; break; }
+        if (__hip_atomic_load(&job->seq, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= expected) break;
+        if (spins < 64) __builtin_amdgcn_s_sleep(2);
+        else __builtin_amdgcn_s_sleep(64);
+        if (++spins > COOP_SPIN_HELPER) {  // give up; a later job would then time out in workgroup 0
+          st = 1;
+          break;
+        }
+      }
+      sflag[0] = st;
+    }
+    __syncthreads();
+    const int st = sflag[0];
+    __syncthreads();
+    if (st) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    coop_work(job, wg, nwg, sflag);
+  }
+}
+
+// One-sided Jacobi on rows in global memory with the helper workgroups (run
+// by workgroup 0; same rules as jacobi_rows_reg, round-robin rounds, each
+// round one job).
+__device__ bool jacobi_coop(double* X, int n, int b, double* J, CoopJob* job, int nwg, double* red, int* iflag) {
+  const int tid = threadIdx.x, bs = blockDim.x;
+  for (int e2 = tid; e2 < n * n; e2 += bs) J[e2] = (e2 / n == e2 % n) ? 1.0 : 0.0;
+  __syncthreads();
+  if (n < 2) return true;
+  const int nn = n + (n & 1);
+  const double tol = DBL_EPSILON * (double)(b > n ? b : n);
+  double fro = 0.0;
+  for (int e2 = tid; e2 < n * b; e2 += bs) fro += X[e2] * X[e2];
+  const double small = block_sum(fro, red) * DBL_EPSILON * DBL_EPSILON;
+  const int len = b > n ? b : n;
+  const int e = len <= 128 ? 2 : len <= 256 ? 4 : len <= 384 ? 6 : 8;
+  lds_i32* sflag = (lds_i32*)iflag;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    if (tid == 0) __hip_atomic_store(&job->rot, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int t = 0; t < nn - 1; ++t) {
+      if (tid == 0) {
+        job->X = X;
+        job->J = J;
+        job->n = n;
+        job->b = b;
+        job->ldx = b;
+        job->ldj = n;
+        job->t = t;
+        job->nn = nn;
+        job->e = e;
+        job->tol = tol;
+        job->small = small;
+        const int s = job->seq + 1;
+        __hip_atomic_store(&job->seq, s, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        sflag[1] = s;
+      }
+      __syncthreads();
+      const int s = sflag[1];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this workgroup's rows of the previous round
+      coop_work(job, 0, nwg, sflag);
+      if (tid == 0) {  // wait for every workgroup's share
+        const unsigned int target = (unsigned int)s * (unsigned int)nwg;
+        int spins = 0;
+        while (__hip_atomic_load(&job->done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+          __builtin_amdgcn_s_sleep(1);
+          if (++spins > COOP_SPIN_MASTER) {
+            __hip_atomic_store(&job->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+        }
+      }
+      __syncthreads();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    if (tid == 0) sflag[2] = __hip_atomic_load(&job->rot, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) |
+                             __hip_atomic_load(&job->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) * 2;
+    __syncthreads();
+    const int st = sflag[2];
+    __syncthreads();
+    if (st & 2) return false;  // a wait failed: report non-convergence
+    if (!st) return true;
+  }
+  return false;
+}
+
 // One move of the sweep, with the workspace `ws` either in LDS or in global
 // scratch.  Two separate instantiations (one per memory) let the compiler use
 // LDS instructions on the LDS path instead of generic flat addressing.
 template <int G, bool REG>
-__device__ __forceinline__ int run_move(double* ws, const bool gws, double* jpad, lds_f64* pnl, const int pnl_cap,
-                                        const int pmode, const int rows,
+__device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job, const int nwg, double* jpad,
+                                        lds_f64* pnl, const int pnl_cap, const int pmode, const int rows,
                                         const int b, const int tr, const int rpl, const int bx,
                                         const int64_t Au, const int64_t nu, const int64_t Bu, double* Xc,
                                         const double* Uc, double* Ud, double* gY, double cutoff, long long maxdim,
@@ -1589,6 +1779,10 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, double* jpad
         for (int e = tid; e < n * n; e += bs) Jb[e] = Jl[(e / n) * ldj + e % n];
         __syncthreads();
       }
+    } else if (gws && job && nwg > 1 && n >= 32 && b <= 512) {
+      // rows in global memory and helper workgroups present: each round's
+      // pairs are spread over all of them
+      conv = jacobi_coop(X, n, b, Jb, job, nwg, red, iflag);
     } else if (gws && b <= 512) {
       // workspace in global memory: both rows of a pair (and of J) in
       // registers, 32 lanes per pair (two pairs per wavefront; 64 for rows
@@ -1689,6 +1883,11 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, double* jpad
 
 __global__ void __launch_bounds__(512) truncate_kernel(const SweepArgs* __restrict__ all) {
   extern __shared__ double lds[];
+  // a single-network launch may carry helper workgroups after workgroup 0
+  if (all[0].nwg > 1 && blockIdx.x > 0) {
+    coop_helper(all[0].job, (int)blockIdx.x, all[0].nwg, (lds_i32*)(lds + 16));
+    return;
+  }
   const SweepArgs& a = all[blockIdx.x];
   const TopoView& t = a.t;
   const int tid = threadIdx.x, bs = blockDim.x;
@@ -1760,17 +1959,17 @@ __global__ void __launch_bounds__(512) truncate_kernel(const SweepArgs* __restri
     int m;
     if (need <= a.ws_lds) {
       if (small)
-        m = run_move<8, true>(wsl, false, jpad, nullptr, 0, 0, rows, b, tr, rpl, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<8, true>(wsl, false, nullptr, 1, jpad, nullptr, 0, 0, rows, b, tr, rpl, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
                         iflag, red, dbc);
       else
-        m = run_move<64, false>(wsl, false, nullptr, nullptr, 0, 0, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<64, false>(wsl, false, nullptr, 1, nullptr, nullptr, 0, 0, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
                          iflag, red, dbc);
     } else {
       if (small)
-        m = run_move<8, false>(a.gscr, true, nullptr, pnl, pnl_cap, pmode, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<8, false>(a.gscr, true, a.job, a.nwg, nullptr, pnl, pnl_cap, pmode, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
                         iflag, red, dbc);
       else
-        m = run_move<64, false>(a.gscr, true, nullptr, pnl, pnl_cap, pmode, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<64, false>(a.gscr, true, a.job, a.nwg, nullptr, pnl, pnl_cap, pmode, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
                          iflag, red, dbc);
     }
     QTT_PHASE(st, 5);
@@ -1781,6 +1980,7 @@ __global__ void __launch_bounds__(512) truncate_kernel(const SweepArgs* __restri
   }
   for (int v = tid; v < t.nv; v += bs) a.bond_g[v] = bond[v];
   if (tid == 0) a.err_out[0] = dbc[0];
+  if (a.nwg > 1 && tid == 0) __hip_atomic_store(&a.job->quit, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 }  // namespace
@@ -1874,14 +2074,24 @@ int launch_truncate_batch(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double c
     maxnv = std::max(maxnv, net->t.nv);
   }
   const size_t args_bytes = (size_t)nnet * sizeof(SweepArgs);
-  // the tail slack covers the whole-block row reads of the Jacobi rounds (< 256 doubles past a row)
-  const size_t bytes = (tot_doubles + nnet + 8) * sizeof(double) + args_bytes + tot_steps * sizeof(int) + 4096;
+  // A single network with cores beyond LDS gets helper workgroups for the
+  // Jacobi rounds of its largest moves.
+  int nwg = 1;
+  if (nnet == 1) {
+    int64_t maxcap = 1;
+    for (int v = 0; v < nets[0]->t.nv; v++) maxcap = std::max(maxcap, nets[0]->cap[v]);
+    if (maxcap > QTT_LDS_DOUBLES) nwg = std::max(1, std::min(32, ctx->cus));
+  }
+  // the tail slack covers the whole-block row reads of the Jacobi rounds (< 512 doubles past a row)
+  const size_t bytes = (tot_doubles + nnet + 8) * sizeof(double) + args_bytes + tot_steps * sizeof(int) + 4096 +
+                       sizeof(CoopJob) + 256;
   double* scr = qtt_scratch(ctx, bytes);
   if (!scr) return qtt_fail(ctx, QTT_ENOMEM, "truncate scratch");
   double* d_err = scr;
   double* wsp = scr + nnet + 8;
   SweepArgs* d_args = (SweepArgs*)(wsp + tot_doubles);
   int* d_steps = (int*)((char*)d_args + ((args_bytes + 255) & ~size_t(255)));
+  CoopJob* d_job = (CoopJob*)((char*)d_steps + ((tot_steps * sizeof(int) + 255) & ~size_t(255)));
   std::vector<int> allsteps;
   allsteps.reserve(tot_steps);
   for (int i = 0; i < nnet; i++) allsteps.insert(allsteps.end(), steps[i].begin(), steps[i].end());
@@ -1907,9 +2117,12 @@ int launch_truncate_batch(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double c
     sa.ws_lds = lds_doubles - 64 - (net->t.nv + 1) / 2;
     sa.err_out = d_err + i;
     sa.status = ctx->d_flag;
+    sa.job = nwg > 1 ? d_job : nullptr;
+    sa.nwg = nwg;
     if (sa.ws_lds < 0) return qtt_fail(ctx, QTT_EINVAL, "truncate: too many vertices");
   }
   int rc = qtt_hip_check(ctx, hipMemcpyAsync(d_args, hargs.data(), args_bytes, hipMemcpyHostToDevice, ctx->stream), "sweep args");
+  if (!rc && nwg > 1) rc = qtt_hip_check(ctx, hipMemsetAsync(d_job, 0, sizeof(CoopJob), ctx->stream), "coop job");
   if (!rc && tot_steps)
     rc = qtt_hip_check(ctx, hipMemcpyAsync(d_steps, allsteps.data(), tot_steps * sizeof(int), hipMemcpyHostToDevice, ctx->stream), "steps");
   if (rc) return rc;
@@ -1918,7 +2131,22 @@ int launch_truncate_batch(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double c
                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes),
                      "truncate LDS attribute");
   if (rc) return rc;
-  truncate_kernel<<<nnet, 512, lds_bytes, ctx->stream>>>(d_args);
+  bool coop = false;
+  if (nwg > 1) {  // helpers: one cooperative launch, so every workgroup is resident
+    void* kargs[] = {(void*)&d_args};
+    hipError_t e = hipLaunchCooperativeKernel((const void*)truncate_kernel, dim3(nwg), dim3(512), kargs, lds_bytes,
+                                              ctx->stream);
+    if (e == hipSuccess) {
+      coop = true;
+    } else {  // not available: run without helpers
+      (void)hipGetLastError();
+      hargs[0].nwg = 1;
+      hargs[0].job = nullptr;
+      rc = qtt_hip_check(ctx, hipMemcpyAsync(d_args, hargs.data(), args_bytes, hipMemcpyHostToDevice, ctx->stream), "sweep args");
+      if (rc) return rc;
+    }
+  }
+  if (!coop) truncate_kernel<<<nnet, 512, lds_bytes, ctx->stream>>>(d_args);
   rc = qtt_hip_check(ctx, hipGetLastError(), "truncate_kernel");
   if (rc) return rc;
   std::vector<double> herr(nnet);
@@ -1940,6 +2168,15 @@ int launch_truncate_batch(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double c
   int fl[2];
   rc = qtt_hip_check(ctx, hipMemcpy(fl, ctx->d_flag, sizeof(fl), hipMemcpyDeviceToHost), "flags");
   if (rc) return rc;
+  if (coop) {
+    CoopJob hj;
+    rc = qtt_hip_check(ctx, hipMemcpy(&hj, d_job, sizeof(hj), hipMemcpyDeviceToHost), "coop job");
+    if (rc) return rc;
+    if (hj.err) {
+      (void)hipMemset(ctx->d_flag, 0, 2 * sizeof(int));
+      return qtt_fail(ctx, QTT_EHIP, "truncate: a helper workgroup wait exceeded its limit");
+    }
+  }
   if (fl[1]) {
     (void)hipMemset(ctx->d_flag, 0, 2 * sizeof(int));
     return qtt_fail(ctx, QTT_ENOCONV, "truncate: Jacobi SVD did not converge");
