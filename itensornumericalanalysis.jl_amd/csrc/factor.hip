@@ -245,12 +245,22 @@ struct CoopJob {
   int err;   // a wait exceeded its limit
   unsigned int done;
   int pad;
-  // the job: one round t of a one-sided Jacobi sweep on the rows of X (n x b,
+  int type;  // 1: Jacobi round, 2: mode product columns, 3: copy
+  // type 1: one round t of a one-sided Jacobi sweep on the rows of X (n x b,
   // ld ldx) with the rotations kept in J (n x n, ld ldj), both in global memory
   double* X;
   double* J;
   int n, b, ldx, ldj, t, nn, e;  // e: entries per lane of the row loads (2, 4, 6 or 8)
   double tol, small;
+  // type 2: Y = R (x) U over the columns (aa, bb), each workgroup a range of
+  // them (mode_product_staged); type 3: D[0, ytot) <- Y, each a range
+  const double* R;
+  const int* perm;
+  const double* U;
+  double* Y;
+  double* D;
+  long long ytot;
+  int use_perm, m, rb, Au, nu, Bu;
 };
 
 constexpr int COOP_SPIN_MASTER = 1 << 22;  // polls of workgroup 0 for a job's completion
@@ -1425,10 +1435,10 @@ constexpr int RB = 32;
 
 __device__ __noinline__ void mode_product_staged(const double* Rrows, const int* perm, bool use_perm, int m_in,
                                                  int b_in, const double* Uc, double* gY, int Au_in, int nu_in,
-                                                 int Bu_in, lds_f64* Rs) {
+                                                 int Bu_in, lds_f64* Rs, int j0_in = 0, int j1_in = -1) {
   const int tid = threadIdx.x, bs = blockDim.x;
   const int m = uni(m_in), b = uni(b_in), Au = uni(Au_in), nu = uni(nu_in), Bu = uni(Bu_in);
-  const int ncol = Au * Bu;
+  const int j0 = uni(j0_in), ncol = j1_in < 0 ? Au * Bu : uni(j1_in);  // columns [j0, ncol)
   for (int i0 = 0; i0 < m; i0 += RB) {
     const int rb = m - i0 < RB ? m - i0 : RB;
     for (int e = tid; e < RB * nu; e += bs) {
@@ -1438,7 +1448,7 @@ __device__ __noinline__ void mode_product_staged(const double* Rrows, const int*
       Rs[c * RB + q] = v;
     }
     __syncthreads();
-    for (int j = tid; j < ncol; j += bs) {
+    for (int j = j0 + tid; j < ncol; j += bs) {
       const int aa = j / Bu, bb = j - aa * Bu;
       const double* up = Uc + (int64_t)aa * nu * Bu + bb;
       double acc[RB];
@@ -1533,23 +1543,72 @@ __device__ __noinline__ bool coop_pairs(double* X, double* J, int n_in, int b_in
   return any;
 }
 
-// This workgroup's share of the posted job, then its completion count.
-__device__ void coop_work(CoopJob* job, int wg, int nwg, lds_i32* sflag) {
+// This workgroup's share of the posted job, then its completion count.  lw:
+// LDS the job may use (the staged R block of a mode product).
+__device__ void coop_work(CoopJob* job, int wg, int nwg, lds_f64* lw) {
   const int tid = threadIdx.x;
-  const int e = job->e;
-  bool r;
-  if (e <= 2) r = coop_pairs<2>(job->X, job->J, job->n, job->b, job->ldx, job->ldj, job->t, job->nn, job->tol, job->small, wg, nwg);
-  else if (e <= 4) r = coop_pairs<4>(job->X, job->J, job->n, job->b, job->ldx, job->ldj, job->t, job->nn, job->tol, job->small, wg, nwg);
-  else if (e <= 6) r = coop_pairs<6>(job->X, job->J, job->n, job->b, job->ldx, job->ldj, job->t, job->nn, job->tol, job->small, wg, nwg);
-  else r = coop_pairs<8>(job->X, job->J, job->n, job->b, job->ldx, job->ldj, job->t, job->nn, job->tol, job->small, wg, nwg);
-  if (r && (tid & 63) == 0) __hip_atomic_store(&job->rot, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __syncthreads();  // every wavefront's rows are written before thread 0 releases them
+  const int type = job->type;
+  if (type == 1) {
+    const int e = job->e;
+    bool r;
+    if (e <= 2) r = coop_pairs<2>(job->X, job->J, job->n, job->b, job->ldx, job->ldj, job->t, job->nn, job->tol, job->small, wg, nwg);
+    else if (e <= 4) r = coop_pairs<4>(job->X, job->J, job->n, job->b, job->ldx, job->ldj, job->t, job->nn, job->tol, job->small, wg, nwg);
+    else if (e <= 6) r = coop_pairs<6>(job->X, job->J, job->n, job->b, job->ldx, job->ldj, job->t, job->nn, job->tol, job->small, wg, nwg);
+    else r = coop_pairs<8>(job->X, job->J, job->n, job->b, job->ldx, job->ldj, job->t, job->nn, job->tol, job->small, wg, nwg);
+    if (r && (tid & 63) == 0) __hip_atomic_store(&job->rot, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else if (type == 2) {
+    const int ncol = job->Au * job->Bu;
+    const int chunk = ((ncol + nwg - 1) / nwg + 63) & ~63;
+    const int j0 = min(ncol, wg * chunk), j1 = min(ncol, j0 + chunk);
+    if (j0 < j1)
+      mode_product_staged(job->R, job->perm, job->use_perm != 0, job->m, job->rb, job->U, job->Y, job->Au, job->nu,
+                          job->Bu, lw, j0, j1);
+  } else if (type == 3) {
+    const long long tot = job->ytot;
+    const long long chunk = ((tot + nwg - 1) / nwg + 63) & ~63LL;
+    const long long e0 = min(tot, (long long)wg * chunk), e1 = min(tot, e0 + chunk);
+    for (long long x = e0 + tid; x < e1; x += blockDim.x) job->D[x] = job->Y[x];
+  }
+  __syncthreads();  // every wavefront's writes are done before thread 0 releases them
   if (tid == 0) __hip_atomic_fetch_add(&job->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-  (void)sflag;
+}
+
+// Workgroup 0: post the job whose fields thread 0 has written, take its own
+// share, and wait until every workgroup has done its share.  Returns false if
+// the wait exceeded its limit.
+__device__ bool coop_post_run(CoopJob* job, int nwg, int* iflag, lds_f64* lw) {
+  const int tid = threadIdx.x;
+  lds_i32* sflag = (lds_i32*)iflag;
+  if (tid == 0) {
+    const int s = job->seq + 1;
+    __hip_atomic_store(&job->seq, s, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    sflag[1] = s;
+  }
+  __syncthreads();
+  const int s = sflag[1];
+  coop_work(job, 0, nwg, lw);
+  if (tid == 0) {
+    const unsigned int target = (unsigned int)s * (unsigned int)nwg;
+    int spins = 0, bad = 0;
+    while (__hip_atomic_load(&job->done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins > COOP_SPIN_MASTER) {
+        __hip_atomic_store(&job->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        bad = 1;
+        break;
+      }
+    }
+    sflag[3] = bad;
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  const int bad = sflag[3];
+  __syncthreads();
+  return !bad;
 }
 
 // Helper workgroups: take every posted job until workgroup 0 quits.
-__device__ void coop_helper(CoopJob* job, int wg, int nwg, lds_i32* sflag) {
+__device__ void coop_helper(CoopJob* job, int wg, int nwg, lds_i32* sflag, lds_f64* lw) {
   const int tid = threadIdx.x;
   for (int expected = 1;; ++expected) {
     if (tid == 0) {
@@ -1571,7 +1630,7 @@ __device__ void coop_helper(CoopJob* job, int wg, int nwg, lds_i32* sflag) {
     __syncthreads();
     if (st) return;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    coop_work(job, wg, nwg, sflag);
+    coop_work(job, wg, nwg, lw);
   }
 }
 
@@ -1595,6 +1654,7 @@ __device__ bool jacobi_coop(double* X, int n, int b, double* J, CoopJob* job, in
     if (tid == 0) __hip_atomic_store(&job->rot, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (int t = 0; t < nn - 1; ++t) {
       if (tid == 0) {
+        job->type = 1;
         job->X = X;
         job->J = J;
         job->n = n;
@@ -1606,27 +1666,8 @@ __device__ bool jacobi_coop(double* X, int n, int b, double* J, CoopJob* job, in
         job->e = e;
         job->tol = tol;
         job->small = small;
-        const int s = job->seq + 1;
-        __hip_atomic_store(&job->seq, s, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        sflag[1] = s;
       }
-      __syncthreads();
-      const int s = sflag[1];
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this workgroup's rows of the previous round
-      coop_work(job, 0, nwg, sflag);
-      if (tid == 0) {  // wait for every workgroup's share
-        const unsigned int target = (unsigned int)s * (unsigned int)nwg;
-        int spins = 0;
-        while (__hip_atomic_load(&job->done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-          __builtin_amdgcn_s_sleep(1);
-          if (++spins > COOP_SPIN_MASTER) {
-            __hip_atomic_store(&job->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            break;
-          }
-        }
-      }
-      __syncthreads();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      if (!coop_post_run(job, nwg, iflag, nullptr)) break;
     }
     if (tid == 0) sflag[2] = __hip_atomic_load(&job->rot, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) |
                              __hip_atomic_load(&job->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) * 2;
@@ -1637,6 +1678,34 @@ __device__ bool jacobi_coop(double* X, int n, int b, double* J, CoopJob* job, in
     if (!st) return true;
   }
   return false;
+}
+
+// u <- R (x) u with the helper workgroups: the columns (aa, bb) split over all
+// workgroups into Y, then Y copied into u's core (a second job: the core's
+// new layout overlaps columns of other workgroups).
+__device__ bool mode_product_coop(const double* Rrows, const int* perm, bool use_perm, int m, int b, const double* Uc,
+                                  double* gY, double* Ud, int Au, int nu, int Bu, CoopJob* job, int nwg, int* iflag,
+                                  lds_f64* lw) {
+  if (threadIdx.x == 0) {
+    job->type = 2;
+    job->R = Rrows;
+    job->perm = perm;
+    job->use_perm = use_perm ? 1 : 0;
+    job->m = m;
+    job->rb = b;
+    job->U = Uc;
+    job->Y = gY;
+    job->Au = Au;
+    job->nu = nu;
+    job->Bu = Bu;
+  }
+  if (!coop_post_run(job, nwg, iflag, lw)) return false;
+  if (threadIdx.x == 0) {
+    job->type = 3;
+    job->D = Ud;
+    job->ytot = (long long)Au * m * Bu;
+  }
+  return coop_post_run(job, nwg, iflag, lw);
 }
 
 // One move of the sweep, with the workspace `ws` either in LDS or in global
@@ -1869,14 +1938,22 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
   QTT_PHASE(-1, 3);  // x written
   // u <- R (x)_{au} u : Y[(aa*m + i)*Bu + bb] = sum_c R[i][c] U[(aa*nu + c)*Bu + bb]
   const int ytot = (int)(Au * m * Bu);
-  if (REG)
-    mode_product_lds((const lds_f64*)Rrows, (const lds_i32*)perm, use_perm, m, b, Uc, gY, (int)Au, (int)nu, (int)Bu);
-  else if (pnl && (int64_t)nu * RB <= pnl_cap)  // global workspace: stage R in LDS
-    mode_product_staged(Rrows, perm, use_perm, m, b, Uc, gY, (int)Au, (int)nu, (int)Bu, pnl);
-  else
-    mode_product(Rrows, perm, use_perm, m, b, Uc, gY, (int)Au, (int)nu, (int)Bu);
-  __syncthreads();
-  for (int e = tid; e < ytot; e += bs) Ud[e] = gY[e];
+  if (!REG && gws && job && nwg > 1 && pnl && (int64_t)nu * RB <= pnl_cap &&
+      (int64_t)nu * RB <= QTT_LDS_DOUBLES - 64 && Au * Bu * nu * (int64_t)m >= (int64_t(1) << 22)) {
+    // a large product in global memory: columns spread over the helper workgroups
+    if (!mode_product_coop(Rrows, perm, use_perm, m, b, Uc, gY, Ud, (int)Au, (int)nu, (int)Bu, job, nwg, iflag, pnl)) {
+      if (tid == 0) status[1] = 1;
+    }
+  } else {
+    if (REG)
+      mode_product_lds((const lds_f64*)Rrows, (const lds_i32*)perm, use_perm, m, b, Uc, gY, (int)Au, (int)nu, (int)Bu);
+    else if (pnl && (int64_t)nu * RB <= pnl_cap)  // global workspace: stage R in LDS
+      mode_product_staged(Rrows, perm, use_perm, m, b, Uc, gY, (int)Au, (int)nu, (int)Bu, pnl);
+    else
+      mode_product(Rrows, perm, use_perm, m, b, Uc, gY, (int)Au, (int)nu, (int)Bu);
+    __syncthreads();
+    for (int e = tid; e < ytot; e += bs) Ud[e] = gY[e];
+  }
   QTT_PHASE(-1, 4);  // u written
   return m;
 }
@@ -1885,7 +1962,7 @@ __global__ void __launch_bounds__(512) truncate_kernel(const SweepArgs* __restri
   extern __shared__ double lds[];
   // a single-network launch may carry helper workgroups after workgroup 0
   if (all[0].nwg > 1 && blockIdx.x > 0) {
-    coop_helper(all[0].job, (int)blockIdx.x, all[0].nwg, (lds_i32*)(lds + 16));
+    coop_helper(all[0].job, (int)blockIdx.x, all[0].nwg, (lds_i32*)(lds + 16), (lds_f64*)(lds + 64));
     return;
   }
   const SweepArgs& a = all[blockIdx.x];
