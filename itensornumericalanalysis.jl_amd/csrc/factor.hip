@@ -21,6 +21,7 @@
 //   group per row pair in each Jacobi round (round-robin pairing, all pairs of
 //   a round independent).  Group sums use DPP lane permutes, not LDS.
 #include <cfloat>
+#include <cstring>
 
 #include "qtt_internal.h"
 
@@ -261,7 +262,22 @@ struct CoopJob {
   double* D;
   long long ytot;
   int use_perm, m, rb, Au, nu, Bu;
+  // type 4: the first sums of a tall panel (X = the panel, ldx, n = its rows,
+  // nb columns); type 5: pass t of a tall panel (tol = the reflector's scale,
+  // small = its product with column t+1, dcol per column); both leave 16 sums
+  // per workgroup in Gp.  type 6 / 7: the two MFMA phases of a WY update
+  // (wy_* fields), the first leaving per-unit partial tiles in Pg, the second
+  // reading -U from Ug.
+  int nb;
+  double dcol[16];
+  double* Gp;  // [workgroup][16]
+  double* Pg;  // [unit][256]
+  double* Ug;  // [tile][16][16]
+  double* wy_M;
+  const double* wy_V;
+  int wy_ldm, wy_ldv, wy_mr, wy_cg, wy_c1, wy_nct, wy_nch, wy_chunk;
 };
+constexpr int COOP_PG_UNITS = 512;  // partial tiles Pg holds
 
 constexpr int COOP_SPIN_MASTER = 1 << 22;  // polls of workgroup 0 for a job's completion
 constexpr int COOP_SPIN_HELPER = 1 << 24;  // polls of a helper for the next job (long moves in between)
@@ -1058,40 +1074,112 @@ __device__ __forceinline__ void tall_reduce16(double acc, lds_f64* buf, lds_f64*
   __syncthreads();
 }
 
+__device__ bool coop_post_run(CoopJob* job, int nwg, int* iflag, lds_f64* lw);
+
+// The two kinds of pass over panel rows [r0, r1) (lane layout: a 16-lane group
+// per row, a lane per column c = lane & 15; rows step by 4 per wavefront).
+constexpr int PANEL_U = 8;  // row steps in flight per wait
+
+// sums for column 0: slot 0 its norm^2, slot c its dot with column c
+__device__ __forceinline__ double panel_pre_rows(const double* P, int ld, int nb, int r0, int r1) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, bs = blockDim.x;
+  const int c = lane & 15, rstep = (bs >> 6) * 4, rsub = wv * 4 + (lane >> 4);
+  const bool cin = c < nb;
+  double acc = 0.0;
+  for (int rb = r0; rb < r1; rb += PANEL_U * rstep) {
+    double x[PANEL_U], x0[PANEL_U];  // this lane's column and column 0 (the same load for the whole group)
+#pragma unroll
+    for (int u = 0; u < PANEL_U; ++u) {
+      const int r = rb + u * rstep + rsub;
+      const bool in = r < r1;
+      x[u] = (in && cin) ? P[(int64_t)r * ld + c] : 0.0;
+      x0[u] = in ? P[(int64_t)r * ld] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < PANEL_U; ++u) acc = __builtin_fma(x0[u], x[u], acc);
+  }
+  return acc;
+}
+
+// apply reflector j (scale, this lane's dc; dn for column j+1) to rows
+// [r0, r1) (all below row j) and return this lane's slot of the sums for
+// column j+1: slot c > j+1 column c, c == j+1 its norm^2, c == j reflector j,
+// c < j reflector c (rows below j+1 only)
+__device__ __forceinline__ double panel_pass_rows(double* P, int ld, int nb, int j, double scale, double dc,
+                                                  double dn, int r0, int r1) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, bs = blockDim.x;
+  const int c = lane & 15, rstep = (bs >> 6) * 4, rsub = wv * 4 + (lane >> 4);
+  const bool cin = c < nb, nxt = j + 1 < nb;
+  double acc = 0.0;
+  for (int rb = r0; rb < r1; rb += PANEL_U * rstep) {
+    // this lane's column, and columns j and j+1 of the same row (one load for
+    // the whole group each): no cross-lane traffic in the pass
+    double x[PANEL_U], xj[PANEL_U], xj1[PANEL_U];
+#pragma unroll
+    for (int u = 0; u < PANEL_U; ++u) {
+      const int r = rb + u * rstep + rsub;
+      const bool in = r < r1;
+      const double* row = P + (int64_t)r * ld;
+      x[u] = (in && cin) ? row[c] : 0.0;
+      xj[u] = in ? row[j] : 0.0;
+      xj1[u] = (in && nxt) ? row[j + 1] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < PANEL_U; ++u) {
+      const int r = rb + u * rstep + rsub;
+      const double v = scale * xj[u];  // reflector j at row r
+      double y = x[u];
+      if (c > j) y = __builtin_fma(-dc, v, y);
+      if (r < r1 && cin && c >= j) P[(int64_t)r * ld + c] = c == j ? v : y;
+      const double xn = __builtin_fma(-dn, v, xj1[u]);
+      const double o = c == j ? v : y;
+      if (r >= j + 2 && r < r1 && nxt) acc = __builtin_fma(xn, o, acc);
+    }
+  }
+  return acc;
+}
+
 // Factor the panel P = W[p0:, p0:p0+nb] (global, ld): R entries on and above
 // its diagonal, reflectors below (unit diagonal implied), tau[p0 + j], and the
 // panel's T into Tl (LDS, PNB x PNB; padding rows and columns zero).  Each
 // wavefront reads four panel rows per load, a 16-lane group per row and a lane
 // per column, so the loads are coalesced and each lane keeps the sum of its
 // own column's slot; the group's lanes also read columns j and j+1 of their
-// row (one broadcast load each) instead of exchanging them.
+// row (one broadcast load each) instead of exchanging them.  With helper
+// workgroups (job), every pass is a job: each workgroup takes a range of rows
+// and leaves its 16 sums in job->Gp, which workgroup 0 adds in a fixed order.
 __device__ __noinline__ void tall_panel(double* W, int rows_in, int ld_in, int p0_in, int nb_in, double* tau,
-                                        lds_f64* Tl, lds_f64* S, lds_f64* buf) {
-  const int tid = threadIdx.x, bs = blockDim.x, lane = tid & 63, wv = tid >> 6;
-  const int rows = uni(rows_in), ld = uni(ld_in), p0 = uni(p0_in), nb = uni(nb_in);
+                                        lds_f64* Tl, lds_f64* S, lds_f64* buf, CoopJob* job, int nwg_in,
+                                        int* iflag, lds_f64* lw) {
+  const int tid = threadIdx.x, bs = blockDim.x, lane = tid & 63;
+  const int rows = uni(rows_in), ld = uni(ld_in), p0 = uni(p0_in), nb = uni(nb_in), nwg = uni(nwg_in);
   double* P = W + (int64_t)p0 * ld + p0;
   const int mr = rows - p0;
-  const int c = lane & 15;                     // this lane's column
-  const int rstep = (bs >> 6) * 4;             // rows per workgroup step
-  const int rsub = wv * 4 + (lane >> 4);       // this lane's row within a step
+  const int c = lane & 15;  // this lane's column
   const bool cin = c < nb;
-  constexpr int U = 8;                         // steps in flight per wait
-  for (int e = tid; e < PNB * PNB; e += bs) Tl[e] = 0.0;
-  {  // sums for column 0 over rows 1..: slot 0 its norm^2, slot c its dot with column c
-    double acc = 0.0;
-    for (int rb = 1; rb < mr; rb += U * rstep) {
-      double x[U], x0[U];  // this lane's column and column 0 (the same load for the whole group)
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int r = rb + u * rstep + rsub;
-        const bool in = r < mr;
-        x[u] = (in && cin) ? P[(int64_t)r * ld + c] : 0.0;
-        x0[u] = in ? P[(int64_t)r * ld] : 0.0;
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u) acc = __builtin_fma(x0[u], x[u], acc);
+  const bool coop = job != nullptr && nwg > 1;
+  // the sums over all rows of one pass into S
+  auto gather = [&]() {
+    if (tid < PNB) {
+      double t = 0.0;
+      for (int w = 0; w < nwg; ++w) t += job->Gp[w * PNB + tid];
+      S[tid] = t;
     }
-    tall_reduce16(acc, buf, S);
+    __syncthreads();
+  };
+  for (int e = tid; e < PNB * PNB; e += bs) Tl[e] = 0.0;
+  if (coop) {
+    if (tid == 0) {
+      job->type = 4;
+      job->X = P;
+      job->ldx = ld;
+      job->n = mr;
+      job->nb = nb;
+    }
+    coop_post_run(job, nwg, iflag, lw);
+    gather();
+  } else {
+    tall_reduce16(panel_pre_rows(P, ld, nb, 1, mr), buf, S);
   }
   for (int j = 0; j < nb; ++j) {
     // slot j: norm^2 of column j below row j; slot c > j: its dot with column c;
@@ -1124,35 +1212,24 @@ __device__ __noinline__ void tall_panel(double* W, int rows_in, int ld_in, int p
     }
     // row j of R right of the diagonal: a_c[j] - tau_j (v_j . a_c) (v_j[j] = 1)
     if (tid < PNB && c > j && cin) P[(int64_t)j * ld + c] = rjc - dc;
-    double acc = 0.0;
-    for (int rb = j + 1; rb < mr; rb += U * rstep) {
-      // this lane's column, and columns j and j+1 of the same row (one load
-      // for the whole group each): no cross-lane traffic in the pass
-      double x[U], xj[U], xj1[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int r = rb + u * rstep + rsub;
-        const bool in = r < mr;
-        const double* row = P + (int64_t)r * ld;
-        x[u] = (in && cin) ? row[c] : 0.0;
-        xj[u] = in ? row[j] : 0.0;
-        xj1[u] = (in && nxt) ? row[j + 1] : 0.0;
+    if (coop) {
+      if (tid < PNB) job->dcol[tid] = dc;
+      if (tid == 0) {
+        job->type = 5;
+        job->X = P;
+        job->ldx = ld;
+        job->n = mr;
+        job->nb = nb;
+        job->t = j;
+        job->tol = scale;
+        job->small = dn;
       }
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int r = rb + u * rstep + rsub;
-        const double v = scale * xj[u];  // reflector j at row r
-        double y = x[u];
-        if (c > j) y = __builtin_fma(-dc, v, y);
-        if (r < mr && cin && c >= j) P[(int64_t)r * ld + c] = c == j ? v : y;
-        // sums for column j+1 over rows below j+1: slot c > j+1 column c,
-        // c == j+1 its norm^2, c == j reflector j, c < j reflector c
-        const double xn = __builtin_fma(-dn, v, xj1[u]);
-        const double o = c == j ? v : y;
-        if (r >= j + 2 && r < mr && nxt) acc = __builtin_fma(xn, o, acc);
-      }
+      coop_post_run(job, nwg, iflag, lw);
+      if (nxt) gather();
+    } else {
+      const double acc = panel_pass_rows(P, ld, nb, j, scale, dc, dn, j + 1, mr);
+      if (nxt) tall_reduce16(acc, buf, S);
     }
-    if (j + 1 < nb) tall_reduce16(acc, buf, S);
   }
   __syncthreads();
 }
@@ -1173,57 +1250,116 @@ __device__ __forceinline__ double vref(const double* V, int ldv, int r, int j, i
   return (j < nb && r < mr) ? (r > j ? V[(int64_t)r * ldv + j] : (r == j ? 1.0 : 0.0)) : 0.0;
 }
 
+// W partial of one (tile, row chunk) unit: rows [rs, re), 16 columns from
+// col0 (columns past c1 read as zero).  Returns this lane's 4 entries.
+__device__ __forceinline__ f64x4 wy_p1_unit(const double* M, int ldm, int mr, int col0, int c1, const double* V,
+                                            int ldv, int nb, int rs, int re) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  const int col = col0 + li;
+  const bool con = col < c1;
+  f64x4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+  int r = rs;
+  for (; r + 16 <= re; r += 16) {  // 16 rows (four steps) in flight per wait
+    double a[4], b[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int rr = r + 4 * q + lk;
+      a[q] = vref(V, ldv, rr, li, mr, nb);
+      b[q] = con ? M[(int64_t)rr * ldm + col] : 0.0;
+    }
+    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0], b[0], acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1], b[1], acc1, 0, 0, 0);
+    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[2], b[2], acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[3], b[3], acc1, 0, 0, 0);
+  }
+  for (; r < re; r += 4) {
+    const int rr = r + lk;
+    const bool rin = rr < re;
+    const double a = rin ? vref(V, ldv, rr, li, mr, nb) : 0.0;
+    const double b = (rin && con) ? M[(int64_t)rr * ldm + col] : 0.0;
+    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc0, 0, 0, 0);
+  }
+  return acc0 + acc1;
+}
+
+// M -= V U on one (tile, 16-row block) unit; Ut holds -U of the tile as
+// [j][c] (16 x 16), in LDS or global memory.
+template <typename UP>
+__device__ __forceinline__ void wy_p3_unit(double* M, int ldm, int mr, int col0, int c1, const double* V, int ldv,
+                                           int nb, int rb, UP Ut) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  const int col = col0 + li;
+  const bool con = col < c1;
+  double bu[4], a[4];
+  f64x4 cm;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    bu[q] = Ut[(4 * q + lk) * 16 + li];               // B = -U[4q + k][c]
+    a[q] = vref(V, ldv, rb + li, 4 * q + lk, mr, nb);  // A = V[row][4q + k]
+    const int rr = rb + lk + 4 * q;
+    cm[q] = (con && rr < mr) ? M[(int64_t)rr * ldm + col] : 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) cm = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], bu[q], cm, 0, 0, 0);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int rr = rb + lk + 4 * e;
+    if (con && rr < mr) M[(int64_t)rr * ldm + col] = cm[e];
+  }
+}
+
 __device__ __noinline__ void tall_wy(double* M, int ldm_in, int mr_in, int c0_in, int c1_in, const double* V,
-                                     int ldv_in, int nb_in, const lds_f64* Tl, bool trans, lds_f64* L) {
+                                     int ldv_in, int nb_in, const lds_f64* Tl, bool trans, lds_f64* L,
+                                     CoopJob* job, int nwg_in, int* iflag) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
   const int ldm = uni(ldm_in), mr = uni(mr_in), c0 = uni(c0_in), c1 = uni(c1_in), ldv = uni(ldv_in),
-            nb = uni(nb_in);
-  const int li = lane & 15, lk = lane >> 4;
+            nb = uni(nb_in), nwg = uni(nwg_in);
+  const bool coop = job != nullptr && nwg > 1;
   lds_f64* Pw = L;                  // partial W tiles: [unit][e * 64 + lane]
   lds_f64* Wt = L + 32 * 256;       // W tiles [tile][j][c]
   lds_f64* Ut = Wt + 8 * 256;       // U tiles [tile][j][c]
   for (int cg = c0; cg < c1; cg += 128) {
-    const int nct = (min(128, c1 - cg) + 15) >> 4;        // 16-column tiles in this group
-    const int nch = max(1, min(32 / nct, (mr + 63) / 64));  // row chunks per tile
+    const int nct = (min(128, c1 - cg) + 15) >> 4;  // 16-column tiles in this group
+    // row chunks per tile: enough units for every wavefront (of all workgroups with helpers)
+    const int nch = coop ? max(1, min(min((nwg * nwv + nct - 1) / nct, (mr + 63) / 64), COOP_PG_UNITS / nct))
+                         : max(1, min(32 / nct, (mr + 63) / 64));
     const int nunit = nct * nch, chunk = (((mr + nch - 1) / nch) + 3) & ~3;
     // W = V^T M
-    for (int u = wv; u < nunit; u += nwv) {
-      const int t = u / nch, h = u - t * nch;
-      const int col = cg + t * 16 + li;
-      const bool con = col < c1;
-      const int rs = h * chunk, re = min(mr, rs + chunk);
-      f64x4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
-      int r = rs;
-      for (; r + 16 <= re; r += 16) {  // 16 rows (four steps) in flight per wait
-        double a[4], b[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int rr = r + 4 * q + lk;
-          a[q] = vref(V, ldv, rr, li, mr, nb);
-          b[q] = con ? M[(int64_t)rr * ldm + col] : 0.0;
-        }
-        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0], b[0], acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1], b[1], acc1, 0, 0, 0);
-        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[2], b[2], acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[3], b[3], acc1, 0, 0, 0);
+    if (coop) {
+      if (tid == 0) {
+        job->type = 6;
+        job->wy_M = M;
+        job->wy_ldm = ldm;
+        job->wy_mr = mr;
+        job->wy_cg = cg;
+        job->wy_c1 = c1;
+        job->wy_V = V;
+        job->wy_ldv = ldv;
+        job->nb = nb;
+        job->wy_nct = nct;
+        job->wy_nch = nch;
+        job->wy_chunk = chunk;
       }
-      for (; r < re; r += 4) {
-        const int rr = r + lk;
-        const bool rin = rr < re;
-        const double a = rin ? vref(V, ldv, rr, li, mr, nb) : 0.0;
-        const double b = (rin && con) ? M[(int64_t)rr * ldm + col] : 0.0;
-        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc0, 0, 0, 0);
-      }
+      coop_post_run(job, nwg, iflag, nullptr);
+    } else {
+      for (int u = wv; u < nunit; u += nwv) {
+        const int t = u / nch, h = u - t * nch;
+        const int rs = h * chunk, re = min(mr, rs + chunk);
+        const f64x4 acc = wy_p1_unit(M, ldm, mr, cg + t * 16, c1, V, ldv, nb, rs, re);
 #pragma unroll
-      for (int e = 0; e < 4; ++e) Pw[u * 256 + e * 64 + lane] = acc0[e] + acc1[e];
+        for (int e = 0; e < 4; ++e) Pw[u * 256 + e * 64 + lane] = acc[e];
+      }
+      __syncthreads();
     }
-    __syncthreads();
     // reduce the row chunks: W[t][j][c] with j = lane/16 + 4e, c = lane%16
     for (int x = tid; x < nct * 256; x += blockDim.x) {
       const int t = x >> 8, y = x & 255;
       const int e = y >> 6, ln = y & 63;
       double sum = 0.0;
-      for (int h = 0; h < nch; ++h) sum += Pw[(t * nch + h) * 256 + y];
+      if (coop)
+        for (int h = 0; h < nch; ++h) sum += job->Pg[(int64_t)(t * nch + h) * 256 + y];
+      else
+        for (int h = 0; h < nch; ++h) sum += Pw[(t * nch + h) * 256 + y];
       Wt[t * 256 + ((ln >> 4) + 4 * e) * 16 + (ln & 15)] = sum;
     }
     __syncthreads();
@@ -1234,51 +1370,41 @@ __device__ __noinline__ void tall_wy(double* M, int ldm_in, int mr_in, int c0_in
 #pragma unroll
       for (int j = 0; j < PNB; ++j) sum = __builtin_fma(trans ? Tl[j * PNB + i] : Tl[i * PNB + j], Wt[t * 256 + j * 16 + c], sum);
       Ut[t * 256 + i * 16 + c] = -sum;
+      if (coop) job->Ug[t * 256 + i * 16 + c] = -sum;
     }
     __syncthreads();
     // M -= V U, one (tile, 16-row block) at a time
     const int nrb = (mr + 15) >> 4;
-    for (int x = wv; x < nct * nrb; x += nwv) {
-      const int t = x / nrb, rb = (x - t * nrb) * 16;
-      const int col = cg + t * 16 + li;
-      const bool con = col < c1;
-      double bu[4], a[4];
-      f64x4 cm;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        bu[q] = Ut[t * 256 + (4 * q + lk) * 16 + li];        // B = -U[4q + k][c]
-        a[q] = vref(V, ldv, rb + li, 4 * q + lk, mr, nb);     // A = V[row][4q + k]
-        const int rr = rb + lk + 4 * q;
-        cm[q] = (con && rr < mr) ? M[(int64_t)rr * ldm + col] : 0.0;
+    if (coop) {
+      if (tid == 0) job->type = 7;  // the other wy_* fields stay from the first phase
+      coop_post_run(job, nwg, iflag, nullptr);
+    } else {
+      for (int x = wv; x < nct * nrb; x += nwv) {
+        const int t = x / nrb, rb = (x - t * nrb) * 16;
+        wy_p3_unit(M, ldm, mr, cg + t * 16, c1, V, ldv, nb, rb, (const lds_f64*)(Ut + t * 256));
       }
-#pragma unroll
-      for (int q = 0; q < 4; ++q) cm = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], bu[q], cm, 0, 0, 0);
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int rr = rb + lk + 4 * e;
-        if (con && rr < mr) M[(int64_t)rr * ldm + col] = cm[e];
-      }
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
 
 // QR of W (global rows x cols, ld) for the tall case: as qr_panel, plus every
 // panel's T in Tg (PNB x PNB each).
 __device__ __noinline__ void qr_tall(double* W, int rows_in, int cols_in, int ld_in, double* tau, double* Tg,
-                                     lds_f64* L) {
+                                     lds_f64* L, CoopJob* job, int nwg_in, int* iflag) {
   const int tid = threadIdx.x, bs = blockDim.x;
-  const int rows = uni(rows_in), cols = uni(cols_in), ld = uni(ld_in);
+  const int rows = uni(rows_in), cols = uni(cols_in), ld = uni(ld_in), nwg = uni(nwg_in);
   const int k = rows < cols ? rows : cols;
   lds_f64* Tl = L + 8 * TALL_REGION + 16 * 128;
   lds_f64* S = Tl + PNB * PNB;
   lds_f64* buf = S + 16;
   for (int p0 = 0; p0 < k; p0 += PNB) {
     const int nb = k - p0 < PNB ? k - p0 : PNB;
-    tall_panel(W, rows, ld, p0, nb, tau, Tl, S, buf);
+    tall_panel(W, rows, ld, p0, nb, tau, Tl, S, buf, job, nwg, iflag, L);
     for (int e = tid; e < PNB * PNB; e += bs) Tg[(p0 / PNB) * PNB * PNB + e] = Tl[e];
     if (p0 + nb < cols)
-      tall_wy(W + (int64_t)p0 * ld, ld, rows - p0, p0 + nb, cols, W + (int64_t)p0 * ld + p0, ld, nb, Tl, true, L);
+      tall_wy(W + (int64_t)p0 * ld, ld, rows - p0, p0 + nb, cols, W + (int64_t)p0 * ld + p0, ld, nb, Tl, true, L,
+              job, nwg, iflag);
     __syncthreads();
   }
 }
@@ -1288,9 +1414,11 @@ __device__ __noinline__ void qr_tall(double* W, int rows_in, int cols_in, int ld
 // (rows split as (aa, bb), bb < bx).
 __device__ __noinline__ void q_form_tall(const double* W, int rows_in, int k_in, int ld_in, const double* Tg,
                                          double* Mq, double* Xc, int bx_in, const double* Jb, int n_in,
-                                         const int* perm, int m_in, lds_f64* L) {
+                                         const int* perm, int m_in, lds_f64* L, CoopJob* job, int nwg_in,
+                                         int* iflag) {
   const int tid = threadIdx.x, bs = blockDim.x;
-  const int rows = uni(rows_in), k = uni(k_in), ld = uni(ld_in), bx = uni(bx_in), n = uni(n_in), m = uni(m_in);
+  const int rows = uni(rows_in), k = uni(k_in), ld = uni(ld_in), bx = uni(bx_in), n = uni(n_in), m = uni(m_in),
+            nwg = uni(nwg_in);
   const int ncol = Jb ? m : k;
   lds_f64* Tl = L + 8 * TALL_REGION + 16 * 128;
   for (int64_t e = tid; e < (int64_t)rows * ncol; e += bs) {
@@ -1304,7 +1432,7 @@ __device__ __noinline__ void q_form_tall(const double* W, int rows_in, int k_in,
     __syncthreads();
     // Q: only columns >= p0 differ from the identity yet; U: every column
     tall_wy(Mq + (int64_t)p0 * ncol, ncol, rows - p0, Jb ? 0 : p0, ncol, W + (int64_t)p0 * ld + p0, ld, nb, Tl,
-            false, L);
+            false, L, job, nwg, iflag);
   }
   for (int64_t e = tid; e < (int64_t)rows * ncol; e += bs) {
     const int bb = (int)(e % bx);
@@ -1322,8 +1450,9 @@ __device__ __noinline__ void q_form_tall(const double* W, int rows_in, int k_in,
 // layout (TALL_LDS doubles); the panel lives in its first part, which tall_wy
 // reuses once the panel is written back.
 __device__ __noinline__ void qr_panel(double* W, int rows_in, int cols_in, int ld_in, double* tau, double* Tg,
-                                      lds_f64* L) {
+                                      lds_f64* L, CoopJob* job, int nwg_in, int* iflag) {
   const int tid = threadIdx.x, bs = blockDim.x;
+  const int nwg = uni(nwg_in);
   const int rows = uni(rows_in), cols = uni(cols_in), ld = uni(ld_in);
   const int k = rows < cols ? rows : cols;
   lds_f64* Pp = L;                             // mr x PLD
@@ -1354,7 +1483,8 @@ __device__ __noinline__ void qr_panel(double* W, int rows_in, int cols_in, int l
     for (int e = tid; e < PNB * PNB; e += bs) Tg[(p0 / PNB) * PNB * PNB + e] = Tl[e];
     QTT_PT(3);
     if (p0 + nb < cols)
-      tall_wy(W + (int64_t)p0 * ld, ld, mr, p0 + nb, cols, W + (int64_t)p0 * ld + p0, ld, nb, Tl, true, L);
+      tall_wy(W + (int64_t)p0 * ld, ld, mr, p0 + nb, cols, W + (int64_t)p0 * ld + p0, ld, nb, Tl, true, L, job, nwg,
+              iflag);
     __syncthreads();
     QTT_PT(4);
     if (threadIdx.x == 0) QTT_PANEL_COUNT;
@@ -1568,17 +1698,54 @@ __device__ void coop_work(CoopJob* job, int wg, int nwg, lds_f64* lw) {
     const long long chunk = ((tot + nwg - 1) / nwg + 63) & ~63LL;
     const long long e0 = min(tot, (long long)wg * chunk), e1 = min(tot, e0 + chunk);
     for (long long x = e0 + tid; x < e1; x += blockDim.x) job->D[x] = job->Y[x];
+  } else if (type == 4 || type == 5) {  // a tall-panel pass over this workgroup's rows
+    const int mr = job->n, nb = job->nb, j = job->t;
+    const int start = type == 4 ? 1 : j + 1;
+    const int rc = (mr - start + nwg - 1) / nwg;
+    const int r0 = min(mr, start + wg * rc), r1 = min(mr, r0 + rc);
+    double acc = 0.0;
+    if (r0 < r1) {
+      if (type == 4) {
+        acc = panel_pre_rows(job->X, job->ldx, nb, r0, r1);
+      } else {
+        const int c = tid & 15;
+        acc = panel_pass_rows(job->X, job->ldx, nb, j, job->tol, job->dcol[c], job->small, r0, r1);
+      }
+    }
+    tall_reduce16(acc, lw, lw + 128);
+    if (tid < PNB) job->Gp[wg * PNB + tid] = lw[128 + tid];
+  } else if (type == 6) {  // W partials of the units of this workgroup's wavefronts
+    const int lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
+    const int nch = job->wy_nch, chunk = job->wy_chunk, mr = job->wy_mr;
+    const int nunit = job->wy_nct * nch;
+    for (int u = wg * nwv + wv; u < nunit; u += nwg * nwv) {
+      const int t = u / nch, h = u - t * nch;
+      const int rs = h * chunk, re = min(mr, rs + chunk);
+      const f64x4 acc = wy_p1_unit(job->wy_M, job->wy_ldm, mr, job->wy_cg + t * 16, job->wy_c1, job->wy_V,
+                                   job->wy_ldv, job->nb, rs, re);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) job->Pg[(int64_t)u * 256 + e * 64 + lane] = acc[e];
+    }
+  } else if (type == 7) {  // M -= V U on the (tile, 16-row block) units of this workgroup's wavefronts
+    const int wv = tid >> 6, nwv = blockDim.x >> 6;
+    const int mr = job->wy_mr, nrb = (mr + 15) >> 4, nct = job->wy_nct;
+    for (int x = wg * nwv + wv; x < nct * nrb; x += nwg * nwv) {
+      const int t = x / nrb, rb = (x - t * nrb) * 16;
+      wy_p3_unit(job->wy_M, job->wy_ldm, mr, job->wy_cg + t * 16, job->wy_c1, job->wy_V, job->wy_ldv, job->nb, rb,
+                 (const double*)(job->Ug + t * 256));
+    }
   }
   __syncthreads();  // every wavefront's writes are done before thread 0 releases them
   if (tid == 0) __hip_atomic_fetch_add(&job->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Workgroup 0: post the job whose fields thread 0 has written, take its own
+// Workgroup 0: post the job whose fields this workgroup has written, take its own
 // share, and wait until every workgroup has done its share.  Returns false if
 // the wait exceeded its limit.
 __device__ bool coop_post_run(CoopJob* job, int nwg, int* iflag, lds_f64* lw) {
   const int tid = threadIdx.x;
   lds_i32* sflag = (lds_i32*)iflag;
+  __syncthreads();  // job fields written by any thread of this workgroup come before the release
   if (tid == 0) {
     const int s = job->seq + 1;
     __hip_atomic_store(&job->seq, s, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -1765,15 +1932,15 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
     } else if (pmode) {  // workspace in global memory: blocked QR, WY updates as MFMA tiles
       double* Tg = vbuf;
       double* Mq = Tg + (((int64_t)k + PNB) * PNB);
-      if (pmode == 3) qr_tall(W, rows, b, b, tau, Tg, pnl);  // panels in global memory
-      else qr_panel(W, rows, b, b, tau, Tg, pnl);            // panels in LDS
+      if (pmode == 3) qr_tall(W, rows, b, b, tau, Tg, pnl, job, nwg, iflag);  // panels in global memory
+      else qr_panel(W, rows, b, b, tau, Tg, pnl, job, nwg, iflag);            // panels in LDS
       for (int e = tid; e < k * b; e += bs) {
         const int i = e / b, c = e - i * b;
         Rb[e] = c >= i ? W[(int64_t)i * b + c] : 0.0;
       }
       __syncthreads();
       QTT_PHASE(-1, 2);  // factored
-      q_form_tall(W, rows, k, b, Tg, Mq, Xc, bx, nullptr, 0, nullptr, 0, pnl);  // x <- Q (bond m)
+      q_form_tall(W, rows, k, b, Tg, Mq, Xc, bx, nullptr, 0, nullptr, 0, pnl, job, nwg, iflag);  // x <- Q (bond m)
     } else {
       geqr2<G>(W, rows, b, b, tau);
       for (int e = tid; e < k * b; e += bs) {
@@ -1801,8 +1968,8 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
         else if (rpl == 16) qr_reg<8, 16, 2>(Wl, rows, b, taul, Rl);
         else qr_reg<16, 12, 3>(Wl, rows, b, taul, Rl);
       } else if (pmode) {  // blocked QR; the reflectors stay in W for U below
-        if (pmode == 3) qr_tall(W, rows, b, b, tau, vbuf, pnl);
-        else qr_panel(W, rows, b, b, tau, vbuf, pnl);
+        if (pmode == 3) qr_tall(W, rows, b, b, tau, vbuf, pnl, job, nwg, iflag);
+        else qr_panel(W, rows, b, b, tau, vbuf, pnl, job, nwg, iflag);
         for (int e = tid; e < b * b; e += bs) {
           const int i = e / b, c = e - i * b;
           Rb[e] = c >= i ? W[(int64_t)i * b + c] : 0.0;
@@ -1910,7 +2077,8 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
     use_perm = true;
     // x <- U (bond m): U[row][i] = sum_j Q[row][j] J[perm[i]][j]  (or J[perm[i]][row])
     if (tall && pmode) {
-      q_form_tall(W, rows, b, b, vbuf, vbuf + (((int64_t)b + PNB) * PNB), Xc, bx, Jb, n, perm, m, pnl);
+      q_form_tall(W, rows, b, b, vbuf, vbuf + (((int64_t)b + PNB) * PNB), Xc, bx, Jb, n, perm, m, pnl, job, nwg,
+                  iflag);
     } else if (tall && use_reg) {
       const lds_f64 *Wl = (const lds_f64*)W, *taul = (const lds_f64*)tau, *Jl = (const lds_f64*)Jb;
       const lds_i32* pl = (const lds_i32*)perm;
@@ -2160,8 +2328,10 @@ int launch_truncate_batch(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double c
     if (maxcap > QTT_LDS_DOUBLES) nwg = std::max(1, std::min(32, ctx->cus));
   }
   // the tail slack covers the whole-block row reads of the Jacobi rounds (< 512 doubles past a row)
+  // helpers' exchange buffers after the job: Gp [32][16], Pg [512][256], Ug [8][256]
+  const size_t coop_doubles = 32 * 16 + (size_t)COOP_PG_UNITS * 256 + 8 * 256;
   const size_t bytes = (tot_doubles + nnet + 8) * sizeof(double) + args_bytes + tot_steps * sizeof(int) + 4096 +
-                       sizeof(CoopJob) + 256;
+                       sizeof(CoopJob) + 256 + (nwg > 1 ? coop_doubles * sizeof(double) : 0);
   double* scr = qtt_scratch(ctx, bytes);
   if (!scr) return qtt_fail(ctx, QTT_ENOMEM, "truncate scratch");
   double* d_err = scr;
@@ -2199,7 +2369,16 @@ int launch_truncate_batch(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double c
     if (sa.ws_lds < 0) return qtt_fail(ctx, QTT_EINVAL, "truncate: too many vertices");
   }
   int rc = qtt_hip_check(ctx, hipMemcpyAsync(d_args, hargs.data(), args_bytes, hipMemcpyHostToDevice, ctx->stream), "sweep args");
-  if (!rc && nwg > 1) rc = qtt_hip_check(ctx, hipMemsetAsync(d_job, 0, sizeof(CoopJob), ctx->stream), "coop job");
+  CoopJob hjob;  // zero state, plus the exchange buffers
+  std::memset(&hjob, 0, sizeof(hjob));
+  if (nwg > 1) {
+    double* cb = (double*)((char*)d_job + ((sizeof(CoopJob) + 255) & ~size_t(255)));
+    hjob.Gp = cb;
+    hjob.Pg = cb + 32 * 16;
+    hjob.Ug = hjob.Pg + (size_t)COOP_PG_UNITS * 256;
+  }
+  if (!rc && nwg > 1)
+    rc = qtt_hip_check(ctx, hipMemcpyAsync(d_job, &hjob, sizeof(CoopJob), hipMemcpyHostToDevice, ctx->stream), "coop job");
   if (!rc && tot_steps)
     rc = qtt_hip_check(ctx, hipMemcpyAsync(d_steps, allsteps.data(), tot_steps * sizeof(int), hipMemcpyHostToDevice, ctx->stream), "steps");
   if (rc) return rc;
