@@ -269,6 +269,7 @@ struct CoopJob {
   // (wy_* fields), the first leaving per-unit partial tiles in Pg, the second
   // reading -U from Ug.
   int nb;
+  int bsz, nblk;  // type 8: block round t of a block Jacobi sweep (blocks of bsz rows)
   double dcol[16];
   double* Gp;  // [workgroup][16]
   double* Pg;  // [unit][256]
@@ -1673,6 +1674,88 @@ __device__ __noinline__ bool coop_pairs(double* X, double* J, int n_in, int b_in
   return any;
 }
 
+// Type 8: this workgroup's block pair of block round t: the 2 bsz rows of X
+// and of J are brought into LDS, swept once locally (round-robin over the
+// local rows, one wavefront per pair, both rows of X and J in registers), and
+// written back.  Returns whether any pair rotated.
+template <int E>
+__device__ __noinline__ bool coop_block(double* X, double* J, int n_in, int b_in, int bsz_in, int nblk_in, int t_in,
+                                        double tol, double small, int wg_in, lds_f64* lw) {
+  const int n = uni(n_in), b = uni(b_in), bsz = uni(bsz_in), nblk = uni(nblk_in), t = uni(t_in), wg = uni(wg_in);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, bs = blockDim.x;
+  if (wg >= nblk / 2) return false;
+  const int P = rr_player(wg, t, nblk), Q = rr_player(nblk - 1 - wg, t, nblk);
+  const int nl = 2 * bsz;  // local rows: P's then Q's
+  auto grow = [&](int i) { return i < bsz ? P * bsz + i : Q * bsz + (i - bsz); };
+  const int lx = (b + 1) & ~1, lj = (n + 1) & ~1;  // LDS row strides
+  lds_f64* Xl = lw;
+  lds_f64* Jl = lw + nl * lx;
+  lds_i32* fl = (lds_i32*)(Jl + nl * lj);
+  for (int e = tid; e < nl * lx; e += bs) {
+    const int i = e / lx, c = e - i * lx, r = grow(i);
+    Xl[e] = (r < n && c < b) ? X[(int64_t)r * b + c] : 0.0;
+  }
+  for (int e = tid; e < nl * lj; e += bs) {
+    const int i = e / lj, c = e - i * lj, r = grow(i);
+    Jl[e] = (r < n && c < n) ? J[(int64_t)r * n + c] : 0.0;
+  }
+  if (tid == 0) fl[0] = 0;
+  __syncthreads();
+  for (int lt = 0; lt < nl - 1; ++lt) {
+    if (wv < nl / 2) {
+      int i = rr_player(wv, lt, nl), k = rr_player(nl - 1 - wv, lt, nl);
+      if (grow(i) > grow(k)) {  // the lower global row first, as in the other Jacobi routines
+        const int tmp = i;
+        i = k;
+        k = tmp;
+      }
+      if (grow(k) < n) {
+        const int gl = opaque(lane);
+        double u[E], w[E], ju[E], jw[E];
+        load_row<64, E>(Xl + i * lx, b, gl, u);
+        load_row<64, E>(Xl + k * lx, b, gl, w);
+        load_row<64, E>(Jl + i * lj, n, gl, ju);
+        load_row<64, E>(Jl + k * lj, n, gl, jw);
+        double a0 = 0.0, c0 = 0.0, g0 = 0.0;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          a0 = __builtin_fma(u[e], u[e], a0);
+          c0 = __builtin_fma(w[e], w[e], c0);
+          g0 = __builtin_fma(u[e], w[e], g0);
+        }
+        const double a = jsum<64>(a0), c = jsum<64>(c0), g = jsum<64>(g0);
+        if (g != 0.0 && a > small && c > small && g * g > tol * tol * (a * c)) {
+          const double zeta = (c - a) * rcp_nr(2.0 * g);
+          const double q1 = __builtin_fma(zeta, zeta, 1.0);
+          const double r = q1 * rsq_nr(q1);
+          const double t0 = rcp_nr(fabs(zeta) + r);
+          const double tt = zeta >= 0.0 ? t0 : -t0;
+          const double cs = rsq_nr(__builtin_fma(tt, tt, 1.0)), sn = cs * tt;
+          rotate4<E>(u, w, cs, sn);
+          rotate4<E>(ju, jw, cs, sn);
+          store_row<64, E>(Xl + i * lx, b, gl, u);
+          store_row<64, E>(Xl + k * lx, b, gl, w);
+          store_row<64, E>(Jl + i * lj, n, gl, ju);
+          store_row<64, E>(Jl + k * lj, n, gl, jw);
+          if (lane == 0) fl[0] = 1;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < nl * lx; e += bs) {
+    const int i = e / lx, c = e - i * lx, r = grow(i);
+    if (r < n && c < b) X[(int64_t)r * b + c] = Xl[e];
+  }
+  for (int e = tid; e < nl * lj; e += bs) {
+    const int i = e / lj, c = e - i * lj, r = grow(i);
+    if (r < n && c < n) J[(int64_t)r * n + c] = Jl[e];
+  }
+  const bool any = fl[0] != 0;
+  __syncthreads();
+  return any;
+}
+
 // This workgroup's share of the posted job, then its completion count.  lw:
 // LDS the job may use (the staged R block of a mode product).
 __device__ void coop_work(CoopJob* job, int wg, int nwg, lds_f64* lw) {
@@ -1698,6 +1781,14 @@ __device__ void coop_work(CoopJob* job, int wg, int nwg, lds_f64* lw) {
     const long long chunk = ((tot + nwg - 1) / nwg + 63) & ~63LL;
     const long long e0 = min(tot, (long long)wg * chunk), e1 = min(tot, e0 + chunk);
     for (long long x = e0 + tid; x < e1; x += blockDim.x) job->D[x] = job->Y[x];
+  } else if (type == 8) {  // a block round of a block Jacobi sweep
+    const int e = job->e;
+    bool r;
+    if (e <= 2) r = coop_block<2>(job->X, job->J, job->n, job->b, job->bsz, job->nblk, job->t, job->tol, job->small, wg, lw);
+    else if (e <= 4) r = coop_block<4>(job->X, job->J, job->n, job->b, job->bsz, job->nblk, job->t, job->tol, job->small, wg, lw);
+    else if (e <= 6) r = coop_block<6>(job->X, job->J, job->n, job->b, job->bsz, job->nblk, job->t, job->tol, job->small, wg, lw);
+    else r = coop_block<8>(job->X, job->J, job->n, job->b, job->bsz, job->nblk, job->t, job->tol, job->small, wg, lw);
+    if (r && tid == 0) __hip_atomic_store(&job->rot, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   } else if (type == 4 || type == 5) {  // a tall-panel pass over this workgroup's rows
     const int mr = job->n, nb = job->nb, j = job->t;
     const int start = type == 4 ? 1 : j + 1;
@@ -1804,7 +1895,8 @@ __device__ void coop_helper(CoopJob* job, int wg, int nwg, lds_i32* sflag, lds_f
 // One-sided Jacobi on rows in global memory with the helper workgroups (run
 // by workgroup 0; same rules as jacobi_rows_reg, round-robin rounds, each
 // round one job).
-__device__ bool jacobi_coop(double* X, int n, int b, double* J, CoopJob* job, int nwg, double* red, int* iflag) {
+__device__ bool jacobi_coop(double* X, int n, int b, double* J, CoopJob* job, int nwg, double* red, int* iflag,
+                            lds_f64* lw, int lw_cap) {
   const int tid = threadIdx.x, bs = blockDim.x;
   for (int e2 = tid; e2 < n * n; e2 += bs) J[e2] = (e2 / n == e2 % n) ? 1.0 : 0.0;
   __syncthreads();
@@ -1817,11 +1909,19 @@ __device__ bool jacobi_coop(double* X, int n, int b, double* J, CoopJob* job, in
   const int len = b > n ? b : n;
   const int e = len <= 128 ? 2 : len <= 256 ? 4 : len <= 384 ? 6 : 8;
   lds_i32* sflag = (lds_i32*)iflag;
+  // block rounds: pairs of blocks of bsz rows, one pair per workgroup, swept
+  // locally in LDS (fewer rounds, each with more work); else row-pair rounds
+  int bsz = max(4, (n + 2 * nwg - 1) / (2 * nwg));
+  int nblk = (n + bsz - 1) / bsz;
+  nblk += nblk & 1;
+  const int need = 2 * bsz * (((b + 1) & ~1) + ((n + 1) & ~1)) + 8;
+  const bool blocked = bsz <= 8 && nblk >= 4 && need <= lw_cap && need <= QTT_LDS_DOUBLES - 64;
+  const int rounds = blocked ? nblk - 1 : nn - 1;
   for (int sweep = 0; sweep < 100; ++sweep) {
     if (tid == 0) __hip_atomic_store(&job->rot, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (int t = 0; t < nn - 1; ++t) {
+    for (int t = 0; t < rounds; ++t) {
       if (tid == 0) {
-        job->type = 1;
+        job->type = blocked ? 8 : 1;
         job->X = X;
         job->J = J;
         job->n = n;
@@ -1833,8 +1933,10 @@ __device__ bool jacobi_coop(double* X, int n, int b, double* J, CoopJob* job, in
         job->e = e;
         job->tol = tol;
         job->small = small;
+        job->bsz = bsz;
+        job->nblk = nblk;
       }
-      if (!coop_post_run(job, nwg, iflag, nullptr)) break;
+      if (!coop_post_run(job, nwg, iflag, lw)) break;
     }
     if (tid == 0) sflag[2] = __hip_atomic_load(&job->rot, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) |
                              __hip_atomic_load(&job->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) * 2;
@@ -2018,7 +2120,7 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
     } else if (gws && job && nwg > 1 && n >= 32 && b <= 512) {
       // rows in global memory and helper workgroups present: each round's
       // pairs are spread over all of them
-      conv = jacobi_coop(X, n, b, Jb, job, nwg, red, iflag);
+      conv = jacobi_coop(X, n, b, Jb, job, nwg, red, iflag, pnl, pnl ? pnl_cap : 0);
     } else if (gws && b <= 512) {
       // workspace in global memory: both rows of a pair (and of J) in
       // registers, 32 lanes per pair (two pairs per wavefront; 64 for rows
