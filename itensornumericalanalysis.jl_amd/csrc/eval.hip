@@ -160,6 +160,7 @@ __device__ __forceinline__ void eval_chain_reg_body(const double* __restrict__ p
           if (r0 < sd.chir) {
 #pragma unroll
             for (int rr = 0; rr < 4; ++rr) {
+              if (rr == 2 && r0 + 2 >= sd.chir) break;  // rows past the bond, rounded up to 2, are zero
               const double* __restrict__ q = P + ((r0 + rr) * lpad + l0) * 2;
               const double vr = v[r0 + rr];
 #pragma unroll
@@ -269,6 +270,7 @@ eval_chain_reg2_kernel(const double* __restrict__ pack, const int64_t* __restric
           if (r0 < sd.chir) {
 #pragma unroll
             for (int rr = 0; rr < 4; ++rr) {
+              if (rr == 2 && r0 + 2 >= sd.chir) break;  // rows past the bond, rounded up to 2, are zero
               const double* __restrict__ q = P + ((r0 + rr) * lpad + l0) * 2;
               const double x0 = v0[r0 + rr], x1 = v1[r0 + rr];
 #pragma unroll
