@@ -252,7 +252,7 @@ struct CoopJob {
   double* X;
   double* J;
   int n, b, ldx, ldj, t, nn, e;  // e: entries per lane of the row loads (2, 4, 6 or 8)
-  double tol, small;
+  double tol, small, noise;
   // type 2: Y = R (x) U over the columns (aa, bb), each workgroup a range of
   // them (mode_product_staged); type 3: D[0, ytot) <- Y, each a range
   const double* R;
@@ -739,10 +739,21 @@ __device__ inline int rr_player(int pos, int t, int nn) {
   return pos == 0 ? 0 : 1 + x;
 }
 
+// Rows whose squared norm is at most cutoff * ||X||_F^2 / n are not rotated
+// against each other.  Together they weigh at most cutoff * ||X||_F^2, so the
+// truncation that follows the factorization discards all of them: the squared
+// row norms sum to ||X||_F^2 whatever the rotations, every other row ends
+// above this level, and every pair with a row above it is still rotated to
+// convergence.  What is kept is therefore the same.  A cutoff outside (0, 1)
+// rotates every pair.
+__device__ inline double noise_level(double fro2, double cut, int n) {
+  return (cut > 0.0 && cut < 1.0) ? cut * fro2 / (double)n : 0.0;
+}
+
 // One-sided Jacobi on the rows of X (n x b, ld b), rotations accumulated in J
 // (n x n).  Returns false if it did not converge.
 template <int G>
-__device__ bool jacobi_rows(double* X, int n, int b, double* J, int* flag, double* red) {
+__device__ bool jacobi_rows(double* X, int n, int b, double* J, int* flag, double* red, double cut) {
   const int tid = threadIdx.x, bs = blockDim.x;
   const int gl = tid & (G - 1), gid = tid / G, ng = bs / G;
   for (int e = tid; e < n * n; e += bs) J[e] = (e / n == e % n) ? 1.0 : 0.0;
@@ -754,7 +765,9 @@ __device__ bool jacobi_rows(double* X, int n, int b, double* J, int* flag, doubl
   // rows below eps * ||X||_F are numerically zero and are left alone
   double fro = 0.0;
   for (int64_t e = tid; e < (int64_t)n * b; e += bs) fro += X[e] * X[e];
-  const double small = block_sum(fro, red) * DBL_EPSILON * DBL_EPSILON;
+  const double fsum = block_sum(fro, red);
+  const double small = fsum * DBL_EPSILON * DBL_EPSILON;
+  const double noise = noise_level(fsum, cut, n);
   for (int sweep = 0; sweep < 100; ++sweep) {
     if (tid == 0) *flag = 0;
     __syncthreads();
@@ -789,7 +802,7 @@ __device__ bool jacobi_rows(double* X, int n, int b, double* J, int* flag, doubl
         const double a = gsum<G>(a0 + a1);
         const double c = gsum<G>(c0 + c1);
         const double g = gsum<G>(g0 + g1);
-        if (g != 0.0 && a > small && c > small && fabs(g) > tol * sqrt(a) * sqrt(c)) {
+        if (g != 0.0 && a > small && c > small && (a > noise || c > noise) && fabs(g) > tol * sqrt(a) * sqrt(c)) {
           const double zeta = (c - a) / (2.0 * g);
           const double tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
           const double cs = 1.0 / sqrt(1.0 + tt * tt), sn = cs * tt;
@@ -905,7 +918,7 @@ __device__ __forceinline__ void rotate4(double (&u)[JE], double (&w)[JE], double
 // where a row's tail past its end lies inside the padded workspace too.)
 template <int G, int JE = JEPL, typename F = lds_f64>
 __device__ __noinline__ bool jacobi_rows_reg(F* X, int ldx_in, int n_in, int b_in, F* J, int ldj_in,
-                                             lds_i32* flag, double* red) {
+                                             lds_i32* flag, double* red, double cut) {
   const int ldx = uni(ldx_in), n = uni(n_in), b = uni(b_in), ldj = uni(ldj_in);
   const int tid = threadIdx.x, bs = blockDim.x;
   const int gid = tid / G, ng = bs / G;
@@ -919,7 +932,9 @@ __device__ __noinline__ bool jacobi_rows_reg(F* X, int ldx_in, int n_in, int b_i
     const double x = X[(e / b) * ldx + e % b];
     fro += x * x;
   }
-  const double small = block_sum(fro, red) * DBL_EPSILON * DBL_EPSILON;
+  const double fsum = block_sum(fro, red);
+  const double small = fsum * DBL_EPSILON * DBL_EPSILON;
+  const double noise = noise_level(fsum, cut, n);
   QTT_JT_INIT;
   for (int sweep = 0; sweep < 100; ++sweep) {
     if (tid == 0) *flag = 0;
@@ -959,7 +974,7 @@ __device__ __noinline__ bool jacobi_rows_reg(F* X, int ldx_in, int n_in, int b_i
         const double a = jsum<G>(a0 + a1), c = jsum<G>(c0 + c1), g = jsum<G>(g0 + g1);
         QTT_JT(0);
         // |g| > tol sqrt(a c), compared squared (a c stays finite for core entries)
-        if (g != 0.0 && a > small && c > small && g * g > tol * tol * (a * c)) {
+        if (g != 0.0 && a > small && c > small && (a > noise || c > noise) && g * g > tol * tol * (a * c)) {
           const double zeta = (c - a) * rcp_nr(2.0 * g);
           const double q1 = __builtin_fma(zeta, zeta, 1.0);
           const double r = q1 * rsq_nr(q1);  // sqrt(1 + zeta^2)
@@ -1624,7 +1639,7 @@ __device__ __noinline__ void mode_product_staged(const double* Rrows, const int*
 // J), as in jacobi_rows_reg.  Returns whether any of its pairs rotated.
 template <int E>
 __device__ __noinline__ bool coop_pairs(double* X, double* J, int n_in, int b_in, int ldx_in, int ldj_in, int t_in,
-                                        int nn_in, double tol, double small, int wg_in, int nwg_in) {
+                                        int nn_in, double tol, double small, double noise, int wg_in, int nwg_in) {
   const int n = uni(n_in), b = uni(b_in), ldx = uni(ldx_in), ldj = uni(ldj_in), t = uni(t_in), nn = uni(nn_in),
             wg = uni(wg_in), nwg = uni(nwg_in);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
@@ -1655,7 +1670,7 @@ __device__ __noinline__ bool coop_pairs(double* X, double* J, int n_in, int b_in
       g0 = __builtin_fma(u[e], w[e], g0);
     }
     const double a = jsum<64>(a0), c = jsum<64>(c0), g = jsum<64>(g0);
-    if (g != 0.0 && a > small && c > small && g * g > tol * tol * (a * c)) {
+    if (g != 0.0 && a > small && c > small && (a > noise || c > noise) && g * g > tol * tol * (a * c)) {
       const double zeta = (c - a) * rcp_nr(2.0 * g);
       const double q1 = __builtin_fma(zeta, zeta, 1.0);
       const double r = q1 * rsq_nr(q1);
@@ -1680,7 +1695,7 @@ __device__ __noinline__ bool coop_pairs(double* X, double* J, int n_in, int b_in
 // written back.  Returns whether any pair rotated.
 template <int E>
 __device__ __noinline__ bool coop_block(double* X, double* J, int n_in, int b_in, int bsz_in, int nblk_in, int t_in,
-                                        double tol, double small, int wg_in, lds_f64* lw) {
+                                        double tol, double small, double noise, int wg_in, lds_f64* lw) {
   const int n = uni(n_in), b = uni(b_in), bsz = uni(bsz_in), nblk = uni(nblk_in), t = uni(t_in), wg = uni(wg_in);
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, bs = blockDim.x;
   if (wg >= nblk / 2) return false;
@@ -1724,7 +1739,7 @@ __device__ __noinline__ bool coop_block(double* X, double* J, int n_in, int b_in
           g0 = __builtin_fma(u[e], w[e], g0);
         }
         const double a = jsum<64>(a0), c = jsum<64>(c0), g = jsum<64>(g0);
-        if (g != 0.0 && a > small && c > small && g * g > tol * tol * (a * c)) {
+        if (g != 0.0 && a > small && c > small && (a > noise || c > noise) && g * g > tol * tol * (a * c)) {
           const double zeta = (c - a) * rcp_nr(2.0 * g);
           const double q1 = __builtin_fma(zeta, zeta, 1.0);
           const double r = q1 * rsq_nr(q1);
@@ -1764,10 +1779,10 @@ __device__ void coop_work(CoopJob* job, int wg, int nwg, lds_f64* lw) {
   if (type == 1) {
     const int e = job->e;
     bool r;
-    if (e <= 2) r = coop_pairs<2>(job->X, job->J, job->n, job->b, job->ldx, job->ldj, job->t, job->nn, job->tol, job->small, wg, nwg);
-    else if (e <= 4) r = coop_pairs<4>(job->X, job->J, job->n, job->b, job->ldx, job->ldj, job->t, job->nn, job->tol, job->small, wg, nwg);
-    else if (e <= 6) r = coop_pairs<6>(job->X, job->J, job->n, job->b, job->ldx, job->ldj, job->t, job->nn, job->tol, job->small, wg, nwg);
-    else r = coop_pairs<8>(job->X, job->J, job->n, job->b, job->ldx, job->ldj, job->t, job->nn, job->tol, job->small, wg, nwg);
+    if (e <= 2) r = coop_pairs<2>(job->X, job->J, job->n, job->b, job->ldx, job->ldj, job->t, job->nn, job->tol, job->small, job->noise, wg, nwg);
+    else if (e <= 4) r = coop_pairs<4>(job->X, job->J, job->n, job->b, job->ldx, job->ldj, job->t, job->nn, job->tol, job->small, job->noise, wg, nwg);
+    else if (e <= 6) r = coop_pairs<6>(job->X, job->J, job->n, job->b, job->ldx, job->ldj, job->t, job->nn, job->tol, job->small, job->noise, wg, nwg);
+    else r = coop_pairs<8>(job->X, job->J, job->n, job->b, job->ldx, job->ldj, job->t, job->nn, job->tol, job->small, job->noise, wg, nwg);
     if (r && (tid & 63) == 0) __hip_atomic_store(&job->rot, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   } else if (type == 2) {
     const int ncol = job->Au * job->Bu;
@@ -1784,10 +1799,10 @@ __device__ void coop_work(CoopJob* job, int wg, int nwg, lds_f64* lw) {
   } else if (type == 8) {  // a block round of a block Jacobi sweep
     const int e = job->e;
     bool r;
-    if (e <= 2) r = coop_block<2>(job->X, job->J, job->n, job->b, job->bsz, job->nblk, job->t, job->tol, job->small, wg, lw);
-    else if (e <= 4) r = coop_block<4>(job->X, job->J, job->n, job->b, job->bsz, job->nblk, job->t, job->tol, job->small, wg, lw);
-    else if (e <= 6) r = coop_block<6>(job->X, job->J, job->n, job->b, job->bsz, job->nblk, job->t, job->tol, job->small, wg, lw);
-    else r = coop_block<8>(job->X, job->J, job->n, job->b, job->bsz, job->nblk, job->t, job->tol, job->small, wg, lw);
+    if (e <= 2) r = coop_block<2>(job->X, job->J, job->n, job->b, job->bsz, job->nblk, job->t, job->tol, job->small, job->noise, wg, lw);
+    else if (e <= 4) r = coop_block<4>(job->X, job->J, job->n, job->b, job->bsz, job->nblk, job->t, job->tol, job->small, job->noise, wg, lw);
+    else if (e <= 6) r = coop_block<6>(job->X, job->J, job->n, job->b, job->bsz, job->nblk, job->t, job->tol, job->small, job->noise, wg, lw);
+    else r = coop_block<8>(job->X, job->J, job->n, job->b, job->bsz, job->nblk, job->t, job->tol, job->small, job->noise, wg, lw);
     if (r && tid == 0) __hip_atomic_store(&job->rot, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   } else if (type == 4 || type == 5) {  // a tall-panel pass over this workgroup's rows
     const int mr = job->n, nb = job->nb, j = job->t;
@@ -1896,7 +1911,7 @@ __device__ void coop_helper(CoopJob* job, int wg, int nwg, lds_i32* sflag, lds_f
 // by workgroup 0; same rules as jacobi_rows_reg, round-robin rounds, each
 // round one job).
 __device__ bool jacobi_coop(double* X, int n, int b, double* J, CoopJob* job, int nwg, double* red, int* iflag,
-                            lds_f64* lw, int lw_cap) {
+                            lds_f64* lw, int lw_cap, double cut) {
   const int tid = threadIdx.x, bs = blockDim.x;
   for (int e2 = tid; e2 < n * n; e2 += bs) J[e2] = (e2 / n == e2 % n) ? 1.0 : 0.0;
   __syncthreads();
@@ -1905,7 +1920,9 @@ __device__ bool jacobi_coop(double* X, int n, int b, double* J, CoopJob* job, in
   const double tol = DBL_EPSILON * (double)(b > n ? b : n);
   double fro = 0.0;
   for (int e2 = tid; e2 < n * b; e2 += bs) fro += X[e2] * X[e2];
-  const double small = block_sum(fro, red) * DBL_EPSILON * DBL_EPSILON;
+  const double fsum = block_sum(fro, red);
+  const double small = fsum * DBL_EPSILON * DBL_EPSILON;
+  const double noise = noise_level(fsum, cut, n);
   const int len = b > n ? b : n;
   const int e = len <= 128 ? 2 : len <= 256 ? 4 : len <= 384 ? 6 : 8;
   lds_i32* sflag = (lds_i32*)iflag;
@@ -1933,6 +1950,7 @@ __device__ bool jacobi_coop(double* X, int n, int b, double* J, CoopJob* job, in
         job->e = e;
         job->tol = tol;
         job->small = small;
+        job->noise = noise;
         job->bsz = bsz;
         job->nblk = nblk;
       }
@@ -2090,6 +2108,7 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
       n = b;
     }
     QTT_PHASE(-1, 2);  // factored (tall) / gathered (wide)
+    const double cut = cutoff;  // pairs of rows below its noise level are left alone
     // lanes per row pair sized so a row is at most 4 entries per lane
     bool conv;
     if (REG && G == 8 && b <= 256) {
@@ -2107,10 +2126,10 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
         __syncthreads();
       }
       lds_i32* fl = (lds_i32*)iflag;
-      if (b <= 32) conv = jacobi_rows_reg<8>(Xl, ldx, n, b, Jl, ldj, fl, red);
-      else if (b <= 64) conv = jacobi_rows_reg<16>(Xl, ldx, n, b, Jl, ldj, fl, red);
-      else if (b <= 128) conv = jacobi_rows_reg<32>(Xl, ldx, n, b, Jl, ldj, fl, red);
-      else conv = jacobi_rows_reg<64>(Xl, ldx, n, b, Jl, ldj, fl, red);
+      if (b <= 32) conv = jacobi_rows_reg<8>(Xl, ldx, n, b, Jl, ldj, fl, red, cut);
+      else if (b <= 64) conv = jacobi_rows_reg<16>(Xl, ldx, n, b, Jl, ldj, fl, red, cut);
+      else if (b <= 128) conv = jacobi_rows_reg<32>(Xl, ldx, n, b, Jl, ldj, fl, red, cut);
+      else conv = jacobi_rows_reg<64>(Xl, ldx, n, b, Jl, ldj, fl, red, cut);
       if (jpad) {
         __syncthreads();
         for (int e = tid; e < n * b; e += bs) X[e] = Xl[(e / b) * ldx + e % b];
@@ -2120,20 +2139,20 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
     } else if (gws && job && nwg > 1 && n >= 32 && b <= 512) {
       // rows in global memory and helper workgroups present: each round's
       // pairs are spread over all of them
-      conv = jacobi_coop(X, n, b, Jb, job, nwg, red, iflag, pnl, pnl ? pnl_cap : 0);
+      conv = jacobi_coop(X, n, b, Jb, job, nwg, red, iflag, pnl, pnl ? pnl_cap : 0, cut);
     } else if (gws && b <= 512) {
       // workspace in global memory: both rows of a pair (and of J) in
       // registers, 32 lanes per pair (two pairs per wavefront; 64 for rows
       // over 384), all loads of a pair in flight at once
       lds_i32* fl = (lds_i32*)iflag;
-      if (b <= 128) conv = jacobi_rows_reg<32, 4, double>(X, b, n, b, Jb, n, fl, red);
-      else if (b <= 256) conv = jacobi_rows_reg<32, 8, double>(X, b, n, b, Jb, n, fl, red);
-      else if (b <= 384) conv = jacobi_rows_reg<32, 12, double>(X, b, n, b, Jb, n, fl, red);
-      else conv = jacobi_rows_reg<64, 8, double>(X, b, n, b, Jb, n, fl, red);
+      if (b <= 128) conv = jacobi_rows_reg<32, 4, double>(X, b, n, b, Jb, n, fl, red, cut);
+      else if (b <= 256) conv = jacobi_rows_reg<32, 8, double>(X, b, n, b, Jb, n, fl, red, cut);
+      else if (b <= 384) conv = jacobi_rows_reg<32, 12, double>(X, b, n, b, Jb, n, fl, red, cut);
+      else conv = jacobi_rows_reg<64, 8, double>(X, b, n, b, Jb, n, fl, red, cut);
     } else {
       // eight lanes per pair: every group of a wavefront has a pair of its
       // own, so the rows of 64 pairs are read at once
-      conv = jacobi_rows<8>(X, n, b, Jb, iflag, red);
+      conv = jacobi_rows<8>(X, n, b, Jb, iflag, red, cut);
     }
     if (!conv) {
       if (tid == 0) status[1] = 1;

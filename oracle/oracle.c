@@ -475,8 +475,15 @@ int or_qr(int r, int c, const double* M, double* Q, double* R) {
 
 /* One-sided Jacobi on the rows of X (n x c, n <= c) with the rotations
  * accumulated in J (n x n, starts as identity): J X_in = X_out, rows of X_out
- * mutually orthogonal.  Cyclic row ordering. */
-static int jacobi_rows(int n, int c, double* X, double* J) {
+ * mutually orthogonal.  Cyclic row ordering.
+ * cut in (0, 1): the relative cutoff of the truncation that follows.  Two rows
+ * whose squared norms are both at most cut ||X||_F^2 / n are not rotated
+ * against each other: together such rows weigh at most cut ||X||_F^2 (the
+ * squared row norms always sum to ||X||_F^2), every other row ends above that
+ * level and orthogonal to all rows, so the truncation discards exactly those
+ * rows and keeps the same rows it would keep after a full SVD.  Any other cut
+ * rotates every pair. */
+static int jacobi_rows(int n, int c, double* X, double* J, double cut) {
   for (int i = 0; i < n; i++)
     for (int j = 0; j < n; j++) J[i * n + j] = (i == j) ? 1.0 : 0.0;
   if (n < 2) return OR_OK;
@@ -488,6 +495,7 @@ static int jacobi_rows(int n, int c, double* X, double* J) {
   double fro = 0.0;
   for (long e = 0; e < (long)n * c; e++) fro += X[e] * X[e];
   const double small = fro * DBL_EPSILON * DBL_EPSILON;
+  const double noise = (cut > 0.0 && cut < 1.0) ? cut * fro / (double)n : 0.0;
   for (int sweep = 0; sweep < 100; sweep++) {
     int rot = 0;
     for (int p = 0; p < n - 1; p++)
@@ -501,6 +509,7 @@ static int jacobi_rows(int n, int c, double* X, double* J) {
           g += xp[k] * xq[k];
         }
         if (g == 0.0 || a <= small || b <= small || fabs(g) <= tol * sqrt(a) * sqrt(b)) continue;
+        if (a <= noise && b <= noise) continue;
         double zeta = (b - a) / (2.0 * g);
         double tt = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
         double cs = 1.0 / sqrt(1.0 + tt * tt), sn = cs * tt;
@@ -548,8 +557,9 @@ long or_trunc_rule(const double* p, long n, double cutoff, long maxdim, double* 
 
 /* Factor M (r x c) = U (r x k) Y (k x c) with U orthonormal and the rows of Y
  * orthogonal, sorted by norm (singular value) descending.  k = min(r, c).
- * When r > c a Householder QR reduces M to its c x c triangle first. */
-static int svd_factor(int r, int c, const double* M, double* U, double* Y, double* sig) {
+ * When r > c a Householder QR reduces M to its c x c triangle first.  cut:
+ * the truncation cutoff that follows (see jacobi_rows; 0 for a full SVD). */
+static int svd_factor(int r, int c, const double* M, double* U, double* Y, double* sig, double cut) {
   int k = r < c ? r : c;
   double* Qm = NULL;
   double* X = (double*)malloc(sizeof(double) * (size_t)k * c);
@@ -560,7 +570,7 @@ static int svd_factor(int r, int c, const double* M, double* U, double* Y, doubl
     memcpy(X, M, sizeof(double) * (size_t)r * c);
   }
   double* J = (double*)malloc(sizeof(double) * (size_t)k * k);
-  int rc = jacobi_rows(k, c, X, J);
+  int rc = jacobi_rows(k, c, X, J, cut);
   /* sort rows by norm */
   int* idx = (int*)malloc(sizeof(int) * (size_t)k);
   double* nrm = (double*)malloc(sizeof(double) * (size_t)k);
@@ -604,8 +614,8 @@ static int svd_factor(int r, int c, const double* M, double* U, double* Y, doubl
   return rc;
 }
 
-int or_svd(int r, int c, const double* M, double* U, double* S, double* Vt) {
-  int rc = svd_factor(r, c, M, U, Vt, S);
+int or_svd(int r, int c, const double* M, double* U, double* S, double* Vt, double cut) {
+  int rc = svd_factor(r, c, M, U, Vt, S, cut);
   int k = r < c ? r : c;
   for (int i = 0; i < k; i++)
     if (S[i] > 0)
@@ -659,7 +669,7 @@ static int move_center(sweep_t* s, int x, int u, int truncate, double cutoff, lo
     or_qr((int)rows, (int)b, s->w1, Qx, Rx);
     m = k;
   } else {
-    int rc = svd_factor((int)rows, (int)b, s->w1, Qx, Rx, s->sig);
+    int rc = svd_factor((int)rows, (int)b, s->w1, Qx, Rx, s->sig, cutoff);
     if (rc) return rc;
     double* p = (double*)malloc(sizeof(double) * (size_t)k);
     for (long i = 0; i < k; i++) p[i] = s->sig[i] * s->sig[i];

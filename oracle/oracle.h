@@ -78,7 +78,7 @@ int or_eval_grad(int nv, const int* parent, const int* sdim, const long* bond,
 /* Dense factorizations used above, exposed for checks against LAPACK.
  * or_svd: M (r x c, row-major) = U (r x k) diag(S) Vt (k x c), k = min(r, c),
  * singular values descending. */
-int or_svd(int r, int c, const double* M, double* U, double* S, double* Vt);
+int or_svd(int r, int c, const double* M, double* U, double* S, double* Vt, double cut);
 /* or_qr: M (r x c) = Q (r x k) R (k x c), k = min(r, c). */
 int or_qr(int r, int c, const double* M, double* Q, double* R);
 

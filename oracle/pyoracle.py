@@ -39,7 +39,7 @@ def lib():
         L.or_truncate.argtypes = [C.c_int, ip, ip, lp, dpp, C.c_double, C.c_long, dp]
         L.or_eval_grad.argtypes = [C.c_int, ip, ip, lp, dpp, ip, ip, C.c_int, ip, dp, C.c_long, dp, dpp,
                                    C.c_int]
-        L.or_svd.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp]
+        L.or_svd.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp, C.c_double]
         L.or_qr.argtypes = [C.c_int, C.c_int, dp, dp, dp]
         L.or_trunc_rule.argtypes = [dp, C.c_long, C.c_double, C.c_long, dp]
         L.or_trunc_rule.restype = C.c_long
@@ -191,14 +191,18 @@ def eval_grad(parent, sdim, cores, vcoord, vdigit, Lc, xs, ybar, nthreads=None):
     return gg
 
 
-def svd(M):
+def svd(M, cut=0.0):
+    """Full SVD (cut = 0), or the factorization a truncation with relative
+    cutoff cut starts from: rows below cut * ||M||_F^2 / k are not rotated
+    against each other."""
     M = np.ascontiguousarray(M, dtype=np.float64)
     r, c = M.shape
     k = min(r, c)
     U, S, Vt = np.zeros((r, k)), np.zeros(k), np.zeros((k, c))
     _, mp = _d(M)
     rc = lib().or_svd(r, c, mp, U.ctypes.data_as(C.POINTER(C.c_double)),
-                      S.ctypes.data_as(C.POINTER(C.c_double)), Vt.ctypes.data_as(C.POINTER(C.c_double)))
+                      S.ctypes.data_as(C.POINTER(C.c_double)), Vt.ctypes.data_as(C.POINTER(C.c_double)),
+                      float(cut))
     if rc:
         raise RuntimeError(f"or_svd failed: {rc}")
     return U, S, Vt

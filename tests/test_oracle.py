@@ -93,6 +93,39 @@ def test_svd_graded_spectrum(po, rng):
     assert np.allclose(S, s, rtol=1e-6)
 
 
+@pytest.mark.parametrize("shape", [(24, 40), (40, 24)])
+def test_svd_noise_rows_left_alone(po, rng, shape):
+    # 5 singular values of order 1 and a floor of 1e-6 .. 1e-8: with cutoff
+    # 1e-10 every floor value lies below cutoff * ||M||^2 / k, so the floor rows
+    # are not rotated against each other.  What the truncation keeps must be
+    # what LAPACK's SVD gives: the same rank, the same kept singular triplets,
+    # and the same discarded weight.
+    r, c = shape
+    k = min(shape)
+    Q1, _ = np.linalg.qr(rng.standard_normal((r, k)))
+    Q2, _ = np.linalg.qr(rng.standard_normal((c, k)))
+    s = np.r_[[1.0, 0.7, 0.4, 0.2, 0.1], np.logspace(-6, -8, k - 5)]
+    M = (Q1 * s) @ Q2.T
+    cut = 1e-10
+    U, S, Vt = po.svd(M, cut)
+    _, S0, _ = po.svd(M)
+    s_ref = np.linalg.svd(M, compute_uv=False)
+    assert np.all(s_ref[5:] ** 2 <= cut * np.sum(s_ref ** 2) / k)
+    m_cut, e_cut = po.trunc_rule(S ** 2, cut, 1 << 30)
+    m_ref, e_ref = po.trunc_rule(s_ref ** 2, cut, 1 << 30)
+    assert m_cut == m_ref == 5
+    assert np.allclose(S[:5], s_ref[:5], rtol=1e-13)
+    assert math.isclose(e_cut, e_ref, rel_tol=1e-9)
+    # kept part: U_5 S_5 Vt_5 equals LAPACK's rank-5 truncation
+    Ur, Sr, Vr = np.linalg.svd(M, full_matrices=False)
+    assert np.allclose((U[:, :5] * S[:5]) @ Vt[:5], (Ur[:, :5] * Sr[:5]) @ Vr[:5], atol=1e-14)
+    # and the whole factorization still reproduces M (U orthonormal)
+    assert np.allclose((U * S) @ Vt, M, atol=1e-14)
+    assert np.allclose(U.T @ U, np.eye(k), atol=1e-13)
+    # the full SVD still resolves the floor (the rule only applies with a cutoff)
+    assert np.allclose(S0, s_ref, rtol=1e-6)
+
+
 def test_svd_rank_deficient(po, rng):
     A = rng.standard_normal((20, 3)) @ rng.standard_normal((3, 12))
     U, S, Vt = po.svd(A)
