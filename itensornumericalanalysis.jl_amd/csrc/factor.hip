@@ -2419,15 +2419,18 @@ int launch_truncate_batch(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double c
   if (nnet <= 0) return QTT_OK;
   std::vector<std::vector<int>> steps(nnet);
   std::vector<int64_t> ws(nnet), soff(nnet);
+  std::vector<double> cost(nnet);
   int64_t tot_doubles = 0, tot_steps = 0;
   int maxnv = 1;
   for (int i = 0; i < nnet; i++) {
     const qtt_net* net = nets[i];
     steps[i] = truncation_schedule(net->t);
     int64_t maxcap = 1, maxb = 1;
+    cost[i] = 0.0;
     for (int v = 0; v < net->t.nv; v++) {
       maxcap = std::max(maxcap, net->cap[v]);
       maxb = std::max(maxb, net->bond[v]);
+      cost[i] += (double)net->cap[v] * (double)net->bond[v];  // ~ a move's factorization work
     }
     if (maxcap >= (int64_t(1) << 31) / 8)
       return qtt_fail(ctx, QTT_EINVAL, "truncate: a core is too large for the sweep kernel");
@@ -2464,11 +2467,18 @@ int launch_truncate_batch(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double c
   allsteps.reserve(tot_steps);
   for (int i = 0; i < nnet; i++) allsteps.insert(allsteps.end(), steps[i].begin(), steps[i].end());
   const int lds_doubles = QTT_LDS_DOUBLES;
+  // Workgroups are dispatched in index order and the networks of a batch are
+  // independent: the costliest networks get the first workgroups, so that a
+  // large one does not start last (longest-first list scheduling).
+  std::vector<int> ord(nnet);
+  for (int i = 0; i < nnet; i++) ord[i] = i;
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return cost[x] > cost[y]; });
   std::vector<SweepArgs> hargs(nnet);
   double* w = wsp;
-  for (int i = 0; i < nnet; i++) {
+  for (int j = 0; j < nnet; j++) {
+    const int i = ord[j];
     qtt_net* net = nets[i];
-    SweepArgs& sa = hargs[i];
+    SweepArgs& sa = hargs[j];
     sa.data = net->d_data;
     sa.off = net->d_off;
     sa.bond_g = net->d_bond;

@@ -336,6 +336,25 @@ def test_comb_full_size_integrate(q, po, eng):
         assert q.integrate(w.f) == pytest.approx(ref, rel=TOL)
 
 
+def test_comb_full_size_operate_error_bound(q, eng):
+    # The 3D comb at full size: 3 x 16 digits, bond 128, so W f has bond 384.
+    # The oracle needs about 75 s for this operate.  So the check uses the
+    # truncation's own identities instead, from inner products on the GPU:
+    # - ||W f - g||^2 is at most the sum of the discarded weights times
+    #   ||W f||^2, and close to it (the sequential bound is tight here);
+    # - g is nearly the orthogonal projection of W f: <W f, g> = <g, g>.
+    from itensornumericalanalysis_jl_amd import workloads
+    w = workloads.comb_3d()
+    Wf = q.operate(w.op, w.f, truncate=False)
+    g, err = q.operate(w.op, w.f, cutoff=w.cutoff, maxdim=w.maxdim, return_error=True)
+    assert max(Wf.bonds()) == 384 and max(g.bonds()) == 128
+    nw, cr, ng = q.inner(Wf, Wf), q.inner(Wf, g), q.inner(g, g)
+    d2 = (nw - 2 * cr + ng) / nw
+    assert 0 < d2 <= err * (1 + 1e-9)
+    assert d2 == pytest.approx(err, rel=1e-3)
+    assert cr == pytest.approx(ng, rel=1e-9)
+
+
 # ------------------------------------------------------------------ batches (config-5 shape, reduced)
 def test_operate_batch_and_integrate_batch(q, po, eng, rng):
     from itensornumericalanalysis_jl_amd import workloads
