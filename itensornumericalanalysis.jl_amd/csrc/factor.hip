@@ -1573,64 +1573,55 @@ __device__ __forceinline__ void mode_product(const double* Rrows, const int* per
   mode_product_body(Rrows, perm, use_perm, m, b, Uc, gY, Au, nu, Bu);
 }
 
-// Mode product with R in global memory (the global-workspace path): R is
-// staged in LDS RB rows at a time, transposed (Rs[c][q], RB doubles per c), so
-// each step reads RB entries of R with 16-byte broadcast loads; one thread per
-// (aa, bb) column keeps RB sums and reads its column of u once per block.
-constexpr int RB = 32;
-
-__device__ __noinline__ void mode_product_staged(const double* Rrows, const int* perm, bool use_perm, int m_in,
-                                                 int b_in, const double* Uc, double* gY, int Au_in, int nu_in,
-                                                 int Bu_in, lds_f64* Rs, int j0_in = 0, int j1_in = -1) {
-  const int tid = threadIdx.x, bs = blockDim.x;
-  const int m = uni(m_in), b = uni(b_in), Au = uni(Au_in), nu = uni(nu_in), Bu = uni(Bu_in);
-  const int j0 = uni(j0_in), ncol = j1_in < 0 ? Au * Bu : uni(j1_in);  // columns [j0, ncol)
-  for (int i0 = 0; i0 < m; i0 += RB) {
-    const int rb = m - i0 < RB ? m - i0 : RB;
-    for (int e = tid; e < RB * nu; e += bs) {
-      const int q = e / nu, c = e - q * nu;
-      double v = 0.0;
-      if (q < rb) v = Rrows[(int64_t)(use_perm ? perm[i0 + q] : i0 + q) * b + c];
-      Rs[c * RB + q] = v;
-    }
-    __syncthreads();
-    for (int j = j0 + tid; j < ncol; j += bs) {
-      const int aa = j / Bu, bb = j - aa * Bu;
-      const double* up = Uc + (int64_t)aa * nu * Bu + bb;
-      double acc[RB];
+// Mode product with R in global memory (the global-workspace path) as FP64
+// MFMA tiles, v_mfma_f64_16x16x4f64: Y[(aa*m + i)*Bu + bb] = sum_c
+// R[perm(i)][c] U[(aa*nu + c)*Bu + bb] over the output columns j = aa*Bu + bb
+// in [j0, j1).  A unit is 32 output rows (two 16-row tiles sharing each B
+// operand) by 16 columns; A = R rows (lane: row li, column lk of the k-step),
+// B = U (lane: k-row lk, column li), D element e = row lk + 4e, column li.
+__device__ __noinline__ void mode_product_mfma(const double* Rrows, const int* perm, bool use_perm, int m_in,
+                                               int b_in, const double* Uc, double* gY, int Au_in, int nu_in,
+                                               int Bu_in, int j0_in, int j1_in) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int m = uni(m_in), b = uni(b_in), nu = uni(nu_in), Bu = uni(Bu_in), j0 = uni(j0_in), j1 = uni(j1_in);
+  (void)Au_in;
+  const int nit = (m + 31) >> 5, njt = (j1 - j0 + 15) >> 4;
+  for (int u = wv; u < nit * njt; u += nwv) {
+    const int it = u / njt, jt = u - it * njt;
+    const int i0 = it * 32, jc = j0 + jt * 16 + li;
+    const bool jon = jc < j1;
+    const int aa = jon ? jc / Bu : 0, bb = jon ? jc - aa * Bu : 0;
+    const double* up = Uc + (int64_t)aa * nu * Bu + bb;
+    const int ra = i0 + li, rb = i0 + 16 + li;
+    const bool aon = ra < m, bon = rb < m;
+    const double* rpa = Rrows + (int64_t)(aon ? (use_perm ? perm[ra] : ra) : 0) * b;
+    const double* rpb = Rrows + (int64_t)(bon ? (use_perm ? perm[rb] : rb) : 0) * b;
+    f64x4 da = {0.0, 0.0, 0.0, 0.0}, db = {0.0, 0.0, 0.0, 0.0};
+    for (int c0 = 0; c0 < nu; c0 += 32) {  // 8 k-steps: all 24 loads in flight before the MFMAs
+      double bv[8], av[8], aw[8];
 #pragma unroll
-      for (int q = 0; q < RB; ++q) acc[q] = 0.0;
-      int c = 0;
-      for (; c + 4 <= nu; c += 4) {
-        double uv[4];
-#pragma unroll
-        for (int q2 = 0; q2 < 4; ++q2) uv[q2] = up[(int64_t)(c + q2) * Bu];
-#pragma unroll
-        for (int q2 = 0; q2 < 4; ++q2) {
-          const lds_f64x2* rp = (const lds_f64x2*)(Rs + (c + q2) * RB);
-#pragma unroll
-          for (int q = 0; q < RB / 2; ++q) {
-            const f64x2 r2 = rp[q];
-            acc[2 * q] = __builtin_fma(r2.x, uv[q2], acc[2 * q]);
-            acc[2 * q + 1] = __builtin_fma(r2.y, uv[q2], acc[2 * q + 1]);
-          }
-        }
-      }
-      for (; c < nu; ++c) {
-        const double u = up[(int64_t)c * Bu];
-        const lds_f64x2* rp = (const lds_f64x2*)(Rs + c * RB);
-#pragma unroll
-        for (int q = 0; q < RB / 2; ++q) {
-          const f64x2 r2 = rp[q];
-          acc[2 * q] = __builtin_fma(r2.x, u, acc[2 * q]);
-          acc[2 * q + 1] = __builtin_fma(r2.y, u, acc[2 * q + 1]);
-        }
+      for (int q = 0; q < 8; ++q) {
+        const int c = c0 + 4 * q + lk;
+        const bool con = c < nu;
+        bv[q] = (jon && con) ? up[(int64_t)c * Bu] : 0.0;
+        av[q] = (aon && con) ? rpa[c] : 0.0;
+        aw[q] = (bon && con) ? rpb[c] : 0.0;
       }
 #pragma unroll
-      for (int q = 0; q < RB; ++q)
-        if (q < rb) gY[((int64_t)aa * m + i0 + q) * Bu + bb] = acc[q];
+      for (int q = 0; q < 8; ++q) {
+        da = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], bv[q], da, 0, 0, 0);
+        db = __builtin_amdgcn_mfma_f64_16x16x4f64(aw[q], bv[q], db, 0, 0, 0);
+      }
     }
-    __syncthreads();
+    if (jon) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int i = i0 + lk + 4 * e;
+        if (i < m) gY[((int64_t)aa * m + i) * Bu + bb] = da[e];
+        if (i + 16 < m) gY[((int64_t)aa * m + i + 16) * Bu + bb] = db[e];
+      }
+    }
   }
 }
 
@@ -1789,8 +1780,8 @@ __device__ void coop_work(CoopJob* job, int wg, int nwg, lds_f64* lw) {
     const int chunk = ((ncol + nwg - 1) / nwg + 63) & ~63;
     const int j0 = min(ncol, wg * chunk), j1 = min(ncol, j0 + chunk);
     if (j0 < j1)
-      mode_product_staged(job->R, job->perm, job->use_perm != 0, job->m, job->rb, job->U, job->Y, job->Au, job->nu,
-                          job->Bu, lw, j0, j1);
+      mode_product_mfma(job->R, job->perm, job->use_perm != 0, job->m, job->rb, job->U, job->Y, job->Au, job->nu,
+                        job->Bu, j0, j1);
   } else if (type == 3) {
     const long long tot = job->ytot;
     const long long chunk = ((tot + nwg - 1) / nwg + 63) & ~63LL;
@@ -2227,8 +2218,7 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
   QTT_PHASE(-1, 3);  // x written
   // u <- R (x)_{au} u : Y[(aa*m + i)*Bu + bb] = sum_c R[i][c] U[(aa*nu + c)*Bu + bb]
   const int ytot = (int)(Au * m * Bu);
-  if (!REG && gws && job && nwg > 1 && pnl && (int64_t)nu * RB <= pnl_cap &&
-      (int64_t)nu * RB <= QTT_LDS_DOUBLES - 64 && Au * Bu * nu * (int64_t)m >= (int64_t(1) << 22)) {
+  if (!REG && gws && job && nwg > 1 && Au * Bu * nu * (int64_t)m >= (int64_t(1) << 22)) {
     // a large product in global memory: columns spread over the helper workgroups
     if (!mode_product_coop(Rrows, perm, use_perm, m, b, Uc, gY, Ud, (int)Au, (int)nu, (int)Bu, job, nwg, iflag, pnl)) {
       if (tid == 0) status[1] = 1;
@@ -2236,8 +2226,8 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
   } else {
     if (REG)
       mode_product_lds((const lds_f64*)Rrows, (const lds_i32*)perm, use_perm, m, b, Uc, gY, (int)Au, (int)nu, (int)Bu);
-    else if (pnl && (int64_t)nu * RB <= pnl_cap)  // global workspace: stage R in LDS
-      mode_product_staged(Rrows, perm, use_perm, m, b, Uc, gY, (int)Au, (int)nu, (int)Bu, pnl);
+    else if (gws)  // global workspace: MFMA tiles
+      mode_product_mfma(Rrows, perm, use_perm, m, b, Uc, gY, (int)Au, (int)nu, (int)Bu, 0, (int)(Au * Bu));
     else
       mode_product(Rrows, perm, use_perm, m, b, Uc, gY, (int)Au, (int)nu, (int)Bu);
     __syncthreads();
