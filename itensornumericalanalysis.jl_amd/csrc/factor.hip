@@ -254,7 +254,7 @@ struct CoopJob {
   int n, b, ldx, ldj, t, nn, e;  // e: entries per lane of the row loads (2, 4, 6 or 8)
   double tol, small, noise;
   // type 2: Y = R (x) U over the columns (aa, bb), each workgroup a range of
-  // them (mode_product_staged); type 3: D[0, ytot) <- Y, each a range
+  // them (mode_product_mfma); type 3: D[0, ytot) <- Y, each a range
   const double* R;
   const int* perm;
   const double* U;
@@ -1763,7 +1763,7 @@ __device__ __noinline__ bool coop_block(double* X, double* J, int n_in, int b_in
 }
 
 // This workgroup's share of the posted job, then its completion count.  lw:
-// LDS the job may use (the staged R block of a mode product).
+// LDS the job may use (the row blocks of a block Jacobi round).
 __device__ void coop_work(CoopJob* job, int wg, int nwg, lds_f64* lw) {
   const int tid = threadIdx.x;
   const int type = job->type;
