@@ -1014,23 +1014,40 @@ __device__ __noinline__ bool jacobi_rows_reg(F* X, int ldx_in, int n_in, int b_i
 constexpr int PNB = 16;
 constexpr int PLD = PNB + 1;  // row stride of the panel that is read by columns: odd, so no bank conflicts
 
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
 // T (nb x nb upper triangular, ld PNB) of the reflectors V (unit lower
 // trapezoidal, mr x nb, ld PLD, reflector entries below the diagonal of the
-// panel) and their scalars tau.  Y is nb x nb scratch.
-__device__ void build_T(const lds_f64* V, int mr, int nb, const lds_f64* tau, lds_f64* T, lds_f64* Y) {
-  const int tid = threadIdx.x, bs = blockDim.x;
+// panel) and their scalars tau.  Y is PNB x PNB scratch, S 1024 doubles.
+__device__ void build_T(const lds_f64* V, int mr, int nb, const lds_f64* tau, lds_f64* T, lds_f64* Y, lds_f64* S) {
+  const int tid = threadIdx.x;
   nb = PNB;  // padded: the extra reflectors have tau = 0, so their rows and columns of T are 0
-  // Y[j][i] = v_j . v_i for j < i (rows past the panel: none); one wavefront
-  // per pair, the rows split over its lanes
-  const int lane = tid & 63, wv = tid >> 6, nw = bs >> 6;
-  for (int e = wv; e < nb * nb; e += nw) {
-    const int j = e / nb, i = e - j * nb;
-    if (j < i && i < mr) {
-      double acc = 0.0;
-      for (int r = i + 1 + lane; r < mr; r += 64) acc = __builtin_fma(V[r * PLD + j], V[r * PLD + i], acc);
-      acc = sum64(acc) + V[i * PLD + j];  // v_i is zero above row i and one at row i
-      if (lane == 0) Y[j * PNB + i] = acc;
-    }
+  // Y = V^T V with v_j unit at row j and zero above it: FP64 MFMA over the
+  // rows (k-steps of 4 rows), the operand of lane (li, lk) is v_li[row] for
+  // both A and B; wavefront w takes the k-steps w, w + 8, ...  (512 threads)
+  const int lane = tid & 63, wv = tid >> 6, li = lane & 15, lk = lane >> 4;
+  f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+  const int nk = (mr + 3) >> 2;
+  for (int ks = wv; ks < nk; ks += 8) {
+    const int r = 4 * ks + lk;
+    const double v = r >= mr ? 0.0 : r > li ? V[r * PLD + li] : (r == li ? 1.0 : 0.0);
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(v, v, acc, 0, 0, 0);
+  }
+  // the 8 partials summed in a fixed order through S (1024 doubles):
+  // (w0 + w4) + (w1 + w5) and (w2 + w6) + (w3 + w7)
+  if (wv >= 4) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) S[(wv - 4) * 256 + e * 64 + lane] = acc[e];
+  }
+  __syncthreads();
+  if (wv < 4) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) S[wv * 256 + e * 64 + lane] += acc[e];
+  }
+  __syncthreads();
+  if (tid < 256) {  // element e of lane ln is Y[ln / 16 + 4 e][ln % 16]
+    const int e = tid >> 6, ln = tid & 63, o = e * 64 + ln;
+    Y[((ln >> 4) + 4 * e) * PNB + (ln & 15)] = (S[o] + S[256 + o]) + (S[512 + o] + S[768 + o]);
   }
   __syncthreads();
   if (tid < nb) {  // T[:, i] = -tau_i T[:, 0:i] Y[0:i, i]; lane r owns row r
@@ -1260,7 +1277,6 @@ __device__ __noinline__ void tall_panel(double* W, int rows_in, int ld_in, int p
 //   W = V^T M: per (16-column tile, row chunk) unit, A = V^T, B = M, 4 rows a step;
 //   U = op(T) W (16 x 16 per tile, in LDS);
 //   M -= V U: per (tile, 16-row block), C = M, A = V, B = -U, 4 reflectors a step.
-typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ double vref(const double* V, int ldv, int r, int j, int mr, int nb) {
   return (j < nb && r < mr) ? (r > j ? V[(int64_t)r * ldv + j] : (r == j ? 1.0 : 0.0)) : 0.0;
@@ -1495,7 +1511,7 @@ __device__ __noinline__ void qr_panel(double* W, int rows_in, int cols_in, int l
     for (int e = nb + tid; e < PNB; e += bs) tl[e] = 0.0;  // padding reflectors
     __syncthreads();
     QTT_PT(2);
-    build_T(Pp, mr, nb, tl, Tl, Yl);
+    build_T(Pp, mr, nb, tl, Tl, Yl, tl + PNB);  // partial sums after tl, below Tl
     for (int e = tid; e < PNB * PNB; e += bs) Tg[(p0 / PNB) * PNB * PNB + e] = Tl[e];
     QTT_PT(3);
     if (p0 + nb < cols)
