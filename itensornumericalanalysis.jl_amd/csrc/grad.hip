@@ -221,11 +221,13 @@ __device__ __forceinline__ void step_to_child(const double* __restrict__ P, cons
 #pragma unroll
       for (int l0 = 0; l0 < CHI; l0 += 4) {
         if (l0 < sd.chil) {
+          // i outer, rr inner: consecutive multiply-adds go to 8 different
+          // accumulators (r, digit), so none waits on the one before
 #pragma unroll
-          for (int rr = 0; rr < 4; ++rr) {
-            const double* __restrict__ q = P + ((r0 + rr) * lpad + l0) * 2;
+          for (int i = 0; i < 4; ++i) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
+            for (int rr = 0; rr < 4; ++rr) {
+              const double* __restrict__ q = P + ((r0 + rr) * lpad + l0) * 2;
               a0[rr] = __builtin_fma(q[2 * i], L[l0 + i], a0[rr]);
               a1[rr] = __builtin_fma(q[2 * i + 1], L[l0 + i], a1[rr]);
             }
