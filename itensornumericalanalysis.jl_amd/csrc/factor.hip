@@ -1697,9 +1697,9 @@ __device__ __noinline__ bool coop_pairs(double* X, double* J, int n_in, int b_in
 }
 
 // Type 8: this workgroup's block pair of block round t: the 2 bsz rows of X
-// and of J are brought into LDS, swept once locally (round-robin over the
-// local rows, one wavefront per pair, both rows of X and J in registers), and
-// written back.  Returns whether any pair rotated.
+// and of J are brought into LDS, their pairs rotated (one wavefront per pair,
+// both rows of X and J in registers), and written back.  Returns whether any
+// pair rotated.
 template <int E>
 __device__ __noinline__ bool coop_block(double* X, double* J, int n_in, int b_in, int bsz_in, int nblk_in, int t_in,
                                         double tol, double small, double noise, int wg_in, lds_f64* lw) {
@@ -1723,9 +1723,15 @@ __device__ __noinline__ bool coop_block(double* X, double* J, int n_in, int b_in
   }
   if (tid == 0) fl[0] = 0;
   __syncthreads();
-  for (int lt = 0; lt < nl - 1; ++lt) {
-    if (wv < nl / 2) {
-      int i = rr_player(wv, lt, nl), k = rr_player(nl - 1 - wv, lt, nl);
+  // round 0 sweeps all local pairs (so every pair inside a block is met once
+  // per sweep); the later rounds take only the pairs across the two blocks,
+  // bsz steps of bsz disjoint pairs (P row w with Q row (w + step) mod bsz)
+  const int nsteps = t == 0 ? nl - 1 : bsz;
+  const int nwv = bs >> 6;
+  for (int lt = 0; lt < nsteps; ++lt) {
+    for (int pw = wv; pw < bsz; pw += nwv) {  // bsz pairs per step, one wavefront each
+      int i = t == 0 ? rr_player(pw, lt, nl) : pw;
+      int k = t == 0 ? rr_player(nl - 1 - pw, lt, nl) : bsz + (pw + lt) % bsz;
       if (grow(i) > grow(k)) {  // the lower global row first, as in the other Jacobi routines
         const int tmp = i;
         i = k;
@@ -1933,13 +1939,23 @@ __device__ bool jacobi_coop(double* X, int n, int b, double* J, CoopJob* job, in
   const int len = b > n ? b : n;
   const int e = len <= 128 ? 2 : len <= 256 ? 4 : len <= 384 ? 6 : 8;
   lds_i32* sflag = (lds_i32*)iflag;
-  // block rounds: pairs of blocks of bsz rows, one pair per workgroup, swept
-  // locally in LDS (fewer rounds, each with more work); else row-pair rounds
+  // block rounds: pairs of blocks of bsz rows, one pair per workgroup, their
+  // rows in LDS (fewer rounds, each with more work); else row-pair rounds.
+  // A round costs a handoff and the blocks' trip through LDS whatever bsz, so
+  // bsz is the largest of 16, 8, 4 whose block pair fits LDS (at least as many
+  // rows as the workgroups can take, and at least 4 blocks)
+  const int ldsum = ((b + 1) & ~1) + ((n + 1) & ~1);
+  const int cap = min(lw_cap, QTT_LDS_DOUBLES - 64);
   int bsz = max(4, (n + 2 * nwg - 1) / (2 * nwg));
+  for (int c = 16; c > bsz; c >>= 1)
+    if (2 * c * ldsum + 8 <= cap && (n + c - 1) / c >= 4) {
+      bsz = c;
+      break;
+    }
   int nblk = (n + bsz - 1) / bsz;
   nblk += nblk & 1;
-  const int need = 2 * bsz * (((b + 1) & ~1) + ((n + 1) & ~1)) + 8;
-  const bool blocked = bsz <= 8 && nblk >= 4 && need <= lw_cap && need <= QTT_LDS_DOUBLES - 64;
+  const int need = 2 * bsz * ldsum + 8;
+  const bool blocked = bsz <= 16 && nblk >= 4 && need <= cap;
   const int rounds = blocked ? nblk - 1 : nn - 1;
   for (int sweep = 0; sweep < 100; ++sweep) {
     if (tid == 0) __hip_atomic_store(&job->rot, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
