@@ -218,10 +218,57 @@ eval_chain_batch_kernel(const EvalItem* __restrict__ items, const int* __restric
                            (int64_t)(blockIdx.x - it.block0) * blockDim.x + threadIdx.x);
 }
 
+// One site of the two-point chain kernel: y = T[:, d, :] x for each point's
+// digit d (masks m0, m1), over the child bond; entries of y past the parent
+// bond are zero.
+template <int CHI>
+__device__ __forceinline__ void chain2_site(const double* __restrict__ P, const SiteDesc sd, uint64_t m0, uint64_t m1,
+                                            const double (&x0)[CHI], const double (&x1)[CHI], double (&y0)[CHI],
+                                            double (&y1)[CHI]) {
+  const int lpad = (sd.chil + 3) & ~3;
+#pragma unroll
+  for (int l0 = 0; l0 < CHI; l0 += 4) {
+    if (l0 < sd.chil) {
+      double a0[4] = {0.0, 0.0, 0.0, 0.0}, a1[4] = {0.0, 0.0, 0.0, 0.0};
+      double b0[4] = {0.0, 0.0, 0.0, 0.0}, b1[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int r0 = 0; r0 < CHI; r0 += 4) {
+        if (r0 < sd.chir) {
+#pragma unroll
+          for (int rr = 0; rr < 4; ++rr) {
+            if (rr == 2 && r0 + 2 >= sd.chir) break;  // rows past the bond, rounded up to 2, are zero
+            const double* __restrict__ q = P + ((r0 + rr) * lpad + l0) * 2;
+            const double u0 = x0[r0 + rr], u1 = x1[r0 + rr];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              a0[i] = __builtin_fma(q[2 * i], u0, a0[i]);
+              a1[i] = __builtin_fma(q[2 * i + 1], u0, a1[i]);
+              b0[i] = __builtin_fma(q[2 * i], u1, b0[i]);
+              b1[i] = __builtin_fma(q[2 * i + 1], u1, b1[i]);
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        y0[l0 + i] = bitsel(a0[i], a1[i], m0);
+        y1[l0 + i] = bitsel(b0[i], b1[i], m1);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        y0[l0 + i] = 0.0;
+        y1[l0 + i] = 0.0;
+      }
+    }
+  }
+}
+
 // Two points per lane (p and p + blockDim.x of the block's span): every core
 // entry loaded through the scalar path feeds four FMAs instead of two, which
 // halves the scalar loads, the address arithmetic and the loop overhead per
-// point.  Same arithmetic per point as eval_chain_reg_kernel.
+// point.  Same arithmetic per point as eval_chain_reg_kernel.  Sites are taken
+// two at a time, v -> w then w -> v, so no register copy follows a site.
 template <int CHI>
 __global__ void __launch_bounds__(256)
 eval_chain_reg2_kernel(const double* __restrict__ pack, const int64_t* __restrict__ poff,
@@ -248,9 +295,10 @@ eval_chain_reg2_kernel(const double* __restrict__ pack, const int64_t* __restric
   }
   v0[0] = 1.0;
   v1[0] = 1.0;
-  for (int k = nv - 1; k >= 0; --k) {
-    const SiteDesc sd = desc[k];
-    uint64_t m0 = 0, m1 = 0;
+  // the digit masks of site k for both points
+  auto masks = [&](const SiteDesc& sd, uint64_t& m0, uint64_t& m1) {
+    m0 = 0;
+    m1 = 0;
     if (sd.coord >= 0) {
       const int c = sd.coord;
       const uint64_t k0 = c == 0 ? kw0[0] : c == 1 ? kw0[1] : c == 2 ? kw0[2] : kw0[3];
@@ -258,52 +306,28 @@ eval_chain_reg2_kernel(const double* __restrict__ pack, const int64_t* __restric
       m0 = 0 - ((k0 >> sd.shift) & 1u);
       m1 = 0 - ((k1 >> sd.shift) & 1u);
     }
-    const double* __restrict__ P = pack + poff[k];
-    const int lpad = (sd.chil + 3) & ~3;
-#pragma unroll
-    for (int l0 = 0; l0 < CHI; l0 += 4) {
-      if (l0 < sd.chil) {
-        double a0[4] = {0.0, 0.0, 0.0, 0.0}, a1[4] = {0.0, 0.0, 0.0, 0.0};
-        double b0[4] = {0.0, 0.0, 0.0, 0.0}, b1[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int r0 = 0; r0 < CHI; r0 += 4) {
-          if (r0 < sd.chir) {
-#pragma unroll
-            for (int rr = 0; rr < 4; ++rr) {
-              if (rr == 2 && r0 + 2 >= sd.chir) break;  // rows past the bond, rounded up to 2, are zero
-              const double* __restrict__ q = P + ((r0 + rr) * lpad + l0) * 2;
-              const double x0 = v0[r0 + rr], x1 = v1[r0 + rr];
-#pragma unroll
-              for (int i = 0; i < 4; ++i) {
-                a0[i] = __builtin_fma(q[2 * i], x0, a0[i]);
-                a1[i] = __builtin_fma(q[2 * i + 1], x0, a1[i]);
-                b0[i] = __builtin_fma(q[2 * i], x1, b0[i]);
-                b1[i] = __builtin_fma(q[2 * i + 1], x1, b1[i]);
-              }
-            }
-          }
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          w0[l0 + i] = bitsel(a0[i], a1[i], m0);
-          w1[l0 + i] = bitsel(b0[i], b1[i], m1);
-        }
-      } else {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          w0[l0 + i] = 0.0;
-          w1[l0 + i] = 0.0;
-        }
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < CHI; ++i) {
-      v0[i] = w0[i];
-      v1[i] = w1[i];
-    }
+  };
+  int k = nv - 1;
+  for (; k >= 1; k -= 2) {
+    uint64_t m0, m1;
+    const SiteDesc sa = desc[k];
+    masks(sa, m0, m1);
+    chain2_site<CHI>(pack + poff[k], sa, m0, m1, v0, v1, w0, w1);
+    const SiteDesc sb = desc[k - 1];
+    masks(sb, m0, m1);
+    chain2_site<CHI>(pack + poff[k - 1], sb, m0, m1, w0, w1, v0, v1);
   }
-  if (act0) out[p0] = bad0 ? __builtin_nan("") : v0[0];
-  if (act1) out[p1] = bad1 ? __builtin_nan("") : v1[0];
+  double r0 = v0[0], r1 = v1[0];
+  if (k == 0) {  // an odd number of sites: the last one lands in w
+    uint64_t m0, m1;
+    const SiteDesc sa = desc[0];
+    masks(sa, m0, m1);
+    chain2_site<CHI>(pack + poff[0], sa, m0, m1, v0, v1, w0, w1);
+    r0 = w0[0];
+    r1 = w1[0];
+  }
+  if (act0) out[p0] = bad0 ? __builtin_nan("") : r0;
+  if (act1) out[p1] = bad1 ? __builtin_nan("") : r1;
   if (bad0 || bad1) flag[0] = 1;
 }
 
