@@ -125,6 +125,44 @@ eval_chain_f64_kernel(const double* __restrict__ pack, const int64_t* __restrict
 // padded to a multiple of 4, so four 64-byte scalar loads are issued back to
 // back for each group of 4 rows before their 32 FMAs.
 
+// One site of the one-point chain kernel: y = T[:, d, :] x over the child
+// bond for this lane's digit d (mask dmask); entries of y past the parent bond
+// are zero (the next site's padded rows multiply them by 0).
+template <int CHI>
+__device__ __forceinline__ void chain1_site(const double* __restrict__ P, const SiteDesc sd, uint64_t dmask,
+                                            const double (&x)[CHI], double (&y)[CHI]) {
+  const int lpad = (sd.chil + 3) & ~3;
+#pragma unroll
+  for (int l0 = 0; l0 < CHI; l0 += 4) {
+    if (l0 < sd.chil) {
+      double a0[4] = {0.0, 0.0, 0.0, 0.0}, a1[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int r0 = 0; r0 < CHI; r0 += 4) {
+        if (r0 < sd.chir) {
+#pragma unroll
+          for (int rr = 0; rr < 4; ++rr) {
+            if (rr == 2 && r0 + 2 >= sd.chir) break;  // rows past the bond, rounded up to 2, are zero
+            const double* __restrict__ q = P + ((r0 + rr) * lpad + l0) * 2;
+            const double vr = x[r0 + rr];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              a0[i] = __builtin_fma(q[2 * i], vr, a0[i]);
+              a1[i] = __builtin_fma(q[2 * i + 1], vr, a1[i]);
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) y[l0 + i] = bitsel(a0[i], a1[i], dmask);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) y[l0 + i] = 0.0;
+    }
+  }
+}
+
+// One lane per point; sites two at a time (v -> w, w -> v), so no register
+// copy follows a site.
 template <int CHI>
 __device__ __forceinline__ void eval_chain_reg_body(const double* __restrict__ pack, const int64_t* __restrict__ poff,
                                                     const SiteDesc* __restrict__ desc, int nv, int nc,
@@ -141,48 +179,25 @@ __device__ __forceinline__ void eval_chain_reg_body(const double* __restrict__ p
 #pragma unroll
   for (int i = 0; i < CHI; ++i) v[i] = 0.0;
   v[0] = 1.0;
-  for (int k = nv - 1; k >= 0; --k) {
-    const SiteDesc sd = desc[k];
-    int d = 0;
-    if (sd.coord >= 0) {
-      const uint64_t kk = sd.coord == 0 ? kw[0] : sd.coord == 1 ? kw[1] : sd.coord == 2 ? kw[2] : kw[3];
-      d = (int)((kk >> sd.shift) & 1u);
-    }
-    const uint64_t dmask = 0 - (uint64_t)d;
-    const double* __restrict__ P = pack + poff[k];
-    const int lpad = (sd.chil + 3) & ~3;
-#pragma unroll
-    for (int l0 = 0; l0 < CHI; l0 += 4) {
-      if (l0 < sd.chil) {
-        double a0[4] = {0.0, 0.0, 0.0, 0.0}, a1[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int r0 = 0; r0 < CHI; r0 += 4) {
-          if (r0 < sd.chir) {
-#pragma unroll
-            for (int rr = 0; rr < 4; ++rr) {
-              if (rr == 2 && r0 + 2 >= sd.chir) break;  // rows past the bond, rounded up to 2, are zero
-              const double* __restrict__ q = P + ((r0 + rr) * lpad + l0) * 2;
-              const double vr = v[r0 + rr];
-#pragma unroll
-              for (int i = 0; i < 4; ++i) {
-                a0[i] = __builtin_fma(q[2 * i], vr, a0[i]);
-                a1[i] = __builtin_fma(q[2 * i + 1], vr, a1[i]);
-              }
-            }
-          }
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) w[l0 + i] = bitsel(a0[i], a1[i], dmask);
-      } else {
-        // keep entries past the bond finite: the next site's padded rows multiply them by 0
-#pragma unroll
-        for (int i = 0; i < 4; ++i) w[l0 + i] = 0.0;
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < CHI; ++i) v[i] = w[i];
+  auto mask = [&](const SiteDesc& sd) -> uint64_t {
+    if (sd.coord < 0) return 0;
+    const uint64_t kk = sd.coord == 0 ? kw[0] : sd.coord == 1 ? kw[1] : sd.coord == 2 ? kw[2] : kw[3];
+    return 0 - ((kk >> sd.shift) & 1u);
+  };
+  int k = nv - 1;
+  for (; k >= 1; k -= 2) {
+    const SiteDesc sa = desc[k];
+    chain1_site<CHI>(pack + poff[k], sa, mask(sa), v, w);
+    const SiteDesc sb = desc[k - 1];
+    chain1_site<CHI>(pack + poff[k - 1], sb, mask(sb), w, v);
   }
-  if (active) out[p] = bad ? __builtin_nan("") : v[0];
+  double r = v[0];
+  if (k == 0) {  // an odd number of sites: the last one lands in w
+    const SiteDesc sa = desc[0];
+    chain1_site<CHI>(pack + poff[0], sa, mask(sa), v, w);
+    r = w[0];
+  }
+  if (active) out[p] = bad ? __builtin_nan("") : r;
   if (bad) flag[0] = 1;
 }
 
