@@ -138,7 +138,7 @@ __device__ __forceinline__ void chain1_site(const double* __restrict__ P, const 
       double a0[4] = {0.0, 0.0, 0.0, 0.0}, a1[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int r0 = 0; r0 < CHI; r0 += 4) {
-        if (r0 < sd.chir) {
+        if (r0 == 0 || r0 < sd.chir) {  // a bond has at least one row: the first block always runs
 #pragma unroll
           for (int rr = 0; rr < 4; ++rr) {
             if (rr == 2 && r0 + 2 >= sd.chir) break;  // rows past the bond, rounded up to 2, are zero
@@ -248,7 +248,7 @@ __device__ __forceinline__ void chain2_site(const double* __restrict__ P, const 
       double b0[4] = {0.0, 0.0, 0.0, 0.0}, b1[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int r0 = 0; r0 < CHI; r0 += 4) {
-        if (r0 < sd.chir) {
+        if (r0 == 0 || r0 < sd.chir) {  // a bond has at least one row: the first block always runs
 #pragma unroll
           for (int rr = 0; rr < 4; ++rr) {
             if (rr == 2 && r0 + 2 >= sd.chir) break;  // rows past the bond, rounded up to 2, are zero

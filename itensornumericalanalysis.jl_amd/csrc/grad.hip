@@ -184,7 +184,7 @@ __device__ __forceinline__ void step_to_parent(const double* __restrict__ P, con
       double a0[4] = {0.0, 0.0, 0.0, 0.0}, a1[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int r0 = 0; r0 < CHI; r0 += 4) {
-        if (r0 < sd.chir) {
+        if (r0 == 0 || r0 < sd.chir) {  // a bond has at least one row: the first block always runs
 #pragma unroll
           for (int rr = 0; rr < 4; ++rr) {
             const double* __restrict__ q = P + ((r0 + rr) * lpad + l0) * 2;
@@ -220,7 +220,7 @@ __device__ __forceinline__ void step_to_child(const double* __restrict__ P, cons
       double a0[4] = {0.0, 0.0, 0.0, 0.0}, a1[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int l0 = 0; l0 < CHI; l0 += 4) {
-        if (l0 < sd.chil) {
+        if (l0 == 0 || l0 < sd.chil) {  // at least one parent-bond entry: the first block always runs
           // i outer, rr inner: consecutive multiply-adds go to 8 different
           // accumulators (r, digit), so none waits on the one before
 #pragma unroll
