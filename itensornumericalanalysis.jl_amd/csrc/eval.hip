@@ -126,8 +126,8 @@ eval_chain_f64_kernel(const double* __restrict__ pack, const int64_t* __restrict
 // back for each group of 4 rows before their 32 FMAs.
 
 // One site of the one-point chain kernel: y = T[:, d, :] x over the child
-// bond for this lane's digit d (mask dmask); entries of y past the parent bond
-// are zero (the next site's padded rows multiply them by 0).
+// bond for this lane's digit d (mask dmask), written in blocks of 4 up to the
+// parent bond (zero past the bond inside the last block).
 template <int CHI>
 __device__ __forceinline__ void chain1_site(const double* __restrict__ P, const SiteDesc sd, uint64_t dmask,
                                             const double (&x)[CHI], double (&y)[CHI]) {
@@ -154,10 +154,8 @@ __device__ __forceinline__ void chain1_site(const double* __restrict__ P, const 
       }
 #pragma unroll
       for (int i = 0; i < 4; ++i) y[l0 + i] = bitsel(a0[i], a1[i], dmask);
-    } else {
-#pragma unroll
-      for (int i = 0; i < 4; ++i) y[l0 + i] = 0.0;
     }
+    // blocks past the bond are left as they are (never read by the next site)
   }
 }
 
@@ -234,8 +232,9 @@ eval_chain_batch_kernel(const EvalItem* __restrict__ items, const int* __restric
 }
 
 // One site of the two-point chain kernel: y = T[:, d, :] x for each point's
-// digit d (masks m0, m1), over the child bond; entries of y past the parent
-// bond are zero.
+// digit d (masks m0, m1), over the child bond.  y is written in blocks of 4
+// up to the parent bond; inside the last block the entries past the bond are
+// zero (the packed core is zero there).
 template <int CHI>
 __device__ __forceinline__ void chain2_site(const double* __restrict__ P, const SiteDesc sd, uint64_t m0, uint64_t m1,
                                             const double (&x0)[CHI], const double (&x1)[CHI], double (&y0)[CHI],
@@ -269,13 +268,9 @@ __device__ __forceinline__ void chain2_site(const double* __restrict__ P, const 
         y0[l0 + i] = bitsel(a0[i], a1[i], m0);
         y1[l0 + i] = bitsel(b0[i], b1[i], m1);
       }
-    } else {
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        y0[l0 + i] = 0.0;
-        y1[l0 + i] = 0.0;
-      }
     }
+    // blocks past the bond are left as they are: the next site reads rows only
+    // up to the bond rounded up to 2, which the last computed block covers
   }
 }
 
