@@ -16,8 +16,8 @@
  *       [bond[v]][sdim[v]][bond[c1]]...[bond[cm]]   (children c1 < ... < cm);
  *   - an operator core is [bond[v]][s_in][s_out][bond[c1]]... and acts as
  *       phi[s_out] = sum_{s_in} O[s_in, s_out] psi[s_in].
- * Julia/ITensors are column-major: the shim permutes each ITensor into this
- * order.
+ * Julia/ITensors are column-major: a Julia binding permutes each ITensor into
+ * this order (the row-major dims (d0, ..., dn) are a Julia array's (dn, ..., d0)).
  *
  * Status codes: QTT_OK, QTT_EINVAL (shape/topology/argument), QTT_ERANGE
  * (a coordinate < 0 or NaN -- the reference never terminates there; x >= 1

@@ -14,8 +14,8 @@
  *     is phi[s_out] = sum_{s_in} O[s_in, s_out] psi[s_in]
  *     (src/digit_inds.jl:35-58, src/elementary_operators.jl:296).
  *
- * Parity status of each function is stated in DESIGN.md and in the tests that
- * pin it (closed forms, brute force, the reference's own test values).
+ * Each function is pinned by tests of its own: closed forms, brute force over
+ * the whole grid, numpy's LAPACK and the reference's own test values.
  */
 #ifndef QTT_ORACLE_H
 #define QTT_ORACLE_H
