@@ -1941,13 +1941,14 @@ __device__ bool jacobi_coop(double* X, int n, int b, double* J, CoopJob* job, in
   lds_i32* sflag = (lds_i32*)iflag;
   // block rounds: pairs of blocks of bsz rows, one pair per workgroup, their
   // rows in LDS (fewer rounds, each with more work); else row-pair rounds.
-  // A round costs a handoff and the blocks' trip through LDS whatever bsz, so
-  // bsz is the largest of 16, 8, 4 whose block pair fits LDS (at least as many
-  // rows as the workgroups can take, and at least 4 blocks)
+  // A round costs a handoff and the blocks' trip through LDS, a local step
+  // costs one row pair per wavefront: bsz is 8 when the block pair fits LDS
+  // and leaves at least 4 blocks (one pair per wavefront per step), else 4, or
+  // more when the workgroups could not take all blocks
   const int ldsum = ((b + 1) & ~1) + ((n + 1) & ~1);
   const int cap = min(lw_cap, QTT_LDS_DOUBLES - 64);
   int bsz = max(4, (n + 2 * nwg - 1) / (2 * nwg));
-  for (int c = 16; c > bsz; c >>= 1)
+  for (int c = 8; c > bsz; c >>= 1)
     if (2 * c * ldsum + 8 <= cap && (n + c - 1) / c >= 4) {
       bsz = c;
       break;
