@@ -136,20 +136,41 @@ __device__ __forceinline__ void chain1_site(const double* __restrict__ P, const 
   for (int l0 = 0; l0 < CHI; l0 += 4) {
     if (l0 < sd.chil) {
       double a0[4] = {0.0, 0.0, 0.0, 0.0}, a1[4] = {0.0, 0.0, 0.0, 0.0};
+      // rows in pairs, the next pair's scalar loads issued before this pair's
+      // multiply-adds (rows up to the bond rounded up to 4 exist in the pack)
+      const double* __restrict__ Pl = P + l0 * 2;
+      const int rs = lpad * 2;
+      double qa[8], qb[8];
 #pragma unroll
-      for (int r0 = 0; r0 < CHI; r0 += 4) {
-        if (r0 == 0 || r0 < sd.chir) {  // a bond has at least one row: the first block always runs
+      for (int e = 0; e < 8; ++e) {
+        qa[e] = Pl[e];
+        qb[e] = Pl[rs + e];
+      }
 #pragma unroll
-          for (int rr = 0; rr < 4; ++rr) {
-            if (rr == 2 && r0 + 2 >= sd.chir) break;  // rows past the bond, rounded up to 2, are zero
-            const double* __restrict__ q = P + ((r0 + rr) * lpad + l0) * 2;
-            const double vr = x[r0 + rr];
+      for (int r = 0; r < CHI; r += 2) {
+        if (r != 0 && r >= sd.chir) break;
+        const int rn = r + 2 < sd.chir ? r + 2 : r;
+        double na[8], nb[8];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-              a0[i] = __builtin_fma(q[2 * i], vr, a0[i]);
-              a1[i] = __builtin_fma(q[2 * i + 1], vr, a1[i]);
-            }
-          }
+        for (int e = 0; e < 8; ++e) {
+          na[e] = Pl[rn * rs + e];
+          nb[e] = Pl[(rn + 1) * rs + e];
+        }
+        const double u = x[r], w = x[r + 1];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          a0[i] = __builtin_fma(qa[2 * i], u, a0[i]);
+          a1[i] = __builtin_fma(qa[2 * i + 1], u, a1[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          a0[i] = __builtin_fma(qb[2 * i], w, a0[i]);
+          a1[i] = __builtin_fma(qb[2 * i + 1], w, a1[i]);
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          qa[e] = na[e];
+          qb[e] = nb[e];
         }
       }
 #pragma unroll

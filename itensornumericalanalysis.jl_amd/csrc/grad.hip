@@ -182,19 +182,42 @@ __device__ __forceinline__ void step_to_parent(const double* __restrict__ P, con
   for (int l0 = 0; l0 < CHI; l0 += 4) {
     if (l0 < sd.chil) {
       double a0[4] = {0.0, 0.0, 0.0, 0.0}, a1[4] = {0.0, 0.0, 0.0, 0.0};
+      // rows in pairs, the next pair's scalar loads issued before this pair's
+      // multiply-adds (the packed core has rows up to the bond rounded up to 4,
+      // zero past the bond)
+      const double* __restrict__ Pl = P + l0 * 2;
+      const int rs = lpad * 2;
+      double qa[8], qb[8];
 #pragma unroll
-      for (int r0 = 0; r0 < CHI; r0 += 4) {
-        if (r0 == 0 || r0 < sd.chir) {  // a bond has at least one row: the first block always runs
+      for (int e = 0; e < 8; ++e) {
+        qa[e] = Pl[e];
+        qb[e] = Pl[rs + e];
+      }
 #pragma unroll
-          for (int rr = 0; rr < 4; ++rr) {
-            const double* __restrict__ q = P + ((r0 + rr) * lpad + l0) * 2;
-            const double vr = v[r0 + rr];
+      for (int r = 0; r < CHI; r += 2) {
+        if (r != 0 && r >= sd.chir) break;
+        const int rn = r + 2 < sd.chir ? r + 2 : r;
+        double na[8], nb[8];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-              a0[i] = __builtin_fma(q[2 * i], vr, a0[i]);
-              a1[i] = __builtin_fma(q[2 * i + 1], vr, a1[i]);
-            }
-          }
+        for (int e = 0; e < 8; ++e) {
+          na[e] = Pl[rn * rs + e];
+          nb[e] = Pl[(rn + 1) * rs + e];
+        }
+        const double vu = v[r], vw = v[r + 1];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          a0[i] = __builtin_fma(qa[2 * i], vu, a0[i]);
+          a1[i] = __builtin_fma(qa[2 * i + 1], vu, a1[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          a0[i] = __builtin_fma(qb[2 * i], vw, a0[i]);
+          a1[i] = __builtin_fma(qb[2 * i + 1], vw, a1[i]);
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          qa[e] = na[e];
+          qb[e] = nb[e];
         }
       }
 #pragma unroll
