@@ -374,6 +374,21 @@ def test_operate_batch_and_integrate_batch(q, po, eng, rng):
         assert e == pytest.approx(rerr, rel=1e-6, abs=1e-18)
 
 
+def test_operate_batch_order_does_not_matter(q, eng, rng):
+    # the batch truncation dispatches its costliest networks first; each network
+    # is swept by one workgroup on its own, so the order of the list changes no bit
+    from itensornumericalanalysis_jl_amd import workloads
+    ws = workloads.fredholm_batch(nfun=10, npts=16, seed=6)
+    ops, fs = [w.op for w in ws], [w.f for w in ws]
+    ga, ea = q.operate_batch(ops, fs, cutoff=1e-10)
+    gb, eb = q.operate_batch(ops[::-1], fs[::-1], cutoff=1e-10)
+    for x, y, e1, e2 in zip(ga, gb[::-1], ea, eb[::-1]):
+        assert list(x.bonds()) == list(y.bonds())
+        assert e1 == e2
+        for c1, c2 in zip(x.cores, y.cores):
+            assert np.array_equal(c1, c2)
+
+
 def test_tune_evaluate(q, po, eng, rng):
     # the autotuner times the register kernel's launch shapes and later calls use the fastest;
     # every shape runs the same arithmetic per point, so the values do not change
