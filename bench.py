@@ -221,7 +221,11 @@ def main():
         "cpu_baseline": cpu,
         "other_workloads": others,
     }
-    print(json.dumps(rec))
+    print(json.dumps(rec), flush=True)
+    # every engine handle is destroyed here, before interpreter teardown
+    from itensornumericalanalysis_jl_amd import _capi
+    _capi.release_all()
+    print("bench: engine handles released", file=sys.stderr, flush=True)
 
 
 def _time_gpu(fn, reps, torch, stream):

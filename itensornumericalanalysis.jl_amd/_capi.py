@@ -2,9 +2,11 @@
 path: if the library is missing or no GPU is visible these calls raise."""
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 import threading
+import weakref
 
 import numpy as np
 
@@ -103,6 +105,7 @@ class Context:
         if rc:
             raise EngineError(rc, "qtt_ctx_create failed (no HIP device visible?)")
         self.device = device
+        self.stream = None  # the HIP stream work runs on (None: the context's own)
 
     @property
     def handle(self):
@@ -115,6 +118,7 @@ class Context:
 
     def set_stream(self, stream_ptr):
         self.check(lib().qtt_ctx_set_stream(self._h, C.c_void_p(stream_ptr) if stream_ptr else None))
+        self.stream = stream_ptr or None
 
     def sync(self):
         self.check(lib().qtt_sync(self._h))
@@ -125,11 +129,41 @@ class Context:
             self._h = C.c_void_p()
 
     def __del__(self):
+        if _shutting_down:
+            return
         try:
             self.close()
         except Exception:
             pass
 
+
+# Device handles still alive.  At interpreter exit they are all destroyed,
+# networks and operators first, then the context, while the HIP runtime is
+# still up; the destructors that the garbage collector runs later do nothing.
+_live = weakref.WeakSet()
+_shutting_down = False
+
+
+def release_all():
+    """Destroy every live network and operator handle, then the process-wide
+    context (registered with atexit; callable earlier by a program that is done
+    with the engine)."""
+    global _shutting_down, _ctx
+    _shutting_down = True
+    for obj in list(_live):
+        try:
+            obj._release()
+        except Exception:
+            pass
+    ctx, _ctx = _ctx, None
+    if ctx is not None:
+        try:
+            ctx.close()
+        except Exception:
+            pass
+
+
+atexit.register(release_all)
 
 _ctx = None
 
@@ -158,6 +192,7 @@ class DeviceNetwork:
         self.ctx = ctx
         self.topo = topo
         self._h = handle
+        _live.add(self)
 
     @classmethod
     def upload(cls, ctx: Context, topo, cores):
@@ -192,11 +227,16 @@ class DeviceNetwork:
         self.ctx.check(lib().qtt_net_copy(self.ctx.handle, self._h, C.byref(h)))
         return DeviceNetwork(self.ctx, self.topo, h)
 
+    def _release(self):
+        if self._h:
+            lib().qtt_net_destroy(self._h)
+            self._h = C.c_void_p()
+
     def __del__(self):
+        if _shutting_down:
+            return
         try:
-            if self._h:
-                lib().qtt_net_destroy(self._h)
-                self._h = C.c_void_p()
+            self._release()
         except Exception:
             pass
 
@@ -210,15 +250,21 @@ class DeviceOperator:
         self._h = C.c_void_p()
         ctx.check(lib().qtt_mpo_create(ctx.handle, C.byref(st), bond.ctypes.data_as(C.POINTER(C.c_int64)),
                                        _dpp(cores), C.byref(self._h)))
+        _live.add(self)
 
     @property
     def handle(self):
         return self._h
 
+    def _release(self):
+        if self._h:
+            lib().qtt_mpo_destroy(self._h)
+            self._h = C.c_void_p()
+
     def __del__(self):
+        if _shutting_down:
+            return
         try:
-            if self._h:
-                lib().qtt_mpo_destroy(self._h)
-                self._h = C.c_void_p()
+            self._release()
         except Exception:
             pass

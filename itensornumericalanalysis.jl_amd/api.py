@@ -35,13 +35,34 @@ def engine_context():
 
 @contextmanager
 def on_stream(stream_ptr):
-    """Run engine work on an existing HIP stream (e.g. torch.cuda.Stream().cuda_stream)."""
+    """Run engine work on an existing HIP stream (e.g. torch.cuda.Stream().cuda_stream);
+    the stream in use before is restored afterwards."""
     ctx = engine_context()
+    prev = ctx.stream
     ctx.set_stream(stream_ptr)
     try:
         yield ctx
     finally:
-        ctx.set_stream(None)
+        ctx.set_stream(prev)
+
+
+@contextmanager
+def _torch_stream(ctx, tensor):
+    """Launch on torch's current stream of the tensor's device, so the engine's
+    reads of torch inputs and writes of torch outputs are ordered with torch's
+    own work on that stream (the default stream, or the one selected by
+    `with torch.cuda.stream(s)`).  Torch's default stream is the null stream,
+    which the context's own stream (a blocking HIP stream) is ordered with."""
+    import torch
+    s = torch.cuda.current_stream(tensor.device).cuda_stream or None
+    prev = ctx.stream
+    if s != prev:
+        ctx.set_stream(s)
+    try:
+        yield ctx
+    finally:
+        if s != prev:
+            ctx.set_stream(prev)
 
 
 def device_network(f: ITensorNetworkFunction) -> _capi.DeviceNetwork:
@@ -82,7 +103,12 @@ def _points(f, xs, dims):
 
 
 def evaluate(f: ITensorNetworkFunction, xs, dims=None, precision: int = Precision.F64, out=None):
-    """f at one point (scalar result) or at a batch of points (array result)."""
+    """f at one point (scalar result) or at a batch of points (array result).
+
+    With a CUDA tensor of points the call is asynchronous: it is queued on
+    torch's current stream and `out` (a CUDA tensor) is ready when that stream
+    reaches it.  An invalid coordinate then yields NaN at that point and a
+    DigitMapError from the next `engine_context().sync()`."""
     dn = device_network(f)
     ctx = dn.ctx
     try:
@@ -98,8 +124,9 @@ def evaluate(f: ITensorNetworkFunction, xs, dims=None, precision: int = Precisio
         n = xs.shape[0]
         if out is None:
             out = torch.empty(n, dtype=torch.float64, device=xs.device)
-        ctx.check(_capi.lib().qtt_eval(ctx.handle, dn.handle, C.c_void_p(xs.data_ptr()), n,
-                                       C.c_void_p(out.data_ptr()), int(precision)))
+        with _torch_stream(ctx, xs):
+            ctx.check(_capi.lib().qtt_eval(ctx.handle, dn.handle, C.c_void_p(xs.data_ptr()), n,
+                                           C.c_void_p(out.data_ptr()), int(precision)))
         return out
     pts, single = _points(f, xs, dims)
     res = np.empty(pts.shape[0])
@@ -127,8 +154,9 @@ def tune_evaluate(f: ITensorNetworkFunction, xs, out=None):
         out = torch.empty(xs.shape[0], dtype=torch.float64, device=xs.device)
     cfg = (C.c_int * 2)()
     ms = C.c_double(0.0)
-    ctx.check(_capi.lib().qtt_tune_eval(ctx.handle, dn.handle, C.c_void_p(xs.data_ptr()), xs.shape[0],
-                                        C.c_void_p(out.data_ptr()), cfg, C.byref(ms)))
+    with _torch_stream(ctx, xs):
+        ctx.check(_capi.lib().qtt_tune_eval(ctx.handle, dn.handle, C.c_void_p(xs.data_ptr()), xs.shape[0],
+                                            C.c_void_p(out.data_ptr()), cfg, C.byref(ms)))
     return {"points_per_lane": cfg[0], "workgroup": cfg[1], "ms": ms.value}
 
 
@@ -235,7 +263,7 @@ def operate_batch(ops, fs, cutoff=None, maxdim=None, precision: int = Precision.
 def evaluate_batch(fs, xs_list, precision: int = Precision.F64, outs=None):
     """evaluate(fs[i], xs_list[i]) for a batch of functions; each xs_list[i] is a
     contiguous float64 CUDA tensor [npts_i, ndims_i].  Returns the list of CUDA
-    result tensors (ready once the engine's stream is synchronized)."""
+    result tensors, queued on torch's current stream (as `evaluate`)."""
     import torch
     n = len(fs)
     if len(xs_list) != n:
@@ -252,8 +280,11 @@ def evaluate_batch(fs, xs_list, precision: int = Precision.F64, outs=None):
     arr_x = (C.c_void_p * n)(*[x.data_ptr() for x in xs_list])
     arr_o = (C.c_void_p * n)(*[o.data_ptr() for o in outs])
     npts = np.array([x.shape[0] for x in xs_list], dtype=np.int64)
-    ctx.check(_capi.lib().qtt_eval_batch(ctx.handle, n, arr_f, arr_x, npts.ctypes.data_as(C.POINTER(C.c_int64)),
-                                         arr_o, int(precision)))
+    if n == 0:
+        return outs
+    with _torch_stream(ctx, xs_list[0]):
+        ctx.check(_capi.lib().qtt_eval_batch(ctx.handle, n, arr_f, arr_x,
+                                             npts.ctypes.data_as(C.POINTER(C.c_int64)), arr_o, int(precision)))
     return outs
 
 

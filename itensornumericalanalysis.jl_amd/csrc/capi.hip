@@ -73,12 +73,17 @@ static int build_topo(qtt_ctx* ctx, const qtt_topology* tp, HostTopo& t) {
   for (int c = 0; c < t.ncoord; c++)
     if (t.ndigits[c] < 1 || t.ndigits[c] > 62)
       return qtt_fail(ctx, QTT_EINVAL, "topology: digits per coordinate must be in 1..62");
+  std::vector<std::vector<char>> seen(t.ncoord);
+  for (int c = 0; c < t.ncoord; c++) seen[c].assign(t.ndigits[c] + 1, 0);
   for (int v = 0; v < nv; v++) {
     if (t.sdim[v] < 1) return qtt_fail(ctx, QTT_EINVAL, "topology: sdim < 1");
     if (t.vcoord[v] >= 0) {
       if (t.vcoord[v] >= t.ncoord || t.vdigit[v] < 1 || t.vdigit[v] > t.ndigits[t.vcoord[v]])
         return qtt_fail(ctx, QTT_EINVAL, "topology: digit out of range");
       if (t.sdim[v] != 2) return qtt_fail(ctx, QTT_EINVAL, "topology: digit sites must be binary");
+      char& s = seen[t.vcoord[v]][t.vdigit[v]];
+      if (s) return qtt_fail(ctx, QTT_EINVAL, "topology: a digit sits on two vertices");
+      s = 1;
     }
   }
   t.nch.assign(nv, 0);
@@ -266,12 +271,14 @@ int qtt_ctx_create(int device, qtt_ctx** out) {
   c->force_lds_eval = ek && std::strcmp(ek, "lds") == 0;
   const char* ppl = std::getenv("QTT_EVAL_PPL");  // 1 or 2 (default: chosen by bond size)
   c->eval_ppl = ppl ? std::atoi(ppl) : 0;
+  // QTT_JACOBI_MAX_SWEEPS: sweeps before an SVD reports QTT_ENOCONV (default 100)
+  const char* js = std::getenv("QTT_JACOBI_MAX_SWEEPS");
+  if (js) c->jacobi_max_sweeps = std::min(1000, std::max(1, std::atoi(js)));
   *out = c;
   return QTT_OK;
 }
 
-void qtt_ctx_destroy(qtt_ctx* ctx) {
-  if (!ctx) return;
+static void ctx_free(qtt_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   if (ctx->d_scratch) (void)hipFree(ctx->d_scratch);
@@ -279,6 +286,18 @@ void qtt_ctx_destroy(qtt_ctx* ctx) {
   if (ctx->d_flag) (void)hipFree(ctx->d_flag);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
   delete ctx;
+}
+
+void qtt_ctx_destroy(qtt_ctx* ctx) {
+  if (!ctx) return;
+  ctx->closed = true;
+  if (ctx->refs == 0) ctx_free(ctx);  // else the last network or operator destroyed frees it
+}
+
+// a network or operator released its hold on the context
+static void ctx_release(qtt_ctx* ctx) {
+  if (!ctx) return;
+  if (--ctx->refs == 0 && ctx->closed) ctx_free(ctx);
 }
 
 const char* qtt_last_error(const qtt_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
@@ -289,24 +308,41 @@ int qtt_ctx_set_stream(qtt_ctx* ctx, void* s) {
   return QTT_OK;
 }
 
-int qtt_sync(qtt_ctx* ctx) {
-  if (!ctx) return QTT_EINVAL;
+// Wait for the stream, then report and clear the status words of slot pair
+// `base` (0: asynchronous calls, 2: the synchronous call in progress).
+static int collect_status(qtt_ctx* ctx, int base) {
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  int h[4];
-  HIPCHK(ctx, hipMemcpy(h, ctx->d_flag, sizeof(h), hipMemcpyDeviceToHost));
+  int h[2];
+  HIPCHK(ctx, hipMemcpy(h, ctx->d_flag + base, sizeof(h), hipMemcpyDeviceToHost));
   if (h[0] || h[1]) {
-    HIPCHK(ctx, hipMemset(ctx->d_flag, 0, sizeof(h)));
+    HIPCHK(ctx, hipMemset(ctx->d_flag + base, 0, sizeof(h)));
     if (h[1]) return qtt_fail(ctx, QTT_ENOCONV, "SVD did not converge");
     return qtt_fail(ctx, QTT_ERANGE, "a coordinate was negative or NaN");
   }
   return QTT_OK;
 }
 
+int qtt_sync(qtt_ctx* ctx) {
+  if (!ctx) return QTT_EINVAL;
+  return collect_status(ctx, 0);
+}
+
+namespace {
+// Kernels launched while this guard lives write the synchronous status slots.
+struct SyncSlots {
+  qtt_ctx* c;
+  int old;
+  explicit SyncSlots(qtt_ctx* ctx) : c(ctx), old(ctx->flag_base) { c->flag_base = 2; }
+  ~SyncSlots() { c->flag_base = old; }
+};
+}  // namespace
+
 static int alloc_net(qtt_ctx* ctx, const HostTopo& t, const std::vector<int64_t>& bond,
                      const std::vector<int64_t>& cap, qtt_net** out) {
   qtt_net* n = new (std::nothrow) qtt_net;
   if (!n) return qtt_fail(ctx, QTT_ENOMEM, "host allocation");
   n->ctx = ctx;
+  ctx->refs++;
   n->t = t;
   n->bond = bond;
   n->cap = cap;
@@ -414,7 +450,9 @@ void qtt_net_destroy(qtt_net* n) {
   if (n->d_pack_off) (void)hipFree(n->d_pack_off);
   if (n->d_pack32) (void)hipFree(n->d_pack32);
   if (n->d_desc) (void)hipFree(n->d_desc);
+  qtt_ctx* c = n->ctx;
   delete n;
+  ctx_release(c);
 }
 
 int qtt_mpo_create(qtt_ctx* ctx, const qtt_topology* topo, const int64_t* bond,
@@ -429,6 +467,7 @@ int qtt_mpo_create(qtt_ctx* ctx, const qtt_topology* topo, const int64_t* bond,
     return rc ? rc : qtt_fail(ctx, QTT_EINVAL, "bond[0] must be 1");
   }
   m->ctx = ctx;
+  ctx->refs++;
   int nv = m->t.nv;
   m->bond.assign(bond, bond + nv);
   m->off.resize(nv);
@@ -436,7 +475,7 @@ int qtt_mpo_create(qtt_ctx* ctx, const qtt_topology* topo, const int64_t* bond,
   std::vector<int64_t> sz(nv);
   for (int v = 0; v < nv; v++) {
     if (m->bond[v] < 1) {
-      delete m;
+      qtt_mpo_destroy(m);
       return qtt_fail(ctx, QTT_EINVAL, "operator bond < 1");
     }
     int64_t s = m->bond[v] * m->t.sdim[v] * m->t.sdim[v];
@@ -477,12 +516,21 @@ void qtt_mpo_destroy(qtt_mpo* m) {
   if (m->d_data) (void)hipFree(m->d_data);
   if (m->d_off) (void)hipFree(m->d_off);
   if (m->d_bond) (void)hipFree(m->d_bond);
+  qtt_ctx* c = m->ctx;
   delete m;
+  ctx_release(c);
 }
 
 static int check_precision(qtt_ctx* ctx, int precision) {
   if (precision != QTT_F64 && precision != QTT_F32_SPLIT && precision != QTT_F32)
     return qtt_fail(ctx, QTT_EINVAL, "unknown precision");
+  return QTT_OK;
+}
+
+// operator application: the precisions with an implementation of their own
+static int check_apply_precision(qtt_ctx* ctx, int precision) {
+  if (precision != QTT_F64)
+    return qtt_fail(ctx, QTT_EINVAL, "apply: only QTT_F64 is implemented for operator application");
   return QTT_OK;
 }
 
@@ -504,10 +552,11 @@ int qtt_eval_host(qtt_ctx* ctx, const qtt_net* net, const double* xs, int64_t np
   int nc = std::max(net->t.ncoord, 1);
   double* d = nullptr;
   HIPCHK(ctx, hipMalloc(&d, size_t(npts) * (nc + 1) * sizeof(double)));
+  SyncSlots slots(ctx);
   int rc = qtt_hip_check(ctx, hipMemcpyAsync(d, xs, size_t(npts) * nc * sizeof(double), hipMemcpyHostToDevice, ctx->stream), "upload points");
   if (!rc) rc = qtt_eval(ctx, net, d, npts, d + size_t(npts) * nc, precision);
   if (!rc) rc = qtt_hip_check(ctx, hipMemcpyAsync(out, d + size_t(npts) * nc, npts * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "download values");
-  int rs = qtt_sync(ctx);
+  int rs = collect_status(ctx, 2);
   (void)hipFree(d);
   return rc ? rc : rs;
 }
@@ -533,7 +582,7 @@ int qtt_apply(qtt_ctx* ctx, const qtt_mpo* op, const qtt_net* net, double cutoff
               int precision, qtt_net** out, double* trunc_err) {
   if (!ctx || !op || !net || !out) return qtt_fail(ctx, QTT_EINVAL, "null argument");
   *out = nullptr;
-  int rc = check_precision(ctx, precision);
+  int rc = check_apply_precision(ctx, precision);
   if (rc) return rc;
   if (!same_topo(op->t, net->t)) return qtt_fail(ctx, QTT_EINVAL, "apply: operator and network layouts differ");
   const HostTopo& t = net->t;
@@ -565,7 +614,7 @@ int qtt_apply(qtt_ctx* ctx, const qtt_mpo* op, const qtt_net* net, double cutoff
 int qtt_apply_batch(qtt_ctx* ctx, int n, const qtt_mpo* const* ops, const qtt_net* const* nets, double cutoff,
                     int64_t maxdim, int precision, qtt_net** outs, double* errs) {
   if (!ctx || n < 0 || (n && (!ops || !nets || !outs))) return qtt_fail(ctx, QTT_EINVAL, "null argument");
-  int rc = check_precision(ctx, precision);
+  int rc = check_apply_precision(ctx, precision);
   if (rc) return rc;
   for (int i = 0; i < n; i++) {
     outs[i] = nullptr;
@@ -591,6 +640,7 @@ int qtt_apply_batch(qtt_ctx* ctx, int n, const qtt_mpo* const* ops, const qtt_ne
       }
       made[i] = m;
       m->ctx = ctx;
+      ctx->refs++;
       m->t = t;
       m->bond.resize(t.nv);
       m->cap.resize(t.nv);
@@ -709,6 +759,7 @@ int qtt_eval_grad_host(qtt_ctx* ctx, const qtt_net* net, const double* xs, const
   double* d = nullptr;
   size_t bytes = (size_t(npts) * (nc + 1) + gsz) * sizeof(double);
   HIPCHK(ctx, hipMalloc(&d, std::max<size_t>(bytes, 16)));
+  SyncSlots slots(ctx);
   int rc = QTT_OK;
   double* dx = d;
   double* dy = d + size_t(npts) * nc;
@@ -718,7 +769,10 @@ int qtt_eval_grad_host(qtt_ctx* ctx, const qtt_net* net, const double* xs, const
     if (!rc) rc = qtt_hip_check(ctx, hipMemcpyAsync(dy, ybar, size_t(npts) * sizeof(double), hipMemcpyHostToDevice, ctx->stream), "upload");
   }
   if (!rc) rc = qtt_eval_grad(ctx, net, dx, dy, npts, dg, precision);
-  if (!rc) rc = qtt_sync(ctx);
+  {
+    const int rs = collect_status(ctx, 2);
+    if (!rc) rc = rs;
+  }
   int64_t o = 0;
   for (int v = 0; v < net->t.nv && !rc; v++) {
     int64_t s = net->core_size(v);

@@ -656,7 +656,7 @@ static int launch_eval_reg(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t
     if (nb2 > INT32_MAX) return qtt_fail(ctx, QTT_EINVAL, "evaluate: too many points");
 #define QTT_REG2_LAUNCH(C)                                                                         \
   eval_chain_reg2_kernel<C><<<dim3((unsigned)nb2), bsr, 0, ctx->stream>>>(                         \
-      net->d_pack, net->d_pack_off, dd, t.nv, t.ncoord, tv.ndigits, xs, npts, out, ctx->d_flag)
+      net->d_pack, net->d_pack_off, dd, t.nv, t.ncoord, tv.ndigits, xs, npts, out, ctx->flags())
     if (maxb <= 4) QTT_REG2_LAUNCH(4);
     else if (maxb <= 8) QTT_REG2_LAUNCH(8);
     else QTT_REG2_LAUNCH(16);
@@ -667,7 +667,7 @@ static int launch_eval_reg(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t
   if (nbr > INT32_MAX) return qtt_fail(ctx, QTT_EINVAL, "evaluate: too many points");
 #define QTT_REG_LAUNCH(C)                                                                          \
   eval_chain_reg_kernel<C><<<dim3((unsigned)nbr), bsr, 0, ctx->stream>>>(                          \
-      net->d_pack, net->d_pack_off, dd, t.nv, t.ncoord, tv.ndigits, xs, npts, out, ctx->d_flag)
+      net->d_pack, net->d_pack_off, dd, t.nv, t.ncoord, tv.ndigits, xs, npts, out, ctx->flags())
   if (maxb <= 4) QTT_REG_LAUNCH(4);
   else if (maxb <= 8) QTT_REG_LAUNCH(8);
   else if (maxb <= 16) QTT_REG_LAUNCH(16);
@@ -762,13 +762,13 @@ int launch_eval(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t npts, doub
     }
     if (precision == QTT_F64) {
       eval_chain_f64_kernel<<<dim3((unsigned)nb), bs, lds, ctx->stream>>>(
-          net->d_pack, net->d_pack_off, net->d_bond, tv, xs, npts, out, ctx->d_flag, (int)maxb);
+          net->d_pack, net->d_pack_off, net->d_bond, tv, xs, npts, out, ctx->flags(), (int)maxb);
     } else if (precision == QTT_F32_SPLIT) {
       eval_chain_f32_kernel<true><<<dim3((unsigned)nb), bs, lds, ctx->stream>>>(
-          (const float*)net->d_pack32, net->d_pack_off, net->d_bond, tv, xs, npts, out, ctx->d_flag, (int)maxb);
+          (const float*)net->d_pack32, net->d_pack_off, net->d_bond, tv, xs, npts, out, ctx->flags(), (int)maxb);
     } else {
       eval_chain_f32_kernel<false><<<dim3((unsigned)nb), bs, lds, ctx->stream>>>(
-          (const float*)net->d_pack32, net->d_pack_off, net->d_bond, tv, xs, npts, out, ctx->d_flag, (int)maxb);
+          (const float*)net->d_pack32, net->d_pack_off, net->d_bond, tv, xs, npts, out, ctx->flags(), (int)maxb);
     }
     return qtt_hip_check(ctx, hipGetLastError(), "eval_chain kernel");
   }
@@ -792,7 +792,7 @@ int launch_eval(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t npts, doub
   int bs = 256;
   int64_t nb = (nthr + bs - 1) / bs;
   eval_tree_kernel<<<dim3((unsigned)nb), bs, 0, ctx->stream>>>(net->d_data, net->d_off, net->d_bond, tv, d_voff, xs, npts, out,
-                                                               ctx->d_flag, scr, nthr);
+                                                               ctx->flags(), scr, nthr);
   rc = qtt_hip_check(ctx, hipGetLastError(), "eval_tree_kernel");
   if (rc) return rc;
   return qtt_hip_check(ctx, hipStreamSynchronize(ctx->stream), "eval_tree sync");
@@ -851,10 +851,10 @@ int launch_eval_batch(qtt_ctx* ctx, int n, qtt_net* const* nets, const double* c
     if (!rc) rc = qtt_hip_check(ctx, hipMemcpyAsync(d_map, item_of[c].data(), item_of[c].size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream), "batch map");
     if (rc) return rc;
     const dim3 grid((unsigned)item_of[c].size());
-    if (c == 0) eval_chain_batch_kernel<4><<<grid, 256, 0, ctx->stream>>>(d_items, d_map, ctx->d_flag);
-    else if (c == 1) eval_chain_batch_kernel<8><<<grid, 256, 0, ctx->stream>>>(d_items, d_map, ctx->d_flag);
-    else if (c == 2) eval_chain_batch_kernel<16><<<grid, 256, 0, ctx->stream>>>(d_items, d_map, ctx->d_flag);
-    else eval_chain_batch_kernel<32><<<grid, 256, 0, ctx->stream>>>(d_items, d_map, ctx->d_flag);
+    if (c == 0) eval_chain_batch_kernel<4><<<grid, 256, 0, ctx->stream>>>(d_items, d_map, ctx->flags());
+    else if (c == 1) eval_chain_batch_kernel<8><<<grid, 256, 0, ctx->stream>>>(d_items, d_map, ctx->flags());
+    else if (c == 2) eval_chain_batch_kernel<16><<<grid, 256, 0, ctx->stream>>>(d_items, d_map, ctx->flags());
+    else eval_chain_batch_kernel<32><<<grid, 256, 0, ctx->stream>>>(d_items, d_map, ctx->flags());
     rc = qtt_hip_check(ctx, hipGetLastError(), "eval_chain_batch_kernel");
     if (rc) return rc;
   }

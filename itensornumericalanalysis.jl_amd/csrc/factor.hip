@@ -221,6 +221,7 @@ struct SweepArgs {
   int nsteps;
   double cutoff;
   long long maxdim;
+  int max_sweeps;  // Jacobi sweeps before an SVD is reported as not converged
   double* gscr;  // global workspace (used when a matrix does not fit LDS)
   double* gY;    // global buffer for the mode-product result
   int ws_lds;    // doubles of LDS workspace
@@ -739,24 +740,59 @@ __device__ inline int rr_player(int pos, int t, int nn) {
   return pos == 0 ? 0 : 1 + x;
 }
 
-// Rows whose squared norm is at most cutoff * ||X||_F^2 / n are not rotated
-// against each other.  Together they weigh at most cutoff * ||X||_F^2, so the
-// truncation that follows the factorization discards all of them: the squared
-// row norms sum to ||X||_F^2 whatever the rotations, every other row ends
-// above this level, and every pair with a row above it is still rotated to
-// convergence.  What is kept is therefore the same.  A cutoff outside (0, 1)
-// rotates every pair.
+// Rows whose squared norm is at most cutoff * ||X||_F^2 / n (the noise level)
+// are not rotated against each other.  At convergence every row above the
+// level is orthogonal to all rows, so it is an exact singular triplet; the
+// rows below it (the noise block) weigh at most cutoff * ||X||_F^2 together
+// but are not yet mutually orthogonal.  noise_block_exact below decides
+// whether that block can change what the truncation keeps; if it can, the
+// Jacobi runs again with the rule off (cutoff 0, rotations continued in J),
+// which rotates only the unconverged pairs of the block.  A cutoff outside
+// (0, 1) rotates every pair.
 __device__ inline double noise_level(double fro2, double cut, int n) {
   return (cut > 0.0 && cut < 1.0) ? cut * fro2 / (double)n : 0.0;
 }
 
+// true if the noise block of the rows with norms sig[0..n) cannot change the
+// truncation: each of its singular values has a square of at most its weight
+// W, so when W is below the smallest squared norm q of the rows above the
+// level, all of them sort after those rows and are all discarded (W <= cut
+// ||X||_F^2), and what is kept and the discarded weight are those of the full
+// SVD.  The level is widened by 1e-9 relative so that a row on the boundary
+// counts as noise.  Block-wide; slot is a broadcast int in shared memory.
+__device__ bool noise_block_exact(const double* sig, int n, double cut, int* slot) {
+  if (threadIdx.x == 0) {
+    double fro = 0.0, W = 0.0, q = -1.0;
+    int ns = 0;
+    for (int i = 0; i < n; ++i) fro += sig[i] * sig[i];
+    const double level = cut * fro / (double)n * (1.0 + 1e-9);
+    for (int i = 0; i < n; ++i) {
+      const double a = sig[i] * sig[i];
+      if (a <= level) {
+        W += a;
+        ++ns;
+      } else if (q < 0.0 || a < q) {
+        q = a;
+      }
+    }
+    *slot = (ns <= 1 || (q >= 0.0 && W <= q * (1.0 - 1e-9))) ? 1 : 0;
+  }
+  __syncthreads();
+  const int r = *slot;
+  __syncthreads();
+  return r != 0;
+}
+
 // One-sided Jacobi on the rows of X (n x b, ld b), rotations accumulated in J
-// (n x n).  Returns false if it did not converge.
+// (n x n; set to the identity first when init_j).  Returns false if it did
+// not converge.
 template <int G>
-__device__ bool jacobi_rows(double* X, int n, int b, double* J, int* flag, double* red, double cut) {
+__device__ bool jacobi_rows(double* X, int n, int b, double* J, int* flag, double* red, double cut, bool init_j,
+                            int max_sweeps) {
   const int tid = threadIdx.x, bs = blockDim.x;
   const int gl = tid & (G - 1), gid = tid / G, ng = bs / G;
-  for (int e = tid; e < n * n; e += bs) J[e] = (e / n == e % n) ? 1.0 : 0.0;
+  if (init_j)
+    for (int e = tid; e < n * n; e += bs) J[e] = (e / n == e % n) ? 1.0 : 0.0;
   __syncthreads();
   if (n < 2) return true;
   const int nn = n + (n & 1);
@@ -768,7 +804,7 @@ __device__ bool jacobi_rows(double* X, int n, int b, double* J, int* flag, doubl
   const double fsum = block_sum(fro, red);
   const double small = fsum * DBL_EPSILON * DBL_EPSILON;
   const double noise = noise_level(fsum, cut, n);
-  for (int sweep = 0; sweep < 100; ++sweep) {
+  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
     if (tid == 0) *flag = 0;
     __syncthreads();
     for (int t = 0; t < nn - 1; ++t) {
@@ -918,11 +954,14 @@ __device__ __forceinline__ void rotate4(double (&u)[JE], double (&w)[JE], double
 // where a row's tail past its end lies inside the padded workspace too.)
 template <int G, int JE = JEPL, typename F = lds_f64>
 __device__ __noinline__ bool jacobi_rows_reg(F* X, int ldx_in, int n_in, int b_in, F* J, int ldj_in,
-                                             lds_i32* flag, double* red, double cut) {
-  const int ldx = uni(ldx_in), n = uni(n_in), b = uni(b_in), ldj = uni(ldj_in);
+                                             lds_i32* flag, double* red, double cut, int init_j_in,
+                                             int max_sweeps_in) {
+  const int ldx = uni(ldx_in), n = uni(n_in), b = uni(b_in), ldj = uni(ldj_in), init_j = uni(init_j_in);
+  const int max_sweeps = uni(max_sweeps_in);
   const int tid = threadIdx.x, bs = blockDim.x;
   const int gid = tid / G, ng = bs / G;
-  for (int e = tid; e < n * n; e += bs) J[(e / n) * ldj + e % n] = (e / n == e % n) ? 1.0 : 0.0;
+  if (init_j)
+    for (int e = tid; e < n * n; e += bs) J[(e / n) * ldj + e % n] = (e / n == e % n) ? 1.0 : 0.0;
   __syncthreads();
   if (n < 2) return true;
   const int nn = n + (n & 1);
@@ -936,7 +975,7 @@ __device__ __noinline__ bool jacobi_rows_reg(F* X, int ldx_in, int n_in, int b_i
   const double small = fsum * DBL_EPSILON * DBL_EPSILON;
   const double noise = noise_level(fsum, cut, n);
   QTT_JT_INIT;
-  for (int sweep = 0; sweep < 100; ++sweep) {
+  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
     if (tid == 0) *flag = 0;
     __syncthreads();
     QTT_JC(6);
@@ -1924,9 +1963,10 @@ __device__ void coop_helper(CoopJob* job, int wg, int nwg, lds_i32* sflag, lds_f
 // by workgroup 0; same rules as jacobi_rows_reg, round-robin rounds, each
 // round one job).
 __device__ bool jacobi_coop(double* X, int n, int b, double* J, CoopJob* job, int nwg, double* red, int* iflag,
-                            lds_f64* lw, int lw_cap, double cut) {
+                            lds_f64* lw, int lw_cap, double cut, bool init_j, int max_sweeps) {
   const int tid = threadIdx.x, bs = blockDim.x;
-  for (int e2 = tid; e2 < n * n; e2 += bs) J[e2] = (e2 / n == e2 % n) ? 1.0 : 0.0;
+  if (init_j)
+    for (int e2 = tid; e2 < n * n; e2 += bs) J[e2] = (e2 / n == e2 % n) ? 1.0 : 0.0;
   __syncthreads();
   if (n < 2) return true;
   const int nn = n + (n & 1);
@@ -1958,7 +1998,7 @@ __device__ bool jacobi_coop(double* X, int n, int b, double* J, CoopJob* job, in
   const int need = 2 * bsz * ldsum + 8;
   const bool blocked = bsz <= 16 && nblk >= 4 && need <= cap;
   const int rounds = blocked ? nblk - 1 : nn - 1;
-  for (int sweep = 0; sweep < 100; ++sweep) {
+  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
     if (tid == 0) __hip_atomic_store(&job->rot, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (int t = 0; t < rounds; ++t) {
       if (tid == 0) {
@@ -2027,7 +2067,7 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
                                         lds_f64* pnl, const int pnl_cap, const int pmode, const int rows,
                                         const int b, const int tr, const int rpl, const int bx,
                                         const int64_t Au, const int64_t nu, const int64_t Bu, double* Xc,
-                                        const double* Uc, double* Ud, double* gY, double cutoff, long long maxdim,
+                                        const double* Uc, double* Ud, double* gY, double cutoff, long long maxdim, int max_sweeps,
                                         int* status, int* iflag, double* red, double* dbc) {
   const int tid = threadIdx.x, bs = blockDim.x;
   const int k = rows < b ? rows : b;
@@ -2132,57 +2172,73 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
       n = b;
     }
     QTT_PHASE(-1, 2);  // factored (tall) / gathered (wide)
-    const double cut = cutoff;  // pairs of rows below its noise level are left alone
-    // lanes per row pair sized so a row is at most 4 entries per lane
-    bool conv;
-    if (REG && G == 8 && b <= 256) {
-      // (REG: the workspace is in LDS.)  With room for it, the rounds run on a
-      // copy whose row strides are odd numbers of doubles, so the rows that
-      // the groups of a wavefront read together fall in different banks.
-      lds_f64 *Xl = (lds_f64*)X, *Jl = (lds_f64*)Jb;
-      int ldx = b, ldj = n;
-      if (jpad) {
-        ldx = b | 1;
-        ldj = n | 1;
-        Xl = (lds_f64*)jpad;
-        Jl = Xl + n * ldx;
-        for (int e = tid; e < n * b; e += bs) Xl[(e / b) * ldx + e % b] = X[e];
-        __syncthreads();
+    // lanes per row pair sized so a row is at most 4 entries per lane.
+    // cut: pairs of rows below its noise level are left alone; init_j: start
+    // J from the identity (else continue the rotations already in it)
+    auto jacobi = [&](const double cut, const bool init_j) -> bool {
+      bool cv;
+      if (REG && G == 8 && b <= 256) {
+        // (REG: the workspace is in LDS.)  With room for it, the rounds run on a
+        // copy whose row strides are odd numbers of doubles, so the rows that
+        // the groups of a wavefront read together fall in different banks.
+        lds_f64 *Xl = (lds_f64*)X, *Jl = (lds_f64*)Jb;
+        int ldx = b, ldj = n;
+        if (jpad) {
+          ldx = b | 1;
+          ldj = n | 1;
+          Xl = (lds_f64*)jpad;
+          Jl = Xl + n * ldx;
+          for (int e = tid; e < n * b; e += bs) Xl[(e / b) * ldx + e % b] = X[e];
+          if (!init_j)
+            for (int e = tid; e < n * n; e += bs) Jl[(e / n) * ldj + e % n] = Jb[e];
+          __syncthreads();
+        }
+        lds_i32* fl = (lds_i32*)iflag;
+        const int ij = init_j ? 1 : 0;
+        if (b <= 32) cv = jacobi_rows_reg<8>(Xl, ldx, n, b, Jl, ldj, fl, red, cut, ij, max_sweeps);
+        else if (b <= 64) cv = jacobi_rows_reg<16>(Xl, ldx, n, b, Jl, ldj, fl, red, cut, ij, max_sweeps);
+        else if (b <= 128) cv = jacobi_rows_reg<32>(Xl, ldx, n, b, Jl, ldj, fl, red, cut, ij, max_sweeps);
+        else cv = jacobi_rows_reg<64>(Xl, ldx, n, b, Jl, ldj, fl, red, cut, ij, max_sweeps);
+        if (jpad) {
+          __syncthreads();
+          for (int e = tid; e < n * b; e += bs) X[e] = Xl[(e / b) * ldx + e % b];
+          for (int e = tid; e < n * n; e += bs) Jb[e] = Jl[(e / n) * ldj + e % n];
+          __syncthreads();
+        }
+      } else if (gws && job && nwg > 1 && n >= 32 && b <= 512) {
+        // rows in global memory and helper workgroups present: each round's
+        // pairs are spread over all of them
+        cv = jacobi_coop(X, n, b, Jb, job, nwg, red, iflag, pnl, pnl ? pnl_cap : 0, cut, init_j, max_sweeps);
+      } else if (gws && b <= 512) {
+        // workspace in global memory: both rows of a pair (and of J) in
+        // registers, 32 lanes per pair (two pairs per wavefront; 64 for rows
+        // over 384), all loads of a pair in flight at once
+        lds_i32* fl = (lds_i32*)iflag;
+        const int ij = init_j ? 1 : 0;
+        if (b <= 128) cv = jacobi_rows_reg<32, 4, double>(X, b, n, b, Jb, n, fl, red, cut, ij, max_sweeps);
+        else if (b <= 256) cv = jacobi_rows_reg<32, 8, double>(X, b, n, b, Jb, n, fl, red, cut, ij, max_sweeps);
+        else if (b <= 384) cv = jacobi_rows_reg<32, 12, double>(X, b, n, b, Jb, n, fl, red, cut, ij, max_sweeps);
+        else cv = jacobi_rows_reg<64, 8, double>(X, b, n, b, Jb, n, fl, red, cut, ij, max_sweeps);
+      } else {
+        // eight lanes per pair: every group of a wavefront has a pair of its
+        // own, so the rows of 64 pairs are read at once
+        cv = jacobi_rows<8>(X, n, b, Jb, iflag, red, cut, init_j, max_sweeps);
       }
-      lds_i32* fl = (lds_i32*)iflag;
-      if (b <= 32) conv = jacobi_rows_reg<8>(Xl, ldx, n, b, Jl, ldj, fl, red, cut);
-      else if (b <= 64) conv = jacobi_rows_reg<16>(Xl, ldx, n, b, Jl, ldj, fl, red, cut);
-      else if (b <= 128) conv = jacobi_rows_reg<32>(Xl, ldx, n, b, Jl, ldj, fl, red, cut);
-      else conv = jacobi_rows_reg<64>(Xl, ldx, n, b, Jl, ldj, fl, red, cut);
-      if (jpad) {
-        __syncthreads();
-        for (int e = tid; e < n * b; e += bs) X[e] = Xl[(e / b) * ldx + e % b];
-        for (int e = tid; e < n * n; e += bs) Jb[e] = Jl[(e / n) * ldj + e % n];
-        __syncthreads();
-      }
-    } else if (gws && job && nwg > 1 && n >= 32 && b <= 512) {
-      // rows in global memory and helper workgroups present: each round's
-      // pairs are spread over all of them
-      conv = jacobi_coop(X, n, b, Jb, job, nwg, red, iflag, pnl, pnl ? pnl_cap : 0, cut);
-    } else if (gws && b <= 512) {
-      // workspace in global memory: both rows of a pair (and of J) in
-      // registers, 32 lanes per pair (two pairs per wavefront; 64 for rows
-      // over 384), all loads of a pair in flight at once
-      lds_i32* fl = (lds_i32*)iflag;
-      if (b <= 128) conv = jacobi_rows_reg<32, 4, double>(X, b, n, b, Jb, n, fl, red, cut);
-      else if (b <= 256) conv = jacobi_rows_reg<32, 8, double>(X, b, n, b, Jb, n, fl, red, cut);
-      else if (b <= 384) conv = jacobi_rows_reg<32, 12, double>(X, b, n, b, Jb, n, fl, red, cut);
-      else conv = jacobi_rows_reg<64, 8, double>(X, b, n, b, Jb, n, fl, red, cut);
-    } else {
-      // eight lanes per pair: every group of a wavefront has a pair of its
-      // own, so the rows of 64 pairs are read at once
-      conv = jacobi_rows<8>(X, n, b, Jb, iflag, red, cut);
+      return cv;
+    };
+    bool conv = jacobi(cutoff, true);
+    row_norms<G>(X, n, b, sig);
+    __syncthreads();
+    if (conv && cutoff > 0.0 && cutoff < 1.0 && !noise_block_exact(sig, n, cutoff, iflag + 10)) {
+      // the noise block could change what is kept: finish the SVD among its
+      // rows (pairs with a row above the level are converged and skipped)
+      conv = jacobi(0.0, false);
+      row_norms<G>(X, n, b, sig);
+      __syncthreads();
     }
     if (!conv) {
       if (tid == 0) status[1] = 1;
     }
-    row_norms<G>(X, n, b, sig);
-    __syncthreads();
     for (int i = tid; i < n; i += bs) {
       int r = 0;
       const double si = sig[i];
@@ -2348,17 +2404,17 @@ __global__ void __launch_bounds__(512) truncate_kernel(const SweepArgs* __restri
     int m;
     if (need <= a.ws_lds) {
       if (small)
-        m = run_move<8, true>(wsl, false, nullptr, 1, jpad, nullptr, 0, 0, rows, b, tr, rpl, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<8, true>(wsl, false, nullptr, 1, jpad, nullptr, 0, 0, rows, b, tr, rpl, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.max_sweeps, a.status,
                         iflag, red, dbc);
       else
-        m = run_move<64, false>(wsl, false, nullptr, 1, nullptr, nullptr, 0, 0, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<64, false>(wsl, false, nullptr, 1, nullptr, nullptr, 0, 0, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.max_sweeps, a.status,
                          iflag, red, dbc);
     } else {
       if (small)
-        m = run_move<8, false>(a.gscr, true, a.job, a.nwg, nullptr, pnl, pnl_cap, pmode, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<8, false>(a.gscr, true, a.job, a.nwg, nullptr, pnl, pnl_cap, pmode, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.max_sweeps, a.status,
                         iflag, red, dbc);
       else
-        m = run_move<64, false>(a.gscr, true, a.job, a.nwg, nullptr, pnl, pnl_cap, pmode, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.status,
+        m = run_move<64, false>(a.gscr, true, a.job, a.nwg, nullptr, pnl, pnl_cap, pmode, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.max_sweeps, a.status,
                          iflag, red, dbc);
     }
     QTT_PHASE(st, 5);
@@ -2510,6 +2566,7 @@ int launch_truncate_batch(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double c
     sa.nsteps = (int)steps[i].size() / 3;
     sa.cutoff = cutoff;
     sa.maxdim = (long long)maxdim;
+    sa.max_sweeps = ctx->jacobi_max_sweeps;
     sa.gscr = w;
     int64_t maxcap = 1;
     for (int v = 0; v < net->t.nv; v++) maxcap = std::max(maxcap, net->cap[v]);
@@ -2517,7 +2574,7 @@ int launch_truncate_batch(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double c
     w += ws[i];
     sa.ws_lds = lds_doubles - 64 - (net->t.nv + 1) / 2;
     sa.err_out = d_err + i;
-    sa.status = ctx->d_flag;
+    sa.status = ctx->d_flag + 2;  // truncation is synchronous: its own status slots
     sa.job = nwg > 1 ? d_job : nullptr;
     sa.nwg = nwg;
     if (sa.ws_lds < 0) return qtt_fail(ctx, QTT_EINVAL, "truncate: too many vertices");
@@ -2576,19 +2633,19 @@ int launch_truncate_batch(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double c
     if (errs) errs[i] = herr[i];
   }
   int fl[2];
-  rc = qtt_hip_check(ctx, hipMemcpy(fl, ctx->d_flag, sizeof(fl), hipMemcpyDeviceToHost), "flags");
+  rc = qtt_hip_check(ctx, hipMemcpy(fl, ctx->d_flag + 2, sizeof(fl), hipMemcpyDeviceToHost), "flags");
   if (rc) return rc;
   if (coop) {
     CoopJob hj;
     rc = qtt_hip_check(ctx, hipMemcpy(&hj, d_job, sizeof(hj), hipMemcpyDeviceToHost), "coop job");
     if (rc) return rc;
     if (hj.err) {
-      (void)hipMemset(ctx->d_flag, 0, 2 * sizeof(int));
+      (void)hipMemset(ctx->d_flag + 2, 0, 2 * sizeof(int));
       return qtt_fail(ctx, QTT_EHIP, "truncate: a helper workgroup wait exceeded its limit");
     }
   }
   if (fl[1]) {
-    (void)hipMemset(ctx->d_flag, 0, 2 * sizeof(int));
+    (void)hipMemset(ctx->d_flag + 2, 0, 2 * sizeof(int));
     return qtt_fail(ctx, QTT_ENOCONV, "truncate: Jacobi SVD did not converge");
   }
   return QTT_OK;

@@ -489,7 +489,7 @@ int launch_grad(qtt_ctx* ctx, const qtt_net* cnet, const double* xs, const doubl
 #define QTT_GRAD_LAUNCH(C)                                                                                   \
   grad_chain_reg_kernel<C><<<nwg, GT, lds, ctx->stream>>>(net->d_pack, net->d_pack_off, dd, tv.sdim, nv,     \
                                                           tv.ncoord, tv.ndigits, d_goff, d_roff, rtot, xs, ybar, \
-                                                          npts, gsize, Rscr, gpart, ctx->d_flag)
+                                                          npts, gsize, Rscr, gpart, ctx->flags())
       if (chi == 8) QTT_GRAD_LAUNCH(8);
       else if (chi == 16) QTT_GRAD_LAUNCH(16);
       else QTT_GRAD_LAUNCH(32);
@@ -497,7 +497,7 @@ int launch_grad(qtt_ctx* ctx, const qtt_net* cnet, const double* xs, const doubl
     } else {
       grad_chain_kernel<<<nwg, GB, lds, ctx->stream>>>(net->d_pack, net->d_pack_off, net->d_bond, tv, d_goff,
                                                        d_roff, rtot, xs, ybar, npts, gsize, Rscr, gpart,
-                                                       ctx->d_flag, (int)maxb);
+                                                       ctx->flags(), (int)maxb);
     }
     rc = qtt_hip_check(ctx, hipGetLastError(), "grad_chain_kernel");
     if (rc) return rc;

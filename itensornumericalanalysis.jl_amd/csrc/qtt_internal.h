@@ -19,7 +19,18 @@ struct qtt_ctx {
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
   std::string err;
-  int* d_flag = nullptr;  // [0] = invalid coordinate seen, [1] = SVD non-convergence
+  // Status words written by kernels: slots [0] (invalid coordinate) and [1]
+  // (SVD non-convergence) belong to asynchronous calls and are collected by
+  // qtt_sync; slots [2] and [3] hold the same for synchronous entry points,
+  // which read and clear them before returning, so a pending asynchronous
+  // status is never reported by an unrelated synchronous call.
+  int* d_flag = nullptr;
+  int flag_base = 0;  // 0 (asynchronous) or 2 (inside a synchronous entry point)
+  int* flags() const { return d_flag + flag_base; }
+  // Networks and operators keep their context alive: qtt_ctx_destroy only
+  // marks it closed while handles remain, and the last handle destroyed frees it.
+  int refs = 0;
+  bool closed = false;
   double* d_scratch = nullptr;
   size_t scratch_bytes = 0;
   int* d_sync = nullptr;  // counters of the dataflow kernels, zero between launches
@@ -27,6 +38,7 @@ struct qtt_ctx {
   int cus = 256;
   bool force_lds_eval = false;  // QTT_EVAL_KERNEL=lds selects the LDS-vector chain kernel
   int eval_ppl = 0;             // points per lane of the register chain kernel (0: by bond size)
+  int jacobi_max_sweeps = 100;  // Jacobi sweeps before an SVD reports QTT_ENOCONV
   // launch shapes of the register chain kernel picked by qtt_tune_eval, keyed
   // by eval_tune_key (bond class, point-count octave); absent keys use the
   // built-in rule

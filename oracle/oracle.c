@@ -477,15 +477,18 @@ int or_qr(int r, int c, const double* M, double* Q, double* R) {
  * accumulated in J (n x n, starts as identity): J X_in = X_out, rows of X_out
  * mutually orthogonal.  Cyclic row ordering.
  * cut in (0, 1): the relative cutoff of the truncation that follows.  Two rows
- * whose squared norms are both at most cut ||X||_F^2 / n are not rotated
- * against each other: together such rows weigh at most cut ||X||_F^2 (the
- * squared row norms always sum to ||X||_F^2), every other row ends above that
- * level and orthogonal to all rows, so the truncation discards exactly those
- * rows and keeps the same rows it would keep after a full SVD.  Any other cut
- * rotates every pair. */
-static int jacobi_rows(int n, int c, double* X, double* J, double cut) {
-  for (int i = 0; i < n; i++)
-    for (int j = 0; j < n; j++) J[i * n + j] = (i == j) ? 1.0 : 0.0;
+ * whose squared norms are both at most cut ||X||_F^2 / n (the noise level) are
+ * not rotated against each other.  At convergence every row above the level is
+ * orthogonal to all rows, so it is an exact singular triplet; the rows below
+ * it (the noise block) weigh W <= cut ||X||_F^2 together, but are not yet
+ * mutually orthogonal.  svd_factor then checks whether the noise block can
+ * change what the truncation keeps (see noise_block_exact) and, if it can,
+ * calls this again with cut = 0 and init_j = 0 to finish the SVD.  Any other
+ * cut rotates every pair.  init_j = 0 continues from the rotations in J. */
+static int jacobi_rows(int n, int c, double* X, double* J, double cut, int init_j) {
+  if (init_j)
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) J[i * n + j] = (i == j) ? 1.0 : 0.0;
   if (n < 2) return OR_OK;
   /* a computed dot product of length c carries rounding error up to ~c eps,
      so the stopping test must not be tighter than that */
@@ -534,6 +537,40 @@ static int jacobi_rows(int n, int c, double* X, double* J, double cut) {
   return OR_ENOCONV;
 }
 
+/* After jacobi_rows with the noise rule: 1 if the unfinished noise block
+ * cannot change the truncation, i.e. the factorization already keeps what a
+ * full SVD keeps.  Every singular value of the noise block has a square of at
+ * most its weight W; when W is below the smallest squared norm q of the rows
+ * above the level, all of the block's singular values sort after those rows,
+ * and the truncation discards all of them whatever their split (W <= cut
+ * ||X||_F^2), so the kept triplets, their number and the discarded weight are
+ * those of the full SVD.  The level is widened by 1e-9 relative, so a row on
+ * the boundary counts as noise. */
+static int noise_block_exact(int n, int c, const double* X, double cut) {
+  double fro = 0.0, W = 0.0, q = -1.0;
+  int ns = 0;
+  double* a = (double*)malloc(sizeof(double) * (size_t)n);
+  for (int i = 0; i < n; i++) {
+    double s = 0.0;
+    for (int j = 0; j < c; j++) s += X[(long)i * c + j] * X[(long)i * c + j];
+    a[i] = s;
+    fro += s;
+  }
+  const double level = cut * fro / (double)n * (1.0 + 1e-9);
+  for (int i = 0; i < n; i++) {
+    if (a[i] <= level) {
+      W += a[i];
+      ns++;
+    } else if (q < 0.0 || a[i] < q) {
+      q = a[i];
+    }
+  }
+  free(a);
+  if (ns <= 1) return 1; /* a single noise row is orthogonal to all others */
+  if (q < 0.0) return 0;
+  return W <= q * (1.0 - 1e-9);
+}
+
 long or_trunc_rule(const double* p, long n, double cutoff, long maxdim, double* err) {
   double scale = 0.0;
   for (long i = 0; i < n; i++) scale += p[i];
@@ -570,7 +607,11 @@ static int svd_factor(int r, int c, const double* M, double* U, double* Y, doubl
     memcpy(X, M, sizeof(double) * (size_t)r * c);
   }
   double* J = (double*)malloc(sizeof(double) * (size_t)k * k);
-  int rc = jacobi_rows(k, c, X, J, cut);
+  int rc = jacobi_rows(k, c, X, J, cut, 1);
+  /* the noise block could change what is kept: finish the SVD among its rows
+     (pairs with a row above the level are converged and are skipped) */
+  if (rc == OR_OK && cut > 0.0 && cut < 1.0 && !noise_block_exact(k, c, X, cut))
+    rc = jacobi_rows(k, c, X, J, 0.0, 0);
   /* sort rows by norm */
   int* idx = (int*)malloc(sizeof(int) * (size_t)k);
   double* nrm = (double*)malloc(sizeof(double) * (size_t)k);

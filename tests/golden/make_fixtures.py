@@ -1,0 +1,170 @@
+"""Writes the golden fixtures of tests/golden.  Run from the repository root:
+
+    python tests/golden/make_fixtures.py
+
+Expected values come from definitions and closed forms, never from the engine
+or the oracle, so the fixtures pin both:
+
+shift_ops_L6.npz
+    Dense 64 x 64 matrices M[s_in, s_out] (phi = M^T psi on the grid k = 0..63)
+    of the forward / backward shift by n + 1 points, n = 0, 1, with the
+    Dirichlet, Periodic and Neumann boundary rules
+    (src/elementary_operators.jl:32-103; the rules test/test_operators.jl:168-282
+    checks), and of the 1st..4th derivative stencils with Dirichlet boundaries
+    scaled by h^-p (src/elementary_operators.jl:122-176).
+comb_shifts_L6.npz
+    The reference's shift test on named_comb_tree((2, 3)) (test/test_operators.jl:168-282):
+    psi = 1 + 0.5 x + 0.25 x^2 (its poly_itn(s, [1.0, 0.5, 0.25])), given as the
+    cores of an exact bond-3 network on the comb in the engine's vertex order,
+    its values at all 64 grid points, and the shifted values for each boundary,
+    n and direction, from the polynomial itself.
+c1_sine.npz
+    The 1D workload: f = sin(2 pi x) + 0.5 cos(6 pi x) + exp(-x) on a 12-digit
+    chain.  The network's cores (bond 5, from the host builders, checked here
+    against the closed form on the whole grid), 1000 points drawn uniform on
+    [0, 1) (seed 0), f at their grid values, the Dirichlet second derivative
+    there by the 3-point stencil on the closed form, the Riemann-sum integral
+    of f and the telescoping integral of D^2 f, -(f(1 - h) + f(0)) / h.
+"""
+import math
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+
+
+def load_package():
+    sys.path.insert(0, ROOT)
+    import __graft_entry__ as ge
+    return ge._load_package()
+
+
+def shift_matrix(N, delta, boundary):
+    """M[s_in, s_out] of phi(k) = psi(k + delta) on N points."""
+    M = np.zeros((N, N))
+    for k in range(N):
+        j = k + delta
+        if 0 <= j < N:
+            M[j, k] = 1.0
+        elif boundary == "Periodic":
+            M[j % N, k] = 1.0
+        elif boundary == "Neumann":
+            M[k, k] = 1.0
+    return M
+
+
+STENCILS = {1: [0, .5, 0, -.5, 0], 2: [0, 1, -2, 1, 0], 3: [.5, -1, 0, 1, -.5], 4: [1, -4, 6, -4, 1]}
+
+
+def shift_ops(L=6):
+    N = 2 ** L
+    out = {}
+    for boundary in ("Dirichlet", "Periodic", "Neumann"):
+        for n in (0, 1):
+            out[f"forward_{boundary}_n{n}"] = shift_matrix(N, n + 1, boundary)
+            out[f"backward_{boundary}_n{n}"] = shift_matrix(N, -(n + 1), boundary)
+    for p, coefs in STENCILS.items():
+        M = sum(c * shift_matrix(N, sh, "Dirichlet") for c, sh in zip(coefs, [2, 1, 0, -1, -2]) if c)
+        out[f"derivative_{p}"] = M * float(2 ** (p * L))
+    return out
+
+
+def poly_network(q):
+    """1 + 0.5 x + 0.25 x^2 on named_comb_tree((2, 3)) as exact cores.  With
+    x = sum_v d_v w_v (w_v = 2^-digit), a subtree S passes up its moments
+    (1, x_S, x_S^2).  A vertex with digit d and children k combines them:
+    (c + sum_k x_k)^p with c = d w expands by the multinomial theorem, so
+    T[p, d, i_1, ..., i_m] = p! / ((p - sum i)! prod i_k!) c^(p - sum i)
+    (zero when sum i > p), and the root closes with the coefficients."""
+    s = q.continuous_siteinds(q.named_comb_tree((2, 3)))
+    t = q.Topology(s)
+    coef = np.array([1.0, 0.5, 0.25])
+    fact = math.factorial
+    cores = []
+    for v in range(t.nv):
+        w = 2.0 ** -int(t.vdigit[v])
+        m = len(t.children[v])
+        T = np.zeros((3, 2) + (3,) * m)
+        for d in (0, 1):
+            c = d * w
+            for idx in np.ndindex(*((3,) * m)):
+                si = sum(idx)
+                for p in range(3):
+                    if si <= p:
+                        mult = fact(p) / (fact(p - si) * np.prod([fact(i) for i in idx]))
+                        T[(p, d) + idx] = mult * c ** (p - si)
+        if v == 0:
+            T = np.tensordot(coef, T, axes=([0], [0]))[None]
+        cores.append(T)
+    return s, t, cores
+
+
+def comb_shifts(q):
+    s, t, cores = poly_network(q)
+    L = t.nv
+    N = 2 ** L
+    xg = np.arange(N) / N
+    p = lambda x: 1.0 + 0.5 * x + 0.25 * x * x
+    psi = p(xg)
+    out = {"parent": t.parent, "sdim": t.sdim, "vcoord": t.vcoord, "vdigit": t.vdigit,
+           "Lc": t.Lc, "grid": xg, "psi": psi}
+    for v, c in enumerate(cores):
+        out[f"core{v}"] = c
+    for boundary in ("Dirichlet", "Periodic", "Neumann"):
+        for n in (0, 1):
+            for fwd in (True, False):
+                M = shift_matrix(N, (n + 1) if fwd else -(n + 1), boundary)
+                out[f"{'forward' if fwd else 'backward'}_{boundary}_n{n}"] = M.T @ psi
+    return out, cores, t
+
+
+def c1_sine(q, npts=1000, seed=0):
+    from itensornumericalanalysis_jl_amd import workloads
+    w = workloads.sine_1d(npts=npts, seed=seed)
+    L = 12
+    h = 2.0 ** -L
+    fx = workloads.sine_1d_function
+    xg = np.floor(w.xs[:, 0] * 2 ** L) / 2 ** L
+    k = np.rint(xg * 2 ** L).astype(np.int64)
+    grid = np.arange(2 ** L) * h
+    fg = fx(grid)
+    fp = np.concatenate([fg[1:], [0.0]])
+    fm = np.concatenate([[0.0], fg[:-1]])
+    d2 = (fp - 2 * fg + fm) / h ** 2  # Dirichlet: zero outside the grid
+    out = {"xs": w.xs, "f": fx(xg), "d2f": d2[k], "integral_f": float(np.mean(fg)),
+           "integral_d2f": -(fx(1 - h) + fx(0.0)) / h, "parent": w.f.topo.parent,
+           "sdim": w.f.topo.sdim, "vcoord": w.f.topo.vcoord, "vdigit": w.f.topo.vdigit, "Lc": w.f.topo.Lc}
+    for v, c in enumerate(w.f.cores):
+        out[f"core{v}"] = c
+    return out, w
+
+
+def main():
+    q = load_package()
+    sys.path.insert(0, ROOT)
+    from oracle import dense
+
+    ops = shift_ops()
+    np.savez_compressed(os.path.join(HERE, "shift_ops_L6.npz"), **ops)
+
+    cs, cores, t = comb_shifts(q)
+    # the cores reproduce the polynomial on the whole grid
+    T = dense.dense_tensor(t.parent, cores)
+    order = np.argsort(t.vdigit)
+    vals = np.transpose(T, order).reshape(-1)
+    assert np.allclose(vals, cs["psi"], rtol=1e-14), "polynomial cores"
+    np.savez_compressed(os.path.join(HERE, "comb_shifts_L6.npz"), **cs)
+
+    c1, w = c1_sine(q)
+    T = dense.grid_values_1d(w.f.topo.parent, w.f.cores, w.f.topo.vdigit)
+    from itensornumericalanalysis_jl_amd import workloads
+    assert np.allclose(T, workloads.sine_1d_function(np.arange(4096) / 4096), rtol=1e-13, atol=1e-13), "c1 cores"
+    np.savez_compressed(os.path.join(HERE, "c1_sine.npz"), **c1)
+    print("fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()

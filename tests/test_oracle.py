@@ -6,9 +6,7 @@ LAPACK, torch float64 autograd, and the reference's own known answers
 (test/test_indexmaps.jl, test/test_realitensorfunction.jl,
 test/test_integration.jl, test/test_operators.jl).
 """
-import json
 import math
-import os
 
 import numpy as np
 import pytest
@@ -17,8 +15,6 @@ from hypothesis import strategies as st
 
 from oracle import dense
 from helpers import grid_of, oracle_args, oracle_eval, rel_l2
-
-GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
 
 # --------------------------------------------------------------------------- digits
@@ -126,6 +122,58 @@ def test_svd_noise_rows_left_alone(po, rng, shape):
     assert np.allclose(S0, s_ref, rtol=1e-6)
 
 
+def _noise_block_case():
+    """5 x 8 rows with squared norms 0.898, 0.045 and three parallel rows of
+    0.019 (total 1).  With cutoff 0.1 the noise level is 0.1 / 5 = 0.02: the
+    three parallel rows lie below it and are not rotated against each other,
+    yet together they hold a singular value of square 0.057, above the row of
+    0.045.  LAPACK keeps {0.898, 0.057} and discards 0.045."""
+    M = np.zeros((5, 8))
+    M[0, 0] = math.sqrt(0.898)
+    M[1, 1] = math.sqrt(0.045)
+    M[2:5, 2] = math.sqrt(0.019)
+    return M
+
+
+def test_svd_noise_block_finished(po):
+    M = _noise_block_case()
+    cut = 0.1
+    U, S, Vt = po.svd(M, cut)
+    s_ref = np.linalg.svd(M, compute_uv=False)
+    assert np.allclose(S ** 2, s_ref ** 2, atol=1e-15)
+    m, e = po.trunc_rule(S ** 2, cut, 1 << 30)
+    m_ref, e_ref = po.trunc_rule(s_ref ** 2, cut, 1 << 30)
+    assert (m, m_ref) == (2, 2)
+    assert math.isclose(e, 0.045, rel_tol=1e-12) and math.isclose(e_ref, 0.045, rel_tol=1e-12)
+    Ur, Sr, Vr = np.linalg.svd(M, full_matrices=False)
+    assert np.allclose((U[:, :2] * S[:2]) @ Vt[:2], (Ur[:, :2] * Sr[:2]) @ Vr[:2], atol=1e-15)
+
+
+def noise_block_network():
+    """The same matrix on the edge of a two-vertex network without digits:
+    vertex 0 is the identity on a site of dimension 8, vertex 1 carries M^T
+    ([bond 8][site 5]), so the edge's SVD move factors the rows of M."""
+    from types import SimpleNamespace
+    M = _noise_block_case()
+    topo = SimpleNamespace(nv=2, parent=np.array([-1, 0], np.int32), sdim=np.array([8, 5], np.int32),
+                           vcoord=np.array([-1, -1], np.int32), vdigit=np.zeros(2, np.int32), ncoord=0,
+                           Lc=np.zeros(0, np.int32), children=[[1], []])
+    topo.core_shape = lambda v, b: (int(b[v]), int(topo.sdim[v])) + tuple(int(b[c]) for c in topo.children[v])
+    cores = [np.eye(8).reshape(1, 8, 8), M.T.copy()]
+    Ur, Sr, Vr = np.linalg.svd(M, full_matrices=False)
+    dense_ref = ((Ur[:, :2] * Sr[:2]) @ Vr[:2]).T  # f(i0, i1) = M[i1, i0], rank-2 truncated
+    return topo, cores, dense_ref
+
+
+def test_truncate_noise_block_finished(po):
+    topo, cores, dense_ref = noise_block_network()
+    out, err = po.truncate(topo.parent, topo.sdim, cores, 0.1)
+    assert [c.shape[0] for c in out] == [1, 2]
+    assert math.isclose(err, 0.045, rel_tol=1e-12)
+    dense_out = out[0].reshape(8, 2) @ out[1]
+    assert np.allclose(dense_out, dense_ref, atol=1e-15)
+
+
 def test_svd_rank_deficient(po, rng):
     A = rng.standard_normal((20, 3)) @ rng.standard_normal((3, 12))
     U, S, Vt = po.svd(A)
@@ -190,13 +238,15 @@ def test_builders_reference_values(q, po, name, k, a, c):
     assert math.isclose(y, c * func(k * 0.625 + a), rel_tol=1e-12)
 
 
-def test_const_linkdim(q, po):
-    # test/test_realitensorfunction.jl:41-56
-    g = q.named_grid((3, 3))
+def test_const_linkdim(q, po, rng):
+    # test/test_realitensorfunction.jl:41-56 (c = 1.5, linkdim 4, on a tree of
+    # nine vertices here; the reference's 3 x 3 grid has loops)
     s = q.continuous_siteinds(q.named_comb_tree((3, 3)))
     f = q.const_itn(s, c=1.5, linkdim=4)
+    assert f.maxlinkdim() == 4
+    xs = rng.uniform(0, 1, (64, 1))
+    assert np.allclose(oracle_eval(po, f, xs), 1.5, rtol=1e-13)
     assert math.isclose(oracle_eval(po, f, [[0.5]])[0], 1.5, rel_tol=1e-13)
-    assert g.nv() == 9
 
 
 def test_eval_closed_form_chain_2d(q, po, rng):
@@ -471,15 +521,6 @@ def test_gradient_finite_differences(q, po, rng):
             ym, _ = po.evaluate(t.parent, t.sdim, cm, t.vcoord, t.vdigit, t.Lc, xs)
             fd = float(yb @ (yp - ym)) / (2 * h)
             assert math.isclose(grads[v].flat[flat], fd, rel_tol=1e-6, abs_tol=1e-8)
-
-
-def test_reference_known_answers_file():
-    """The golden file lists the reference's known-answer tests used above."""
-    with open(os.path.join(GOLDEN, "reference_known_answers.json")) as fh:
-        cases = json.load(fh)
-    assert {c["source"].split(":")[0] for c in cases} >= {
-        "test/test_indexmaps.jl", "test/test_integration.jl", "test/test_realitensorfunction.jl",
-        "test/test_operators.jl"}
 
 
 def test_integrate_vertices_without_digits(q, po, rng):
