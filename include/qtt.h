@@ -159,6 +159,24 @@ int qtt_integrate_batch(qtt_ctx* ctx, int n, const qtt_net* const* nets, double 
 int qtt_eval_batch(qtt_ctx* ctx, int n, const qtt_net* const* nets, const double* const* xs,
                    const int64_t* npts, double* const* out, int precision);
 
+/* ---- evaluation without allocation (no reference counterpart) -------------
+ * For callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  Same result
+ * as qtt_eval with QTT_F64 on a chain network, but the packed copy of the
+ * cores, its offsets and its site descriptors are staged in a fixed device
+ * array of the library (one per GPU, held until the process ends) instead of
+ * a per-network allocation: the call allocates nothing and copies nothing
+ * from the host.  Limits: at most QTT_NOALLOC_MAX_SITES sites and
+ * QTT_NOALLOC_PACK_DOUBLES doubles of packed cores (each site takes
+ * 2 * round4(left bond) * round4(right bond)); bonds as the LDS chain kernel
+ * allows (<= 160).  Trees, larger networks and other precisions return
+ * QTT_EINVAL.  Every call on a GPU shares that GPU's array: calls that
+ * overlap on one GPU (other streams or contexts) race and must be ordered by
+ * the caller.  Asynchronous, as qtt_eval. */
+#define QTT_NOALLOC_MAX_SITES 1024
+#define QTT_NOALLOC_PACK_DOUBLES (1 << 20)
+int qtt_eval_noalloc(qtt_ctx* ctx, const qtt_net* net, const double* xs, int64_t npts, double* out);
+
 /* ---- autotuning (no reference counterpart) --------------------------------
  * Times every launch shape of the float64 register chain kernel (points per
  * lane 1 or 2, workgroups of 64, 128 or 256) on these device points (xs

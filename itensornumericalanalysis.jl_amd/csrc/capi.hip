@@ -264,6 +264,12 @@ int qtt_ctx_create(int device, qtt_ctx** out) {
     delete c;
     return QTT_EHIP;
   }
+  if (qtt_noalloc_init(c) != QTT_OK) {  // the device address of the fixed scratch, with the device current
+    (void)hipFree(c->d_flag);
+    (void)hipStreamDestroy(c->own_stream);
+    delete c;
+    return QTT_EHIP;
+  }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cus = prop.multiProcessorCount;
   c->stream = c->own_stream;
@@ -542,6 +548,12 @@ int qtt_eval(qtt_ctx* ctx, const qtt_net* net, const double* xs, int64_t npts, d
   if (rc) return rc;
   if (npts == 0) return QTT_OK;
   return launch_eval(ctx, const_cast<qtt_net*>(net), xs, npts, out, precision);
+}
+
+int qtt_eval_noalloc(qtt_ctx* ctx, const qtt_net* net, const double* xs, int64_t npts, double* out) {
+  if (!ctx || !net || (!xs && npts) || (!out && npts) || npts < 0)
+    return qtt_fail(ctx, QTT_EINVAL, "null argument");
+  return launch_eval_noalloc(ctx, net, xs, npts, out);
 }
 
 int qtt_eval_host(qtt_ctx* ctx, const qtt_net* net, const double* xs, int64_t npts, double* out,

@@ -645,18 +645,30 @@ static uint32_t eval_tune_key(int64_t maxb, int64_t npts) {
 
 // The register chain kernel (float64 chains, bonds <= 32) with ppl points per
 // lane (2 only for bonds <= 16) and wg threads per workgroup (64..256).
+static int launch_eval_reg_on(qtt_ctx* ctx, const qtt_net* net, const double* pack, const int64_t* pack_off,
+                              const SiteDesc* dd, const double* xs, int64_t npts, double* out, int64_t maxb, int ppl,
+                              int wg);
+
 static int launch_eval_reg(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t npts, double* out, int64_t maxb,
                            int ppl, int wg) {
+  return launch_eval_reg_on(ctx, net, net->d_pack, net->d_pack_off, (const SiteDesc*)net->d_desc, xs, npts, out,
+                            maxb, ppl, wg);
+}
+
+// The register chain kernels on a packed copy of the chain (pack, pack_off,
+// dd: as ensure_pack lays them out).
+static int launch_eval_reg_on(qtt_ctx* ctx, const qtt_net* net, const double* pack, const int64_t* pack_off,
+                              const SiteDesc* dd, const double* xs, int64_t npts, double* out, int64_t maxb, int ppl,
+                              int wg) {
   const HostTopo& t = net->t;
   TopoView tv = net->view();
   const int bsr = wg;
-  const SiteDesc* dd = (const SiteDesc*)net->d_desc;
   if (ppl == 2 && maxb <= 16) {
     const int64_t nb2 = (npts + 2 * bsr - 1) / (2 * bsr);
     if (nb2 > INT32_MAX) return qtt_fail(ctx, QTT_EINVAL, "evaluate: too many points");
 #define QTT_REG2_LAUNCH(C)                                                                         \
   eval_chain_reg2_kernel<C><<<dim3((unsigned)nb2), bsr, 0, ctx->stream>>>(                         \
-      net->d_pack, net->d_pack_off, dd, t.nv, t.ncoord, tv.ndigits, xs, npts, out, ctx->flags())
+      pack, pack_off, dd, t.nv, t.ncoord, tv.ndigits, xs, npts, out, ctx->flags())
     if (maxb <= 4) QTT_REG2_LAUNCH(4);
     else if (maxb <= 8) QTT_REG2_LAUNCH(8);
     else QTT_REG2_LAUNCH(16);
@@ -667,7 +679,7 @@ static int launch_eval_reg(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t
   if (nbr > INT32_MAX) return qtt_fail(ctx, QTT_EINVAL, "evaluate: too many points");
 #define QTT_REG_LAUNCH(C)                                                                          \
   eval_chain_reg_kernel<C><<<dim3((unsigned)nbr), bsr, 0, ctx->stream>>>(                          \
-      net->d_pack, net->d_pack_off, dd, t.nv, t.ncoord, tv.ndigits, xs, npts, out, ctx->flags())
+      pack, pack_off, dd, t.nv, t.ncoord, tv.ndigits, xs, npts, out, ctx->flags())
   if (maxb <= 4) QTT_REG_LAUNCH(4);
   else if (maxb <= 8) QTT_REG_LAUNCH(8);
   else if (maxb <= 16) QTT_REG_LAUNCH(16);
@@ -860,4 +872,117 @@ int launch_eval_batch(qtt_ctx* ctx, int n, qtt_net* const* nets, const double* c
   }
   // the host tables must outlive their asynchronous copies
   return qtt_hip_check(ctx, hipStreamSynchronize(ctx->stream), "evaluate batch sync");
+}
+
+// ---------------------------------------------------------------------------
+// Evaluation without allocation (qtt_eval_noalloc): the packed chain, its
+// offsets and its site descriptors are staged in `scratch`, a fixed array of
+// the library, instead of the per-network copy that ensure_pack allocates.
+// Layout in doubles: [pack: QTT_NOALLOC_PACK_DOUBLES][offsets: one int64 per
+// site][descriptors: 4 ints per site], QTT_NOALLOC_MAX_SITES sites.  Every
+// call rewrites what it reads, so nothing needs clearing; calls that overlap
+// on one GPU race on the array.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[QTT_NOALLOC_PACK_DOUBLES + 3 * QTT_NOALLOC_MAX_SITES];
+
+namespace {
+
+// Device address of `scratch` on each device (hipGetSymbolAddress, taken once
+// per device with that device current); host code uses only these.
+constexpr int kMaxDevices = 64;
+double* g_scratch_dev[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// Offsets and descriptors of a packed chain, as ensure_pack computes them on
+// the host (one thread: a prefix sum over at most QTT_NOALLOC_MAX_SITES sites).
+__global__ void chain_meta_kernel(const int* __restrict__ bond, TopoView t, int64_t* __restrict__ poff,
+                                  int* __restrict__ desc) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  int64_t o = 0;
+  for (int k = 0; k < t.nv; ++k) {
+    const int chil = bond[k];
+    const int chir = (k == t.nv - 1) ? 1 : bond[k + 1];
+    const int64_t lpad = (chil + 3) & ~3, rpad = (chir + 3) & ~3;
+    poff[k] = o;
+    o += rpad * lpad * 2;
+    const int c = t.vcoord[k];
+    desc[4 * k + 0] = chil;
+    desc[4 * k + 1] = chir;
+    desc[4 * k + 2] = c;
+    desc[4 * k + 3] = c >= 0 ? t.ndigits[c] - t.vdigit[k] : 0;
+  }
+}
+
+}  // namespace
+
+int qtt_noalloc_init(qtt_ctx* ctx) {
+  if (ctx->device < 0 || ctx->device >= kMaxDevices)
+    return qtt_fail(ctx, QTT_EINVAL, "no-allocation scratch: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[ctx->device]) return QTT_OK;
+  void* p = nullptr;
+  int rc = qtt_hip_check(ctx, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+  if (rc) return rc;
+  g_scratch_dev[ctx->device] = (double*)p;
+  return QTT_OK;
+}
+
+int launch_eval_noalloc(qtt_ctx* ctx, const qtt_net* net, const double* xs, int64_t npts, double* out) {
+  const HostTopo& t = net->t;
+  if (!t.chain || t.ncoord > QTT_MAX_COORD)
+    return qtt_fail(ctx, QTT_EINVAL, "evaluate (no allocation): chains only");
+  if (t.nv > QTT_NOALLOC_MAX_SITES)
+    return qtt_fail(ctx, QTT_EINVAL, "evaluate (no allocation): more sites than QTT_NOALLOC_MAX_SITES");
+  int64_t maxb = 1, elems = 0;
+  for (int k = 0; k < t.nv; k++) {
+    maxb = std::max(maxb, net->bond[k]);
+    const int64_t chil = net->bond[k], chir = (k == t.nv - 1) ? 1 : net->bond[k + 1];
+    elems += ((chir + 3) & ~int64_t(3)) * ((chil + 3) & ~int64_t(3)) * 2;
+  }
+  if (elems > QTT_NOALLOC_PACK_DOUBLES)
+    return qtt_fail(ctx, QTT_EINVAL, "evaluate (no allocation): packed cores exceed QTT_NOALLOC_PACK_DOUBLES");
+  double* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (ctx->device >= 0 && ctx->device < kMaxDevices) base = g_scratch_dev[ctx->device];
+  }
+  if (!base) return qtt_fail(ctx, QTT_EINVAL, "evaluate (no allocation): scratch not initialised");
+  if (npts == 0) return QTT_OK;
+  double* pack = base;
+  int64_t* poff = (int64_t*)(base + QTT_NOALLOC_PACK_DOUBLES);
+  int* desc = (int*)(base + QTT_NOALLOC_PACK_DOUBLES + QTT_NOALLOC_MAX_SITES);
+  const TopoView tv = net->view();
+  chain_meta_kernel<<<1, 64, 0, ctx->stream>>>(net->d_bond, tv, poff, desc);
+  int rc = qtt_hip_check(ctx, hipGetLastError(), "chain_meta_kernel");
+  if (rc) return rc;
+  pack_chain_kernel<<<dim3(64, t.nv), 256, 0, ctx->stream>>>(net->d_data, net->d_off, net->d_bond, tv.sdim, poff,
+                                                             t.nv, pack);
+  rc = qtt_hip_check(ctx, hipGetLastError(), "pack_chain_kernel");
+  if (rc) return rc;
+  if (maxb <= 32) {
+    int ppl = maxb <= 16 && npts >= (int64_t(1) << 19) ? 2 : 1, wg = 256;
+    const auto it = ctx->eval_tuned.find(eval_tune_key(maxb, npts));
+    if (it != ctx->eval_tuned.end()) {
+      ppl = it->second.ppl;
+      wg = it->second.wg;
+    }
+    return launch_eval_reg_on(ctx, net, pack, poff, (const SiteDesc*)desc, xs, npts, out, maxb, ppl, wg);
+  }
+  const size_t per_lane = 2 * size_t(maxb) * sizeof(double);
+  int bs = 256;
+  while (bs > 64 && per_lane * bs > 160 * 1024) bs /= 2;
+  if (per_lane * bs > 160 * 1024)
+    return qtt_fail(ctx, QTT_EINVAL, "evaluate (no allocation): bond dimension too large for the chain kernel");
+  const int64_t nb = (npts + bs - 1) / bs;
+  if (nb > INT32_MAX) return qtt_fail(ctx, QTT_EINVAL, "evaluate: too many points");
+  const size_t lds = per_lane * bs;
+  if (lds > 64 * 1024) {
+    rc = qtt_hip_check(ctx, hipFuncSetAttribute((const void*)eval_chain_f64_kernel,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                       "evaluate LDS attribute");
+    if (rc) return rc;
+  }
+  eval_chain_f64_kernel<<<dim3((unsigned)nb), bs, lds, ctx->stream>>>(pack, poff, net->d_bond, tv, xs, npts, out,
+                                                                      ctx->flags(), (int)maxb);
+  return qtt_hip_check(ctx, hipGetLastError(), "eval_chain_f64_kernel");
 }

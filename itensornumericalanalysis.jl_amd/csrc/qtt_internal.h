@@ -5,6 +5,7 @@
 #include <cstdint>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -128,6 +129,9 @@ int launch_eval(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t npts, doub
 int tune_eval(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t npts, double* out, int* cfg, double* ms);
 int launch_eval_batch(qtt_ctx* ctx, int n, qtt_net* const* nets, const double* const* xs, const int64_t* npts,
                       double* const* out, int precision);
+// the library's fixed evaluate scratch: its device address, once per device (eval.hip)
+int qtt_noalloc_init(qtt_ctx* ctx);
+int launch_eval_noalloc(qtt_ctx* ctx, const qtt_net* net, const double* xs, int64_t npts, double* out);
 int launch_integrate(qtt_ctx* ctx, const qtt_net* net, double weight, double* out_host);
 int launch_inner(qtt_ctx* ctx, const qtt_net* a, const qtt_net* b, double* out_host);
 int launch_integrate_batch(qtt_ctx* ctx, int n, const qtt_net* const* nets, double weight, double* out_host);
