@@ -178,13 +178,15 @@ int qtt_eval_batch(qtt_ctx* ctx, int n, const qtt_net* const* nets, const double
 int qtt_eval_noalloc(qtt_ctx* ctx, const qtt_net* net, const double* xs, int64_t npts, double* out);
 
 /* ---- autotuning (no reference counterpart) --------------------------------
- * Times every launch shape of the float64 register chain kernel (points per
- * lane 1 or 2, workgroups of 64, 128 or 256) on these device points (xs
- * [npts][ncoord], out [npts], both overwritten by the runs) and keeps the
- * fastest in the context for networks of the same bond class and point counts
- * of the same octave; later qtt_eval calls use it.  cfg[2] receives {points
- * per lane, workgroup size} ({0, 0} if the network does not take that kernel:
- * not a chain, bonds over 32), ms the time of one launch.  Synchronous. */
+ * Times every launch shape of the float64 register chain evaluation on these
+ * device points (xs [npts][ncoord]; out [npts] is overwritten by the runs):
+ * digit sites per environment table (0 = no tables, else 8, 10, ... up to 20
+ * and half of the chain's digit sites), points per lane 1 or 2, workgroups of
+ * 64, 128 or 256 threads.  The fastest is kept in the context for networks of
+ * the same bond class and point counts of the same octave; later qtt_eval
+ * calls use it.  cfg[3] receives {points per lane, workgroup size, table
+ * digit sites} ({0, 0, 0} if the network does not take that kernel: not a
+ * chain, bonds over 32), ms the time of one call.  Synchronous. */
 int qtt_tune_eval(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t npts, double* out, int* cfg,
                   double* ms);
 

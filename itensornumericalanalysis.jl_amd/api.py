@@ -142,8 +142,9 @@ def tune_evaluate(f: ITensorNetworkFunction, xs, out=None):
     """Time the float64 register chain kernel's launch shapes on these device
     points (a contiguous float64 CUDA tensor [npts, ndims]) and keep the fastest
     in the engine context for later evaluate calls of the same bond class and
-    point-count octave.  Returns {"points_per_lane", "workgroup", "ms"}
-    (zeros when the network does not take that kernel)."""
+    point-count octave.  Returns {"points_per_lane", "workgroup",
+    "table_digits", "ms"} (zeros when the network does not take that kernel);
+    table_digits is the number of digit sites under each environment table."""
     import torch
     dn = device_network(f)
     ctx = dn.ctx
@@ -152,12 +153,12 @@ def tune_evaluate(f: ITensorNetworkFunction, xs, out=None):
         raise ValueError("device points must be a contiguous float64 CUDA tensor [npts, ndims]")
     if out is None:
         out = torch.empty(xs.shape[0], dtype=torch.float64, device=xs.device)
-    cfg = (C.c_int * 2)()
+    cfg = (C.c_int * 3)()
     ms = C.c_double(0.0)
     with _torch_stream(ctx, xs):
         ctx.check(_capi.lib().qtt_tune_eval(ctx.handle, dn.handle, C.c_void_p(xs.data_ptr()), xs.shape[0],
                                             C.c_void_p(out.data_ptr()), cfg, C.byref(ms)))
-    return {"points_per_lane": cfg[0], "workgroup": cfg[1], "ms": ms.value}
+    return {"points_per_lane": cfg[0], "workgroup": cfg[1], "table_digits": cfg[2], "ms": ms.value}
 
 
 def _sigma(f: ITensorNetworkFunction, take_sum: bool):

@@ -243,7 +243,7 @@ std::vector<int> truncation_schedule(const HostTopo& t) {
 // ---------------------------------------------------------------------------
 extern "C" {
 
-int qtt_version(void) { return 100; }
+int qtt_version(void) { return 200; }
 
 int qtt_ctx_create(int device, qtt_ctx** out) {
   if (!out) return QTT_EINVAL;
@@ -277,6 +277,9 @@ int qtt_ctx_create(int device, qtt_ctx** out) {
   c->force_lds_eval = ek && std::strcmp(ek, "lds") == 0;
   const char* ppl = std::getenv("QTT_EVAL_PPL");  // 1 or 2 (default: chosen by bond size)
   c->eval_ppl = ppl ? std::atoi(ppl) : 0;
+  // QTT_EVAL_TABLE_BITS: digit sites per environment table of the chain evaluate
+  const char* tbits = std::getenv("QTT_EVAL_TABLE_BITS");
+  if (tbits) c->eval_tbits = std::max(0, std::min(24, std::atoi(tbits)));
   // QTT_JACOBI_MAX_SWEEPS: sweeps before an SVD reports QTT_ENOCONV (default 100)
   const char* js = std::getenv("QTT_JACOBI_MAX_SWEEPS");
   if (js) c->jacobi_max_sweeps = std::min(1000, std::max(1, std::atoi(js)));

@@ -557,6 +557,299 @@ eval_tree_kernel(const double* __restrict__ data, const int64_t* __restrict__ of
   }
 }
 
+// ---------------------------------------------------------------------------
+// Chain evaluation with environment tables.
+//
+// f(x) = e^T T_0[d_0] ... T_{n-1}[d_{n-1}] e.  The product of the first ml
+// sites depends only on their digits, so for a batch of many points every one
+// of its 2^bl values (bl = digit sites among them) is computed once, as a
+// table L[p] of row vectors; likewise the last mr sites give a table R[s] of
+// column vectors.  A point then gathers L[p(x)] and R[s(x)] and runs only the
+// middle sites ml .. n-mr-1 on the register chain body: f = L[p] M R[s].
+// p takes the prefix digits in site order (first digit = most significant
+// bit), s the suffix digits from the last site backwards.
+// ---------------------------------------------------------------------------
+
+// One table: the sites first, first + step, ... (count sites; step +1 for L,
+// -1 for R), nbits of them with a digit.  Workgroup b fixes the first
+// nbits - kb digits to the bits of b and expands the last kb in LDS, so it
+// writes the 2^kb consecutive rows (b << kb) ... of the table.
+struct EnvTableArgs {
+  double* tab;   // [2^nbits][w]
+  int first, count, step;
+  int nbits, kb;
+  int w;         // row width: the end bond rounded up to 4 (zero padded)
+  int maxw;      // LDS row stride: the largest bond the build passes through
+  int nblk;      // workgroups of this table (2^(nbits - kb))
+};
+
+// What one step of a table build reads of site k: both digit slices when the
+// step expands a digit, else the slice d ([chil][ns][chir] staged).
+struct EnvStep {
+  const double* T;  // the core, [chil][sd][chir]
+  int chil, chir, sd, d, expand;
+};
+
+__device__ __forceinline__ EnvStep env_step(const double* __restrict__ data, const int64_t* __restrict__ off,
+                                            const int* __restrict__ bond, const TopoView& t, const EnvTableArgs& a,
+                                            int i, int blk, int& seen) {
+  const int k = a.first + i * a.step;
+  EnvStep s;
+  s.T = data + off[k];
+  s.chil = bond[k];
+  s.chir = (k == t.nv - 1) ? 1 : bond[k + 1];
+  s.sd = t.sdim[k];
+  s.d = 0;
+  s.expand = 0;
+  if (t.vcoord[k] >= 0) {
+    const int top = a.nbits - a.kb;
+    if (seen < top) s.d = (blk >> (top - 1 - seen)) & 1;
+    else s.expand = 1;
+    ++seen;
+  }
+  return s;
+}
+
+constexpr int kEnvPre = 8;  // staged core entries per thread: 2 * 32 * 32 / 256
+
+// the entries of step s this thread stages: e = tid + bs * q, as [l][slice][o]
+__device__ __forceinline__ void env_fetch(const EnvStep& s, int tid, int bs, double (&pre)[kEnvPre]) {
+  const int ns = s.expand ? 2 : 1;
+  const int n = s.chil * ns * s.chir;
+#pragma unroll
+  for (int q = 0; q < kEnvPre; ++q) {
+    const int e = tid + bs * q;
+    if (e < n) {
+      const int l = e / (ns * s.chir), rest = e - l * ns * s.chir;
+      const int sl = rest / s.chir, o = rest - sl * s.chir;
+      pre[q] = s.T[((int64_t)l * s.sd + (s.expand ? sl : s.d)) * s.chir + o];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+env_table_kernel(const double* __restrict__ data, const int64_t* __restrict__ off, const int* __restrict__ bond,
+                 TopoView t, EnvTableArgs ta, EnvTableArgs tb) {
+  extern __shared__ double lds[];
+  const bool isb = (int)blockIdx.x >= ta.nblk;
+  const EnvTableArgs& a = isb ? tb : ta;
+  const int blk = isb ? (int)blockIdx.x - ta.nblk : (int)blockIdx.x;
+  const int tid = threadIdx.x, bs = blockDim.x;
+  const bool left = a.step > 0;
+  double* A = lds;
+  double* B = lds + ((size_t)1 << a.kb) * a.maxw;
+  double* Ts = B + ((size_t)1 << a.kb) * a.maxw;  // the step's core slices, [l][slice][o]
+  if (tid == 0) A[0] = 1.0;  // the chain's end bonds are 1
+  int cnt = 1, len = 1, seen = 0;
+  // the first site's slices are fetched before the loop; every step then
+  // fetches the next site's into registers while it computes
+  double pre[kEnvPre];
+  EnvStep cur;
+  if (a.count > 0) {
+    cur = env_step(data, off, bond, t, a, 0, blk, seen);
+    env_fetch(cur, tid, bs, pre);
+  }
+  for (int i = 0; i < a.count; ++i) {
+    const int ns = cur.expand ? 2 : 1;
+    const int n = cur.chil * ns * cur.chir;
+#pragma unroll
+    for (int q = 0; q < kEnvPre; ++q)
+      if (tid + bs * q < n) Ts[tid + bs * q] = pre[q];
+    __syncthreads();
+    EnvStep nxt = cur;
+    if (i + 1 < a.count) {
+      nxt = env_step(data, off, bond, t, a, i + 1, blk, seen);
+      env_fetch(nxt, tid, bs, pre);
+    }
+    const int ncnt = cur.expand ? 2 * cnt : cnt;
+    const int olen = left ? cur.chir : cur.chil;
+    const int chir = cur.chir;
+    for (int e = tid; e < ncnt * olen; e += bs) {
+      const int j = e / olen, o = e - j * olen;
+      const int src = cur.expand ? (j >> 1) : j;
+      const int sl = cur.expand ? (j & 1) : 0;
+      const double* x = A + (size_t)src * a.maxw;
+      double s = 0.0;
+      if (left) {
+        for (int l = 0; l < len; ++l) s = __builtin_fma(x[l], Ts[(l * ns + sl) * chir + o], s);
+      } else {
+        const double* row = Ts + (o * ns + sl) * chir;
+        for (int r = 0; r < len; ++r) s = __builtin_fma(row[r], x[r], s);
+      }
+      B[(size_t)j * a.maxw + o] = s;
+    }
+    __syncthreads();
+    double* tmp = A;
+    A = B;
+    B = tmp;
+    cnt = ncnt;
+    len = olen;
+    cur = nxt;
+  }
+  double* out = a.tab + ((size_t)blk << a.kb) * a.w;
+  for (int e = tid; e < cnt * a.w; e += bs) {
+    const int j = e / a.w, o = e - j * a.w;
+    out[(size_t)j * a.w + o] = o < len ? A[(size_t)j * a.maxw + o] : 0.0;
+  }
+}
+
+// What a point needs of the tables.
+struct EnvView {
+  const double* L;  // [2^bl][lw]
+  const double* R;  // [2^br][rw]
+  int lw, rw;
+  int ml, mr;       // sites covered by L (0 .. ml-1) and by R (n-mr .. n-1)
+};
+
+__device__ __forceinline__ uint64_t word_of(const uint64_t (&kw)[QTT_MAX_COORD], int c) {
+  return c == 0 ? kw[0] : c == 1 ? kw[1] : c == 2 ? kw[2] : kw[3];
+}
+
+// table indices of a point from its digit words
+__device__ __forceinline__ void env_index(const SiteDesc* __restrict__ desc, int nv, const EnvView& ev,
+                                          const uint64_t (&kw)[QTT_MAX_COORD], uint32_t& pi, uint32_t& si) {
+  pi = 0;
+  si = 0;
+  for (int k = 0; k < ev.ml; ++k) {
+    const SiteDesc sd = desc[k];
+    if (sd.coord >= 0) pi = (pi << 1) | (uint32_t)((word_of(kw, sd.coord) >> sd.shift) & 1u);
+  }
+  for (int k = nv - 1; k >= nv - ev.mr; --k) {
+    const SiteDesc sd = desc[k];
+    if (sd.coord >= 0) si = (si << 1) | (uint32_t)((word_of(kw, sd.coord) >> sd.shift) & 1u);
+  }
+}
+
+template <int CHI>
+__device__ __forceinline__ void load_env(const double* __restrict__ row, int w, double (&v)[CHI]) {
+#pragma unroll
+  for (int i = 0; i < CHI; i += 2) {
+    if (i < w) {
+      const double2 q = *(const double2*)(row + i);
+      v[i] = q.x;
+      v[i + 1] = q.y;
+    } else {
+      v[i] = 0.0;
+      v[i + 1] = 0.0;
+    }
+  }
+}
+
+template <int CHI>
+__device__ __forceinline__ double dot_env(const double* __restrict__ row, int w, const double (&v)[CHI]) {
+  double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+  for (int i = 0; i < CHI; i += 2) {
+    if (i < w) {
+      const double2 q = *(const double2*)(row + i);
+      s0 = __builtin_fma(q.x, v[i], s0);
+      s1 = __builtin_fma(q.y, v[i + 1], s1);
+    }
+  }
+  return s0 + s1;
+}
+
+// One point per lane: the middle sites two at a time (v -> w, w -> v).
+template <int CHI>
+__global__ void __launch_bounds__(256)
+eval_chain_env_kernel(const double* __restrict__ pack, const int64_t* __restrict__ poff,
+                      const SiteDesc* __restrict__ desc, int nv, int nc, const int* __restrict__ ndigits, EnvView ev,
+                      const double* __restrict__ xs, int64_t npts, double* __restrict__ out, int* __restrict__ flag) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = p < npts;
+  bool bad = false;
+  uint64_t kw[QTT_MAX_COORD] = {0, 0, 0, 0};
+#pragma unroll
+  for (int c = 0; c < QTT_MAX_COORD; ++c)
+    if (c < nc) kw[c] = digit_word(active ? xs[p * nc + c] : 0.0, ndigits[c], bad);
+  uint32_t pi, si;
+  env_index(desc, nv, ev, kw, pi, si);
+  double v[CHI], w[CHI];
+  load_env<CHI>(ev.R + (size_t)si * ev.rw, ev.rw, v);
+  auto mask = [&](const SiteDesc& sd) -> uint64_t {
+    if (sd.coord < 0) return 0;
+    return 0 - ((word_of(kw, sd.coord) >> sd.shift) & 1u);
+  };
+  int k = nv - ev.mr - 1;
+  for (; k >= ev.ml + 1; k -= 2) {
+    const SiteDesc sa = desc[k];
+    chain1_site<CHI>(pack + poff[k], sa, mask(sa), v, w);
+    const SiteDesc sb = desc[k - 1];
+    chain1_site<CHI>(pack + poff[k - 1], sb, mask(sb), w, v);
+  }
+  double r;
+  if (k == ev.ml) {  // an odd number of middle sites: the last one lands in w
+    const SiteDesc sa = desc[k];
+    chain1_site<CHI>(pack + poff[k], sa, mask(sa), v, w);
+    r = dot_env<CHI>(ev.L + (size_t)pi * ev.lw, ev.lw, w);
+  } else {
+    r = dot_env<CHI>(ev.L + (size_t)pi * ev.lw, ev.lw, v);
+  }
+  if (active) out[p] = bad ? __builtin_nan("") : r;
+  if (bad) flag[0] = 1;
+}
+
+// Two points per lane (p and p + blockDim.x of the block's span), as
+// eval_chain_reg2_kernel: every core entry loaded feeds four multiply-adds.
+template <int CHI>
+__global__ void __launch_bounds__(256)
+eval_chain_env2_kernel(const double* __restrict__ pack, const int64_t* __restrict__ poff,
+                       const SiteDesc* __restrict__ desc, int nv, int nc, const int* __restrict__ ndigits,
+                       EnvView ev, const double* __restrict__ xs, int64_t npts, double* __restrict__ out,
+                       int* __restrict__ flag) {
+  const int64_t p0 = (int64_t)blockIdx.x * (2 * blockDim.x) + threadIdx.x;
+  const int64_t p1 = p0 + blockDim.x;
+  const bool act0 = p0 < npts, act1 = p1 < npts;
+  bool bad0 = false, bad1 = false;
+  uint64_t kw0[QTT_MAX_COORD] = {0, 0, 0, 0}, kw1[QTT_MAX_COORD] = {0, 0, 0, 0};
+#pragma unroll
+  for (int c = 0; c < QTT_MAX_COORD; ++c) {
+    if (c < nc) {
+      kw0[c] = digit_word(act0 ? xs[p0 * nc + c] : 0.0, ndigits[c], bad0);
+      kw1[c] = digit_word(act1 ? xs[p1 * nc + c] : 0.0, ndigits[c], bad1);
+    }
+  }
+  uint32_t pi0, si0, pi1, si1;
+  env_index(desc, nv, ev, kw0, pi0, si0);
+  env_index(desc, nv, ev, kw1, pi1, si1);
+  double v0[CHI], v1[CHI], w0[CHI], w1[CHI];
+  load_env<CHI>(ev.R + (size_t)si0 * ev.rw, ev.rw, v0);
+  load_env<CHI>(ev.R + (size_t)si1 * ev.rw, ev.rw, v1);
+  auto masks = [&](const SiteDesc& sd, uint64_t& m0, uint64_t& m1) {
+    m0 = 0;
+    m1 = 0;
+    if (sd.coord >= 0) {
+      m0 = 0 - ((word_of(kw0, sd.coord) >> sd.shift) & 1u);
+      m1 = 0 - ((word_of(kw1, sd.coord) >> sd.shift) & 1u);
+    }
+  };
+  int k = nv - ev.mr - 1;
+  for (; k >= ev.ml + 1; k -= 2) {
+    uint64_t m0, m1;
+    const SiteDesc sa = desc[k];
+    masks(sa, m0, m1);
+    chain2_site<CHI>(pack + poff[k], sa, m0, m1, v0, v1, w0, w1);
+    const SiteDesc sb = desc[k - 1];
+    masks(sb, m0, m1);
+    chain2_site<CHI>(pack + poff[k - 1], sb, m0, m1, w0, w1, v0, v1);
+  }
+  double r0, r1;
+  if (k == ev.ml) {
+    uint64_t m0, m1;
+    const SiteDesc sa = desc[k];
+    masks(sa, m0, m1);
+    chain2_site<CHI>(pack + poff[k], sa, m0, m1, v0, v1, w0, w1);
+    r0 = dot_env<CHI>(ev.L + (size_t)pi0 * ev.lw, ev.lw, w0);
+    r1 = dot_env<CHI>(ev.L + (size_t)pi1 * ev.lw, ev.lw, w1);
+  } else {
+    r0 = dot_env<CHI>(ev.L + (size_t)pi0 * ev.lw, ev.lw, v0);
+    r1 = dot_env<CHI>(ev.L + (size_t)pi1 * ev.lw, ev.lw, v1);
+  }
+  if (act0) out[p0] = bad0 ? __builtin_nan("") : r0;
+  if (act1) out[p1] = bad1 ? __builtin_nan("") : r1;
+  if (bad0 || bad1) flag[0] = 1;
+}
+
 }  // namespace
 
 static int ensure_pack(qtt_ctx* ctx, qtt_net* net) {
@@ -688,13 +981,130 @@ static int launch_eval_reg_on(qtt_ctx* ctx, const qtt_net* net, const double* pa
   return qtt_hip_check(ctx, hipGetLastError(), "eval_chain_reg_kernel");
 }
 
-// Time every launch shape of the register chain kernel on these points (one
-// warm-up and five timed launches each, HIP events on the context's stream) and
-// keep the fastest for this bond class and point-count octave.  cfg receives
-// {points per lane, workgroup size} ({0, 0}: the network does not take the
-// register kernel), ms the time of one launch.
+// ---------------------------------------------------------------------------
+// Environment tables (see env_table_kernel): plan, build, launch.
+// ---------------------------------------------------------------------------
+struct EnvPlan {
+  int ml = 0, mr = 0;    // sites under the left / right table
+  int bl = 0, br = 0;    // digit sites among them (table rows 2^bl, 2^br)
+  int64_t lw = 4, rw = 4;  // row widths: end bonds rounded up to 4
+};
+
+// tb digit sites per table, at most half of the chain's digit sites each
+static EnvPlan env_plan(const qtt_net* net, int tb) {
+  const HostTopo& t = net->t;
+  int D = 0;
+  for (int k = 0; k < t.nv; k++) D += t.vcoord[k] >= 0;
+  tb = std::max(0, std::min(tb, D / 2));
+  EnvPlan p;
+  int seen = 0, k = 0;
+  while (seen < tb) seen += t.vcoord[k++] >= 0;
+  p.ml = k;
+  p.bl = tb;
+  seen = 0;
+  k = t.nv - 1;
+  while (seen < tb) {
+    seen += t.vcoord[k--] >= 0;
+    p.mr++;
+  }
+  p.br = tb;
+  p.lw = (net->bond[p.ml] + 3) & ~int64_t(3);
+  p.rw = ((p.mr ? net->bond[t.nv - p.mr] : 1) + 3) & ~int64_t(3);
+  return p;
+}
+
+// Digit sites per table when nothing is tuned: enough points that a table
+// row is reused about 32 times, at most 16.
+static int default_table_bits(int64_t npts) {
+  if (npts < (int64_t(1) << 15)) return 0;
+  int lg = 0;
+  while ((int64_t(1) << (lg + 1)) <= npts) ++lg;
+  return std::min(16, lg - 5);
+}
+
+static EnvTableArgs env_table_args(const qtt_net* net, const EnvPlan& pl, bool left, double* tab) {
+  const HostTopo& t = net->t;
+  EnvTableArgs e;
+  e.tab = tab;
+  e.first = left ? 0 : t.nv - 1;
+  e.count = left ? pl.ml : pl.mr;
+  e.step = left ? 1 : -1;
+  e.nbits = left ? pl.bl : pl.br;
+  e.w = (int)(left ? pl.lw : pl.rw);
+  int maxw = 1;
+  for (int i = 0; i < e.count; i++) {
+    const int k = e.first + i * e.step;
+    maxw = std::max<int>(maxw, (int)net->bond[k]);
+    maxw = std::max<int>(maxw, k == t.nv - 1 ? 1 : (int)net->bond[k + 1]);
+  }
+  e.maxw = maxw;
+  // rows expanded per workgroup: both row buffers and the staged core slices
+  // (2 maxw^2) within 64 KiB of LDS
+  const size_t stage = 2 * size_t(maxw) * maxw;
+  int kb = 0;
+  while (kb < e.nbits && (2 * (size_t(2) << kb) * maxw + stage) * sizeof(double) <= 64 * 1024) ++kb;
+  e.kb = kb;
+  e.nblk = 1 << (e.nbits - kb);
+  return e;
+}
+
+static size_t env_lds(const EnvTableArgs& e) {
+  return (2 * (size_t(1) << e.kb) * e.maxw + 2 * size_t(e.maxw) * e.maxw) * sizeof(double);
+}
+
+// Build the two tables in scratch and evaluate with them: two launches.
+static int launch_eval_env(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t npts, double* out, int64_t maxb,
+                           int ppl, int wg, int tb) {
+  const HostTopo& t = net->t;
+  const EnvPlan pl = env_plan(net, tb);
+  if (pl.bl == 0) return launch_eval_reg(ctx, net, xs, npts, out, maxb, ppl, wg);
+  const int64_t nL = (int64_t(1) << pl.bl) * pl.lw, nR = (int64_t(1) << pl.br) * pl.rw;
+  double* scr = qtt_scratch(ctx, (size_t)(nL + nR) * sizeof(double));
+  if (!scr) return qtt_fail(ctx, QTT_ENOMEM, "evaluate: environment tables");
+  const EnvTableArgs a = env_table_args(net, pl, true, scr), b = env_table_args(net, pl, false, scr + nL);
+  const TopoView tv = net->view();
+  env_table_kernel<<<a.nblk + b.nblk, 256, std::max(env_lds(a), env_lds(b)), ctx->stream>>>(
+      net->d_data, net->d_off, net->d_bond, tv, a, b);
+  int rc = qtt_hip_check(ctx, hipGetLastError(), "env_table_kernel");
+  if (rc) return rc;
+  EnvView ev{scr, scr + nL, (int)pl.lw, (int)pl.rw, pl.ml, pl.mr};
+  const SiteDesc* dd = (const SiteDesc*)net->d_desc;
+  const int bsr = wg;
+  if (ppl == 2 && maxb <= 16) {
+    const int64_t nb2 = (npts + 2 * bsr - 1) / (2 * bsr);
+    if (nb2 > INT32_MAX) return qtt_fail(ctx, QTT_EINVAL, "evaluate: too many points");
+#define QTT_ENV2_LAUNCH(C)                                                                         \
+  eval_chain_env2_kernel<C><<<dim3((unsigned)nb2), bsr, 0, ctx->stream>>>(                         \
+      net->d_pack, net->d_pack_off, dd, t.nv, t.ncoord, tv.ndigits, ev, xs, npts, out, ctx->flags())
+    if (maxb <= 4) QTT_ENV2_LAUNCH(4);
+    else if (maxb <= 8) QTT_ENV2_LAUNCH(8);
+    else QTT_ENV2_LAUNCH(16);
+#undef QTT_ENV2_LAUNCH
+    return qtt_hip_check(ctx, hipGetLastError(), "eval_chain_env2_kernel");
+  }
+  const int64_t nbr = (npts + bsr - 1) / bsr;
+  if (nbr > INT32_MAX) return qtt_fail(ctx, QTT_EINVAL, "evaluate: too many points");
+#define QTT_ENV_LAUNCH(C)                                                                          \
+  eval_chain_env_kernel<C><<<dim3((unsigned)nbr), bsr, 0, ctx->stream>>>(                          \
+      net->d_pack, net->d_pack_off, dd, t.nv, t.ncoord, tv.ndigits, ev, xs, npts, out, ctx->flags())
+  if (maxb <= 4) QTT_ENV_LAUNCH(4);
+  else if (maxb <= 8) QTT_ENV_LAUNCH(8);
+  else if (maxb <= 16) QTT_ENV_LAUNCH(16);
+  else QTT_ENV_LAUNCH(32);
+#undef QTT_ENV_LAUNCH
+  return qtt_hip_check(ctx, hipGetLastError(), "eval_chain_env_kernel");
+}
+
+// Time every launch shape of the float64 register chain evaluation on these
+// points (one warm-up and five timed launches each, HIP events on the
+// context's stream) and keep the fastest for this bond class and point-count
+// octave.  Shapes: digit sites per environment table (0: none, else 8 .. 20,
+// at most half of the digit sites), points per lane 1 or 2, workgroups of 64,
+// 128 or 256 threads.  cfg receives {points per lane, workgroup size, table
+// digit sites} ({0, 0, 0}: the network does not take the register kernel), ms
+// the time of one call (table build included).
 int tune_eval(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t npts, double* out, int* cfg, double* ms) {
-  cfg[0] = cfg[1] = 0;
+  cfg[0] = cfg[1] = cfg[2] = 0;
   *ms = 0.0;
   const HostTopo& t = net->t;
   int64_t maxb = 1;
@@ -702,33 +1112,41 @@ int tune_eval(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t npts, double
   if (!t.chain || t.ncoord > QTT_MAX_COORD || maxb > 32 || npts <= 0) return QTT_OK;
   int rc = ensure_pack(ctx, net);
   if (rc) return rc;
+  int D = 0;
+  for (int k = 0; k < t.nv; k++) D += t.vcoord[k] >= 0;
+  std::vector<int> tbs{0};
+  for (int tb = 8; tb <= std::min(20, D / 2); tb += 2) tbs.push_back(tb);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   rc = qtt_hip_check(ctx, hipEventCreate(&e0), "tune event");
   if (!rc) rc = qtt_hip_check(ctx, hipEventCreate(&e1), "tune event");
   float best = 0.0f;
-  int bp = 0, bw = 0;
-  for (int ppl = 1; ppl <= (maxb <= 16 ? 2 : 1) && !rc; ++ppl) {
-    for (int wg = 64; wg <= 256 && !rc; wg *= 2) {
-      rc = launch_eval_reg(ctx, net, xs, npts, out, maxb, ppl, wg);
-      if (!rc) rc = qtt_hip_check(ctx, hipEventRecord(e0, ctx->stream), "tune event");
-      for (int r = 0; r < 5 && !rc; ++r) rc = launch_eval_reg(ctx, net, xs, npts, out, maxb, ppl, wg);
-      if (!rc) rc = qtt_hip_check(ctx, hipEventRecord(e1, ctx->stream), "tune event");
-      if (!rc) rc = qtt_hip_check(ctx, hipEventSynchronize(e1), "tune sync");
-      float f = 0.0f;
-      if (!rc) rc = qtt_hip_check(ctx, hipEventElapsedTime(&f, e0, e1), "tune time");
-      if (!rc && (bp == 0 || f < best)) {
-        best = f;
-        bp = ppl;
-        bw = wg;
+  int bp = 0, bw = 0, bt = 0;
+  for (int tb : tbs) {
+    for (int ppl = 1; ppl <= (maxb <= 16 ? 2 : 1) && !rc; ++ppl) {
+      for (int wg = 64; wg <= 256 && !rc; wg *= 2) {
+        rc = launch_eval_env(ctx, net, xs, npts, out, maxb, ppl, wg, tb);
+        if (!rc) rc = qtt_hip_check(ctx, hipEventRecord(e0, ctx->stream), "tune event");
+        for (int r = 0; r < 5 && !rc; ++r) rc = launch_eval_env(ctx, net, xs, npts, out, maxb, ppl, wg, tb);
+        if (!rc) rc = qtt_hip_check(ctx, hipEventRecord(e1, ctx->stream), "tune event");
+        if (!rc) rc = qtt_hip_check(ctx, hipEventSynchronize(e1), "tune sync");
+        float f = 0.0f;
+        if (!rc) rc = qtt_hip_check(ctx, hipEventElapsedTime(&f, e0, e1), "tune time");
+        if (!rc && (bp == 0 || f < best)) {
+          best = f;
+          bp = ppl;
+          bw = wg;
+          bt = tb;
+        }
       }
     }
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   if (rc) return rc;
-  ctx->eval_tuned[eval_tune_key(maxb, npts)] = {bp, bw};
+  ctx->eval_tuned[eval_tune_key(maxb, npts)] = {bp, bw, bt};
   cfg[0] = bp;
   cfg[1] = bw;
+  cfg[2] = bt;
   *ms = best / 5.0;
   return QTT_OK;
 }
@@ -747,13 +1165,16 @@ int launch_eval(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t npts, doub
       // count; else two points per lane for bonds <= 16 (register room) when
       // there are enough points to fill the GPU at half the workgroup count
       int ppl = maxb <= 16 && npts >= (int64_t(1) << 19) ? 2 : 1, wg = 256;
+      int tb = default_table_bits(npts);
       const auto it = ctx->eval_tuned.find(eval_tune_key(maxb, npts));
       if (it != ctx->eval_tuned.end()) {
         ppl = it->second.ppl;
         wg = it->second.wg;
+        tb = it->second.tbits;
       }
       if (ctx->eval_ppl) ppl = ctx->eval_ppl;
-      return launch_eval_reg(ctx, net, xs, npts, out, maxb, ppl, wg);
+      if (ctx->eval_tbits >= 0) tb = ctx->eval_tbits;
+      return launch_eval_env(ctx, net, xs, npts, out, maxb, ppl, wg, tb);
     }
     // LDS: two vectors of maxb entries per lane (double, or a float pair)
     const size_t per_lane = 2 * size_t(maxb) * sizeof(double);

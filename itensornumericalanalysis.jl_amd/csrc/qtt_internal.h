@@ -44,8 +44,9 @@ struct qtt_ctx {
   // by eval_tune_key (bond class, point-count octave); absent keys use the
   // built-in rule
   struct EvalShape {
-    int ppl, wg;
+    int ppl, wg, tbits;  // tbits: digit sites per environment table (0: none)
   };
+  int eval_tbits = -1;  // QTT_EVAL_TABLE_BITS overrides the table size (-1: tuned or default)
   std::map<uint32_t, EvalShape> eval_tuned;
 };
 

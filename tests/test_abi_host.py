@@ -33,7 +33,7 @@ def test_library_exports_every_symbol(q):
     lib = ctypes.CDLL(_capi.LIBPATH)
     for name in _declared():
         assert hasattr(lib, name), name
-    assert lib.qtt_version() == 100
+    assert lib.qtt_version() == 200
 
 
 def test_engine_refuses_without_device(q):

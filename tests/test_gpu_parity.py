@@ -73,7 +73,74 @@ def test_eval_many_points(q, po, eng, rng, bond):
     y = q.evaluate(f, xs)
     ref = oracle_eval(po, f, xs)
     assert rel_l2(y, ref) < TOL
-    assert np.array_equal(y[:4096], q.evaluate(f, xs[:4096]))  # same arithmetic as one point per lane
+    # a small call takes the plain chain (no environment tables): same values up to rounding
+    assert np.allclose(y[:4096], q.evaluate(f, xs[:4096]), rtol=1e-13, atol=1e-13 * np.abs(y).max())
+
+
+def _eval_with_tables(q, f, xs, tbits, monkeypatch):
+    """f at xs through a context whose chain evaluate takes environment tables
+    of tbits digit sites (clamped to half of the chain's digits); returns the
+    values and the status of the following qtt_sync."""
+    import ctypes
+    import torch
+    from itensornumericalanalysis_jl_amd import _capi
+    monkeypatch.setenv("QTT_EVAL_TABLE_BITS", str(tbits))
+    ctx = _capi.Context(0)
+    monkeypatch.delenv("QTT_EVAL_TABLE_BITS")
+    try:
+        dn = _capi.DeviceNetwork.upload(ctx, f.topo, f.cores)
+        xd = torch.tensor(xs, device="cuda")
+        out = torch.empty(len(xs), dtype=torch.float64, device="cuda")
+        ctx.check(_capi.lib().qtt_eval(ctx.handle, dn.handle, ctypes.c_void_p(xd.data_ptr()), len(xs),
+                                       ctypes.c_void_p(out.data_ptr()), 0))
+        rc = _capi.lib().qtt_sync(ctx.handle)
+        y = out.cpu().numpy()
+        dn._release()
+        return y, rc
+    finally:
+        ctx.close()
+
+
+@pytest.mark.parametrize("tbits", [1, 3, 8, 64])
+@pytest.mark.parametrize("case", ["2d", "3d_odd", "no_digit"])
+def test_eval_environment_tables(q, po, eng, rng, monkeypatch, tbits, case):
+    # f = L[prefix digits] (middle sites) R[suffix digits]: tables of 1, 3 and 8
+    # digit sites, and as many as allowed (half of the digits each: no middle
+    # site on the even chain, one on the odd one); vertices without a digit
+    # inside the tables and in the middle
+    if case == "2d":
+        s = q.continuous_siteinds(q.named_grid((20, 1)), map_dimension=2)
+        bond = 16
+    elif case == "3d_odd":
+        s = q.continuous_siteinds(q.named_grid((21, 1)), map_dimension=3)
+        bond = 7
+    else:
+        g = q.named_grid((15, 1))
+        s = q.continuous_siteinds(g, [[(j, 1) for j in range(2, 8)], [(j, 1) for j in range(9, 15)]])
+        bond = 5
+    f = q.random_itn(s, link_space=bond, rng=rng, std=1.0 / math.sqrt(bond))
+    nd = f.dimension()
+    xs = rng.uniform(0, 1, (40000, nd))
+    xs[0] = 0.0
+    xs[1] = np.nextafter(1.0, 0.0)
+    xs[2] = 1.0
+    y, rc = _eval_with_tables(q, f, xs, tbits, monkeypatch)
+    assert rc == 0
+    ref = oracle_eval(po, f, xs)
+    assert rel_l2(y, ref) < TOL
+    assert np.max(np.abs(y - ref)) < TOL * np.max(np.abs(ref))
+
+
+def test_eval_environment_tables_invalid_point(q, eng, rng, monkeypatch):
+    s = q.continuous_siteinds(q.named_grid((16, 1)), map_dimension=2)
+    f = q.random_itn(s, link_space=6, rng=rng, std=0.4)
+    xs = rng.uniform(0, 1, (50000, 2))
+    xs[123, 1] = float("nan")
+    xs[456, 0] = -0.5
+    y, rc = _eval_with_tables(q, f, xs, 6, monkeypatch)
+    assert rc == 2  # QTT_ERANGE at the next sync
+    assert np.isnan(y[123]) and np.isnan(y[456])
+    assert np.isfinite(np.delete(y, [123, 456])).all()
 
 
 def test_eval_tree(q, po, eng, rng):
@@ -400,8 +467,10 @@ def test_tune_evaluate(q, po, eng, rng):
     before = q.evaluate(f, xd).cpu().numpy()
     cfg = q.tune_evaluate(f, xd)
     assert cfg["points_per_lane"] in (1, 2) and cfg["workgroup"] in (64, 128, 256) and cfg["ms"] > 0
+    assert cfg["table_digits"] in (0, 8, 10, 12, 14, 16)  # at most half of the 32 digit sites
     after = q.evaluate(f, xd).cpu().numpy()
-    assert np.array_equal(before, after)
+    # another table size changes the rounding order, not the values
+    assert np.allclose(before, after, rtol=1e-13, atol=1e-13 * np.abs(before).max())
     assert rel_l2(after[:5000], oracle_eval(po, f, xs[:5000])) < TOL
     # a tree does not take the register kernel: nothing to tune
     st = q.continuous_siteinds(q.named_comb_tree((2, 4)))

@@ -48,7 +48,9 @@ def test_noalloc_matches_eval_two_calls(q, po, eng, rng):
         xd = torch.tensor(xs, device="cuda")
         ref = q.evaluate(f, xd).cpu().numpy()
         got = _noalloc(q, eng, f, xd)
-        assert np.array_equal(got, ref)  # same kernels on the same packed cores
+        # the same chain kernels; qtt_eval may also take environment tables
+        # for many points, which changes only the rounding order
+        assert np.allclose(got, ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max())
         assert rel_l2(got, oracle_eval(po, f, xs)) < 1e-12
 
 
@@ -65,7 +67,7 @@ def test_noalloc_lds_kernel_and_many_points(q, po, eng, rng):
         xd = torch.tensor(xs, device="cuda")
         ref = q.evaluate(f, xd).cpu().numpy()
         got = _noalloc(q, eng, f, xd)
-        assert np.array_equal(got, ref)
+        assert np.allclose(got, ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max())
         assert rel_l2(got[:4000], oracle_eval(po, f, xs[:4000])) < 1e-12
 
 
