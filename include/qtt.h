@@ -132,15 +132,29 @@ int qtt_truncate(qtt_ctx* ctx, qtt_net* net, double cutoff, int64_t maxdim, doub
 /* ---- adjoint (no reference counterpart: the rrule of evaluate) ------------
  * grad = sum_m ybar[m] d f(x_m) / d cores, written as the concatenation of
  * all cores in vertex order (same sizes as the network's cores).
- * xs, ybar, grad: device pointers.  Asynchronous. */
+ * xs, ybar, grad: device pointers.  Asynchronous.  Any tree in QTT_F64;
+ * chains also in QTT_F32 (float32 environments and per-tile sums, float64
+ * accumulation across tiles; about 1e-6 relative).  QTT_F32_SPLIT: EINVAL. */
 int qtt_eval_grad(qtt_ctx* ctx, const qtt_net* net, const double* xs, const double* ybar,
                   int64_t npts, double* grad, int precision);
 /* Host-buffer variant, synchronous; grads[v] receives core v's gradient. */
 int qtt_eval_grad_host(qtt_ctx* ctx, const qtt_net* net, const double* xs, const double* ybar,
                        int64_t npts, double* const* grads, int precision);
 
-/* number of float64 elements of all cores of a network (for gradient buffers) */
+/* Pullback of the exact operator application (no reference counterpart: the
+ * rrule of operate(op, f) without truncation, src/elementary_operators.jl:294-299).
+ * gbar: the cotangent of the exact product's cores (the network that
+ * qtt_apply(op, net, cutoff < 0) returns), concatenated in vertex order;
+ * fbar receives the cotangent of net's cores, wbar (nullable) that of op's
+ * cores, both concatenated in vertex order.  Device pointers; returns once
+ * the work is done.  The truncation that qtt_apply applies after the product
+ * has no pullback here. */
+int qtt_apply_grad(qtt_ctx* ctx, const qtt_mpo* op, const qtt_net* net, const double* gbar, double* fbar,
+                   double* wbar);
+
+/* number of float64 elements of all cores of a network / an operator */
 int64_t qtt_net_size(const qtt_net* net);
+int64_t qtt_mpo_size(const qtt_mpo* op);
 
 /* ---- batches (no reference counterpart: the reference applies operators to
  * one function at a time, src/elementary_operators.jl:305-319) -------------

@@ -16,7 +16,7 @@ from .operators import (add_operators, backward_shift_op, first_derivative_opera
                         forward_shift_op, fourth_derivative_operator, fredholm_operator,
                         identity_operator, laplacian_operator, second_derivative_operator, stencil,
                         third_derivative_operator)
-from .api import (Precision, engine_context, eval_grad, evaluate, evaluate_batch, inner, integrate,
+from .api import (Precision, apply_grad, engine_context, eval_grad, evaluate, evaluate_batch, inner, integrate,
                   integrate_batch, on_stream, operate, operate_batch, truncate, tune_evaluate)
 
 real_continuous_siteinds = continuous_siteinds
@@ -30,5 +30,5 @@ __all__ = [
     "second_derivative_operator", "third_derivative_operator", "fourth_derivative_operator",
     "laplacian_operator", "identity_operator", "fredholm_operator", "add_operators", "evaluate",
     "integrate", "inner", "operate", "truncate", "eval_grad", "Precision", "engine_context",
-    "operate_batch", "integrate_batch", "evaluate_batch", "tune_evaluate",
+    "operate_batch", "integrate_batch", "evaluate_batch", "tune_evaluate", "apply_grad",
 ]

@@ -23,7 +23,7 @@ EXPORTS = [
     "qtt_net_destroy", "qtt_mpo_create", "qtt_mpo_destroy", "qtt_eval", "qtt_eval_host",
     "qtt_integrate", "qtt_inner", "qtt_apply", "qtt_truncate", "qtt_eval_grad",
     "qtt_eval_grad_host", "qtt_net_size", "qtt_apply_batch", "qtt_integrate_batch", "qtt_eval_batch",
-    "qtt_tune_eval", "qtt_eval_noalloc",
+    "qtt_tune_eval", "qtt_eval_noalloc", "qtt_apply_grad", "qtt_mpo_size",
 ]
 
 
@@ -85,6 +85,9 @@ def lib():
         L.qtt_eval_batch.argtypes = [vp, C.c_int, C.POINTER(vp), C.POINTER(vp), ip, C.POINTER(vp), C.c_int]
         L.qtt_tune_eval.argtypes = [vp, vp, vp, C.c_int64, vp, C.POINTER(C.c_int), dp]
         L.qtt_eval_noalloc.argtypes = [vp, vp, vp, C.c_int64, vp]
+        L.qtt_apply_grad.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.qtt_mpo_size.argtypes = [vp]
+        L.qtt_mpo_size.restype = C.c_int64
         _lib = L
     return _lib
 

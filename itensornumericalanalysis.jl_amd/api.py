@@ -241,6 +241,47 @@ def eval_grad(f: ITensorNetworkFunction, xs, ybar, precision: int = Precision.F6
     return grads
 
 
+def apply_grad(op: TreeOperator, f: ITensorNetworkFunction, gbar, want_op: bool = True):
+    """Pullback of operate(op, f) without truncation (the exact product W f):
+    given the cotangent `gbar` of the product's cores (a list of arrays in the
+    product's core shapes, e.g. eval_grad of the product), returns (fbar,
+    wbar): the cotangents of f's cores and of the operator's cores (None when
+    want_op is False)."""
+    import torch
+    dn, dop = device_network(f), device_operator(op)
+    ctx = dn.ctx
+    g = torch.tensor(np.concatenate([np.ravel(np.asarray(c, dtype=np.float64)) for c in gbar]), device="cuda")
+    fb = torch.empty(dn.size(), dtype=torch.float64, device="cuda")
+    wb = torch.empty(int(_capi.lib().qtt_mpo_size(dop.handle)), dtype=torch.float64, device="cuda") if want_op else None
+    if g.numel() != sum(int(np.prod(s)) for s in _product_shapes(op, f)):
+        raise ValueError("gbar must hold the exact product's cores")
+    with _torch_stream(ctx, g):
+        ctx.check(_capi.lib().qtt_apply_grad(ctx.handle, dop.handle, dn.handle, C.c_void_p(g.data_ptr()),
+                                             C.c_void_p(fb.data_ptr()),
+                                             C.c_void_p(wb.data_ptr()) if wb is not None else None))
+    b = dn.bonds()
+    fbar = _split(fb.cpu().numpy(), [f.topo.core_shape(v, b) for v in range(f.topo.nv)])
+    wbar = None
+    if want_op:
+        ob = op.bonds()
+        wbar = _split(wb.cpu().numpy(), [op.topo.op_shape(v, ob) for v in range(op.topo.nv)])
+    return fbar, wbar
+
+
+def _product_shapes(op, f):
+    b = op.bonds() * device_network(f).bonds()
+    return [f.topo.core_shape(v, b) for v in range(f.topo.nv)]
+
+
+def _split(flat, shapes):
+    out, o = [], 0
+    for s in shapes:
+        n = int(np.prod(s))
+        out.append(flat[o:o + n].reshape(s))
+        o += n
+    return out
+
+
 def operate_batch(ops, fs, cutoff=None, maxdim=None, precision: int = Precision.F64):
     """operate(ops[i], fs[i]; cutoff, maxdim) for a batch of independent functions:
     the truncation sweeps of the whole batch run in one launch."""

@@ -422,6 +422,17 @@ int64_t qtt_net_size(const qtt_net* net) {
   return s;
 }
 
+int64_t qtt_mpo_size(const qtt_mpo* op) {
+  if (!op) return 0;
+  int64_t s = 0;
+  for (int v = 0; v < op->t.nv; v++) {
+    int64_t c = op->bond[v] * op->t.sdim[v] * op->t.sdim[v];
+    for (int k = 0; k < op->t.nch[v]; k++) c *= op->bond[op->t.chs[op->t.choff[v] + k]];
+    s += c;
+  }
+  return s;
+}
+
 int qtt_net_read(qtt_ctx* ctx, const qtt_net* net, double* const* cores_out) {
   if (!ctx || !net || !cores_out) return qtt_fail(ctx, QTT_EINVAL, "null argument");
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -763,6 +774,13 @@ int qtt_eval_grad(qtt_ctx* ctx, const qtt_net* net, const double* xs, const doub
   int rc = check_precision(ctx, precision);
   if (rc) return rc;
   return launch_grad(ctx, net, xs, ybar, npts, grad, precision);
+}
+
+int qtt_apply_grad(qtt_ctx* ctx, const qtt_mpo* op, const qtt_net* net, const double* gbar, double* fbar,
+                   double* wbar) {
+  if (!ctx || !op || !net || !gbar || !fbar) return qtt_fail(ctx, QTT_EINVAL, "null argument");
+  if (!same_topo(op->t, net->t)) return qtt_fail(ctx, QTT_EINVAL, "apply gradient: operator and network layouts differ");
+  return launch_apply_grad(ctx, op, net, gbar, fbar, wbar);
 }
 
 int qtt_eval_grad_host(qtt_ctx* ctx, const qtt_net* net, const double* xs, const double* ybar,

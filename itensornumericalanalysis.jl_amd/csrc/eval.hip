@@ -912,6 +912,9 @@ static int ensure_pack(qtt_ctx* ctx, qtt_net* net) {
 
 int qtt_ensure_pack(qtt_ctx* ctx, qtt_net* net) { return ensure_pack(ctx, net); }
 
+static int ensure_pack32(qtt_ctx* ctx, qtt_net* net);
+int qtt_ensure_pack32(qtt_ctx* ctx, qtt_net* net) { return ensure_pack32(ctx, net); }
+
 static int ensure_pack32(qtt_ctx* ctx, qtt_net* net) {
   int rc = ensure_pack(ctx, net);
   if (rc) return rc;

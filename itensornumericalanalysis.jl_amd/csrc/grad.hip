@@ -33,20 +33,26 @@ __device__ inline uint64_t digit_word_g(double x, int L, bool& bad) {
   return (uint64_t)(x * (double)(uint64_t(1) << L));
 }
 
+// F: the arithmetic of the environments and of the per-tile sums (double, or
+// float for the float32 gradient); S: the stride of a pack entry in F units
+// (1 for the float64 pack, 2 for the float (hi, lo) pack, whose hi is read).
+// The partial gradients accumulate in float64 across tiles either way.
+template <typename F, int S>
 __global__ void __launch_bounds__(GB)
-grad_chain_kernel(const double* __restrict__ pack, const int64_t* __restrict__ poff,
+grad_chain_kernel(const F* __restrict__ pack, const int64_t* __restrict__ poff,
                   const int* __restrict__ bond, TopoView t, const int64_t* __restrict__ goff,
                   const int64_t* __restrict__ roff, int64_t rtot, const double* __restrict__ xs,
-                  const double* __restrict__ ybar, int64_t npts, int64_t gsize, double* Rscr,
+                  const double* __restrict__ ybar, int64_t npts, int64_t gsize, F* Rscr,
                   double* gpart, int* __restrict__ flag, int vstride) {
-  extern __shared__ double lds[];
+  extern __shared__ double lds_raw[];
+  F* lds = (F*)lds_raw;
   const int tid = threadIdx.x;
   // LDS: Lv[vstride][GB], Nv[vstride][GB], Rv[vstride][GB], digit per lane
-  double* Lv = lds;
-  double* Nv = lds + (size_t)vstride * GB;
-  double* Rv = lds + (size_t)2 * vstride * GB;
+  F* Lv = lds;
+  F* Nv = lds + (size_t)vstride * GB;
+  F* Rv = lds + (size_t)2 * vstride * GB;
   int* dig = (int*)(lds + (size_t)3 * vstride * GB);
-  double* R = Rscr + (size_t)blockIdx.x * rtot * GB;  // [roff[k] + r][lane]
+  F* R = Rscr + (size_t)blockIdx.x * rtot * GB;  // [roff[k] + r][lane]
   double* G = gpart + (size_t)blockIdx.x * gsize;
   for (int64_t e = tid; e < gsize; e += GB) G[e] = 0.0;
   const int nc = t.ncoord;
@@ -60,11 +66,11 @@ grad_chain_kernel(const double* __restrict__ pack, const int64_t* __restrict__ p
     for (int c = 0; c < QTT_MAX_COORD; ++c)
       if (c < nc) kw[c] = digit_word_g(active ? xs[p * nc + c] : 0.0, t.ndigits[c], bad);
     if (bad) flag[0] = 1;
-    const double yb = (active && !bad) ? ybar[p] : 0.0;
+    const F yb = (F)((active && !bad) ? ybar[p] : 0.0);
     // ---- backward: R_in(k) = vector over the child bond of site k -------
-    double* v = Lv + tid;
-    double* w = Nv + tid;
-    v[0] = 1.0;
+    F* v = Lv + tid;
+    F* w = Nv + tid;
+    v[0] = (F)1;
     for (int k = t.nv - 1; k >= 0; --k) {
       const int chil = bond[k];
       const int chir = (k == t.nv - 1) ? 1 : bond[k + 1];
@@ -75,38 +81,30 @@ grad_chain_kernel(const double* __restrict__ pack, const int64_t* __restrict__ p
         const uint64_t kk = c == 0 ? kw[0] : c == 1 ? kw[1] : c == 2 ? kw[2] : kw[3];
         d = (int)((kk >> (t.ndigits[c] - t.vdigit[k])) & 1u);
       }
-      const double* __restrict__ P = pack + poff[k];
+      const F* __restrict__ P = pack + (size_t)S * poff[k];
       const int lpad = (chil + 3) & ~3;
       for (int l0 = 0; l0 < chil; l0 += 4) {
-        double a0[4] = {0.0, 0.0, 0.0, 0.0}, a1[4] = {0.0, 0.0, 0.0, 0.0};
-        const double* __restrict__ q = P + 2 * l0;
+        F a0[4] = {0, 0, 0, 0}, a1[4] = {0, 0, 0, 0};
+        const F* __restrict__ q = P + 2 * S * l0;
         for (int r = 0; r < chir; ++r) {
-          const double vr = v[(size_t)r * GB];
+          const F vr = v[(size_t)r * GB];
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
-            a0[i] = __builtin_fma(q[2 * i], vr, a0[i]);
-            a1[i] = __builtin_fma(q[2 * i + 1], vr, a1[i]);
+            a0[i] = __builtin_fma(q[(2 * i) * S], vr, a0[i]);
+            a1[i] = __builtin_fma(q[(2 * i + 1) * S], vr, a1[i]);
           }
-          q += 2 * lpad;
-        }
-        const uint64_t dmask = 0 - (uint64_t)d;
-        double sel[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const uint64_t ua = (uint64_t)__double_as_longlong(a0[i]);
-          const uint64_t ub = (uint64_t)__double_as_longlong(a1[i]);
-          sel[i] = __longlong_as_double((long long)((ua & ~dmask) | (ub & dmask)));
+          q += 2 * S * lpad;
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-          if (l0 + i < chil) w[(size_t)(l0 + i) * GB] = sel[i];
+          if (l0 + i < chil) w[(size_t)(l0 + i) * GB] = d ? a1[i] : a0[i];
       }
-      double* tmp = v;
+      F* tmp = v;
       v = w;
       w = tmp;
     }
     // ---- forward: accumulate outer products, advance L ------------------
-    Lv[tid] = 1.0;  // L_0 over the root's parent bond (dimension 1)
+    Lv[tid] = (F)1;  // L_0 over the root's parent bond (dimension 1)
     for (int k = 0; k < t.nv; ++k) {
       const int chil = bond[k];
       const int chir = (k == t.nv - 1) ? 1 : bond[k + 1];
@@ -128,22 +126,22 @@ grad_chain_kernel(const double* __restrict__ pack, const int64_t* __restrict__ p
         const int64_t ls = e / chir;
         const int s = (int)(ls % sd);
         const int l = (int)(ls / sd);
-        double acc = 0.0;
-        const double* lrow = Lv + (size_t)l * GB;
-        const double* rrow = Rv + (size_t)r * GB;
-        for (int j = 0; j < GB; ++j) acc += (dig[j] == s) ? lrow[j] * rrow[j] : 0.0;
-        Gk[e] += acc;
+        F acc = 0;
+        const F* lrow = Lv + (size_t)l * GB;
+        const F* rrow = Rv + (size_t)r * GB;
+        for (int j = 0; j < GB; ++j) acc += (dig[j] == s) ? lrow[j] * rrow[j] : (F)0;
+        Gk[e] += (double)acc;
       }
       // L_{k+1}[r] = sum_l L_k[l] T[l][d][r]
-      const double* __restrict__ P = pack + poff[k];
+      const F* __restrict__ P = pack + (size_t)S * poff[k];
       const int lpad = (chil + 3) & ~3;
       for (int r = 0; r < chir; ++r) {
-        double s0 = 0.0, s1 = 0.0;
-        const double* __restrict__ q = P + (size_t)r * 2 * lpad;
+        F s0 = 0, s1 = 0;
+        const F* __restrict__ q = P + (size_t)r * 2 * S * lpad;
         for (int l = 0; l < chil; ++l) {
-          const double lv = Lv[(size_t)l * GB + tid];
-          s0 = __builtin_fma(q[2 * l], lv, s0);
-          s1 = __builtin_fma(q[2 * l + 1], lv, s1);
+          const F lv = Lv[(size_t)l * GB + tid];
+          s0 = __builtin_fma(q[(2 * l) * S], lv, s0);
+          s1 = __builtin_fma(q[(2 * l + 1) * S], lv, s1);
         }
         Nv[(size_t)r * GB + tid] = d ? s1 : s0;
       }
@@ -417,6 +415,224 @@ grad_chain_reg_kernel(const double* __restrict__ pack, const int64_t* __restrict
   }
 }
 
+// ---------------------------------------------------------------------------
+// Gradient on a general tree (float64), tiles of GTR points per workgroup.
+//   phase A, a lane per point: the up pass gives every vertex's subtree vector
+//     u_v over bond[v] (children before parents: vertices in decreasing id);
+//     the down pass gives every vertex's environment e_v over bond[v] (the
+//     rest of the network contracted, parents first).  Both go to the
+//     workgroup's scratch as [element][lane], with the digit of every vertex;
+//   phase B, a lane per gradient entry (v, p, s, c_1 .. c_m):
+//     G_v[p][s][c] += sum over the tile's points with digit s at v of
+//     ybar e_v[p] prod_i u_{ch_i}[c_i]  (owner-computes, no atomics).
+// The partials are summed by grad_reduce_kernel in a fixed order.
+// ---------------------------------------------------------------------------
+constexpr int GTR = 64;
+
+__global__ void __launch_bounds__(256)
+grad_tree_kernel(const double* __restrict__ data, const int64_t* __restrict__ off, const int* __restrict__ bond,
+                 TopoView t, const int64_t* __restrict__ voff, int64_t vtot, const int64_t* __restrict__ goff,
+                 const double* __restrict__ xs, const double* __restrict__ ybar, int64_t npts, int64_t gsize,
+                 double* __restrict__ scr, double* __restrict__ gpart, int* __restrict__ flag) {
+  const int tid = threadIdx.x, bs = blockDim.x;
+  // scratch of this workgroup: U [vtot][GTR] | E [vtot][GTR] | ybar [GTR] | digits [nv][GTR]
+  double* U = scr + (size_t)blockIdx.x * (2 * vtot + 1 + t.nv) * GTR;
+  double* E = U + (size_t)vtot * GTR;
+  double* Y = E + (size_t)vtot * GTR;
+  int* D = (int*)(Y + GTR);
+  double* G = gpart + (size_t)blockIdx.x * gsize;
+  for (int64_t e = tid; e < gsize; e += bs) G[e] = 0.0;
+  const int nc = t.ncoord;
+  const int64_t ntiles = (npts + GTR - 1) / GTR;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    __syncthreads();
+    if (tid < GTR) {
+      const int lane = tid;
+      const int64_t p = tile * GTR + lane;
+      const bool active = p < npts;
+      bool bad = false;
+      for (int c = 0; c < nc; ++c) (void)digit_word_g(active ? xs[p * nc + c] : 0.0, t.ndigits[c], bad);
+      if (bad) flag[0] = 1;
+      Y[lane] = (active && !bad) ? ybar[p] : 0.0;
+      for (int v = 0; v < t.nv; ++v) {  // any number of coordinates
+        const int c = t.vcoord[v];
+        int d = 0;
+        if (c >= 0 && active && !bad) {
+          bool b2 = false;
+          const uint64_t kk = digit_word_g(xs[p * nc + c], t.ndigits[c], b2);
+          d = (int)((kk >> (t.ndigits[c] - t.vdigit[v])) & 1u);
+        }
+        D[(size_t)v * GTR + lane] = d;
+      }
+      // up pass
+      for (int v = t.nv - 1; v >= 0; --v) {
+        const int m = t.nch[v];
+        const int* ch = t.chs + t.choff[v];
+        int64_t pc = 1;
+        for (int k = 0; k < m; ++k) pc *= bond[ch[k]];
+        const int sd = t.sdim[v];
+        const int d = D[(size_t)v * GTR + lane];
+        for (int pp = 0; pp < bond[v]; ++pp) {
+          const double* base = data + off[v] + ((int64_t)pp * sd + d) * pc;
+          double acc = 0.0;
+          for (int64_t j = 0; j < pc; ++j) {
+            double w = base[j];
+            int64_t r = j;
+            for (int k = m - 1; k >= 0; --k) {
+              const int b = bond[ch[k]];
+              w *= U[(voff[ch[k]] + r % b) * GTR + lane];
+              r /= b;
+            }
+            acc += w;
+          }
+          U[(voff[v] + pp) * GTR + lane] = acc;
+        }
+      }
+      // down pass
+      E[voff[0] * GTR + lane] = 1.0;
+      for (int v = 0; v < t.nv; ++v) {
+        const int m = t.nch[v];
+        const int* ch = t.chs + t.choff[v];
+        int64_t pc = 1;
+        for (int k = 0; k < m; ++k) pc *= bond[ch[k]];
+        const int sd = t.sdim[v];
+        const int d = D[(size_t)v * GTR + lane];
+        for (int kc = 0; kc < m; ++kc) {
+          const int c = ch[kc];
+          for (int j = 0; j < bond[c]; ++j) E[(voff[c] + j) * GTR + lane] = 0.0;
+          for (int pp = 0; pp < bond[v]; ++pp) {
+            const double ev = E[(voff[v] + pp) * GTR + lane];
+            const double* base = data + off[v] + ((int64_t)pp * sd + d) * pc;
+            for (int64_t j = 0; j < pc; ++j) {
+              double w = base[j] * ev;
+              int64_t r = j;
+              int jc = 0;
+              for (int k = m - 1; k >= 0; --k) {
+                const int b = bond[ch[k]];
+                const int idx = (int)(r % b);
+                r /= b;
+                if (k == kc) jc = idx;
+                else w *= U[(voff[ch[k]] + idx) * GTR + lane];
+              }
+              E[(voff[c] + jc) * GTR + lane] += w;
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // phase B
+    for (int v = 0; v < t.nv; ++v) {
+      const int m = t.nch[v];
+      const int* ch = t.chs + t.choff[v];
+      int64_t pc = 1;
+      for (int k = 0; k < m; ++k) pc *= bond[ch[k]];
+      const int sd = t.sdim[v];
+      const int64_t ne = (int64_t)bond[v] * sd * pc;
+      for (int64_t e = tid; e < ne; e += bs) {
+        const int64_t j = e % pc;
+        const int s = (int)((e / pc) % sd);
+        const int pp = (int)(e / (pc * sd));
+        double acc = 0.0;
+        for (int lane = 0; lane < GTR; ++lane) {
+          if (D[(size_t)v * GTR + lane] != s) continue;
+          double w = Y[lane] * E[(voff[v] + pp) * GTR + lane];
+          int64_t r = j;
+          for (int k = m - 1; k >= 0; --k) {
+            const int b = bond[ch[k]];
+            w *= U[(voff[ch[k]] + r % b) * GTR + lane];
+            r /= b;
+          }
+          acc += w;
+        }
+        G[goff[v] + e] += acc;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pullback of the exact operator contraction g = W f (contract_kernel):
+//   g_v[(pw, pa)][so][(cw_1, ca_1) ...] = sum_si W_v[pw][si][so][cw..] f_v[pa][si][ca..]
+// so, for a cotangent gbar on g's cores,
+//   fbar_v[pa][si][ca..] = sum_{pw, so, cw..} W_v[pw][si][so][cw..] gbar_v[(pw, pa)][so][(cw, ca)..]
+//   wbar_v[pw][si][so][cw..] = sum_{pa, ca..} f_v[pa][si][ca..] gbar_v[(pw, pa)][so][(cw, ca)..]
+// One thread per output entry; a combined child index is cw * bondA + ca.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int64_t comb_offset(const int* ch, int m, const int* bw, const int* ba, int64_t jw,
+                                               int64_t ja) {
+  // offset in g's child block of the combined indices (cw_k * ba_k + ca_k)
+  int64_t o = 0, S = 1;
+  for (int k = m - 1; k >= 0; --k) {
+    const int64_t bwk = bw[ch[k]], bak = ba[ch[k]];
+    const int64_t cw = jw % bwk, ca = ja % bak;
+    jw /= bwk;
+    ja /= bak;
+    o += (cw * bak + ca) * S;
+    S *= bwk * bak;
+  }
+  return o;
+}
+
+__global__ void __launch_bounds__(256)
+contract_adjoint_kernel(const double* __restrict__ W, const int64_t* __restrict__ woff, const int* __restrict__ bw,
+                        const double* __restrict__ A, const int64_t* __restrict__ aoff, const int* __restrict__ ba,
+                        TopoView t, const double* __restrict__ gbar, const int64_t* __restrict__ gstart,
+                        double* __restrict__ fbar, const int64_t* __restrict__ fstart, double* __restrict__ wbar,
+                        const int64_t* __restrict__ wstart) {
+  // fbar entries first, then wbar entries (if wanted), over one flat index
+  const int64_t nf = fstart[t.nv], nw = wbar ? wstart[t.nv] : 0;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nf + nw;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const bool isf = e < nf;
+    const int64_t ee = isf ? e : e - nf;
+    const int64_t* st = isf ? fstart : wstart;
+    int v = 0;
+    while (st[v + 1] <= ee) ++v;
+    const int64_t loc = ee - st[v];
+    const int m = t.nch[v];
+    const int* ch = t.chs + t.choff[v];
+    const int sd = t.sdim[v];
+    int64_t pcw = 1, pca = 1;
+    for (int k = 0; k < m; ++k) {
+      pcw *= bw[ch[k]];
+      pca *= ba[ch[k]];
+    }
+    const int64_t Ap = ba[v], Dp = bw[v];
+    const int64_t pco = pcw * pca;
+    const double* Wv = W + woff[v];
+    const double* Av = A + aoff[v];
+    const double* Gv = gbar + gstart[v];
+    double s = 0.0;
+    if (isf) {  // loc = (pa * sd + si) * pca + ja
+      const int64_t ja = loc % pca;
+      const int si = (int)((loc / pca) % sd);
+      const int64_t pa = loc / (pca * sd);
+      for (int64_t pw = 0; pw < Dp; ++pw)
+        for (int so = 0; so < sd; ++so) {
+          const double* wrow = Wv + ((pw * sd + si) * sd + so) * pcw;
+          const double* grow = Gv + ((pw * Ap + pa) * sd + so) * pco;
+          for (int64_t jw = 0; jw < pcw; ++jw) {
+            const double wv = wrow[jw];
+            if (wv != 0.0) s = __builtin_fma(wv, grow[comb_offset(ch, m, bw, ba, jw, ja)], s);
+          }
+        }
+      fbar[e] = s;
+    } else {  // loc = ((pw * sd + si) * sd + so) * pcw + jw
+      const int64_t jw = loc % pcw;
+      const int so = (int)((loc / pcw) % sd);
+      const int si = (int)((loc / (pcw * sd)) % sd);
+      const int64_t pw = loc / (pcw * sd * sd);
+      for (int64_t pa = 0; pa < Ap; ++pa) {
+        const double* arow = Av + (pa * sd + si) * pca;
+        const double* grow = Gv + ((pw * Ap + pa) * sd + so) * pco;
+        for (int64_t ja = 0; ja < pca; ++ja) s = __builtin_fma(arow[ja], grow[comb_offset(ch, m, bw, ba, jw, ja)], s);
+      }
+      wbar[ee] = s;
+    }
+  }
+}
+
 __global__ void grad_reduce_kernel(const double* __restrict__ gpart, int nparts, int64_t gsize,
                                    double* __restrict__ grad) {
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < gsize;
@@ -431,16 +647,95 @@ __global__ void grad_reduce_kernel(const double* __restrict__ gpart, int nparts,
 
 // implemented in eval.hip
 int qtt_ensure_pack(qtt_ctx* ctx, qtt_net* net);
+int qtt_ensure_pack32(qtt_ctx* ctx, qtt_net* net);
+
+// The tree kernel (float64, any tree): tiles of GTR points, partials summed
+// in a fixed order.
+static int launch_grad_tree(qtt_ctx* ctx, qtt_net* net, const double* xs, const double* ybar, int64_t npts,
+                            double* grad) {
+  const HostTopo& t = net->t;
+  const int nv = t.nv;
+  std::vector<int64_t> voff(nv), goff(nv);
+  int64_t vtot = 0, gsize = 0;
+  for (int v = 0; v < nv; v++) {
+    voff[v] = vtot;
+    vtot += net->bond[v];
+    goff[v] = gsize;
+    gsize += net->core_size(v);
+  }
+  const int64_t ntiles = (npts + GTR - 1) / GTR;
+  int nwg = (int)std::max<int64_t>(1, std::min<int64_t>(ntiles, 2 * (int64_t)ctx->cus));
+  const size_t per_wg = (size_t)(2 * vtot + 1 + nv) * GTR;
+  const size_t bytes = ((size_t)nwg * (per_wg + gsize) + 2 * nv + 16) * sizeof(double);
+  double* scr = qtt_scratch(ctx, bytes);
+  if (!scr) return qtt_fail(ctx, QTT_ENOMEM, "gradient scratch");
+  double* wscr = scr;
+  double* gpart = wscr + (size_t)nwg * per_wg;
+  int64_t* d_voff = (int64_t*)(gpart + (size_t)nwg * gsize);
+  int64_t* d_goff = d_voff + nv;
+  int rc = qtt_hip_check(ctx, hipMemcpyAsync(d_voff, voff.data(), nv * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream), "voff");
+  if (!rc) rc = qtt_hip_check(ctx, hipMemcpyAsync(d_goff, goff.data(), nv * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream), "goff");
+  if (rc) return rc;
+  grad_tree_kernel<<<nwg, 256, 0, ctx->stream>>>(net->d_data, net->d_off, net->d_bond, net->view(), d_voff, vtot,
+                                                 d_goff, xs, ybar, npts, gsize, wscr, gpart, ctx->flags());
+  rc = qtt_hip_check(ctx, hipGetLastError(), "grad_tree_kernel");
+  if (rc) return rc;
+  int64_t nb = std::min<int64_t>((gsize + 255) / 256, 4096);
+  grad_reduce_kernel<<<(unsigned)std::max<int64_t>(nb, 1), 256, 0, ctx->stream>>>(gpart, nwg, gsize, grad);
+  rc = qtt_hip_check(ctx, hipGetLastError(), "grad_reduce_kernel");
+  if (rc) return rc;
+  // the host offset vectors must outlive their asynchronous copies
+  return qtt_hip_check(ctx, hipStreamSynchronize(ctx->stream), "grad sync");
+}
+
+int launch_apply_grad(qtt_ctx* ctx, const qtt_mpo* op, const qtt_net* net, const double* gbar, double* fbar,
+                      double* wbar) {
+  const HostTopo& t = net->t;
+  const int nv = t.nv;
+  std::vector<int64_t> tab(3 * (nv + 1), 0);  // fstart | wstart | gstart
+  int64_t* fs = tab.data();
+  int64_t* ws = fs + nv + 1;
+  int64_t* gs = ws + nv + 1;
+  for (int v = 0; v < nv; v++) {
+    int64_t pa = 1, pw = 1, pg = 1;
+    for (int k = 0; k < t.nch[v]; k++) {
+      const int c = t.chs[t.choff[v] + k];
+      pa *= net->bond[c];
+      pw *= op->bond[c];
+      pg *= net->bond[c] * op->bond[c];
+    }
+    const int64_t sd = t.sdim[v];
+    fs[v + 1] = fs[v] + net->bond[v] * sd * pa;
+    ws[v + 1] = ws[v] + op->bond[v] * sd * sd * pw;
+    gs[v + 1] = gs[v] + op->bond[v] * net->bond[v] * sd * pg;
+  }
+  double* scr = qtt_scratch(ctx, tab.size() * sizeof(int64_t) + 64);
+  if (!scr) return qtt_fail(ctx, QTT_ENOMEM, "apply gradient scratch");
+  int64_t* d = (int64_t*)scr;
+  int rc = qtt_hip_check(ctx, hipMemcpyAsync(d, tab.data(), tab.size() * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream), "offsets");
+  if (rc) return rc;
+  const int64_t total = fs[nv] + (wbar ? ws[nv] : 0);
+  const int64_t nb = std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 65536));
+  contract_adjoint_kernel<<<(unsigned)nb, 256, 0, ctx->stream>>>(op->d_data, op->d_off, op->d_bond, net->d_data,
+                                                                 net->d_off, net->d_bond, net->view(), gbar,
+                                                                 d + 2 * (nv + 1), fbar, d, wbar, d + (nv + 1));
+  rc = qtt_hip_check(ctx, hipGetLastError(), "contract_adjoint_kernel");
+  if (rc) return rc;
+  return qtt_hip_check(ctx, hipStreamSynchronize(ctx->stream), "apply gradient sync");
+}
 
 int launch_grad(qtt_ctx* ctx, const qtt_net* cnet, const double* xs, const double* ybar,
                 int64_t npts, double* grad, int precision) {
   qtt_net* net = const_cast<qtt_net*>(cnet);
   const HostTopo& t = net->t;
-  if (!t.chain || t.ncoord > QTT_MAX_COORD)
-    return qtt_fail(ctx, QTT_EINVAL, "gradient: implemented for chain networks");
-  if (precision != QTT_F64)
-    return qtt_fail(ctx, QTT_EINVAL, "gradient: float64 only");
-  int rc = qtt_ensure_pack(ctx, net);
+  if (precision != QTT_F64 && precision != QTT_F32)
+    return qtt_fail(ctx, QTT_EINVAL, "gradient: QTT_F64 or QTT_F32");
+  if (!t.chain || t.ncoord > QTT_MAX_COORD) {
+    if (precision != QTT_F64) return qtt_fail(ctx, QTT_EINVAL, "gradient: float32 is implemented for chains");
+    return launch_grad_tree(ctx, net, xs, ybar, npts, grad);
+  }
+  const bool f32 = precision == QTT_F32;
+  int rc = f32 ? qtt_ensure_pack32(ctx, net) : qtt_ensure_pack(ctx, net);
   if (rc) return rc;
   const int nv = t.nv;
   std::vector<int64_t> goff(nv), roff(nv);
@@ -455,10 +750,11 @@ int launch_grad(qtt_ctx* ctx, const qtt_net* cnet, const double* xs, const doubl
   bool binary = true;
   for (int k = 0; k < nv; k++) binary = binary && t.sdim[k] <= 2;
   const int chi = maxb <= 8 ? 8 : maxb <= 16 ? 16 : 32;
-  const bool reg = binary && maxb <= 32;
+  const bool reg = binary && maxb <= 32 && !f32;
   const int tilep = reg ? GT : GB;
+  const size_t fsz = f32 ? sizeof(float) : sizeof(double);
   const size_t lds = reg ? ((size_t)2 * chi * (GT / 2 + 1) + 4) * sizeof(double)
-                         : (size_t)3 * maxb * GB * sizeof(double) + GB * sizeof(int);
+                         : (size_t)3 * maxb * GB * fsz + GB * sizeof(int);
   if (lds > 160 * 1024) return qtt_fail(ctx, QTT_EINVAL, "gradient: bond dimension too large");
   const int64_t ntiles = (npts + tilep - 1) / tilep;
   const int per_cu = reg ? std::max(1, std::min(4, (int)((160 * 1024) / lds))) : 2;
@@ -474,7 +770,7 @@ int launch_grad(qtt_ctx* ctx, const qtt_net* cnet, const double* xs, const doubl
   if (!rc) rc = qtt_hip_check(ctx, hipMemcpyAsync(d_roff, roff.data(), nv * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream), "roff");
   if (rc) return rc;
   const TopoView tv = net->view();
-  const void* fn = !reg ? (const void*)grad_chain_kernel
+  const void* fn = !reg ? (f32 ? (const void*)grad_chain_kernel<float, 2> : (const void*)grad_chain_kernel<double, 1>)
                    : chi == 8 ? (const void*)grad_chain_reg_kernel<8>
                    : chi == 16 ? (const void*)grad_chain_reg_kernel<16>
                                : (const void*)grad_chain_reg_kernel<32>;
@@ -495,9 +791,14 @@ int launch_grad(qtt_ctx* ctx, const qtt_net* cnet, const double* xs, const doubl
       else QTT_GRAD_LAUNCH(32);
 #undef QTT_GRAD_LAUNCH
     } else {
-      grad_chain_kernel<<<nwg, GB, lds, ctx->stream>>>(net->d_pack, net->d_pack_off, net->d_bond, tv, d_goff,
-                                                       d_roff, rtot, xs, ybar, npts, gsize, Rscr, gpart,
-                                                       ctx->flags(), (int)maxb);
+      if (f32)  // the float (hi, lo) pack, hi read; float vectors parked in the scratch
+        grad_chain_kernel<float, 2><<<nwg, GB, lds, ctx->stream>>>(
+            (const float*)net->d_pack32, net->d_pack_off, net->d_bond, tv, d_goff, d_roff, rtot, xs, ybar, npts,
+            gsize, (float*)Rscr, gpart, ctx->flags(), (int)maxb);
+      else
+        grad_chain_kernel<double, 1><<<nwg, GB, lds, ctx->stream>>>(net->d_pack, net->d_pack_off, net->d_bond, tv,
+                                                                    d_goff, d_roff, rtot, xs, ybar, npts, gsize, Rscr,
+                                                                    gpart, ctx->flags(), (int)maxb);
     }
     rc = qtt_hip_check(ctx, hipGetLastError(), "grad_chain_kernel");
     if (rc) return rc;

@@ -144,6 +144,8 @@ int launch_truncate_batch(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double c
                           double* errs);
 int launch_grad(qtt_ctx* ctx, const qtt_net* net, const double* xs, const double* ybar,
                 int64_t npts, double* grad, int precision);
+int launch_apply_grad(qtt_ctx* ctx, const qtt_mpo* op, const qtt_net* net, const double* gbar, double* fbar,
+                      double* wbar);
 
 // schedule of the truncation sweep (capi.hip): triples (x, u, truncate)
 std::vector<int> truncation_schedule(const HostTopo& t);

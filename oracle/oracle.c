@@ -16,6 +16,7 @@
  *                 ITensors/NDTensors 0.9 relative cutoff rule on squared singular
  *                 values)
  *   or_eval_grad  no reference counterpart (adjoint of or_eval)
+ *   or_contract_adjoint  no reference counterpart (adjoint of or_contract)
  * The parity status of each is recorded with its tests.
  */
 #include "oracle.h"
@@ -409,6 +410,81 @@ int or_contract(int nv, const int* parent, const int* sdim, const long* bondW, c
               if (wv == 0.0) continue;
               for (long ja = 0; ja < pca; ja++) c[offw[jw] + offa[ja]] += wv * a[ja];
             }
+          }
+        }
+    free(offw);
+    free(offa);
+  }
+  tree_free(&t);
+  return OR_OK;
+}
+
+/* Pullback of or_contract (no reference counterpart: the rrule of operate
+ * without truncation, src/elementary_operators.jl:294-299).  gbar holds the
+ * cotangent of the cores of C = W A in or_contract's layout; then
+ *   abar_v[pa][si][ca..]     = sum_{pw, so, cw..} W_v[pw][si][so][cw..] gbar_v[(pw, pa)][so][(cw, ca)..]
+ *   wbar_v[pw][si][so][cw..] = sum_{pa, ca..}     A_v[pa][si][ca..]     gbar_v[(pw, pa)][so][(cw, ca)..]
+ * abar and wbar (nullable) are overwritten. */
+int or_contract_adjoint(int nv, const int* parent, const int* sdim, const long* bondW,
+                        const double* const* coresW, const long* bondA, const double* const* coresA,
+                        const double* const* gbar, double* const* abar, double* const* wbar) {
+  tree_t t;
+  int rc = tree_build(&t, nv, parent);
+  if (rc) return rc;
+  if (check_bonds(nv, bondW) || check_bonds(nv, bondA)) {
+    tree_free(&t);
+    return OR_EINVAL;
+  }
+  for (int v = 0; v < nv; v++) {
+    int m = t.nch[v];
+    const int* ch = tree_ch(&t, v);
+    int d = sdim[v];
+    long pcw = 1, pca = 1;
+    for (int k = 0; k < m; k++) {
+      pcw *= bondW[ch[k]];
+      pca *= bondA[ch[k]];
+    }
+    long pco = pcw * pca;
+    long Dp = bondW[v], Ap = bondA[v];
+    long* offw = (long*)malloc(sizeof(long) * (size_t)pcw);
+    long* offa = (long*)malloc(sizeof(long) * (size_t)pca);
+    for (long jw = 0; jw < pcw; jw++) {
+      long r = jw, off = 0, S = 1;
+      for (int k = m - 1; k >= 0; k--) {
+        long bw = bondW[ch[k]], ba = bondA[ch[k]];
+        off += (r % bw) * ba * S;
+        r /= bw;
+        S *= bw * ba;
+      }
+      offw[jw] = off;
+    }
+    for (long ja = 0; ja < pca; ja++) {
+      long r = ja, off = 0, S = 1;
+      for (int k = m - 1; k >= 0; k--) {
+        long bw = bondW[ch[k]], ba = bondA[ch[k]];
+        off += (r % ba) * S;
+        r /= ba;
+        S *= bw * ba;
+      }
+      offa[ja] = off;
+    }
+    memset(abar[v], 0, sizeof(double) * (size_t)(Ap * d * pca));
+    if (wbar) memset(wbar[v], 0, sizeof(double) * (size_t)(Dp * d * d * pcw));
+    for (long pw = 0; pw < Dp; pw++)
+      for (long pa = 0; pa < Ap; pa++)
+        for (int so = 0; so < d; so++) {
+          const double* g = gbar[v] + ((pw * Ap + pa) * d + so) * pco;
+          for (int si = 0; si < d; si++) {
+            const double* w = coresW[v] + ((pw * d + si) * d + so) * pcw;
+            const double* a = coresA[v] + (pa * d + si) * pca;
+            double* ab = abar[v] + (pa * d + si) * pca;
+            double* wb = wbar ? wbar[v] + ((pw * d + si) * d + so) * pcw : NULL;
+            for (long jw = 0; jw < pcw; jw++)
+              for (long ja = 0; ja < pca; ja++) {
+                double gv = g[offw[jw] + offa[ja]];
+                ab[ja] += w[jw] * gv;
+                if (wb) wb[jw] += a[ja] * gv;
+              }
           }
         }
     free(offw);

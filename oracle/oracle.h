@@ -60,6 +60,13 @@ int or_contract(int nv, const int* parent, const int* sdim, const long* bondW,
                 const double* const* coresW, const long* bondA,
                 const double* const* coresA, double* const* out);
 
+/* Pullback of or_contract: abar[v] (A's core shapes) and wbar[v] (W's, nullable)
+ * from gbar[v] (the shapes of or_contract's output). */
+int or_contract_adjoint(int nv, const int* parent, const int* sdim, const long* bondW,
+                        const double* const* coresW, const long* bondA,
+                        const double* const* coresA, const double* const* gbar,
+                        double* const* abar, double* const* wbar);
+
 /* In-place truncation (post-order edges from vertex 0; for each edge child->parent
  * orthogonalise toward the child, SVD the edge, keep per the ITensors relative
  * cutoff rule on squared singular values and maxdim, push S V^T into the parent).

@@ -36,6 +36,7 @@ def lib():
         L.or_integrate.argtypes = [C.c_int, ip, ip, lp, dpp, C.c_double, dp]
         L.or_inner.argtypes = [C.c_int, ip, ip, lp, dpp, lp, dpp, dp]
         L.or_contract.argtypes = [C.c_int, ip, ip, lp, dpp, lp, dpp, dpp]
+        L.or_contract_adjoint.argtypes = [C.c_int, ip, ip, lp, dpp, lp, dpp, dpp, dpp, dpp]
         L.or_truncate.argtypes = [C.c_int, ip, ip, lp, dpp, C.c_double, C.c_long, dp]
         L.or_eval_grad.argtypes = [C.c_int, ip, ip, lp, dpp, ip, ip, C.c_int, ip, dp, C.c_long, dp, dpp,
                                    C.c_int]
@@ -148,6 +149,27 @@ def contract(parent, sdim, coresW, coresA):
     if rc:
         raise RuntimeError(f"or_contract failed: {rc}")
     return oo
+
+
+def contract_adjoint(parent, sdim, coresW, coresA, gbar, want_w=True):
+    """Pullback of contract: (abar, wbar) for the cotangent gbar of its output."""
+    bw, ba = bonds(coresW), bonds(coresA)
+    abar = [np.zeros_like(np.asarray(c, dtype=np.float64)) for c in coresA]
+    wbar = [np.zeros_like(np.asarray(c, dtype=np.float64)) for c in coresW] if want_w else None
+    p, pp = _i(parent)
+    s, sp = _i(sdim)
+    b1, b1p = _l(bw)
+    b2, b2p = _l(ba)
+    cw, cwp = _dpp(coresW)
+    ca, cap = _dpp(coresA)
+    gg, gp = _dpp(gbar)
+    aa, ap = _dpp(abar)
+    if want_w:
+        ww, wp = _dpp(wbar)
+    rc = lib().or_contract_adjoint(len(p), pp, sp, b1p, cwp, b2p, cap, gp, ap, wp if want_w else None)
+    if rc:
+        raise RuntimeError(f"or_contract_adjoint failed: {rc}")
+    return aa, (ww if want_w else None)
 
 
 def truncate(parent, sdim, cores, cutoff=0.0, maxdim=1 << 30):

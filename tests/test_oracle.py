@@ -374,6 +374,47 @@ def test_fredholm_operator_dense(q):
     assert np.allclose(M, ref, rtol=1e-13)
 
 
+@pytest.mark.parametrize("graph", ["chain", "comb"])
+def test_contract_adjoint_identity(q, po, rng, graph):
+    # C = contract(W, A) is bilinear, so for any cotangent G:
+    # <G, C> = <abar, A> = <wbar, W>  (the adjoint identity of both arguments)
+    g = q.named_grid((7, 1)) if graph == "chain" else q.named_comb_tree((3, 3))
+    s = q.continuous_siteinds(g)
+    f = q.random_itn(s, link_space=3, rng=rng)
+    op = q.second_derivative_operator(s, left_boundary="Periodic", right_boundary="Neumann")
+    t = f.topo
+    C = po.contract(t.parent, t.sdim, op.cores, f.cores)
+    G = [rng.standard_normal(c.shape) for c in C]
+    ab, wb = po.contract_adjoint(t.parent, t.sdim, op.cores, f.cores, G)
+    lhs = sum(float(np.sum(x * y)) for x, y in zip(G, C))
+    assert math.isclose(sum(float(np.sum(x * y)) for x, y in zip(ab, f.cores)), lhs, rel_tol=1e-12)
+    assert math.isclose(sum(float(np.sum(x * y)) for x, y in zip(wb, op.cores)), lhs, rel_tol=1e-12)
+
+
+def test_contract_adjoint_through_evaluate(q, po, rng):
+    # d/d f of sum_m ybar_m (W f)(x_m): with the dense operator M (phi = M^T psi on
+    # the grid) this is the gradient of sum_y c(y) f(y), c = M[:, k(x_m)] ybar, i.e.
+    # the evaluate adjoint of f at every grid point y with weights c(y)
+    L = 6
+    s = q.continuous_siteinds(q.named_grid((L, 1)))
+    f = q.random_itn(s, link_space=3, rng=rng)
+    op = q.first_derivative_operator(s)
+    t = f.topo
+    xs = rng.uniform(0, 1, (40, 1))
+    yb = rng.standard_normal(40)
+    C = po.contract(t.parent, t.sdim, op.cores, f.cores)
+    gC = po.eval_grad(t.parent, t.sdim, C, t.vcoord, t.vdigit, t.Lc, xs, yb)
+    ab, _ = po.contract_adjoint(t.parent, t.sdim, op.cores, f.cores, gC, want_w=False)
+    M = dense.dense_operator(t.parent, op.cores)  # vertex order = digit order on this chain
+    k = np.floor(xs[:, 0] * 2 ** L).astype(int)
+    c = M[:, k] @ yb
+    grid = (np.arange(2 ** L) / 2 ** L)[:, None]
+    ref = po.eval_grad(t.parent, t.sdim, f.cores, t.vcoord, t.vdigit, t.Lc, grid, c)
+    num = math.sqrt(sum(float(np.sum((a - b) ** 2)) for a, b in zip(ab, ref)))
+    den = math.sqrt(sum(float(np.sum(b ** 2)) for b in ref))
+    assert num / den < 1e-12
+
+
 def test_contract_matches_dense(q, po, rng):
     s = q.continuous_siteinds(q.named_comb_tree((2, 3)))
     f = q.random_itn(s, link_space=3, rng=rng)
