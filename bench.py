@@ -123,17 +123,29 @@ def main():
     y32 = out32.cpu().numpy()
     err_split = float(np.linalg.norm(y32 - y64) / np.linalg.norm(y64))
 
-    # operate + truncate (device-resident input and output), float64 truncation
-    op_times = []
-    for i in range(args.warmup + args.steps):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        gi = q.operate(w.op, fc, cutoff=w.cutoff, maxdim=w.maxdim)
-        torch.cuda.synchronize()
-        if i >= args.warmup:
-            op_times.append((time.perf_counter() - t0) * 1e3)
-        del gi
-    op_ms = float(np.median(op_times))
+    # operate + truncate (device-resident input and output); the bulk products
+    # (contraction, mode products) in float64, float-float and float32, the QR,
+    # SVD and cutoff always in float64
+    def time_operate(prec):
+        times = []
+        for i in range(args.warmup + args.steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            gi = q.operate(w.op, fc, cutoff=w.cutoff, maxdim=w.maxdim, precision=prec)
+            torch.cuda.synchronize()
+            if i >= args.warmup:
+                times.append((time.perf_counter() - t0) * 1e3)
+        return float(np.median(times)), gi
+
+    op_ms, _ = time_operate(0)
+    mixed_operate = {}
+    for name, prec in (("f32_split", 1), ("f32", 2)):
+        ms, gm = time_operate(prec)
+        ym = q.evaluate(gm, xs_d[:200000]).cpu().numpy()
+        yd = out64[:200000].cpu().numpy()
+        mixed_operate[name] = {"operate_truncate_ms": ms,
+                               "maxnorm_rel_err_vs_fp64": float(np.max(np.abs(ym - yd)) / np.max(np.abs(yd))),
+                               "result_bonds_max": int(max(gm.bonds()))}
 
     pts64 = args.npts * args.steps / wall64
     pts32 = args.npts * args.steps / wall32
@@ -208,6 +220,7 @@ def main():
                                "operate+truncate cutoff 1e-10 maxdim 128, evaluate 1e6 points",
                    "npts": args.npts, "result_bonds_max": max(bonds), "parallelism": "single GPU"},
         "operate_truncate_ms": op_ms,
+        "operate_truncate_mixed": mixed_operate,
         "fp64_points_per_s": pts64,
         "split_points_per_s": pts32,
         "split_rel_err_vs_fp64": err_split,

@@ -280,6 +280,11 @@ int qtt_ctx_create(int device, qtt_ctx** out) {
   // QTT_EVAL_TABLE_BITS: digit sites per environment table of the chain evaluate
   const char* tbits = std::getenv("QTT_EVAL_TABLE_BITS");
   if (tbits) c->eval_tbits = std::max(0, std::min(24, std::atoi(tbits)));
+  // QTT_TRUNC_HELPERS: helper workgroups of a single-network truncation whose
+  // cores exceed LDS (default and at most 31, the exchange buffers' size; 0:
+  // workgroup 0 alone)
+  const char* th = std::getenv("QTT_TRUNC_HELPERS");
+  if (th) c->trunc_helpers = std::min(31, std::max(0, std::atoi(th)));
   // QTT_JACOBI_MAX_SWEEPS: sweeps before an SVD reports QTT_ENOCONV (default 100)
   const char* js = std::getenv("QTT_JACOBI_MAX_SWEEPS");
   if (js) c->jacobi_max_sweeps = std::min(1000, std::max(1, std::atoi(js)));
@@ -547,12 +552,9 @@ static int check_precision(qtt_ctx* ctx, int precision) {
   return QTT_OK;
 }
 
-// operator application: the precisions with an implementation of their own
-static int check_apply_precision(qtt_ctx* ctx, int precision) {
-  if (precision != QTT_F64)
-    return qtt_fail(ctx, QTT_EINVAL, "apply: only QTT_F64 is implemented for operator application");
-  return QTT_OK;
-}
+// operator application: the bulk products (exact contraction, mode products)
+// run in the precision asked for; QR, SVD and the cutoff stay float64
+static int check_apply_precision(qtt_ctx* ctx, int precision) { return check_precision(ctx, precision); }
 
 int qtt_eval(qtt_ctx* ctx, const qtt_net* net, const double* xs, int64_t npts, double* out,
              int precision) {
@@ -621,13 +623,13 @@ int qtt_apply(qtt_ctx* ctx, const qtt_mpo* op, const qtt_net* net, double cutoff
   }
   qtt_net* n = nullptr;
   if ((rc = alloc_net(ctx, t, b, cap, &n))) return rc;
-  if ((rc = launch_contract(ctx, op, net, n))) {
+  if ((rc = launch_contract(ctx, op, net, n, precision))) {
     qtt_net_destroy(n);
     return rc;
   }
   if (trunc_err) *trunc_err = 0.0;
   if (cutoff >= 0.0) {
-    rc = launch_truncate(ctx, n, cutoff, maxdim > 0 ? maxdim : (int64_t(1) << 40), trunc_err);
+    rc = launch_truncate(ctx, n, cutoff, maxdim > 0 ? maxdim : (int64_t(1) << 40), trunc_err, precision);
     if (rc) {
       qtt_net_destroy(n);
       return rc;
@@ -719,11 +721,12 @@ int qtt_apply_batch(qtt_ctx* ctx, int n, const qtt_mpo* const* ops, const qtt_ne
       m->d_bond = (int*)(d + t.nv * sizeof(int64_t) + ti.size() * sizeof(int));
     }
     rc = qtt_hip_check(ctx, hipMemcpyAsync(mpool, meta.data(), mtot, hipMemcpyHostToDevice, ctx->stream), "batch metadata");
-    if (!rc) rc = launch_contract_batch(ctx, n, ops, nets, made.data());
+    if (!rc) rc = launch_contract_batch(ctx, n, ops, nets, made.data(), precision);
     if (!rc) rc = qtt_hip_check(ctx, hipStreamSynchronize(ctx->stream), "batch contract sync");  // meta outlives the copy
   }
   if (!rc && cutoff >= 0.0)
-    rc = launch_truncate_batch(ctx, made.data(), n, cutoff, maxdim > 0 ? maxdim : (int64_t(1) << 40), errs);
+    rc = launch_truncate_batch(ctx, made.data(), n, cutoff, maxdim > 0 ? maxdim : (int64_t(1) << 40), errs,
+                               precision);
   else if (!rc && errs)
     for (int i = 0; i < n; i++) errs[i] = 0.0;
   if (rc) {
@@ -763,8 +766,8 @@ int qtt_tune_eval(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t npts, do
 int qtt_truncate(qtt_ctx* ctx, qtt_net* net, double cutoff, int64_t maxdim, double* trunc_err) {
   if (!ctx || !net) return qtt_fail(ctx, QTT_EINVAL, "null argument");
   if (maxdim < 0) return qtt_fail(ctx, QTT_EINVAL, "maxdim < 0");
-  return launch_truncate(ctx, net, cutoff < 0 ? 0.0 : cutoff, maxdim > 0 ? maxdim : (int64_t(1) << 40),
-                         trunc_err);
+  return launch_truncate(ctx, net, cutoff < 0 ? 0.0 : cutoff, maxdim > 0 ? maxdim : (int64_t(1) << 40), trunc_err,
+                         QTT_F64);
 }
 
 int qtt_eval_grad(qtt_ctx* ctx, const qtt_net* net, const double* xs, const double* ybar,

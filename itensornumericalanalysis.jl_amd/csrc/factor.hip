@@ -131,8 +131,47 @@ __device__ inline double block_sum(double v, double* red) {
 // ---------------------------------------------------------------------------
 // exact operator application
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Arithmetic of the bulk products (the exact contraction and the truncation's
+// mode products) for the three precisions of qtt_apply:
+//   QTT_F64        float64;
+//   QTT_F32_SPLIT  float-float: operands split into (hi, lo) floats, products
+//                  and sums by error-free transformations (Dekker / Knuth),
+//                  about 48 bits;
+//   QTT_F32        float32 operands and accumulation.
+// The QR, the SVD and the cutoff decision stay float64 in all three.
+// ---------------------------------------------------------------------------
+struct ffv {
+  float h, l;
+};
+
+__device__ __forceinline__ ffv ff_split(double a) {
+  const float h = (float)a;
+  return {h, (float)(a - (double)h)};
+}
+
+// c += a * b (dropping a.l * b.l); every + and * rounded on its own, since
+// contraction into FMA would lose the low words
+__device__ __forceinline__ void ff_madd(ffv& c, ffv a, ffv b) {
+#pragma clang fp contract(off)
+  const float p = a.h * b.h;
+  float pe = __builtin_fmaf(a.h, b.h, -p);
+  pe = __builtin_fmaf(a.h, b.l, pe);
+  pe = __builtin_fmaf(a.l, b.h, pe);
+  const float s = c.h + p;
+  const float bb = s - c.h;
+  float e = (c.h - (s - bb)) + (p - bb);
+  e = e + c.l + pe;
+  const float h = s + e;
+  c.l = e - (h - s);
+  c.h = h;
+}
+
+__device__ __forceinline__ double ff_value(ffv c) { return (double)c.h + (double)c.l; }
+
 // element e (< total) of the contracted network: vertex by binary search over
 // cstart, then the flattened indices of the output core
+template <int MODE>
 __device__ __forceinline__ void contract_elem(const double* __restrict__ W, const int64_t* __restrict__ woff,
                                               const int* __restrict__ wbond, const double* __restrict__ A,
                                               const int64_t* __restrict__ aoff, const int* __restrict__ abond,
@@ -163,12 +202,37 @@ __device__ __forceinline__ void contract_elem(const double* __restrict__ W, cons
   local /= d;
   const int64_t ap = abond[v];
   const int64_t pw = local / ap, pa = local % ap;
-  double s = 0.0;
-  for (int si = 0; si < d; ++si)
-    s += W[woff[v] + ((pw * d + si) * d + so) * sw + jw] * A[aoff[v] + (pa * d + si) * sa + ja];
-  C[coff[v] + (e - cstart[v])] = s;
+  double out;
+  if (MODE == QTT_F64) {
+    double s = 0.0;
+    for (int si = 0; si < d; ++si)
+      s += W[woff[v] + ((pw * d + si) * d + so) * sw + jw] * A[aoff[v] + (pa * d + si) * sa + ja];
+    out = s;
+  } else {
+    // each product of float words on operands scaled by powers of two to
+    // near 1 (exact), scaled back and summed over the d site values in float64
+    double s = 0.0;
+    for (int si = 0; si < d; ++si) {
+      const double wv = W[woff[v] + ((pw * d + si) * d + so) * sw + jw];
+      const double av = A[aoff[v] + (pa * d + si) * sa + ja];
+      if (wv == 0.0 || av == 0.0) continue;
+      const int ew = ilogb(wv), ea = ilogb(av);
+      double p;
+      if (MODE == QTT_F32_SPLIT) {
+        ffv t = {0.0f, 0.0f};
+        ff_madd(t, ff_split(ldexp(wv, -ew)), ff_split(ldexp(av, -ea)));
+        p = ff_value(t);
+      } else {
+        p = (double)((float)ldexp(wv, -ew) * (float)ldexp(av, -ea));
+      }
+      s += ldexp(p, ew + ea);
+    }
+    out = s;
+  }
+  C[coff[v] + (e - cstart[v])] = out;
 }
 
+template <int MODE>
 __global__ void __launch_bounds__(256)
 contract_kernel(const double* __restrict__ W, const int64_t* __restrict__ woff, const int* __restrict__ wbond,
                 const double* __restrict__ A, const int64_t* __restrict__ aoff, const int* __restrict__ abond,
@@ -176,7 +240,7 @@ contract_kernel(const double* __restrict__ W, const int64_t* __restrict__ woff, 
                 const int64_t* __restrict__ cstart, int64_t total) {
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x)
-    contract_elem(W, woff, wbond, A, aoff, abond, t, C, coff, cstart, e);
+    contract_elem<MODE>(W, woff, wbond, A, aoff, abond, t, C, coff, cstart, e);
 }
 
 // A batch of contractions in one launch: the elements of all networks form one
@@ -194,6 +258,7 @@ struct ContractItem {
   const int64_t* cstart;
 };
 
+template <int MODE>
 __global__ void __launch_bounds__(256)
 contract_batch_kernel(const ContractItem* __restrict__ items, const int64_t* __restrict__ estart, int n,
                       int64_t total) {
@@ -205,7 +270,8 @@ contract_batch_kernel(const ContractItem* __restrict__ items, const int64_t* __r
       if (estart[mid] <= e) lo = mid; else hi = mid;
     }
     const ContractItem& it = items[lo];
-    contract_elem(it.W, it.woff, it.wbond, it.A, it.aoff, it.abond, it.t, it.C, it.coff, it.cstart, e - estart[lo]);
+    contract_elem<MODE>(it.W, it.woff, it.wbond, it.A, it.aoff, it.abond, it.t, it.C, it.coff, it.cstart,
+                        e - estart[lo]);
   }
 }
 
@@ -222,6 +288,7 @@ struct SweepArgs {
   double cutoff;
   long long maxdim;
   int max_sweeps;  // Jacobi sweeps before an SVD is reported as not converged
+  int prec;        // arithmetic of the mode products (QTT_F64, QTT_F32_SPLIT, QTT_F32)
   double* gscr;  // global workspace (used when a matrix does not fit LDS)
   double* gY;    // global buffer for the mode-product result
   int ws_lds;    // doubles of LDS workspace
@@ -263,6 +330,7 @@ struct CoopJob {
   double* D;
   long long ytot;
   int use_perm, m, rb, Au, nu, Bu;
+  int prec;  // arithmetic of the mode product (QTT_F64, QTT_F32_SPLIT, QTT_F32)
   // type 4: the first sums of a tall panel (X = the panel, ldx, n = its rows,
   // nb columns); type 5: pass t of a tall panel (tol = the reflector's scale,
   // small = its product with column t+1, dcol per column); both leave 16 sums
@@ -1576,15 +1644,18 @@ __device__ inline void row_norms(const double* X, int n, int b, double* sig) {
 // Mode product Y[(aa*m + i)*Bu + bb] = sum_c R[row_i][c] U[(aa*nu + c)*Bu + bb]
 // (row_i = perm[i] or i).  One task = one (aa, bb) column and 8 output rows:
 // u is read once per 8 rows, eight of its entries in flight per wait.
-template <typename RP, typename IP>
+// MODE: QTT_F64, QTT_F32_SPLIT or QTT_F32 (see ff_madd); output columns
+// j = aa * Bu + bb in [j0, j1).
+template <int MODE, typename RP, typename IP>
 __device__ __forceinline__ void mode_product_body(RP Rrows, IP perm, bool use_perm, int m, int b,
                                                   const double* __restrict__ Uc, double* __restrict__ gY, int Au,
-                                                  int nu, int Bu) {
+                                                  int nu, int Bu, int j0, int j1) {
   const int tid = threadIdx.x, bs = blockDim.x;
-  const int ncol = Au * Bu;
+  const int ncol = j1 - j0;
   const int nib = (m + 7) >> 3;
+  (void)Au;
   for (int task = tid; task < ncol * nib; task += bs) {
-    const int j = task % ncol, ib = task / ncol;
+    const int j = j0 + task % ncol, ib = task / ncol;
     const int aa = j / Bu, bb = j - aa * Bu;
     RP rr[8];
 #pragma unroll
@@ -1592,40 +1663,158 @@ __device__ __forceinline__ void mode_product_body(RP Rrows, IP perm, bool use_pe
       const int i = min(ib * 8 + q, m - 1);
       rr[q] = Rrows + (use_perm ? perm[i] : i) * b;
     }
-    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const double* up = Uc + (int64_t)aa * nu * Bu + bb;
-    int c = 0;
-    for (; c + 8 <= nu; c += 8) {
-      double uv[8];
+    double res[8];
+    if constexpr (MODE == QTT_F64) {
+      double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      int c = 0;
+      for (; c + 8 <= nu; c += 8) {
+        double uv[8];
 #pragma unroll
-      for (int q2 = 0; q2 < 8; ++q2) uv[q2] = up[(int64_t)(c + q2) * Bu];
+        for (int q2 = 0; q2 < 8; ++q2) uv[q2] = up[(int64_t)(c + q2) * Bu];
 #pragma unroll
-      for (int q2 = 0; q2 < 8; ++q2)
+        for (int q2 = 0; q2 < 8; ++q2)
 #pragma unroll
-        for (int q = 0; q < 8; ++q) acc[q] = __builtin_fma(rr[q][c + q2], uv[q2], acc[q]);
-    }
-    for (; c < nu; ++c) {
-      const double uv = up[(int64_t)c * Bu];
+          for (int q = 0; q < 8; ++q) acc[q] = __builtin_fma(rr[q][c + q2], uv[q2], acc[q]);
+      }
+      for (; c < nu; ++c) {
+        const double uv = up[(int64_t)c * Bu];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) acc[q] = __builtin_fma(rr[q][c], uv, acc[q]);
+        for (int q = 0; q < 8; ++q) acc[q] = __builtin_fma(rr[q][c], uv, acc[q]);
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) res[q] = acc[q];
+    } else {
+      // Float words cover a far narrower exponent range than doubles (a
+      // network's norm can sit near 1e-33): the R rows and the u column are
+      // scaled by powers of two (exact) to a maximum near 1 first, and the
+      // result scaled back in float64.
+      int eu = 0, er[8];
+      {
+        double mx = 0.0;
+        for (int c = 0; c < nu; ++c) mx = fmax(mx, fabs(up[(int64_t)c * Bu]));
+        eu = mx > 0.0 ? ilogb(mx) : 0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          double mr = 0.0;
+          for (int c = 0; c < nu; ++c) mr = fmax(mr, fabs((double)rr[q][c]));
+          er[q] = mr > 0.0 ? ilogb(mr) : 0;
+        }
+      }
+      if constexpr (MODE == QTT_F32_SPLIT) {
+        ffv acc[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc[q] = {0.0f, 0.0f};
+        for (int c = 0; c < nu; ++c) {
+          const ffv uv = ff_split(ldexp(up[(int64_t)c * Bu], -eu));
+#pragma unroll
+          for (int q = 0; q < 8; ++q) ff_madd(acc[q], ff_split(ldexp((double)rr[q][c], -er[q])), uv);
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) res[q] = ldexp(ff_value(acc[q]), er[q] + eu);
+      } else {
+        float acc[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        for (int c = 0; c < nu; ++c) {
+          const float uv = (float)ldexp(up[(int64_t)c * Bu], -eu);
+#pragma unroll
+          for (int q = 0; q < 8; ++q) acc[q] = __builtin_fmaf((float)ldexp((double)rr[q][c], -er[q]), uv, acc[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) res[q] = ldexp((double)acc[q], er[q] + eu);
+      }
     }
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const int i = ib * 8 + q;
-      if (i < m) gY[((int64_t)aa * m + i) * Bu + bb] = acc[q];
+      if (i < m) gY[((int64_t)aa * m + i) * Bu + bb] = res[q];
     }
   }
 }
 
 // LDS-resident R
+template <int MODE>
 __device__ __forceinline__ void mode_product_lds(const lds_f64* Rrows, const lds_i32* perm, bool use_perm, int m, int b,
                                               const double* Uc, double* gY, int Au, int nu, int Bu) {
-  mode_product_body(Rrows, perm, use_perm, m, b, Uc, gY, Au, nu, Bu);
+  mode_product_body<MODE>(Rrows, perm, use_perm, m, b, Uc, gY, Au, nu, Bu, 0, Au * Bu);
 }
 
+template <int MODE>
 __device__ __forceinline__ void mode_product(const double* Rrows, const int* perm, bool use_perm, int m, int b,
                                              const double* Uc, double* gY, int Au, int nu, int Bu) {
-  mode_product_body(Rrows, perm, use_perm, m, b, Uc, gY, Au, nu, Bu);
+  mode_product_body<MODE>(Rrows, perm, use_perm, m, b, Uc, gY, Au, nu, Bu, 0, Au * Bu);
+}
+
+// The mode product in float32 on MFMA tiles, v_mfma_f32_16x16x4f32: the same
+// units as mode_product_mfma below (32 rows by 16 columns per wavefront);
+// operands rounded to float, float accumulation.  D element e of lane =
+// row 4 lk + e, column li.
+__device__ __noinline__ void mode_product_mfma32(const double* Rrows, const int* perm, bool use_perm, int m_in,
+                                                 int b_in, const double* Uc, double* gY, int nu_in, int Bu_in,
+                                                 int j0_in, int j1_in) {
+  typedef float f32x4 __attribute__((ext_vector_type(4)));
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int m = uni(m_in), b = uni(b_in), nu = uni(nu_in), Bu = uni(Bu_in), j0 = uni(j0_in), j1 = uni(j1_in);
+  const int nit = (m + 31) >> 5, njt = (j1 - j0 + 15) >> 4;
+  for (int u = wv; u < nit * njt; u += nwv) {
+    const int it = u / njt, jt = u - it * njt;
+    const int i0 = it * 32, jc = j0 + jt * 16 + li;
+    const bool jon = jc < j1;
+    const int aa = jon ? jc / Bu : 0, bb = jon ? jc - aa * Bu : 0;
+    const double* up = Uc + (int64_t)aa * nu * Bu + bb;
+    const int ra = i0 + li, rb = i0 + 16 + li;
+    const bool aon = ra < m, bon = rb < m;
+    const double* rpa = Rrows + (int64_t)(aon ? (use_perm ? perm[ra] : ra) : 0) * b;
+    const double* rpb = Rrows + (int64_t)(bon ? (use_perm ? perm[rb] : rb) : 0) * b;
+    // powers of two bringing each R row and u column to a maximum near 1 (the
+    // float range is far narrower than the double one); the four lanes that
+    // share a row / column (lk = 0..3) each scan a quarter of it
+    double mu = 0.0, ma = 0.0, mb = 0.0;
+    for (int c = lk; c < nu; c += 4) {
+      if (jon) mu = fmax(mu, fabs(up[(int64_t)c * Bu]));
+      if (aon) ma = fmax(ma, fabs(rpa[c]));
+      if (bon) mb = fmax(mb, fabs(rpb[c]));
+    }
+    mu = fmax(mu, __shfl_xor(mu, 16));
+    mu = fmax(mu, __shfl_xor(mu, 32));
+    ma = fmax(ma, __shfl_xor(ma, 16));
+    ma = fmax(ma, __shfl_xor(ma, 32));
+    mb = fmax(mb, __shfl_xor(mb, 16));
+    mb = fmax(mb, __shfl_xor(mb, 32));
+    const int eu = mu > 0.0 ? ilogb(mu) : 0, ea = ma > 0.0 ? ilogb(ma) : 0, eb = mb > 0.0 ? ilogb(mb) : 0;
+    f32x4 da = {0.0f, 0.0f, 0.0f, 0.0f}, db = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int c0 = 0; c0 < nu; c0 += 32) {
+      float bv[8], av[8], aw[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int c = c0 + 4 * q + lk;
+        const bool con = c < nu;
+        bv[q] = (jon && con) ? (float)ldexp(up[(int64_t)c * Bu], -eu) : 0.0f;
+        av[q] = (aon && con) ? (float)ldexp(rpa[c], -ea) : 0.0f;
+        aw[q] = (bon && con) ? (float)ldexp(rpb[c], -eb) : 0.0f;
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        da = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q], bv[q], da, 0, 0, 0);
+        db = __builtin_amdgcn_mfma_f32_16x16x4f32(aw[q], bv[q], db, 0, 0, 0);
+      }
+    }
+    // the row exponents of this lane's output rows 4 lk + e live in lanes 4 lk + e
+    int era[4], erb[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      era[e] = __shfl(ea, 4 * lk + e);
+      erb[e] = __shfl(eb, 4 * lk + e);
+    }
+    if (jon) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int i = i0 + 4 * lk + e;
+        if (i < m) gY[((int64_t)aa * m + i) * Bu + bb] = ldexp((double)da[e], era[e] + eu);
+        if (i + 16 < m) gY[((int64_t)aa * m + i + 16) * Bu + bb] = ldexp((double)db[e], erb[e] + eu);
+      }
+    }
+  }
 }
 
 // Mode product with R in global memory (the global-workspace path) as FP64
@@ -1840,9 +2029,17 @@ __device__ void coop_work(CoopJob* job, int wg, int nwg, lds_f64* lw) {
     const int ncol = job->Au * job->Bu;
     const int chunk = ((ncol + nwg - 1) / nwg + 63) & ~63;
     const int j0 = min(ncol, wg * chunk), j1 = min(ncol, j0 + chunk);
-    if (j0 < j1)
-      mode_product_mfma(job->R, job->perm, job->use_perm != 0, job->m, job->rb, job->U, job->Y, job->Au, job->nu,
-                        job->Bu, j0, j1);
+    if (j0 < j1) {
+      if (job->prec == QTT_F32)
+        mode_product_mfma32(job->R, job->perm, job->use_perm != 0, job->m, job->rb, job->U, job->Y, job->nu,
+                            job->Bu, j0, j1);
+      else if (job->prec == QTT_F32_SPLIT)
+        mode_product_body<QTT_F32_SPLIT>(job->R, job->perm, job->use_perm != 0, job->m, job->rb, job->U, job->Y,
+                                         job->Au, job->nu, job->Bu, j0, j1);
+      else
+        mode_product_mfma(job->R, job->perm, job->use_perm != 0, job->m, job->rb, job->U, job->Y, job->Au, job->nu,
+                          job->Bu, j0, j1);
+    }
   } else if (type == 3) {
     const long long tot = job->ytot;
     const long long chunk = ((tot + nwg - 1) / nwg + 63) & ~63LL;
@@ -2036,9 +2233,10 @@ __device__ bool jacobi_coop(double* X, int n, int b, double* J, CoopJob* job, in
 // new layout overlaps columns of other workgroups).
 __device__ bool mode_product_coop(const double* Rrows, const int* perm, bool use_perm, int m, int b, const double* Uc,
                                   double* gY, double* Ud, int Au, int nu, int Bu, CoopJob* job, int nwg, int* iflag,
-                                  lds_f64* lw) {
+                                  lds_f64* lw, int prec) {
   if (threadIdx.x == 0) {
     job->type = 2;
+    job->prec = prec;
     job->R = Rrows;
     job->perm = perm;
     job->use_perm = use_perm ? 1 : 0;
@@ -2067,7 +2265,7 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
                                         lds_f64* pnl, const int pnl_cap, const int pmode, const int rows,
                                         const int b, const int tr, const int rpl, const int bx,
                                         const int64_t Au, const int64_t nu, const int64_t Bu, double* Xc,
-                                        const double* Uc, double* Ud, double* gY, double cutoff, long long maxdim, int max_sweeps,
+                                        const double* Uc, double* Ud, double* gY, double cutoff, long long maxdim, int max_sweeps, int prec,
                                         int* status, int* iflag, double* red, double* dbc) {
   const int tid = threadIdx.x, bs = blockDim.x;
   const int k = rows < b ? rows : b;
@@ -2309,16 +2507,32 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
   const int ytot = (int)(Au * m * Bu);
   if (!REG && gws && job && nwg > 1 && Au * Bu * nu * (int64_t)m >= (int64_t(1) << 22)) {
     // a large product in global memory: columns spread over the helper workgroups
-    if (!mode_product_coop(Rrows, perm, use_perm, m, b, Uc, gY, Ud, (int)Au, (int)nu, (int)Bu, job, nwg, iflag, pnl)) {
+    if (!mode_product_coop(Rrows, perm, use_perm, m, b, Uc, gY, Ud, (int)Au, (int)nu, (int)Bu, job, nwg, iflag, pnl,
+                           prec)) {
       if (tid == 0) status[1] = 1;
     }
   } else {
-    if (REG)
-      mode_product_lds((const lds_f64*)Rrows, (const lds_i32*)perm, use_perm, m, b, Uc, gY, (int)Au, (int)nu, (int)Bu);
-    else if (gws)  // global workspace: MFMA tiles
-      mode_product_mfma(Rrows, perm, use_perm, m, b, Uc, gY, (int)Au, (int)nu, (int)Bu, 0, (int)(Au * Bu));
-    else
-      mode_product(Rrows, perm, use_perm, m, b, Uc, gY, (int)Au, (int)nu, (int)Bu);
+    // the bulk product in the precision asked for (QR, SVD and cutoff stay float64)
+    if (REG) {
+      const lds_f64* Rl = (const lds_f64*)Rrows;
+      const lds_i32* pl = (const lds_i32*)perm;
+      if (prec == QTT_F32) mode_product_lds<QTT_F32>(Rl, pl, use_perm, m, b, Uc, gY, (int)Au, (int)nu, (int)Bu);
+      else if (prec == QTT_F32_SPLIT)
+        mode_product_lds<QTT_F32_SPLIT>(Rl, pl, use_perm, m, b, Uc, gY, (int)Au, (int)nu, (int)Bu);
+      else mode_product_lds<QTT_F64>(Rl, pl, use_perm, m, b, Uc, gY, (int)Au, (int)nu, (int)Bu);
+    } else if (gws) {  // global workspace: MFMA tiles (float-float on the VALU)
+      if (prec == QTT_F32)
+        mode_product_mfma32(Rrows, perm, use_perm, m, b, Uc, gY, (int)nu, (int)Bu, 0, (int)(Au * Bu));
+      else if (prec == QTT_F32_SPLIT)
+        mode_product<QTT_F32_SPLIT>(Rrows, perm, use_perm, m, b, Uc, gY, (int)Au, (int)nu, (int)Bu);
+      else
+        mode_product_mfma(Rrows, perm, use_perm, m, b, Uc, gY, (int)Au, (int)nu, (int)Bu, 0, (int)(Au * Bu));
+    } else {
+      if (prec == QTT_F32) mode_product<QTT_F32>(Rrows, perm, use_perm, m, b, Uc, gY, (int)Au, (int)nu, (int)Bu);
+      else if (prec == QTT_F32_SPLIT)
+        mode_product<QTT_F32_SPLIT>(Rrows, perm, use_perm, m, b, Uc, gY, (int)Au, (int)nu, (int)Bu);
+      else mode_product<QTT_F64>(Rrows, perm, use_perm, m, b, Uc, gY, (int)Au, (int)nu, (int)Bu);
+    }
     __syncthreads();
     for (int e = tid; e < ytot; e += bs) Ud[e] = gY[e];
   }
@@ -2404,17 +2618,17 @@ __global__ void __launch_bounds__(512) truncate_kernel(const SweepArgs* __restri
     int m;
     if (need <= a.ws_lds) {
       if (small)
-        m = run_move<8, true>(wsl, false, nullptr, 1, jpad, nullptr, 0, 0, rows, b, tr, rpl, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.max_sweeps, a.status,
+        m = run_move<8, true>(wsl, false, nullptr, 1, jpad, nullptr, 0, 0, rows, b, tr, rpl, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.max_sweeps, a.prec, a.status,
                         iflag, red, dbc);
       else
-        m = run_move<64, false>(wsl, false, nullptr, 1, nullptr, nullptr, 0, 0, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.max_sweeps, a.status,
+        m = run_move<64, false>(wsl, false, nullptr, 1, nullptr, nullptr, 0, 0, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.max_sweeps, a.prec, a.status,
                          iflag, red, dbc);
     } else {
       if (small)
-        m = run_move<8, false>(a.gscr, true, a.job, a.nwg, nullptr, pnl, pnl_cap, pmode, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.max_sweeps, a.status,
+        m = run_move<8, false>(a.gscr, true, a.job, a.nwg, nullptr, pnl, pnl_cap, pmode, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.max_sweeps, a.prec, a.status,
                         iflag, red, dbc);
       else
-        m = run_move<64, false>(a.gscr, true, a.job, a.nwg, nullptr, pnl, pnl_cap, pmode, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.max_sweeps, a.status,
+        m = run_move<64, false>(a.gscr, true, a.job, a.nwg, nullptr, pnl, pnl_cap, pmode, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.max_sweeps, a.prec, a.status,
                          iflag, red, dbc);
     }
     QTT_PHASE(st, 5);
@@ -2430,7 +2644,7 @@ __global__ void __launch_bounds__(512) truncate_kernel(const SweepArgs* __restri
 
 }  // namespace
 
-int launch_contract(qtt_ctx* ctx, const qtt_mpo* op, const qtt_net* net, qtt_net* out) {
+int launch_contract(qtt_ctx* ctx, const qtt_mpo* op, const qtt_net* net, qtt_net* out, int precision) {
   const HostTopo& t = net->t;
   std::vector<int64_t> cstart(t.nv + 1, 0);
   for (int v = 0; v < t.nv; v++) cstart[v + 1] = cstart[v] + out->cap[v];
@@ -2441,16 +2655,22 @@ int launch_contract(qtt_ctx* ctx, const qtt_mpo* op, const qtt_net* net, qtt_net
   if (rc) return rc;
   const int64_t total = cstart[t.nv];
   int64_t nb = std::min<int64_t>((total + 255) / 256, 65536);
-  contract_kernel<<<dim3((unsigned)std::max<int64_t>(nb, 1)), 256, 0, ctx->stream>>>(
-      op->d_data, op->d_off, op->d_bond, net->d_data, net->d_off, net->d_bond, net->view(), out->d_data,
-      out->d_off, d_cstart, total);
+  const dim3 grid((unsigned)std::max<int64_t>(nb, 1));
+#define QTT_CONTRACT_LAUNCH(M)                                                                              \
+  contract_kernel<M><<<grid, 256, 0, ctx->stream>>>(op->d_data, op->d_off, op->d_bond, net->d_data,        \
+                                                    net->d_off, net->d_bond, net->view(), out->d_data, out->d_off, \
+                                                    d_cstart, total)
+  if (precision == QTT_F32) QTT_CONTRACT_LAUNCH(QTT_F32);
+  else if (precision == QTT_F32_SPLIT) QTT_CONTRACT_LAUNCH(QTT_F32_SPLIT);
+  else QTT_CONTRACT_LAUNCH(QTT_F64);
+#undef QTT_CONTRACT_LAUNCH
   rc = qtt_hip_check(ctx, hipGetLastError(), "contract_kernel");
   if (rc) return rc;
   return qtt_hip_check(ctx, hipStreamSynchronize(ctx->stream), "contract sync");
 }
 
 int launch_contract_batch(qtt_ctx* ctx, int n, const qtt_mpo* const* ops, const qtt_net* const* nets,
-                          qtt_net* const* outs) {
+                          qtt_net* const* outs, int precision) {
   if (n <= 0) return QTT_OK;
   // scratch: items | estart (n + 1) | cstart of every network (nv + 1 each)
   std::vector<int64_t> estart(n + 1, 0), cbase(n);
@@ -2484,8 +2704,13 @@ int launch_contract_batch(qtt_ctx* ctx, int n, const qtt_mpo* const* ops, const 
   if (rc) return rc;
   const int64_t total = estart[n];
   const int64_t nb = std::min<int64_t>((total + 255) / 256, 65536);
-  contract_batch_kernel<<<dim3((unsigned)std::max<int64_t>(nb, 1)), 256, 0, ctx->stream>>>(
-      (const ContractItem*)scr, d_es, n, total);
+  const dim3 grid((unsigned)std::max<int64_t>(nb, 1));
+  if (precision == QTT_F32)
+    contract_batch_kernel<QTT_F32><<<grid, 256, 0, ctx->stream>>>((const ContractItem*)scr, d_es, n, total);
+  else if (precision == QTT_F32_SPLIT)
+    contract_batch_kernel<QTT_F32_SPLIT><<<grid, 256, 0, ctx->stream>>>((const ContractItem*)scr, d_es, n, total);
+  else
+    contract_batch_kernel<QTT_F64><<<grid, 256, 0, ctx->stream>>>((const ContractItem*)scr, d_es, n, total);
   rc = qtt_hip_check(ctx, hipGetLastError(), "contract_batch_kernel");
   if (rc) return rc;
   // the host tables must outlive the copy
@@ -2494,7 +2719,7 @@ int launch_contract_batch(qtt_ctx* ctx, int n, const qtt_mpo* const* ops, const 
 
 // Truncate a batch of networks, one workgroup each, in one launch.
 int launch_truncate_batch(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double cutoff, int64_t maxdim,
-                          double* errs) {
+                          double* errs, int precision) {
   if (nnet <= 0) return QTT_OK;
   std::vector<std::vector<int>> steps(nnet);
   std::vector<int64_t> ws(nnet), soff(nnet);
@@ -2528,7 +2753,7 @@ int launch_truncate_batch(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double c
   if (nnet == 1) {
     int64_t maxcap = 1;
     for (int v = 0; v < nets[0]->t.nv; v++) maxcap = std::max(maxcap, nets[0]->cap[v]);
-    if (maxcap > QTT_LDS_DOUBLES) nwg = std::max(1, std::min(32, ctx->cus));
+    if (maxcap > QTT_LDS_DOUBLES) nwg = std::max(1, std::min(ctx->trunc_helpers + 1, ctx->cus));
   }
   // the tail slack covers the whole-block row reads of the Jacobi rounds (< 512 doubles past a row)
   // helpers' exchange buffers after the job: Gp [32][16], Pg [512][256], Ug [8][256]
@@ -2567,6 +2792,7 @@ int launch_truncate_batch(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double c
     sa.cutoff = cutoff;
     sa.maxdim = (long long)maxdim;
     sa.max_sweeps = ctx->jacobi_max_sweeps;
+    sa.prec = precision;
     sa.gscr = w;
     int64_t maxcap = 1;
     for (int v = 0; v < net->t.nv; v++) maxcap = std::max(maxcap, net->cap[v]);
@@ -2651,11 +2877,11 @@ int launch_truncate_batch(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double c
   return QTT_OK;
 }
 
-int launch_truncate(qtt_ctx* ctx, qtt_net* net, double cutoff, int64_t maxdim, double* err) {
+int launch_truncate(qtt_ctx* ctx, qtt_net* net, double cutoff, int64_t maxdim, double* err, int precision) {
   if (err) *err = 0.0;
   if (net->t.nv < 2) return QTT_OK;
   double e = 0.0;
-  int rc = launch_truncate_batch(ctx, &net, 1, cutoff, maxdim, &e);
+  int rc = launch_truncate_batch(ctx, &net, 1, cutoff, maxdim, &e, precision);
   if (err) *err = e;
   return rc;
 }

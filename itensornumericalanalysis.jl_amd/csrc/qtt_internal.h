@@ -40,6 +40,7 @@ struct qtt_ctx {
   bool force_lds_eval = false;  // QTT_EVAL_KERNEL=lds selects the LDS-vector chain kernel
   int eval_ppl = 0;             // points per lane of the register chain kernel (0: by bond size)
   int jacobi_max_sweeps = 100;  // Jacobi sweeps before an SVD reports QTT_ENOCONV
+  int trunc_helpers = 31;       // helper workgroups of a single-network truncation (QTT_TRUNC_HELPERS)
   // launch shapes of the register chain kernel picked by qtt_tune_eval, keyed
   // by eval_tune_key (bond class, point-count octave); absent keys use the
   // built-in rule
@@ -136,12 +137,13 @@ int launch_eval_noalloc(qtt_ctx* ctx, const qtt_net* net, const double* xs, int6
 int launch_integrate(qtt_ctx* ctx, const qtt_net* net, double weight, double* out_host);
 int launch_inner(qtt_ctx* ctx, const qtt_net* a, const qtt_net* b, double* out_host);
 int launch_integrate_batch(qtt_ctx* ctx, int n, const qtt_net* const* nets, double weight, double* out_host);
-int launch_contract(qtt_ctx* ctx, const qtt_mpo* op, const qtt_net* net, qtt_net* out);
+// precision: the arithmetic of the bulk products (QTT_F64, QTT_F32_SPLIT, QTT_F32)
+int launch_contract(qtt_ctx* ctx, const qtt_mpo* op, const qtt_net* net, qtt_net* out, int precision);
 int launch_contract_batch(qtt_ctx* ctx, int n, const qtt_mpo* const* ops, const qtt_net* const* nets,
-                          qtt_net* const* outs);
-int launch_truncate(qtt_ctx* ctx, qtt_net* net, double cutoff, int64_t maxdim, double* err);
+                          qtt_net* const* outs, int precision);
+int launch_truncate(qtt_ctx* ctx, qtt_net* net, double cutoff, int64_t maxdim, double* err, int precision);
 int launch_truncate_batch(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double cutoff, int64_t maxdim,
-                          double* errs);
+                          double* errs, int precision);
 int launch_grad(qtt_ctx* ctx, const qtt_net* net, const double* xs, const double* ybar,
                 int64_t npts, double* grad, int precision);
 int launch_apply_grad(qtt_ctx* ctx, const qtt_mpo* op, const qtt_net* net, const double* gbar, double* fbar,
