@@ -172,6 +172,27 @@ int qtt_integrate_batch(qtt_ctx* ctx, int n, const qtt_net* const* nets, double 
  * argument tables copied); the values are ready after qtt_sync. */
 int qtt_eval_batch(qtt_ctx* ctx, int n, const qtt_net* const* nets, const double* const* xs,
                    const int64_t* npts, double* const* out, int precision);
+/* One step of a batch of kernel-operator solves, the loop body of
+ * examples/fredholm_solver.jl:52-63 for n functions at once:
+ *   outs[i] = truncate(contract(ops[i], nets[i]); cutoff, maxdim)   (operate)
+ *   integrals[i] = integrate(outs[i]), weight per digit as qtt_integrate
+ *   vals[i][p] = outs[i](xs[i][p])                                    (evaluate)
+ * Every stage is one batched launch; the stages follow each other on the
+ * stream with no host synchronisation (bond-dependent evaluate layouts are
+ * computed by a kernel) and are captured into one HIP graph unless the
+ * context's stream cannot be captured, a single network needs helper
+ * workgroups for its truncation (a cooperative launch), or QTT_BATCH_GRAPH=0.
+ * Bonds, truncation errors and integrals are copied back once, after the
+ * graph.  cutoff < 0: no truncation (errs 0).  integrals (host [n]) and the
+ * evaluate arguments (xs, npts, vals: device points and values as in
+ * qtt_eval_batch) are nullable.  Evaluation is float64; chains whose final
+ * bonds exceed 32, trees, and more than 4 coordinates are evaluated after the
+ * graph by the single-network path.  precision: as qtt_apply.  Synchronous:
+ * returns when everything is done; invalid points report QTT_ERANGE (the
+ * outputs are then destroyed). */
+int qtt_batch_run(qtt_ctx* ctx, int n, const qtt_mpo* const* ops, const qtt_net* const* nets, double cutoff,
+                  int64_t maxdim, int precision, double weight, const double* const* xs, const int64_t* npts,
+                  double* const* vals, qtt_net** outs, double* errs, double* integrals);
 
 /* ---- evaluation without allocation (no reference counterpart) -------------
  * For callers that cannot allocate device memory: code under HIP graph

@@ -17,7 +17,8 @@ from .operators import (add_operators, backward_shift_op, first_derivative_opera
                         identity_operator, laplacian_operator, second_derivative_operator, stencil,
                         third_derivative_operator)
 from .api import (Precision, apply_grad, engine_context, eval_grad, evaluate, evaluate_batch, inner, integrate,
-                  integrate_batch, on_stream, operate, operate_batch, truncate, tune_evaluate)
+                  integrate_batch, on_stream, operate, operate_batch, operate_batch_pipeline, truncate,
+                  tune_evaluate)
 
 real_continuous_siteinds = continuous_siteinds
 
@@ -30,5 +31,5 @@ __all__ = [
     "second_derivative_operator", "third_derivative_operator", "fourth_derivative_operator",
     "laplacian_operator", "identity_operator", "fredholm_operator", "add_operators", "evaluate",
     "integrate", "inner", "operate", "truncate", "eval_grad", "Precision", "engine_context",
-    "operate_batch", "integrate_batch", "evaluate_batch", "tune_evaluate", "apply_grad",
+    "operate_batch", "operate_batch_pipeline", "integrate_batch", "evaluate_batch", "tune_evaluate", "apply_grad",
 ]

@@ -23,7 +23,7 @@ EXPORTS = [
     "qtt_net_destroy", "qtt_mpo_create", "qtt_mpo_destroy", "qtt_eval", "qtt_eval_host",
     "qtt_integrate", "qtt_inner", "qtt_apply", "qtt_truncate", "qtt_eval_grad",
     "qtt_eval_grad_host", "qtt_net_size", "qtt_apply_batch", "qtt_integrate_batch", "qtt_eval_batch",
-    "qtt_tune_eval", "qtt_eval_noalloc", "qtt_apply_grad", "qtt_mpo_size",
+    "qtt_tune_eval", "qtt_eval_noalloc", "qtt_apply_grad", "qtt_mpo_size", "qtt_batch_run",
 ]
 
 
@@ -83,6 +83,8 @@ def lib():
                                       C.POINTER(vp), dp]
         L.qtt_integrate_batch.argtypes = [vp, C.c_int, C.POINTER(vp), C.c_double, dp]
         L.qtt_eval_batch.argtypes = [vp, C.c_int, C.POINTER(vp), C.POINTER(vp), ip, C.POINTER(vp), C.c_int]
+        L.qtt_batch_run.argtypes = [vp, C.c_int, C.POINTER(vp), C.POINTER(vp), C.c_double, C.c_int64, C.c_int,
+                                    C.c_double, C.POINTER(vp), ip, C.POINTER(vp), C.POINTER(vp), dp, dp]
         L.qtt_tune_eval.argtypes = [vp, vp, vp, C.c_int64, vp, C.POINTER(C.c_int), dp]
         L.qtt_eval_noalloc.argtypes = [vp, vp, vp, C.c_int64, vp]
         L.qtt_apply_grad.argtypes = [vp, vp, vp, vp, vp, vp]

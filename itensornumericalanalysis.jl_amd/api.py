@@ -302,6 +302,65 @@ def operate_batch(ops, fs, cutoff=None, maxdim=None, precision: int = Precision.
     return res, errs
 
 
+def operate_batch_pipeline(ops, fs, xs_list=None, cutoff=None, maxdim=None, precision: int = Precision.F64,
+                           take_sum: bool = False, truncate: bool = True):
+    """One step of a batch of kernel-operator solves (the loop body of
+    examples/fredholm_solver.jl:52-63 for many functions at once):
+    g_i = operate(ops[i], fs[i]; cutoff, maxdim), its integral, and g_i at
+    xs_list[i] (contiguous float64 CUDA tensors [npts_i, ndims], or None).
+    The stages run back to back on the device, captured into one HIP graph,
+    with one read-back at the end.  truncate=False: the exact products.
+    Returns (gs, errs, integrals, values); an invalid point raises
+    DigitMapError and no networks are returned."""
+    import torch
+    n = len(fs)
+    if len(ops) != n:
+        raise ValueError("one operator per function")
+    dns = [device_network(f) for f in fs]
+    dops = [device_operator(o) for o in ops]
+    ctx = engine_context()
+    arr_op = (C.c_void_p * n)(*[d.handle.value for d in dops])
+    arr_f = (C.c_void_p * n)(*[d.handle.value for d in dns])
+    outs = (C.c_void_p * n)()
+    errs = np.zeros(n)
+    ints = np.zeros(n)
+    base = fs[0].s.base if n else 2
+    vals = None
+    arr_x = arr_v = npts_p = None
+    if xs_list is not None:
+        if len(xs_list) != n:
+            raise ValueError("one point array per function")
+        for f, x in zip(fs, xs_list):
+            if not (isinstance(x, torch.Tensor) and x.is_cuda and x.dtype == torch.float64 and x.dim() == 2
+                    and x.is_contiguous() and x.shape[1] == f.dimension()):
+                raise ValueError("each point array must be a contiguous float64 CUDA tensor [npts, ndims]")
+        vals = [torch.empty(x.shape[0], dtype=torch.float64, device=x.device) for x in xs_list]
+        arr_x = (C.c_void_p * n)(*[x.data_ptr() for x in xs_list])
+        arr_v = (C.c_void_p * n)(*[v.data_ptr() for v in vals])
+        npts = np.array([x.shape[0] for x in xs_list], dtype=np.int64)
+        npts_p = npts.ctypes.data_as(C.POINTER(C.c_int64))
+    if n == 0:
+        return [], errs, ints, vals
+    lib = _capi.lib()
+    cut = _cut(cutoff) if truncate else -1.0
+
+    def call():
+        return lib.qtt_batch_run(ctx.handle, n, arr_op, arr_f, cut, int(maxdim or 0), int(precision),
+                                 1.0 if take_sum else 1.0 / base, arr_x, npts_p, arr_v, outs,
+                                 _capi._dptr(errs), _capi._dptr(ints))
+    if xs_list is not None:
+        with _torch_stream(ctx, xs_list[0]):
+            rc = call()
+    else:
+        rc = call()
+    if rc == 2:  # QTT_ERANGE
+        raise DigitMapError("a coordinate is negative or NaN (the digit map is defined for x >= 0)")
+    ctx.check(rc)
+    res = [ITensorNetworkFunction(f.s, None, f.topo, device=_capi.DeviceNetwork(ctx, f.topo, C.c_void_p(h)))
+           for f, h in zip(fs, outs)]
+    return res, errs, ints, vals
+
+
 def evaluate_batch(fs, xs_list, precision: int = Precision.F64, outs=None):
     """evaluate(fs[i], xs_list[i]) for a batch of functions; each xs_list[i] is a
     contiguous float64 CUDA tensor [npts_i, ndims_i].  Returns the list of CUDA

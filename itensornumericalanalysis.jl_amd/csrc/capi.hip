@@ -288,6 +288,9 @@ int qtt_ctx_create(int device, qtt_ctx** out) {
   // QTT_JACOBI_MAX_SWEEPS: sweeps before an SVD reports QTT_ENOCONV (default 100)
   const char* js = std::getenv("QTT_JACOBI_MAX_SWEEPS");
   if (js) c->jacobi_max_sweeps = std::min(1000, std::max(1, std::atoi(js)));
+  // QTT_BATCH_GRAPH=0: qtt_batch_run queues its stages directly instead of as one graph
+  const char* bg = std::getenv("QTT_BATCH_GRAPH");
+  c->batch_graph = !(bg && std::strcmp(bg, "0") == 0);
   *out = c;
   return QTT_OK;
 }
@@ -639,89 +642,119 @@ int qtt_apply(qtt_ctx* ctx, const qtt_mpo* op, const qtt_net* net, double cutoff
   return QTT_OK;
 }
 
-int qtt_apply_batch(qtt_ctx* ctx, int n, const qtt_mpo* const* ops, const qtt_net* const* nets, double cutoff,
-                    int64_t maxdim, int precision, qtt_net** outs, double* errs) {
-  if (!ctx || n < 0 || (n && (!ops || !nets || !outs))) return qtt_fail(ctx, QTT_EINVAL, "null argument");
-  int rc = check_apply_precision(ctx, precision);
-  if (rc) return rc;
+// Output networks of an apply batch (bond = operator bond x network bond): all
+// cores in one allocation and offsets + topology + bonds in another, whose
+// contents are uploaded asynchronously from `meta` (the caller keeps it until
+// the stream has passed the copy).  On failure the networks made so far are
+// destroyed and `made` is cleared.
+static int alloc_batch_outputs(qtt_ctx* ctx, int n, const qtt_mpo* const* ops, const qtt_net* const* nets,
+                               std::vector<qtt_net*>& made, std::vector<char>& meta) {
+  made.assign(n, nullptr);
+  auto cleanup = [&]() {
+    for (auto* m : made) qtt_net_destroy(m);
+    made.assign(n, nullptr);
+  };
+  if (n <= 0) return QTT_OK;
+  int rc = QTT_OK;
+  std::vector<int64_t> dbase(n), mbase(n);
+  int64_t dtot = 0, mtot = 0;  // doubles of cores; bytes of metadata
+  for (int i = 0; i < n && !rc; i++) {
+    const HostTopo& t = nets[i]->t;
+    qtt_net* m = new (std::nothrow) qtt_net;
+    if (!m) {
+      rc = qtt_fail(ctx, QTT_ENOMEM, "host allocation");
+      break;
+    }
+    made[i] = m;
+    m->ctx = ctx;
+    ctx->refs++;
+    m->t = t;
+    m->bond.resize(t.nv);
+    m->cap.resize(t.nv);
+    m->off.resize(t.nv);
+    for (int v = 0; v < t.nv; v++) m->bond[v] = ops[i]->bond[v] * nets[i]->bond[v];
+    int64_t o = 0;
+    for (int v = 0; v < t.nv; v++) {
+      int64_t sz = m->bond[v] * t.sdim[v];
+      for (int k = 0; k < t.nch[v]; k++) sz *= m->bond[t.chs[t.choff[v] + k]];
+      m->cap[v] = sz;
+      m->off[v] = o;
+      o += (sz + 1) & ~int64_t(1);  // 16-byte aligned cores
+    }
+    m->total = o;
+    dbase[i] = dtot;
+    dtot += std::max<int64_t>(o, 2);
+    mbase[i] = mtot;
+    const size_t nti = size_t(7) * t.nv + std::max(t.ncoord, 1);
+    mtot += (int64_t)((t.nv * sizeof(int64_t) + nti * sizeof(int) + t.nv * sizeof(int) + 15) & ~size_t(15));
+  }
+  double* dpool = nullptr;
+  char* mpool = nullptr;
+  if (!rc) rc = qtt_hip_check(ctx, hipMalloc(&dpool, dtot * sizeof(double)), "hipMalloc batch cores");
+  if (!rc) rc = qtt_hip_check(ctx, hipMalloc(&mpool, std::max<int64_t>(mtot, 16)), "hipMalloc batch metadata");
+  if (rc) {
+    if (dpool) (void)hipFree(dpool);
+    cleanup();
+    return rc;
+  }
+  std::shared_ptr<void> pool(dpool, [mpool](void* p) {
+    (void)hipFree(p);
+    (void)hipFree(mpool);
+  });
+  meta.assign(mtot, 0);
+  for (int i = 0; i < n; i++) {
+    qtt_net* m = made[i];
+    const HostTopo& t = m->t;
+    char* h = meta.data() + mbase[i];
+    char* d = mpool + mbase[i];
+    const std::vector<int> ti = topo_ints(t);
+    std::vector<int> b32(t.nv);
+    for (int v = 0; v < t.nv; v++) b32[v] = (int)m->bond[v];
+    std::memcpy(h, m->off.data(), t.nv * sizeof(int64_t));
+    std::memcpy(h + t.nv * sizeof(int64_t), ti.data(), ti.size() * sizeof(int));
+    std::memcpy(h + t.nv * sizeof(int64_t) + ti.size() * sizeof(int), b32.data(), t.nv * sizeof(int));
+    m->pool = pool;
+    m->d_data = dpool + dbase[i];
+    m->d_off = (int64_t*)d;
+    m->d_topo = (int*)(d + t.nv * sizeof(int64_t));
+    m->d_bond = (int*)(d + t.nv * sizeof(int64_t) + ti.size() * sizeof(int));
+  }
+  rc = qtt_hip_check(ctx, hipMemcpyAsync(mpool, meta.data(), mtot, hipMemcpyHostToDevice, ctx->stream), "batch metadata");
+  if (rc) {
+    (void)hipStreamSynchronize(ctx->stream);
+    cleanup();
+  }
+  return rc;
+}
+
+static int check_apply_batch(qtt_ctx* ctx, int n, const qtt_mpo* const* ops, const qtt_net* const* nets,
+                             qtt_net** outs) {
   for (int i = 0; i < n; i++) {
     outs[i] = nullptr;
     if (!ops[i] || !nets[i]) return qtt_fail(ctx, QTT_EINVAL, "null operator or network in batch");
     if (!same_topo(ops[i]->t, nets[i]->t))
       return qtt_fail(ctx, QTT_EINVAL, "apply batch: operator and network layouts differ");
   }
-  std::vector<qtt_net*> made(n, nullptr);
+  return QTT_OK;
+}
+
+int qtt_apply_batch(qtt_ctx* ctx, int n, const qtt_mpo* const* ops, const qtt_net* const* nets, double cutoff,
+                    int64_t maxdim, int precision, qtt_net** outs, double* errs) {
+  if (!ctx || n < 0 || (n && (!ops || !nets || !outs))) return qtt_fail(ctx, QTT_EINVAL, "null argument");
+  int rc = check_apply_precision(ctx, precision);
+  if (!rc) rc = check_apply_batch(ctx, n, ops, nets, outs);
+  if (rc) return rc;
+  std::vector<qtt_net*> made;
+  std::vector<char> meta;
+  // All outputs in two allocations with one upload of the metadata, and the
+  // exact contractions in one launch.
+  rc = alloc_batch_outputs(ctx, n, ops, nets, made, meta);
+  if (rc) return rc;
   auto cleanup = [&]() {
     for (auto* m : made) qtt_net_destroy(m);
   };
-  // All outputs in two allocations (cores; offsets + topology + bonds) with
-  // one upload of the metadata, and the exact contractions in one launch.
   if (n > 0) {
-    std::vector<int64_t> dbase(n), mbase(n);
-    int64_t dtot = 0, mtot = 0;  // doubles of cores; bytes of metadata
-    for (int i = 0; i < n && !rc; i++) {
-      const HostTopo& t = nets[i]->t;
-      qtt_net* m = new (std::nothrow) qtt_net;
-      if (!m) {
-        rc = qtt_fail(ctx, QTT_ENOMEM, "host allocation");
-        break;
-      }
-      made[i] = m;
-      m->ctx = ctx;
-      ctx->refs++;
-      m->t = t;
-      m->bond.resize(t.nv);
-      m->cap.resize(t.nv);
-      m->off.resize(t.nv);
-      for (int v = 0; v < t.nv; v++) m->bond[v] = ops[i]->bond[v] * nets[i]->bond[v];
-      int64_t o = 0;
-      for (int v = 0; v < t.nv; v++) {
-        int64_t sz = m->bond[v] * t.sdim[v];
-        for (int k = 0; k < t.nch[v]; k++) sz *= m->bond[t.chs[t.choff[v] + k]];
-        m->cap[v] = sz;
-        m->off[v] = o;
-        o += (sz + 1) & ~int64_t(1);  // 16-byte aligned cores
-      }
-      m->total = o;
-      dbase[i] = dtot;
-      dtot += std::max<int64_t>(o, 2);
-      mbase[i] = mtot;
-      const size_t nti = size_t(7) * t.nv + std::max(t.ncoord, 1);
-      mtot += (int64_t)((t.nv * sizeof(int64_t) + nti * sizeof(int) + t.nv * sizeof(int) + 15) & ~size_t(15));
-    }
-    double* dpool = nullptr;
-    char* mpool = nullptr;
-    if (!rc) rc = qtt_hip_check(ctx, hipMalloc(&dpool, dtot * sizeof(double)), "hipMalloc batch cores");
-    if (!rc) rc = qtt_hip_check(ctx, hipMalloc(&mpool, std::max<int64_t>(mtot, 16)), "hipMalloc batch metadata");
-    if (rc) {
-      if (dpool) (void)hipFree(dpool);
-      cleanup();
-      return rc;
-    }
-    std::shared_ptr<void> pool(dpool, [mpool](void* p) {
-      (void)hipFree(p);
-      (void)hipFree(mpool);
-    });
-    std::vector<char> meta(mtot);
-    for (int i = 0; i < n; i++) {
-      qtt_net* m = made[i];
-      const HostTopo& t = m->t;
-      char* h = meta.data() + mbase[i];
-      char* d = mpool + mbase[i];
-      const std::vector<int> ti = topo_ints(t);
-      std::vector<int> b32(t.nv);
-      for (int v = 0; v < t.nv; v++) b32[v] = (int)m->bond[v];
-      std::memcpy(h, m->off.data(), t.nv * sizeof(int64_t));
-      std::memcpy(h + t.nv * sizeof(int64_t), ti.data(), ti.size() * sizeof(int));
-      std::memcpy(h + t.nv * sizeof(int64_t) + ti.size() * sizeof(int), b32.data(), t.nv * sizeof(int));
-      m->pool = pool;
-      m->d_data = dpool + dbase[i];
-      m->d_off = (int64_t*)d;
-      m->d_topo = (int*)(d + t.nv * sizeof(int64_t));
-      m->d_bond = (int*)(d + t.nv * sizeof(int64_t) + ti.size() * sizeof(int));
-    }
-    rc = qtt_hip_check(ctx, hipMemcpyAsync(mpool, meta.data(), mtot, hipMemcpyHostToDevice, ctx->stream), "batch metadata");
-    if (!rc) rc = launch_contract_batch(ctx, n, ops, nets, made.data(), precision);
+    rc = launch_contract_batch(ctx, n, ops, nets, made.data(), precision);
     if (!rc) rc = qtt_hip_check(ctx, hipStreamSynchronize(ctx->stream), "batch contract sync");  // meta outlives the copy
   }
   if (!rc && cutoff >= 0.0)
@@ -730,9 +763,110 @@ int qtt_apply_batch(qtt_ctx* ctx, int n, const qtt_mpo* const* ops, const qtt_ne
   else if (!rc && errs)
     for (int i = 0; i < n; i++) errs[i] = 0.0;
   if (rc) {
+    (void)hipStreamSynchronize(ctx->stream);
     cleanup();
     return rc;
   }
+  for (int i = 0; i < n; i++) outs[i] = made[i];
+  return QTT_OK;
+}
+
+// The whole batch pipeline of a kernel-operator solve step: contract ->
+// truncate -> integrate -> evaluate, every stage one batched launch, queued
+// back to back with no host synchronisation and, where the stream allows,
+// captured into one HIP graph.  Bonds, truncation errors, integrals and the
+// evaluation classes are copied back once, after the graph.
+int qtt_batch_run(qtt_ctx* ctx, int n, const qtt_mpo* const* ops, const qtt_net* const* nets, double cutoff,
+                  int64_t maxdim, int precision, double weight, const double* const* xs, const int64_t* npts,
+                  double* const* vals, qtt_net** outs, double* errs, double* integrals) {
+  if (!ctx || n < 0 || (n && (!ops || !nets || !outs))) return qtt_fail(ctx, QTT_EINVAL, "null argument");
+  int rc = check_apply_precision(ctx, precision);
+  if (!rc) rc = check_apply_batch(ctx, n, ops, nets, outs);
+  if (rc) return rc;
+  const bool want_eval = xs && npts && vals;
+  if (want_eval)
+    for (int i = 0; i < n; i++)
+      if (npts[i] < 0 || (npts[i] && (!xs[i] || !vals[i]))) return qtt_fail(ctx, QTT_EINVAL, "bad batch entry");
+  if (n == 0) return QTT_OK;
+  SyncSlots slots(ctx);
+  std::vector<qtt_net*> made;
+  std::vector<char> meta;
+  rc = alloc_batch_outputs(ctx, n, ops, nets, made, meta);
+  if (rc) return rc;
+  auto fail = [&](int code) {
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto* m : made) qtt_net_destroy(m);
+    return code;
+  };
+  const bool trunc = cutoff >= 0.0;
+  const int64_t md = maxdim > 0 ? maxdim : (int64_t(1) << 40);
+  // one scratch allocation for every stage's tables and workspaces
+  std::vector<int64_t> zero_npts;
+  const int64_t* np = npts;
+  if (!want_eval) {
+    zero_npts.assign(n, 0);
+    np = zero_npts.data();
+  }
+  auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t cb = al(contract_batch_bytes(n, made.data()));
+  const size_t tb = trunc ? al(truncate_batch_bytes(ctx, made.data(), n)) : 0;
+  const size_t ib = integrals ? al(integrate_batch_bytes(n, made.data())) : 0;
+  const size_t eb = al(eval_pipe_bytes(n, made.data(), np));
+  char* scr = (char*)qtt_scratch(ctx, cb + tb + ib + eb);
+  if (!scr) return fail(qtt_fail(ctx, QTT_ENOMEM, "pipeline scratch"));
+  BatchStage cst, ist, est;
+  TruncStaged tst;
+  rc = contract_batch_stage(ctx, n, ops, nets, made.data(), precision, scr, cst);
+  if (!rc && trunc) rc = truncate_batch_stage(ctx, made.data(), n, cutoff, md, precision, (double*)(scr + cb), tst);
+  if (!rc && integrals) rc = integrate_batch_stage(ctx, n, made.data(), weight, scr + cb + tb, ist);
+  if (!rc) rc = eval_pipe_stage(ctx, n, made.data(), want_eval ? xs : nullptr, np, want_eval ? vals : nullptr,
+                                QTT_F64, scr + cb + tb + ib, est);
+  if (rc) return fail(rc);
+  auto queue = [&]() {
+    int r = contract_batch_launch(ctx, cst);
+    if (!r && trunc) r = truncate_batch_launch(ctx, tst);
+    if (!r && integrals) r = integrate_batch_launch(ctx, ist);
+    if (!r) r = eval_pipe_launch(ctx, est);
+    return r;
+  };
+  // Graph capture needs a stream other than the null stream and kernels that
+  // capture (a truncation with helper workgroups is a cooperative launch).
+  const hipStream_t exec = ctx->stream;
+  const hipStream_t cap = exec ? exec : ctx->own_stream;
+  const bool graph = ctx->batch_graph && cap && tst.nwg <= 1;
+  if (graph) {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    rc = qtt_hip_check(ctx, hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "begin capture");
+    if (!rc) {
+      ctx->stream = cap;
+      const int rq = queue();
+      ctx->stream = exec;
+      const int re = qtt_hip_check(ctx, hipStreamEndCapture(cap, &g), "end capture");
+      rc = rq ? rq : re;
+    }
+    if (!rc) rc = qtt_hip_check(ctx, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "graph instantiate");
+    if (!rc) rc = qtt_hip_check(ctx, hipGraphLaunch(ge, exec), "graph launch");
+    // a graph may be destroyed once launched; its work still runs
+    if (ge) (void)hipGraphExecDestroy(ge);
+    if (g) (void)hipGraphDestroy(g);
+  } else {
+    rc = queue();
+  }
+  if (!rc && trunc) rc = truncate_batch_readback(ctx, tst, made.data());
+  if (!rc && integrals)
+    rc = qtt_hip_check(ctx, hipMemcpyAsync(integrals, ist.d_out, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "integrals");
+  if (!rc) rc = eval_pipe_readback(ctx, est);
+  if (!rc) rc = qtt_hip_check(ctx, hipStreamSynchronize(ctx->stream), "pipeline sync");
+  if (!rc && trunc) rc = truncate_batch_finish(ctx, tst, made.data(), errs);
+  if (!rc && !trunc && errs)
+    for (int i = 0; i < n; i++) errs[i] = 0.0;
+  if (!rc && want_eval) rc = eval_pipe_rest(ctx, est, n, made.data(), xs, npts, vals, QTT_F64);
+  {
+    const int rs = collect_status(ctx, 2);
+    if (!rc) rc = rs;
+  }
+  if (rc) return fail(rc);
   for (int i = 0; i < n; i++) outs[i] = made[i];
   return QTT_OK;
 }

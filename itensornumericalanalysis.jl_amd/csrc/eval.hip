@@ -13,6 +13,8 @@
 // layout, conflict free).
 #include "qtt_internal.h"
 
+#include <cstring>
+
 namespace {
 
 __device__ inline uint64_t digit_word(double x, int L, bool& bad) {
@@ -1296,6 +1298,246 @@ int launch_eval_batch(qtt_ctx* ctx, int n, qtt_net* const* nets, const double* c
   }
   // the host tables must outlive their asynchronous copies
   return qtt_hip_check(ctx, hipStreamSynchronize(ctx->stream), "evaluate batch sync");
+}
+
+// ---------------------------------------------------------------------------
+// Evaluate inside a pipeline (qtt_batch_run): the chains' bonds are known only
+// on the device (a truncation kernel earlier on the stream sets them), so the
+// packing layout and each network's bond class are computed by kernels.  Each
+// network gets a pack region sized for bonds clipped to 32 (the largest the
+// register kernels take); one class-filtered launch per register class then
+// evaluates the networks of that class, and networks whose bonds exceed 32 are
+// left to the single-network path once the bonds are back on the host.
+// ---------------------------------------------------------------------------
+struct PipeItem {
+  EvalItem e;              // pack, poff, desc point into this network's region
+  const double* data;      // source cores
+  const int64_t* off;
+  const int* bond;         // device bonds, final only when the pipeline runs
+  const int* sdim;
+  const int* vcoord;
+  const int* vdigit;
+  const int* ndigits;
+  int64_t pack_cap;        // doubles in the pack region
+};
+
+// Packing offsets and site descriptors of a chain from its bonds (one thread);
+// returns the packed size in doubles.
+__device__ int64_t chain_layout(const int* __restrict__ bond, int nv, const int* __restrict__ vcoord,
+                                const int* __restrict__ vdigit, const int* __restrict__ ndigits,
+                                int64_t* __restrict__ poff, int* __restrict__ desc) {
+  int64_t o = 0;
+  for (int k = 0; k < nv; ++k) {
+    const int chil = bond[k];
+    const int chir = (k == nv - 1) ? 1 : bond[k + 1];
+    const int64_t lpad = (chil + 3) & ~3, rpad = (chir + 3) & ~3;
+    if (poff) poff[k] = o;
+    o += rpad * lpad * 2;
+    const int c = vcoord[k];
+    if (desc) {
+      desc[4 * k + 0] = chil;
+      desc[4 * k + 1] = chir;
+      desc[4 * k + 2] = c;
+      desc[4 * k + 3] = c >= 0 ? ndigits[c] - vdigit[k] : 0;
+    }
+  }
+  return o;
+}
+
+// One thread per network: bond class (0..3 for bonds <= 4, 8, 16, 32; 4 when
+// larger or when the packed chain exceeds its region) and the layout tables.
+__global__ void pipe_meta_kernel(const PipeItem* __restrict__ items, int n, int* __restrict__ cls) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const PipeItem& it = items[i];
+  int maxb = 1;
+  for (int k = 0; k < it.e.nv; ++k) maxb = max(maxb, it.bond[k]);
+  const int64_t need = chain_layout(it.bond, it.e.nv, it.vcoord, it.vdigit, it.ndigits, nullptr, nullptr);
+  int c = maxb <= 4 ? 0 : maxb <= 8 ? 1 : maxb <= 16 ? 2 : maxb <= 32 ? 3 : 4;
+  if (need > it.pack_cap) c = 4;
+  if (c < 4)
+    chain_layout(it.bond, it.e.nv, it.vcoord, it.vdigit, it.ndigits, const_cast<int64_t*>(it.e.poff),
+                 (int*)const_cast<SiteDesc*>(it.e.desc));
+  cls[i] = c;
+}
+
+// grid (x-blocks, site, networks): the packed copy of site k of each network
+// z, blockIdx.z + j * gridDim.z
+__global__ void pipe_pack_kernel(const PipeItem* __restrict__ items, const int* __restrict__ cls, int n) {
+  const int k = blockIdx.y;
+  for (int z = blockIdx.z; z < n; z += gridDim.z) {
+    const PipeItem& it = items[z];
+    if (cls[z] >= 4 || k >= it.e.nv) continue;
+    const int chil = it.bond[k];
+    const int chir = (k == it.e.nv - 1) ? 1 : it.bond[k + 1];
+    const int lpad = (chil + 3) & ~3;
+    const int rpad = (chir + 3) & ~3;
+    const int sd = it.sdim[k];
+    const int64_t ne = (int64_t)rpad * lpad * 2;
+    const double* T = it.data + it.off[k];
+    double* P = const_cast<double*>(it.e.pack) + it.e.poff[k];
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x) {
+      const int s = (int)(e & 1);
+      const int64_t rest = e >> 1;
+      const int l = (int)(rest % lpad);
+      const int r = (int)(rest / lpad);
+      double val = 0.0;
+      if (l < chil && r < chir) val = T[((int64_t)l * sd + (sd == 1 ? 0 : s)) * chir + r];
+      P[e] = val;
+    }
+  }
+}
+
+// Blocks of every network's points; a block whose network is not of class
+// `want` returns at once.
+template <int CHI>
+__global__ void __launch_bounds__(256)
+eval_pipe_kernel(const PipeItem* __restrict__ items, const int* __restrict__ item_of, const int* __restrict__ cls,
+                 int want, int* __restrict__ flag) {
+  const int i = item_of[blockIdx.x];
+  if (cls[i] != want) return;
+  const EvalItem& it = items[i].e;
+  eval_chain_reg_body<CHI>(it.pack, it.poff, it.desc, it.nv, it.nc, it.ndigits, it.xs, it.npts, it.out, flag,
+                           (int64_t)(blockIdx.x - it.block0) * blockDim.x + threadIdx.x);
+}
+
+namespace {
+int64_t pipe_pack_bound(const qtt_net* net) {
+  const int nv = net->t.nv;
+  int64_t elems = 0;
+  for (int k = 0; k < nv; k++) {
+    const int64_t chil = std::min<int64_t>(net->bond[k], 32);
+    const int64_t chir = k == nv - 1 ? 1 : std::min<int64_t>(net->bond[k + 1], 32);
+    elems += ((chil + 3) & ~int64_t(3)) * ((chir + 3) & ~int64_t(3)) * 2;
+  }
+  return elems;
+}
+
+bool pipe_eligible(const qtt_net* net, int64_t npts, int precision) {
+  return npts > 0 && precision == QTT_F64 && net->t.chain && net->t.ncoord <= QTT_MAX_COORD;
+}
+
+size_t al256(size_t b) { return (b + 255) & ~size_t(255); }
+}  // namespace
+
+size_t eval_pipe_bytes(int n, qtt_net* const* nets, const int64_t* npts) {
+  size_t bytes = al256((size_t)n * sizeof(PipeItem)) + al256((size_t)n * sizeof(int)) + 512;
+  int64_t blocks = 0;
+  for (int i = 0; i < n; i++) {
+    if (npts[i] <= 0) continue;
+    const int nv = nets[i]->t.nv;
+    bytes += al256(pipe_pack_bound(nets[i]) * sizeof(double)) + al256(nv * sizeof(int64_t)) +
+             al256(4 * nv * sizeof(int));
+    blocks += (npts[i] + 255) / 256;
+  }
+  return bytes + al256((size_t)blocks * sizeof(int));
+}
+
+int eval_pipe_stage(qtt_ctx* ctx, int n, qtt_net* const* nets, const double* const* xs, const int64_t* npts,
+                    double* const* out, int precision, char* scr, BatchStage& st) {
+  st.n = 0;
+  st.eval_host.clear();
+  st.cls.clear();
+  std::vector<PipeItem> items;
+  std::vector<int> item_of;
+  char* d_items = scr;
+  char* p = scr + al256((size_t)n * sizeof(PipeItem));
+  int* d_cls = (int*)p;
+  p += al256((size_t)n * sizeof(int));
+  for (int i = 0; i < n; i++) {
+    if (npts[i] <= 0) continue;
+    qtt_net* net = nets[i];
+    if (!pipe_eligible(net, npts[i], precision)) {
+      st.eval_host.push_back(i);
+      continue;
+    }
+    const int nv = net->t.nv;
+    const TopoView tv = net->view();
+    PipeItem it;
+    it.pack_cap = pipe_pack_bound(net);
+    it.e.pack = (const double*)p;
+    p += al256(it.pack_cap * sizeof(double));
+    it.e.poff = (const int64_t*)p;
+    p += al256(nv * sizeof(int64_t));
+    it.e.desc = (const SiteDesc*)p;
+    p += al256(4 * nv * sizeof(int));
+    it.e.ndigits = tv.ndigits;
+    it.e.xs = xs[i];
+    it.e.out = out[i];
+    it.e.npts = npts[i];
+    it.e.block0 = (int64_t)item_of.size();
+    it.e.nv = nv;
+    it.e.nc = net->t.ncoord;
+    it.data = net->d_data;
+    it.off = net->d_off;
+    it.bond = net->d_bond;
+    it.sdim = tv.sdim;
+    it.vcoord = tv.vcoord;
+    it.vdigit = tv.vdigit;
+    it.ndigits = tv.ndigits;
+    const int64_t nb = (npts[i] + 255) / 256;
+    if (it.e.block0 + nb > INT32_MAX) return qtt_fail(ctx, QTT_EINVAL, "evaluate batch: too many points");
+    item_of.insert(item_of.end(), (size_t)nb, (int)items.size());
+    items.push_back(it);
+    st.cls.push_back(i);  // network index of each item, replaced by classes at readback
+  }
+  int* d_map = (int*)p;
+  st.n = (int)items.size();
+  st.total = (int64_t)item_of.size();
+  st.dev = d_items;
+  st.aux = (char*)d_map;
+  st.d_out = (double*)d_cls;
+  st.maxnv = 1;
+  for (const auto& it : items) st.maxnv = std::max(st.maxnv, it.e.nv);
+  if (st.n == 0) return QTT_OK;
+  const size_t ib = items.size() * sizeof(PipeItem), mb = item_of.size() * sizeof(int);
+  st.host.assign(ib + mb, 0);
+  std::memcpy(st.host.data(), items.data(), ib);
+  std::memcpy(st.host.data() + ib, item_of.data(), mb);
+  int rc = qtt_hip_check(ctx, hipMemcpyAsync(d_items, st.host.data(), ib, hipMemcpyHostToDevice, ctx->stream), "pipeline items");
+  if (!rc) rc = qtt_hip_check(ctx, hipMemcpyAsync(d_map, st.host.data() + ib, mb, hipMemcpyHostToDevice, ctx->stream), "pipeline map");
+  return rc;
+}
+
+int eval_pipe_launch(qtt_ctx* ctx, const BatchStage& st) {
+  if (st.n <= 0) return QTT_OK;
+  const PipeItem* items = (const PipeItem*)st.dev;
+  const int* d_map = (const int*)st.aux;
+  int* d_cls = (int*)st.d_out;
+  pipe_meta_kernel<<<(st.n + 63) / 64, 64, 0, ctx->stream>>>(items, st.n, d_cls);
+  int rc = qtt_hip_check(ctx, hipGetLastError(), "pipe_meta_kernel");
+  if (rc) return rc;
+  pipe_pack_kernel<<<dim3(8, (unsigned)st.maxnv, (unsigned)std::min(st.n, 65535)), 256, 0, ctx->stream>>>(items, d_cls,
+                                                                                                       st.n);
+  rc = qtt_hip_check(ctx, hipGetLastError(), "pipe_pack_kernel");
+  if (rc) return rc;
+  const dim3 grid((unsigned)st.total);
+  eval_pipe_kernel<4><<<grid, 256, 0, ctx->stream>>>(items, d_map, d_cls, 0, ctx->flags());
+  eval_pipe_kernel<8><<<grid, 256, 0, ctx->stream>>>(items, d_map, d_cls, 1, ctx->flags());
+  eval_pipe_kernel<16><<<grid, 256, 0, ctx->stream>>>(items, d_map, d_cls, 2, ctx->flags());
+  eval_pipe_kernel<32><<<grid, 256, 0, ctx->stream>>>(items, d_map, d_cls, 3, ctx->flags());
+  return qtt_hip_check(ctx, hipGetLastError(), "eval_pipe_kernel");
+}
+
+int eval_pipe_readback(qtt_ctx* ctx, BatchStage& st) {
+  if (st.n <= 0) return QTT_OK;
+  // st.cls holds the network index of each item; the classes go after them
+  st.cls.resize(2 * (size_t)st.n);
+  return qtt_hip_check(ctx, hipMemcpyAsync(st.cls.data() + st.n, st.d_out, st.n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream),
+                       "pipeline classes");
+}
+
+int eval_pipe_rest(qtt_ctx* ctx, const BatchStage& st, int n, qtt_net* const* nets, const double* const* xs,
+                   const int64_t* npts, double* const* out, int precision) {
+  std::vector<int> rest = st.eval_host;
+  for (int j = 0; j < st.n; j++)
+    if (st.cls[st.n + j] >= 4) rest.push_back(st.cls[j]);
+  for (int i : rest) {
+    if (i < 0 || i >= n) return qtt_fail(ctx, QTT_EINVAL, "pipeline: bad network index");
+    int rc = launch_eval(ctx, nets[i], xs[i], npts[i], out[i], precision);
+    if (rc) return rc;
+  }
+  return QTT_OK;
 }
 
 // ---------------------------------------------------------------------------

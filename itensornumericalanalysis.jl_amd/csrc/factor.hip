@@ -2669,8 +2669,24 @@ int launch_contract(qtt_ctx* ctx, const qtt_mpo* op, const qtt_net* net, qtt_net
   return qtt_hip_check(ctx, hipStreamSynchronize(ctx->stream), "contract sync");
 }
 
-int launch_contract_batch(qtt_ctx* ctx, int n, const qtt_mpo* const* ops, const qtt_net* const* nets,
-                          qtt_net* const* outs, int precision) {
+// Exact contraction of a batch in one launch, in two phases so that a
+// pipeline can queue the kernel without host synchronisation: stage (item
+// table, element starts and core starts into device memory at `scr`,
+// asynchronously from st.host, which must live until the stream passes the
+// copy) and launch.
+size_t contract_batch_bytes(int n, qtt_net* const* outs) {
+  if (n <= 0) return 0;
+  int64_t ctot = 0;
+  for (int i = 0; i < n; i++) ctot += outs[i]->t.nv + 1;
+  const size_t ib = ((size_t)n * sizeof(ContractItem) + 15) & ~size_t(15);
+  return ib + (size_t)(n + 1 + ctot) * sizeof(int64_t) + 64;
+}
+
+int contract_batch_stage(qtt_ctx* ctx, int n, const qtt_mpo* const* ops, const qtt_net* const* nets,
+                         qtt_net* const* outs, int precision, char* scr, BatchStage& st) {
+  st.n = n;
+  st.precision = precision;
+  st.dev = scr;
   if (n <= 0) return QTT_OK;
   // scratch: items | estart (n + 1) | cstart of every network (nv + 1 each)
   std::vector<int64_t> estart(n + 1, 0), cbase(n);
@@ -2683,11 +2699,9 @@ int launch_contract_batch(qtt_ctx* ctx, int n, const qtt_mpo* const* ops, const 
   }
   const size_t ib = ((size_t)n * sizeof(ContractItem) + 15) & ~size_t(15);
   const size_t bytes = ib + (size_t)(n + 1 + ctot) * sizeof(int64_t);
-  char* scr = (char*)qtt_scratch(ctx, bytes + 64);
-  if (!scr) return qtt_fail(ctx, QTT_ENOMEM, "contract scratch");
-  std::vector<char> h(bytes);
-  ContractItem* hi = (ContractItem*)h.data();
-  int64_t* hes = (int64_t*)(h.data() + ib);
+  st.host.assign(bytes, 0);
+  ContractItem* hi = (ContractItem*)st.host.data();
+  int64_t* hes = (int64_t*)(st.host.data() + ib);
   int64_t* hcs = hes + n + 1;
   const int64_t* d_es = (const int64_t*)(scr + ib);
   const int64_t* d_cs = d_es + n + 1;
@@ -2700,53 +2714,96 @@ int launch_contract_batch(qtt_ctx* ctx, int n, const qtt_mpo* const* ops, const 
     hi[i] = {ops[i]->d_data, ops[i]->d_off, ops[i]->d_bond, nets[i]->d_data, nets[i]->d_off, nets[i]->d_bond,
              nets[i]->view(), o->d_data, o->d_off, d_cs + cbase[i]};
   }
-  int rc = qtt_hip_check(ctx, hipMemcpyAsync(scr, h.data(), bytes, hipMemcpyHostToDevice, ctx->stream), "contract items");
-  if (rc) return rc;
-  const int64_t total = estart[n];
-  const int64_t nb = std::min<int64_t>((total + 255) / 256, 65536);
+  st.total = estart[n];
+  st.aux = (char*)d_es;
+  return qtt_hip_check(ctx, hipMemcpyAsync(scr, st.host.data(), bytes, hipMemcpyHostToDevice, ctx->stream), "contract items");
+}
+
+int contract_batch_launch(qtt_ctx* ctx, const BatchStage& st) {
+  if (st.n <= 0) return QTT_OK;
+  const ContractItem* items = (const ContractItem*)st.dev;
+  const int64_t* d_es = (const int64_t*)st.aux;
+  const int64_t nb = std::min<int64_t>((st.total + 255) / 256, 65536);
   const dim3 grid((unsigned)std::max<int64_t>(nb, 1));
-  if (precision == QTT_F32)
-    contract_batch_kernel<QTT_F32><<<grid, 256, 0, ctx->stream>>>((const ContractItem*)scr, d_es, n, total);
-  else if (precision == QTT_F32_SPLIT)
-    contract_batch_kernel<QTT_F32_SPLIT><<<grid, 256, 0, ctx->stream>>>((const ContractItem*)scr, d_es, n, total);
+  if (st.precision == QTT_F32)
+    contract_batch_kernel<QTT_F32><<<grid, 256, 0, ctx->stream>>>(items, d_es, st.n, st.total);
+  else if (st.precision == QTT_F32_SPLIT)
+    contract_batch_kernel<QTT_F32_SPLIT><<<grid, 256, 0, ctx->stream>>>(items, d_es, st.n, st.total);
   else
-    contract_batch_kernel<QTT_F64><<<grid, 256, 0, ctx->stream>>>((const ContractItem*)scr, d_es, n, total);
-  rc = qtt_hip_check(ctx, hipGetLastError(), "contract_batch_kernel");
+    contract_batch_kernel<QTT_F64><<<grid, 256, 0, ctx->stream>>>(items, d_es, st.n, st.total);
+  return qtt_hip_check(ctx, hipGetLastError(), "contract_batch_kernel");
+}
+
+int launch_contract_batch(qtt_ctx* ctx, int n, const qtt_mpo* const* ops, const qtt_net* const* nets,
+                          qtt_net* const* outs, int precision) {
+  if (n <= 0) return QTT_OK;
+  char* scr = (char*)qtt_scratch(ctx, contract_batch_bytes(n, outs));
+  if (!scr) return qtt_fail(ctx, QTT_ENOMEM, "contract scratch");
+  BatchStage st;
+  int rc = contract_batch_stage(ctx, n, ops, nets, outs, precision, scr, st);
+  if (!rc) rc = contract_batch_launch(ctx, st);
   if (rc) return rc;
   // the host tables must outlive the copy
   return qtt_hip_check(ctx, hipStreamSynchronize(ctx->stream), "contract batch sync");
 }
 
-// Truncate a batch of networks, one workgroup each, in one launch.
-int launch_truncate_batch(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double cutoff, int64_t maxdim,
-                          double* errs, int precision) {
-  if (nnet <= 0) return QTT_OK;
+// Truncate a batch of networks, one workgroup each, in one launch.  In
+// phases, so that a pipeline can queue the launch between other kernels
+// without host synchronisation (qtt_batch_run): stage (plan, argument tables
+// into device memory at `scr`), launch (the kernel only), readback (queued
+// copies of errors and bonds), finish (after the stream passed the copies:
+// host bond mirrors, errors, status).
+struct TruncStaged::Impl {
+  std::vector<SweepArgs> hargs;
+  std::vector<int> allsteps;
+  CoopJob hjob;
+  std::vector<double> herr;
+  std::vector<std::vector<int>> b32;
+  SweepArgs* d_args = nullptr;
+  double* d_err = nullptr;
+  CoopJob* d_job = nullptr;
+};
+
+TruncStaged::TruncStaged() : impl(new Impl) {}
+TruncStaged::~TruncStaged() { delete impl; }
+
+static size_t trunc_layout(qtt_net* const* nets, int nnet, int nwg, std::vector<int64_t>* ws_out,
+                           std::vector<int64_t>* soff_out, std::vector<std::vector<int>>* steps_out,
+                           int64_t* tot_doubles_out, int64_t* tot_steps_out) {
   std::vector<std::vector<int>> steps(nnet);
   std::vector<int64_t> ws(nnet), soff(nnet);
-  std::vector<double> cost(nnet);
   int64_t tot_doubles = 0, tot_steps = 0;
-  int maxnv = 1;
   for (int i = 0; i < nnet; i++) {
     const qtt_net* net = nets[i];
     steps[i] = truncation_schedule(net->t);
     int64_t maxcap = 1, maxb = 1;
-    cost[i] = 0.0;
     for (int v = 0; v < net->t.nv; v++) {
       maxcap = std::max(maxcap, net->cap[v]);
       maxb = std::max(maxb, net->bond[v]);
-      cost[i] += (double)net->cap[v] * (double)net->bond[v];  // ~ a move's factorization work
     }
-    if (maxcap >= (int64_t(1) << 31) / 8)
-      return qtt_fail(ctx, QTT_EINVAL, "truncate: a core is too large for the sweep kernel");
     // workspace bound: W (or the reflector rows, <= max(2 core, 64 maxb)), R, J,
     // staging rows (<= 2 core), vectors; plus the Y buffer
     ws[i] = 4 * maxcap + 2 * maxb * maxb + 68 * maxb + 64 + maxcap;
     soff[i] = tot_steps;
     tot_doubles += ws[i];
     tot_steps += (int64_t)steps[i].size();
-    maxnv = std::max(maxnv, net->t.nv);
   }
   const size_t args_bytes = (size_t)nnet * sizeof(SweepArgs);
+  // the tail slack covers the whole-block row reads of the Jacobi rounds (< 512 doubles past a row)
+  // helpers' exchange buffers after the job: Gp [32][16], Pg [512][256], Ug [8][256]
+  const size_t coop_doubles = 32 * 16 + (size_t)COOP_PG_UNITS * 256 + 8 * 256;
+  const size_t bytes = (tot_doubles + nnet + 8) * sizeof(double) + ((args_bytes + 255) & ~size_t(255)) +
+                       ((tot_steps * sizeof(int) + 255) & ~size_t(255)) + 4096 + sizeof(CoopJob) + 256 +
+                       (nwg > 1 ? coop_doubles * sizeof(double) : 0);
+  if (ws_out) *ws_out = ws;
+  if (soff_out) *soff_out = soff;
+  if (steps_out) *steps_out = std::move(steps);
+  if (tot_doubles_out) *tot_doubles_out = tot_doubles;
+  if (tot_steps_out) *tot_steps_out = tot_steps;
+  return bytes;
+}
+
+static int trunc_nwg(qtt_ctx* ctx, qtt_net* const* nets, int nnet) {
   // A single network with cores beyond LDS gets helper workgroups for the
   // Jacobi rounds of its largest moves.
   int nwg = 1;
@@ -2755,21 +2812,45 @@ int launch_truncate_batch(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double c
     for (int v = 0; v < nets[0]->t.nv; v++) maxcap = std::max(maxcap, nets[0]->cap[v]);
     if (maxcap > QTT_LDS_DOUBLES) nwg = std::max(1, std::min(ctx->trunc_helpers + 1, ctx->cus));
   }
-  // the tail slack covers the whole-block row reads of the Jacobi rounds (< 512 doubles past a row)
-  // helpers' exchange buffers after the job: Gp [32][16], Pg [512][256], Ug [8][256]
-  const size_t coop_doubles = 32 * 16 + (size_t)COOP_PG_UNITS * 256 + 8 * 256;
-  const size_t bytes = (tot_doubles + nnet + 8) * sizeof(double) + args_bytes + tot_steps * sizeof(int) + 4096 +
-                       sizeof(CoopJob) + 256 + (nwg > 1 ? coop_doubles * sizeof(double) : 0);
-  double* scr = qtt_scratch(ctx, bytes);
-  if (!scr) return qtt_fail(ctx, QTT_ENOMEM, "truncate scratch");
+  return nwg;
+}
+
+size_t truncate_batch_bytes(qtt_ctx* ctx, qtt_net* const* nets, int nnet) {
+  if (nnet <= 0) return 0;
+  return trunc_layout(nets, nnet, trunc_nwg(ctx, nets, nnet), nullptr, nullptr, nullptr, nullptr, nullptr);
+}
+
+int truncate_batch_stage(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double cutoff, int64_t maxdim,
+                         int precision, double* scr, TruncStaged& st) {
+  st.nnet = nnet;
+  if (nnet <= 0) return QTT_OK;
+  TruncStaged::Impl* im = st.impl;
+  for (int i = 0; i < nnet; i++) {
+    int64_t maxcap = 1;
+    for (int v = 0; v < nets[i]->t.nv; v++) maxcap = std::max(maxcap, nets[i]->cap[v]);
+    if (maxcap >= (int64_t(1) << 31) / 8)
+      return qtt_fail(ctx, QTT_EINVAL, "truncate: a core is too large for the sweep kernel");
+  }
+  const int nwg = trunc_nwg(ctx, nets, nnet);
+  std::vector<int64_t> ws, soff;
+  std::vector<std::vector<int>> steps;
+  int64_t tot_doubles = 0, tot_steps = 0;
+  (void)trunc_layout(nets, nnet, nwg, &ws, &soff, &steps, &tot_doubles, &tot_steps);
+  std::vector<double> cost(nnet);
+  for (int i = 0; i < nnet; i++) {
+    cost[i] = 0.0;
+    for (int v = 0; v < nets[i]->t.nv; v++)
+      cost[i] += (double)nets[i]->cap[v] * (double)nets[i]->bond[v];  // ~ a move's factorization work
+  }
+  const size_t args_bytes = (size_t)nnet * sizeof(SweepArgs);
   double* d_err = scr;
   double* wsp = scr + nnet + 8;
   SweepArgs* d_args = (SweepArgs*)(wsp + tot_doubles);
   int* d_steps = (int*)((char*)d_args + ((args_bytes + 255) & ~size_t(255)));
   CoopJob* d_job = (CoopJob*)((char*)d_steps + ((tot_steps * sizeof(int) + 255) & ~size_t(255)));
-  std::vector<int> allsteps;
-  allsteps.reserve(tot_steps);
-  for (int i = 0; i < nnet; i++) allsteps.insert(allsteps.end(), steps[i].begin(), steps[i].end());
+  im->allsteps.clear();
+  im->allsteps.reserve(tot_steps);
+  for (int i = 0; i < nnet; i++) im->allsteps.insert(im->allsteps.end(), steps[i].begin(), steps[i].end());
   const int lds_doubles = QTT_LDS_DOUBLES;
   // Workgroups are dispatched in index order and the networks of a batch are
   // independent: the costliest networks get the first workgroups, so that a
@@ -2777,12 +2858,17 @@ int launch_truncate_batch(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double c
   std::vector<int> ord(nnet);
   for (int i = 0; i < nnet; i++) ord[i] = i;
   std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return cost[x] > cost[y]; });
-  std::vector<SweepArgs> hargs(nnet);
+  im->hargs.assign(nnet, SweepArgs{});
   double* w = wsp;
+  std::vector<double*> wbase(nnet);
+  for (int i = 0; i < nnet; i++) {  // workspaces in network order, as the layout counted them
+    wbase[i] = w;
+    w += ws[i];
+  }
   for (int j = 0; j < nnet; j++) {
     const int i = ord[j];
     qtt_net* net = nets[i];
-    SweepArgs& sa = hargs[j];
+    SweepArgs& sa = im->hargs[j];
     sa.data = net->d_data;
     sa.off = net->d_off;
     sa.bond_g = net->d_bond;
@@ -2793,11 +2879,10 @@ int launch_truncate_batch(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double c
     sa.maxdim = (long long)maxdim;
     sa.max_sweeps = ctx->jacobi_max_sweeps;
     sa.prec = precision;
-    sa.gscr = w;
+    sa.gscr = wbase[i];
     int64_t maxcap = 1;
     for (int v = 0; v < net->t.nv; v++) maxcap = std::max(maxcap, net->cap[v]);
-    sa.gY = w + ws[i] - maxcap;
-    w += ws[i];
+    sa.gY = wbase[i] + ws[i] - maxcap;
     sa.ws_lds = lds_doubles - 64 - (net->t.nv + 1) / 2;
     sa.err_out = d_err + i;
     sa.status = ctx->d_flag + 2;  // truncation is synchronous: its own status slots
@@ -2805,65 +2890,83 @@ int launch_truncate_batch(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double c
     sa.nwg = nwg;
     if (sa.ws_lds < 0) return qtt_fail(ctx, QTT_EINVAL, "truncate: too many vertices");
   }
-  int rc = qtt_hip_check(ctx, hipMemcpyAsync(d_args, hargs.data(), args_bytes, hipMemcpyHostToDevice, ctx->stream), "sweep args");
-  CoopJob hjob;  // zero state, plus the exchange buffers
-  std::memset(&hjob, 0, sizeof(hjob));
+  int rc = qtt_hip_check(ctx, hipMemcpyAsync(d_args, im->hargs.data(), args_bytes, hipMemcpyHostToDevice, ctx->stream), "sweep args");
+  std::memset(&im->hjob, 0, sizeof(im->hjob));  // zero state, plus the exchange buffers
   if (nwg > 1) {
     double* cb = (double*)((char*)d_job + ((sizeof(CoopJob) + 255) & ~size_t(255)));
-    hjob.Gp = cb;
-    hjob.Pg = cb + 32 * 16;
-    hjob.Ug = hjob.Pg + (size_t)COOP_PG_UNITS * 256;
+    im->hjob.Gp = cb;
+    im->hjob.Pg = cb + 32 * 16;
+    im->hjob.Ug = im->hjob.Pg + (size_t)COOP_PG_UNITS * 256;
   }
   if (!rc && nwg > 1)
-    rc = qtt_hip_check(ctx, hipMemcpyAsync(d_job, &hjob, sizeof(CoopJob), hipMemcpyHostToDevice, ctx->stream), "coop job");
+    rc = qtt_hip_check(ctx, hipMemcpyAsync(d_job, &im->hjob, sizeof(CoopJob), hipMemcpyHostToDevice, ctx->stream), "coop job");
   if (!rc && tot_steps)
-    rc = qtt_hip_check(ctx, hipMemcpyAsync(d_steps, allsteps.data(), tot_steps * sizeof(int), hipMemcpyHostToDevice, ctx->stream), "steps");
+    rc = qtt_hip_check(ctx, hipMemcpyAsync(d_steps, im->allsteps.data(), tot_steps * sizeof(int), hipMemcpyHostToDevice, ctx->stream), "steps");
   if (rc) return rc;
   const size_t lds_bytes = (size_t)lds_doubles * sizeof(double);
   rc = qtt_hip_check(ctx, hipFuncSetAttribute((const void*)truncate_kernel,
                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes),
                      "truncate LDS attribute");
   if (rc) return rc;
-  bool coop = false;
-  if (nwg > 1) {  // helpers: one cooperative launch, so every workgroup is resident
-    void* kargs[] = {(void*)&d_args};
-    hipError_t e = hipLaunchCooperativeKernel((const void*)truncate_kernel, dim3(nwg), dim3(512), kargs, lds_bytes,
+  st.nwg = nwg;
+  st.coop = false;
+  im->d_args = d_args;
+  im->d_err = d_err;
+  im->d_job = d_job;
+  return QTT_OK;
+}
+
+int truncate_batch_launch(qtt_ctx* ctx, TruncStaged& st) {
+  if (st.nnet <= 0) return QTT_OK;
+  TruncStaged::Impl* im = st.impl;
+  const size_t lds_bytes = (size_t)QTT_LDS_DOUBLES * sizeof(double);
+  st.coop = false;
+  if (st.nwg > 1) {  // helpers: one cooperative launch, so every workgroup is resident
+    void* kargs[] = {(void*)&im->d_args};
+    hipError_t e = hipLaunchCooperativeKernel((const void*)truncate_kernel, dim3(st.nwg), dim3(512), kargs, lds_bytes,
                                               ctx->stream);
     if (e == hipSuccess) {
-      coop = true;
+      st.coop = true;
     } else {  // not available: run without helpers
       (void)hipGetLastError();
-      hargs[0].nwg = 1;
-      hargs[0].job = nullptr;
-      rc = qtt_hip_check(ctx, hipMemcpyAsync(d_args, hargs.data(), args_bytes, hipMemcpyHostToDevice, ctx->stream), "sweep args");
+      im->hargs[0].nwg = 1;
+      im->hargs[0].job = nullptr;
+      int rc = qtt_hip_check(ctx, hipMemcpyAsync(im->d_args, im->hargs.data(), sizeof(SweepArgs), hipMemcpyHostToDevice, ctx->stream), "sweep args");
       if (rc) return rc;
     }
   }
-  if (!coop) truncate_kernel<<<nnet, 512, lds_bytes, ctx->stream>>>(d_args);
-  rc = qtt_hip_check(ctx, hipGetLastError(), "truncate_kernel");
-  if (rc) return rc;
-  std::vector<double> herr(nnet);
-  rc = qtt_hip_check(ctx, hipMemcpyAsync(herr.data(), d_err, nnet * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "err");
-  if (rc) return rc;
-  std::vector<std::vector<int>> b32(nnet);
-  for (int i = 0; i < nnet && !rc; i++) {
-    b32[i].resize(nets[i]->t.nv);
-    rc = qtt_hip_check(ctx, hipMemcpyAsync(b32[i].data(), nets[i]->d_bond, nets[i]->t.nv * sizeof(int), hipMemcpyDeviceToHost, ctx->stream), "bonds");
+  if (!st.coop) truncate_kernel<<<st.nnet, 512, lds_bytes, ctx->stream>>>(im->d_args);
+  return qtt_hip_check(ctx, hipGetLastError(), "truncate_kernel");
+}
+
+int truncate_batch_readback(qtt_ctx* ctx, TruncStaged& st, qtt_net* const* nets) {
+  if (st.nnet <= 0) return QTT_OK;
+  TruncStaged::Impl* im = st.impl;
+  im->herr.assign(st.nnet, 0.0);
+  int rc = qtt_hip_check(ctx, hipMemcpyAsync(im->herr.data(), im->d_err, st.nnet * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "err");
+  im->b32.assign(st.nnet, std::vector<int>());
+  for (int i = 0; i < st.nnet && !rc; i++) {
+    im->b32[i].resize(nets[i]->t.nv);
+    rc = qtt_hip_check(ctx, hipMemcpyAsync(im->b32[i].data(), nets[i]->d_bond, nets[i]->t.nv * sizeof(int), hipMemcpyDeviceToHost, ctx->stream), "bonds");
   }
-  if (!rc) rc = qtt_hip_check(ctx, hipStreamSynchronize(ctx->stream), "truncate sync");
-  if (rc) return rc;
-  for (int i = 0; i < nnet; i++) {
-    for (int v = 0; v < nets[i]->t.nv; v++) nets[i]->bond[v] = b32[i][v];
+  return rc;
+}
+
+int truncate_batch_finish(qtt_ctx* ctx, TruncStaged& st, qtt_net* const* nets, double* errs) {
+  if (st.nnet <= 0) return QTT_OK;
+  TruncStaged::Impl* im = st.impl;
+  for (int i = 0; i < st.nnet; i++) {
+    for (int v = 0; v < nets[i]->t.nv; v++) nets[i]->bond[v] = im->b32[i][v];
     nets[i]->pack_valid = false;
     nets[i]->pack32_valid = false;
-    if (errs) errs[i] = herr[i];
+    if (errs) errs[i] = im->herr[i];
   }
   int fl[2];
-  rc = qtt_hip_check(ctx, hipMemcpy(fl, ctx->d_flag + 2, sizeof(fl), hipMemcpyDeviceToHost), "flags");
+  int rc = qtt_hip_check(ctx, hipMemcpy(fl, ctx->d_flag + 2, sizeof(fl), hipMemcpyDeviceToHost), "flags");
   if (rc) return rc;
-  if (coop) {
+  if (st.coop) {
     CoopJob hj;
-    rc = qtt_hip_check(ctx, hipMemcpy(&hj, d_job, sizeof(hj), hipMemcpyDeviceToHost), "coop job");
+    rc = qtt_hip_check(ctx, hipMemcpy(&hj, im->d_job, sizeof(hj), hipMemcpyDeviceToHost), "coop job");
     if (rc) return rc;
     if (hj.err) {
       (void)hipMemset(ctx->d_flag + 2, 0, 2 * sizeof(int));
@@ -2875,6 +2978,20 @@ int launch_truncate_batch(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double c
     return qtt_fail(ctx, QTT_ENOCONV, "truncate: Jacobi SVD did not converge");
   }
   return QTT_OK;
+}
+
+int launch_truncate_batch(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double cutoff, int64_t maxdim,
+                          double* errs, int precision) {
+  if (nnet <= 0) return QTT_OK;
+  double* scr = qtt_scratch(ctx, truncate_batch_bytes(ctx, nets, nnet));
+  if (!scr) return qtt_fail(ctx, QTT_ENOMEM, "truncate scratch");
+  TruncStaged st;
+  int rc = truncate_batch_stage(ctx, nets, nnet, cutoff, maxdim, precision, scr, st);
+  if (!rc) rc = truncate_batch_launch(ctx, st);
+  if (!rc) rc = truncate_batch_readback(ctx, st, nets);
+  if (!rc) rc = qtt_hip_check(ctx, hipStreamSynchronize(ctx->stream), "truncate sync");
+  if (rc) return rc;
+  return truncate_batch_finish(ctx, st, nets, errs);
 }
 
 int launch_truncate(qtt_ctx* ctx, qtt_net* net, double cutoff, int64_t maxdim, double* err, int precision) {

@@ -41,6 +41,7 @@ struct qtt_ctx {
   int eval_ppl = 0;             // points per lane of the register chain kernel (0: by bond size)
   int jacobi_max_sweeps = 100;  // Jacobi sweeps before an SVD reports QTT_ENOCONV
   int trunc_helpers = 31;       // helper workgroups of a single-network truncation (QTT_TRUNC_HELPERS)
+  bool batch_graph = true;      // qtt_batch_run captures its stages into one HIP graph (QTT_BATCH_GRAPH)
   // launch shapes of the register chain kernel picked by qtt_tune_eval, keyed
   // by eval_tune_key (bond class, point-count octave); absent keys use the
   // built-in rule
@@ -144,6 +145,61 @@ int launch_contract_batch(qtt_ctx* ctx, int n, const qtt_mpo* const* ops, const 
 int launch_truncate(qtt_ctx* ctx, qtt_net* net, double cutoff, int64_t maxdim, double* err, int precision);
 int launch_truncate_batch(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double cutoff, int64_t maxdim,
                           double* errs, int precision);
+// Staged batch launches, for pipelines that queue several batch kernels with
+// no host synchronisation between them (qtt_batch_run): *_bytes sizes the
+// device tables, *_stage fills them at `scr` with asynchronous copies from
+// `host` (which must live until the stream passes them), *_launch queues the
+// kernel only, so it can be captured into a HIP graph.
+struct BatchStage {
+  int n = 0;
+  int precision = QTT_F64;
+  double weight = 1.0;
+  int64_t total = 0;
+  int maxnv = 1;
+  char* dev = nullptr;  // device tables
+  char* aux = nullptr;  // a second table inside them
+  double* d_out = nullptr;
+  std::vector<char> host;
+  std::vector<int> cls;  // evaluate: class per network read back after the run
+  std::vector<int> eval_host;  // evaluate: networks left to the single-network path
+};
+size_t contract_batch_bytes(int n, qtt_net* const* outs);
+int contract_batch_stage(qtt_ctx* ctx, int n, const qtt_mpo* const* ops, const qtt_net* const* nets,
+                         qtt_net* const* outs, int precision, char* scr, BatchStage& st);
+int contract_batch_launch(qtt_ctx* ctx, const BatchStage& st);
+struct TruncStaged {
+  struct Impl;
+  Impl* impl;
+  int nnet = 0, nwg = 1;
+  bool coop = false;
+  TruncStaged();
+  ~TruncStaged();
+  TruncStaged(const TruncStaged&) = delete;
+  TruncStaged& operator=(const TruncStaged&) = delete;
+};
+size_t truncate_batch_bytes(qtt_ctx* ctx, qtt_net* const* nets, int nnet);
+int truncate_batch_stage(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double cutoff, int64_t maxdim,
+                         int precision, double* scr, TruncStaged& st);
+int truncate_batch_launch(qtt_ctx* ctx, TruncStaged& st);
+// queue the copies of errors and bonds; after the stream is synchronised,
+// finish updates the host bond mirrors and reports the status words
+int truncate_batch_readback(qtt_ctx* ctx, TruncStaged& st, qtt_net* const* nets);
+int truncate_batch_finish(qtt_ctx* ctx, TruncStaged& st, qtt_net* const* nets, double* errs);
+// integrate: bonds on the host may be upper bounds of the device bonds
+size_t integrate_batch_bytes(int n, const qtt_net* const* nets);
+int integrate_batch_stage(qtt_ctx* ctx, int n, const qtt_net* const* nets, double weight, char* scr,
+                          BatchStage& st);
+int integrate_batch_launch(qtt_ctx* ctx, const BatchStage& st);
+// evaluate of chains whose bonds are known only on the device: packing and
+// the bond class are decided by kernels; networks whose final bonds exceed
+// the register kernels' (32) are evaluated by eval_pipe_rest afterwards
+size_t eval_pipe_bytes(int n, qtt_net* const* nets, const int64_t* npts);
+int eval_pipe_stage(qtt_ctx* ctx, int n, qtt_net* const* nets, const double* const* xs, const int64_t* npts,
+                    double* const* out, int precision, char* scr, BatchStage& st);
+int eval_pipe_launch(qtt_ctx* ctx, const BatchStage& st);
+int eval_pipe_readback(qtt_ctx* ctx, BatchStage& st);
+int eval_pipe_rest(qtt_ctx* ctx, const BatchStage& st, int n, qtt_net* const* nets, const double* const* xs,
+                   const int64_t* npts, double* const* out, int precision);
 int launch_grad(qtt_ctx* ctx, const qtt_net* net, const double* xs, const double* ybar,
                 int64_t npts, double* grad, int precision);
 int launch_apply_grad(qtt_ctx* ctx, const qtt_mpo* op, const qtt_net* net, const double* gbar, double* fbar,

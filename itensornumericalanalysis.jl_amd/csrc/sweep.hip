@@ -499,10 +499,28 @@ int launch_integrate(qtt_ctx* ctx, const qtt_net* net, double weight, double* ou
   return qtt_hip_check(ctx, hipStreamSynchronize(ctx->stream), "integrate sync");
 }
 
-int launch_integrate_batch(qtt_ctx* ctx, int n, const qtt_net* const* nets, double weight, double* out_host) {
+// Integrals of a batch, one workgroup each, in one launch.  The vectors are
+// laid out from the host bonds, which may be upper bounds of the device
+// bonds the kernel reads (a pipeline stages this before the truncation that
+// lowers them).  Staged like the other batch launches: tables at `scr`,
+// results left on the device at st.d_out.
+size_t integrate_batch_bytes(int n, const qtt_net* const* nets) {
+  if (n <= 0) return 0;
+  int64_t vall = 0, oall = 0;
+  for (int i = 0; i < n; i++) {
+    oall += nets[i]->t.nv;
+    for (int v = 0; v < nets[i]->t.nv; v++) vall += nets[i]->bond[v];
+  }
+  return (n + vall + 8) * sizeof(double) + oall * sizeof(int64_t) + (size_t)n * sizeof(IntegrateArgs) + 256;
+}
+
+int integrate_batch_stage(qtt_ctx* ctx, int n, const qtt_net* const* nets, double weight, char* scr,
+                          BatchStage& st) {
+  st.n = n;
+  st.weight = weight;
   if (n <= 0) return QTT_OK;
   std::vector<int64_t> voff;
-  std::vector<int64_t> vbase(n), obase(n), vtot(n);
+  std::vector<int64_t> vbase(n), obase(n);
   int64_t vall = 0, oall = 0;
   for (int i = 0; i < n; i++) {
     const qtt_net* net = nets[i];
@@ -514,18 +532,17 @@ int launch_integrate_batch(qtt_ctx* ctx, int n, const qtt_net* const* nets, doub
       voff.push_back(tot);
       tot += net->bond[v];
     }
-    vtot[i] = tot;
     vall += tot;
   }
   const size_t args_bytes = (size_t)n * sizeof(IntegrateArgs);
-  const size_t bytes = (n + vall + 8) * sizeof(double) + oall * sizeof(int64_t) + args_bytes + 256;
-  double* scr = qtt_scratch(ctx, bytes);
-  if (!scr) return qtt_fail(ctx, QTT_ENOMEM, "integrate scratch");
-  double* d_out = scr;
-  double* vec = scr + n + 8;
+  double* d_out = (double*)scr;
+  double* vec = d_out + n + 8;
   int64_t* d_voff = (int64_t*)(vec + vall);
   IntegrateArgs* d_args = (IntegrateArgs*)(d_voff + oall);
-  std::vector<IntegrateArgs> h(n);
+  // host copy: voff, then the argument table, contiguous as on the device
+  st.host.assign(oall * sizeof(int64_t) + args_bytes, 0);
+  std::memcpy(st.host.data(), voff.data(), oall * sizeof(int64_t));
+  IntegrateArgs* h = (IntegrateArgs*)(st.host.data() + oall * sizeof(int64_t));
   for (int i = 0; i < n; i++) {
     h[i].data = nets[i]->d_data;
     h[i].off = nets[i]->d_off;
@@ -535,12 +552,26 @@ int launch_integrate_batch(qtt_ctx* ctx, int n, const qtt_net* const* nets, doub
     h[i].vec = vec + vbase[i];
     h[i].out = d_out + i;
   }
-  int rc = qtt_hip_check(ctx, hipMemcpyAsync(d_voff, voff.data(), oall * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream), "voff");
-  if (!rc) rc = qtt_hip_check(ctx, hipMemcpyAsync(d_args, h.data(), args_bytes, hipMemcpyHostToDevice, ctx->stream), "args");
-  if (rc) return rc;
-  integrate_batch_kernel<<<n, 512, 0, ctx->stream>>>(d_args, weight);
-  rc = qtt_hip_check(ctx, hipGetLastError(), "integrate_batch_kernel");
-  if (!rc) rc = qtt_hip_check(ctx, hipMemcpyAsync(out_host, d_out, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "integrals");
+  st.dev = (char*)d_args;
+  st.d_out = d_out;
+  return qtt_hip_check(ctx, hipMemcpyAsync(d_voff, st.host.data(), st.host.size(), hipMemcpyHostToDevice, ctx->stream),
+                       "integrate args");
+}
+
+int integrate_batch_launch(qtt_ctx* ctx, const BatchStage& st) {
+  if (st.n <= 0) return QTT_OK;
+  integrate_batch_kernel<<<st.n, 512, 0, ctx->stream>>>((const IntegrateArgs*)st.dev, st.weight);
+  return qtt_hip_check(ctx, hipGetLastError(), "integrate_batch_kernel");
+}
+
+int launch_integrate_batch(qtt_ctx* ctx, int n, const qtt_net* const* nets, double weight, double* out_host) {
+  if (n <= 0) return QTT_OK;
+  char* scr = (char*)qtt_scratch(ctx, integrate_batch_bytes(n, nets));
+  if (!scr) return qtt_fail(ctx, QTT_ENOMEM, "integrate scratch");
+  BatchStage st;
+  int rc = integrate_batch_stage(ctx, n, nets, weight, scr, st);
+  if (!rc) rc = integrate_batch_launch(ctx, st);
+  if (!rc) rc = qtt_hip_check(ctx, hipMemcpyAsync(out_host, st.d_out, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "integrals");
   if (rc) return rc;
   return qtt_hip_check(ctx, hipStreamSynchronize(ctx->stream), "integrate batch sync");
 }
