@@ -92,6 +92,30 @@ void qtt_net_destroy(qtt_net* net);
 int qtt_mpo_create(qtt_ctx* ctx, const qtt_topology* topo, const int64_t* bond,
                    const double* const* cores, qtt_mpo** out);
 void qtt_mpo_destroy(qtt_mpo* mpo);
+/* bonds and cores of an operator network (as qtt_net_bonds / qtt_net_read;
+ * cores in the operator layout [bond][s_in][s_out][children...]) */
+int qtt_mpo_bonds(qtt_ctx* ctx, const qtt_mpo* mpo, int64_t* bond_out);
+int qtt_mpo_read(qtt_ctx* ctx, const qtt_mpo* mpo, double* const* cores_out);
+/* operator_proj(f) (src/elementary_operators.jl:246-258): the diagonal
+ * operator |f><delta| of a network, O[s_in][s_out] = f[s] when s_in = s_out = s,
+ * so operate(O, g) = f g.  Built on the device from the network's cores;
+ * same layout and bonds.  The operator of map_to_zero_operator
+ * (src/elementary_operators.jl:213-229) is this projection of
+ * 1 - sum of point deltas.  Synchronous. */
+int qtt_operator_proj(qtt_ctx* ctx, const qtt_net* f, qtt_mpo** out);
+
+/* ---- network files (no reference counterpart: the reference has no on-disk
+ * format; this is the interchange format of a binding and of test fixtures) --
+ * Little-endian: "QTTN", uint32 version (1), kind (0 function, 1 operator),
+ * nv, ncoord, flags (bit 0: names); int32 parent, sdim, vcoord, vdigit [nv],
+ * ndigits [ncoord]; int64 bond [nv]; names (uint32 length + UTF-8 bytes per
+ * vertex, when flagged); float64 cores in vertex order in the layouts above.
+ * names (nullable) are stored as given and ignored on load.  Save writes a
+ * temporary file and renames it; load validates the header and the size. */
+int qtt_net_save(qtt_ctx* ctx, const qtt_net* net, const char* path, const char* const* names);
+int qtt_net_load(qtt_ctx* ctx, const char* path, qtt_net** out);
+int qtt_mpo_save(qtt_ctx* ctx, const qtt_mpo* mpo, const char* path, const char* const* names);
+int qtt_mpo_load(qtt_ctx* ctx, const char* path, qtt_mpo** out);
 
 /* ---- evaluate ------------------------------------------------------------
  * Replaces evaluate(f, xs, dims; alg) (src/itensornetworkfunction.jl:96-106:
@@ -151,6 +175,33 @@ int qtt_eval_grad_host(qtt_ctx* ctx, const qtt_net* net, const double* xs, const
  * has no pullback here. */
 int qtt_apply_grad(qtt_ctx* ctx, const qtt_mpo* op, const qtt_net* net, const double* gbar, double* fbar,
                    double* wbar);
+
+/* ---- function algebra ------------------------------------------------------
+ * Hadamard product (g f)(x) = g(x) f(x), multiply(g, f)
+ * (src/elementary_operators.jl:260-282).  The product lives on `topo`, the
+ * union of the two layouts: g_of[o] / f_of[o] give the vertex of g / f at
+ * output vertex o (-1: that function has no vertex there and contributes 1).
+ * Every vertex and edge of both operands must appear in topo, digits at the
+ * same vertices.  Bond of an output edge: g's bond times f's (1 where an
+ * operand lacks the edge); output index i = i_g * bond_f + i_f.  Synchronous;
+ * *out receives a new network. */
+int qtt_multiply(qtt_ctx* ctx, const qtt_net* g, const qtt_net* f, const qtt_topology* topo, const int32_t* g_of,
+                 const int32_t* f_of, qtt_net** out);
+/* (f + g)(x), the direct sum of two networks on the same layout
+ * (ITensorNetworks.add as overloaded in src/fixes.jl:40-95): bonds add, f's
+ * block first.  Synchronous. */
+int qtt_add(qtt_ctx* ctx, const qtt_net* f, const qtt_net* g, qtt_net** out);
+/* partial_integrate(f, dims; merge_vertices, take_sum) (src/integration.jl:35-54):
+ * sums the digits of every coordinate c with integrate[c] != 0 (each digit
+ * weighted by `weight`: 0.5 for the left Riemann sum, 1.0 for the plain sum)
+ * and contracts the vertices left without a digit into the remaining ones
+ * (merge_internal_tensors, src/utils.jl:99-113).  topo is the output layout,
+ * out_vertex[o] the network vertex that output vertex o keeps; its edges must
+ * be network edges (so the kept vertices are connected) and its digits those
+ * that remain.  An output layout with every vertex (integrated ones without
+ * digit) is merge_vertices = false.  Synchronous. */
+int qtt_partial_integrate(qtt_ctx* ctx, const qtt_net* net, const int32_t* integrate, double weight,
+                          const qtt_topology* topo, const int32_t* out_vertex, qtt_net** out);
 
 /* number of float64 elements of all cores of a network / an operator */
 int64_t qtt_net_size(const qtt_net* net);

@@ -24,6 +24,8 @@ EXPORTS = [
     "qtt_integrate", "qtt_inner", "qtt_apply", "qtt_truncate", "qtt_eval_grad",
     "qtt_eval_grad_host", "qtt_net_size", "qtt_apply_batch", "qtt_integrate_batch", "qtt_eval_batch",
     "qtt_tune_eval", "qtt_eval_noalloc", "qtt_apply_grad", "qtt_mpo_size", "qtt_batch_run",
+    "qtt_multiply", "qtt_add", "qtt_partial_integrate", "qtt_operator_proj", "qtt_mpo_bonds", "qtt_mpo_read",
+    "qtt_net_save", "qtt_net_load", "qtt_mpo_save", "qtt_mpo_load",
 ]
 
 
@@ -83,6 +85,19 @@ def lib():
                                       C.POINTER(vp), dp]
         L.qtt_integrate_batch.argtypes = [vp, C.c_int, C.POINTER(vp), C.c_double, dp]
         L.qtt_eval_batch.argtypes = [vp, C.c_int, C.POINTER(vp), C.POINTER(vp), ip, C.POINTER(vp), C.c_int]
+        i32p = C.POINTER(C.c_int32)
+        L.qtt_multiply.argtypes = [vp, vp, vp, C.POINTER(QttTopology), i32p, i32p, C.POINTER(vp)]
+        L.qtt_add.argtypes = [vp, vp, vp, C.POINTER(vp)]
+        L.qtt_operator_proj.argtypes = [vp, vp, C.POINTER(vp)]
+        L.qtt_mpo_bonds.argtypes = [vp, vp, ip]
+        L.qtt_mpo_read.argtypes = [vp, vp, dpp]
+        cpp = C.POINTER(C.c_char_p)
+        L.qtt_net_save.argtypes = [vp, vp, C.c_char_p, cpp]
+        L.qtt_net_load.argtypes = [vp, C.c_char_p, C.POINTER(vp)]
+        L.qtt_mpo_save.argtypes = [vp, vp, C.c_char_p, cpp]
+        L.qtt_mpo_load.argtypes = [vp, C.c_char_p, C.POINTER(vp)]
+        L.qtt_partial_integrate.argtypes = [vp, vp, i32p, C.c_double, C.POINTER(QttTopology), i32p,
+                                            C.POINTER(vp)]
         L.qtt_batch_run.argtypes = [vp, C.c_int, C.POINTER(vp), C.POINTER(vp), C.c_double, C.c_int64, C.c_int,
                                     C.c_double, C.POINTER(vp), ip, C.POINTER(vp), C.POINTER(vp), dp, dp]
         L.qtt_tune_eval.argtypes = [vp, vp, vp, C.c_int64, vp, C.POINTER(C.c_int), dp]
@@ -248,15 +263,32 @@ class DeviceNetwork:
 
 
 class DeviceOperator:
-    def __init__(self, ctx: Context, topo, cores):
-        st, keep = _topology_struct(topo)
-        cores = [np.ascontiguousarray(c, dtype=np.float64) for c in cores]
-        bond = np.array([c.shape[0] for c in cores], dtype=np.int64)
+    """An operator network resident on the device (qtt_mpo)."""
+
+    def __init__(self, ctx: Context, topo, cores=None, handle=None):
         self.ctx = ctx
+        self.topo = topo
         self._h = C.c_void_p()
-        ctx.check(lib().qtt_mpo_create(ctx.handle, C.byref(st), bond.ctypes.data_as(C.POINTER(C.c_int64)),
-                                       _dpp(cores), C.byref(self._h)))
+        if handle is not None:
+            self._h = handle
+        else:
+            st, keep = _topology_struct(topo)
+            cores = [np.ascontiguousarray(c, dtype=np.float64) for c in cores]
+            bond = np.array([c.shape[0] for c in cores], dtype=np.int64)
+            ctx.check(lib().qtt_mpo_create(ctx.handle, C.byref(st), bond.ctypes.data_as(C.POINTER(C.c_int64)),
+                                           _dpp(cores), C.byref(self._h)))
         _live.add(self)
+
+    def bonds(self):
+        b = np.zeros(self.topo.nv, dtype=np.int64)
+        self.ctx.check(lib().qtt_mpo_bonds(self.ctx.handle, self._h, b.ctypes.data_as(C.POINTER(C.c_int64))))
+        return b
+
+    def download(self):
+        b = self.bonds()
+        cores = [np.empty(self.topo.op_shape(v, b)) for v in range(self.topo.nv)]
+        self.ctx.check(lib().qtt_mpo_read(self.ctx.handle, self._h, _dpp(cores)))
+        return cores
 
     @property
     def handle(self):

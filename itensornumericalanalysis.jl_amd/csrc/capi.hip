@@ -71,8 +71,8 @@ static int build_topo(qtt_ctx* ctx, const qtt_topology* tp, HostTopo& t) {
     if (t.parent[v] < 0 || t.parent[v] >= v)
       return qtt_fail(ctx, QTT_EINVAL, "topology: parent[v] must satisfy 0 <= parent[v] < v");
   for (int c = 0; c < t.ncoord; c++)
-    if (t.ndigits[c] < 1 || t.ndigits[c] > 62)
-      return qtt_fail(ctx, QTT_EINVAL, "topology: digits per coordinate must be in 1..62");
+    if (t.ndigits[c] < 0 || t.ndigits[c] > 62)  // 0: a coordinate the function does not depend on
+      return qtt_fail(ctx, QTT_EINVAL, "topology: digits per coordinate must be in 0..62");
   std::vector<std::vector<char>> seen(t.ncoord);
   for (int c = 0; c < t.ncoord; c++) seen[c].assign(t.ndigits[c] + 1, 0);
   for (int v = 0; v < nv; v++) {
@@ -386,6 +386,20 @@ static int alloc_net(qtt_ctx* ctx, const HostTopo& t, const std::vector<int64_t>
   return QTT_OK;
 }
 
+extern "C++" {
+int topo_from_abi(qtt_ctx* ctx, const qtt_topology* tp, HostTopo& t) { return build_topo(ctx, tp, t); }
+
+int alloc_net_bonds(qtt_ctx* ctx, const HostTopo& t, const std::vector<int64_t>& bond, qtt_net** out) {
+  std::vector<int64_t> cap(t.nv);
+  for (int v = 0; v < t.nv; v++) {
+    int64_t s = bond[v] * t.sdim[v];
+    for (int k = 0; k < t.nch[v]; k++) s *= bond[t.chs[t.choff[v] + k]];
+    cap[v] = s;
+  }
+  return alloc_net(ctx, t, bond, cap, out);
+}
+}  // extern "C++"
+
 int qtt_net_create(qtt_ctx* ctx, const qtt_topology* topo, const int64_t* bond,
                    const double* const* cores, qtt_net** out) {
   if (!ctx || !out || !bond || !cores) return qtt_fail(ctx, QTT_EINVAL, "null argument");
@@ -483,38 +497,32 @@ void qtt_net_destroy(qtt_net* n) {
   ctx_release(c);
 }
 
-int qtt_mpo_create(qtt_ctx* ctx, const qtt_topology* topo, const int64_t* bond,
-                   const double* const* cores, qtt_mpo** out) {
-  if (!ctx || !out || !bond || !cores) return qtt_fail(ctx, QTT_EINVAL, "null argument");
+extern "C++" {
+// An operator network with these bonds, cores allocated but not written.
+int alloc_mpo(qtt_ctx* ctx, const HostTopo& t, const std::vector<int64_t>& bond, qtt_mpo** out) {
   *out = nullptr;
+  if (bond.empty() || bond[0] != 1) return qtt_fail(ctx, QTT_EINVAL, "bond[0] must be 1");
   qtt_mpo* m = new (std::nothrow) qtt_mpo;
   if (!m) return qtt_fail(ctx, QTT_ENOMEM, "host allocation");
-  int rc = build_topo(ctx, topo, m->t);
-  if (rc || bond[0] != 1) {
-    delete m;
-    return rc ? rc : qtt_fail(ctx, QTT_EINVAL, "bond[0] must be 1");
-  }
+  m->t = t;
   m->ctx = ctx;
   ctx->refs++;
-  int nv = m->t.nv;
-  m->bond.assign(bond, bond + nv);
+  const int nv = m->t.nv;
+  m->bond = bond;
   m->off.resize(nv);
   int64_t o = 0;
-  std::vector<int64_t> sz(nv);
   for (int v = 0; v < nv; v++) {
-    if (m->bond[v] < 1) {
+    if (m->bond[v] < 1 || m->bond[v] > (1 << 20)) {
       qtt_mpo_destroy(m);
-      return qtt_fail(ctx, QTT_EINVAL, "operator bond < 1");
+      return qtt_fail(ctx, QTT_EINVAL, "operator bond out of range");
     }
-    int64_t s = m->bond[v] * m->t.sdim[v] * m->t.sdim[v];
-    for (int k = 0; k < m->t.nch[v]; k++) s *= m->bond[m->t.chs[m->t.choff[v] + k]];
-    sz[v] = s;
     m->off[v] = o;
-    o += (s + 1) & ~int64_t(1);
+    o += (mpo_core_size(m, v) + 1) & ~int64_t(1);
   }
   m->total = o;
   std::vector<int> b32(nv);
   for (int v = 0; v < nv; v++) b32[v] = (int)m->bond[v];
+  int rc;
   if ((rc = qtt_hip_check(ctx, hipMalloc(&m->d_data, std::max<int64_t>(o, 2) * sizeof(double)), "hipMalloc operator")) ||
       (rc = qtt_hip_check(ctx, hipMalloc(&m->d_off, nv * sizeof(int64_t)), "hipMalloc")) ||
       (rc = qtt_hip_check(ctx, hipMalloc(&m->d_bond, nv * sizeof(int)), "hipMalloc")) ||
@@ -523,18 +531,50 @@ int qtt_mpo_create(qtt_ctx* ctx, const qtt_topology* topo, const int64_t* bond,
     qtt_mpo_destroy(m);
     return rc;
   }
-  for (int v = 0; v < nv; v++) {
-    if (!cores[v]) {
-      qtt_mpo_destroy(m);
-      return qtt_fail(ctx, QTT_EINVAL, "null operator core");
-    }
-    rc = qtt_hip_check(ctx, hipMemcpy(m->d_data + m->off[v], cores[v], sz[v] * sizeof(double), hipMemcpyHostToDevice), "upload operator core");
+  *out = m;
+  return QTT_OK;
+}
+
+int64_t mpo_core_size(const qtt_mpo* m, int v) {
+  int64_t s = m->bond[v] * m->t.sdim[v] * m->t.sdim[v];
+  for (int k = 0; k < m->t.nch[v]; k++) s *= m->bond[m->t.chs[m->t.choff[v] + k]];
+  return s;
+}
+}  // extern "C++"
+
+int qtt_mpo_create(qtt_ctx* ctx, const qtt_topology* topo, const int64_t* bond,
+                   const double* const* cores, qtt_mpo** out) {
+  if (!ctx || !out || !bond || !cores) return qtt_fail(ctx, QTT_EINVAL, "null argument");
+  *out = nullptr;
+  HostTopo t;
+  int rc = build_topo(ctx, topo, t);
+  if (rc) return rc;
+  for (int v = 0; v < t.nv; v++)
+    if (!cores[v]) return qtt_fail(ctx, QTT_EINVAL, "null operator core");
+  qtt_mpo* m = nullptr;
+  if ((rc = alloc_mpo(ctx, t, std::vector<int64_t>(bond, bond + t.nv), &m))) return rc;
+  for (int v = 0; v < t.nv; v++) {
+    rc = qtt_hip_check(ctx, hipMemcpy(m->d_data + m->off[v], cores[v], mpo_core_size(m, v) * sizeof(double), hipMemcpyHostToDevice), "upload operator core");
     if (rc) {
       qtt_mpo_destroy(m);
       return rc;
     }
   }
   *out = m;
+  return QTT_OK;
+}
+
+int qtt_mpo_bonds(qtt_ctx* ctx, const qtt_mpo* m, int64_t* bond_out) {
+  if (!ctx || !m || !bond_out) return qtt_fail(ctx, QTT_EINVAL, "null argument");
+  std::copy(m->bond.begin(), m->bond.end(), bond_out);
+  return QTT_OK;
+}
+
+int qtt_mpo_read(qtt_ctx* ctx, const qtt_mpo* m, double* const* cores_out) {
+  if (!ctx || !m || !cores_out) return qtt_fail(ctx, QTT_EINVAL, "null argument");
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  for (int v = 0; v < m->t.nv; v++)
+    HIPCHK(ctx, hipMemcpy(cores_out[v], m->d_data + m->off[v], mpo_core_size(m, v) * sizeof(double), hipMemcpyDeviceToHost));
   return QTT_OK;
 }
 

@@ -125,6 +125,12 @@ int qtt_fail(qtt_ctx* ctx, int code, const std::string& msg);
 TopoView topo_view(const int* d_topo, const HostTopo& t);
 int qtt_hip_check(qtt_ctx* ctx, hipError_t e, const char* what);
 double* qtt_scratch(qtt_ctx* ctx, size_t bytes);
+// a checked HostTopo from the C-ABI description; a new network with these
+// bonds (cores uninitialised)
+int topo_from_abi(qtt_ctx* ctx, const qtt_topology* tp, HostTopo& t);
+int alloc_net_bonds(qtt_ctx* ctx, const HostTopo& t, const std::vector<int64_t>& bond, qtt_net** out);
+int alloc_mpo(qtt_ctx* ctx, const HostTopo& t, const std::vector<int64_t>& bond, qtt_mpo** out);
+int64_t mpo_core_size(const qtt_mpo* m, int v);
 
 // kernel launchers (eval.hip, sweep.hip, factor.hip, grad.hip)
 int launch_eval(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t npts, double* out,

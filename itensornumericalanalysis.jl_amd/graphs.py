@@ -50,6 +50,19 @@ class NamedGraph:
                     stack.append(u)
         return len(seen) == n
 
+    def induced(self, keep):
+        """Subgraph on the vertices in `keep` (graph order), with the edges among them."""
+        keep = set(keep)
+        verts = [v for v in self.vertices if v in keep]
+        return NamedGraph(verts, [(a, b) for a, b in self.edges if a in keep and b in keep])
+
+    def union(self, other: "NamedGraph"):
+        """This graph's vertices and edges, then the other's that are new."""
+        verts = self.vertices + [v for v in other.vertices if v not in self._index]
+        have = {frozenset(e) for e in self.edges}
+        edges = list(self.edges) + [e for e in other.edges if frozenset(e) not in have]
+        return NamedGraph(verts, edges)
+
     def __repr__(self):
         return f"NamedGraph(nv={self.nv()}, ne={len(self.edges)})"
 

@@ -30,10 +30,13 @@ def default_dimension_vertices(g: NamedGraph, map_dimension: int = 1):
 class IndsNetworkMap:
     """A graph with one binary digit per vertex and the map from digits to coordinates."""
 
-    def __init__(self, g: NamedGraph, dimension_vertices, base: int = 2):
+    def __init__(self, g: NamedGraph, dimension_vertices, base: int = 2, ambient: NamedGraph | None = None):
         if base != 2:
             raise NotImplementedError("only base 2 digits are supported by the engine")
         self.graph = g
+        # the graph this layout was reduced from (its own graph otherwise):
+        # products of functions on parts of it live on its induced subgraphs
+        self.ambient = ambient if ambient is not None else g
         self.base = base
         self._dimvs = [list(d) for d in dimension_vertices]
         self._vertex_site = {}
@@ -116,8 +119,70 @@ class IndsNetworkMap:
         pts = [i * (a / 2.0 ** L) for i in range(0, N + 2)]
         return [x for x in pts if x < 1]
 
+    # --- derived layouts -------------------------------------------------
+    @staticmethod
+    def _trim(dimvs):
+        # the dimension count is the largest dimension with digits
+        # (src/IndexMaps/abstractindexmap.jl:72-73)
+        while dimvs and not dimvs[-1]:
+            dimvs = dimvs[:-1]
+        return dimvs
+
+    def reduced(self, dims):
+        """The layout of the given dimensions only: the subgraph of the vertices
+        carrying their digits (reduced_indsnetworkmap, src/indsnetworkmap.jl:29-46).
+        Other dimensions keep their numbers with no digits."""
+        dims = [dims] if isinstance(dims, int) else list(dims)
+        keep = [v for d in dims for v in self._dimvs[d - 1]]
+        g = self.ambient.induced(keep)
+        if not g.is_tree():
+            raise NotImplementedError("the vertices of these dimensions do not form a connected tree")
+        dimvs = self._trim([list(vs) if d in dims else [] for d, vs in enumerate(self._dimvs, start=1)])
+        return IndsNetworkMap(self.ambient.induced(keep), dimvs, self.base, self.ambient)
+
+    def without_dimensions(self, dims):
+        """Same graph, the digits of `dims` removed (their vertices carry none)."""
+        dims = [dims] if isinstance(dims, int) else list(dims)
+        dimvs = self._trim([[] if d in dims else list(vs) for d, vs in enumerate(self._dimvs, start=1)])
+        return IndsNetworkMap(self.graph, dimvs, self.base, self.ambient)
+
+    def union(self, other: "IndsNetworkMap"):
+        """The layout of a product of functions on these two layouts: the union
+        of the graphs and of the digit maps (merge of the index maps,
+        src/elementary_operators.jl:261)."""
+        if self.base != other.base:
+            raise ValueError("different digit bases")
+        if self.ambient is other.ambient:
+            # parts of one layout: the induced subgraph on both vertex sets, so
+            # parts that do not touch are joined by the layout's own edges
+            verts = set(self.graph.vertices) | set(other.graph.vertices)
+            g = self.ambient.induced(verts)
+        else:
+            g = self.graph.union(other.graph)
+        n = max(len(self._dimvs), len(other._dimvs))
+        dimvs = []
+        for d in range(n):
+            a = self._dimvs[d] if d < len(self._dimvs) else []
+            b = other._dimvs[d] if d < len(other._dimvs) else []
+            if a[:len(b)] != b[:len(a)]:
+                raise ValueError(f"the layouts place the digits of dimension {d + 1} on different vertices")
+            dimvs.append(list(a if len(a) >= len(b) else b))
+        if not g.is_tree():
+            raise NotImplementedError("the union of the two layouts is not a tree")
+        return IndsNetworkMap(g, dimvs, self.base, self.ambient if self.ambient is other.ambient else None)
+
+    def same_layout(self, other: "IndsNetworkMap"):
+        return (self.graph.vertices == other.graph.vertices
+                and {frozenset(e) for e in self.graph.edges} == {frozenset(e) for e in other.graph.edges}
+                and self._dimvs == other._dimvs)
+
     def __repr__(self):
         return f"IndsNetworkMap(nv={self.graph.nv()}, dims={[len(d) for d in self._dimvs]})"
+
+
+def reduced_indsnetworkmap(s: IndsNetworkMap, dims) -> IndsNetworkMap:
+    """reduced_indsnetworkmap(s, dims) (src/indsnetworkmap.jl:29-46)."""
+    return s.reduced(dims)
 
 
 def continuous_siteinds(g: NamedGraph, dimension_vertices=None, *, map_dimension: int = 1,

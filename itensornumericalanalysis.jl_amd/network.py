@@ -147,13 +147,14 @@ class ITensorNetworkFunction:
     def dimension_vertices(self, dim):
         return self.s.dimension_vertices(dim)
 
-    # --- arithmetic (host-side construction, as the reference's `+`, `*`) --
+    # --- arithmetic (on the device, as the reference's `+` and `*`) --------
     def __add__(self, other):
-        from .builders import add
+        from .algebra import add
         return add(self, other)
 
     def __mul__(self, other):
-        from .builders import multiply, scale
+        from .algebra import multiply
+        from .builders import scale
         if isinstance(other, ITensorNetworkFunction):
             return multiply(self, other)
         return scale(self, float(other))
@@ -167,19 +168,31 @@ class ITensorNetworkFunction:
 class TreeOperator:
     """An operator network (MPO on a chain, TTNO on a tree) on a digit layout."""
 
-    def __init__(self, s: IndsNetworkMap, cores, topo: Topology | None = None):
+    def __init__(self, s: IndsNetworkMap, cores=None, topo: Topology | None = None, device=None):
         self.s = s
         self.topo = topo if topo is not None else Topology(s)
-        cores = [np.ascontiguousarray(c, dtype=np.float64) for c in cores]
-        bond = bonds_of(self.topo, cores)
-        for v in range(self.topo.nv):
-            if cores[v].shape != self.topo.op_shape(v, bond):
-                raise ValueError(f"operator core {v} has shape {cores[v].shape}")
-        self.cores = cores
-        self._device = None
+        self._cores = None
+        self._device = device  # engine handle (DeviceOperator) or None
+        if cores is not None:
+            cores = [np.ascontiguousarray(c, dtype=np.float64) for c in cores]
+            bond = bonds_of(self.topo, cores)
+            for v in range(self.topo.nv):
+                if cores[v].shape != self.topo.op_shape(v, bond):
+                    raise ValueError(f"operator core {v} has shape {cores[v].shape}")
+            self._cores = cores
+        elif device is None:
+            raise ValueError("an operator needs host cores or a device handle")
+
+    @property
+    def cores(self):
+        if self._cores is None:
+            self._cores = self._device.download()
+        return self._cores
 
     def bonds(self):
-        return bonds_of(self.topo, self.cores)
+        if self._cores is None:
+            return self._device.bonds()
+        return bonds_of(self.topo, self._cores)
 
     def maxlinkdim(self):
         return int(self.bonds().max())

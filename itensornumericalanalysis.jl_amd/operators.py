@@ -47,6 +47,14 @@ def boundary_terms(s: IndsNetworkMap, boundary: str, dim: int, isfwd: bool, n: i
     return [(1.0, {s.vertex(dim, j): op for j in range(1, L - n + 1)})]
 
 
+def point_to_opsum(s: IndsNetworkMap, x: float, dim: int = 1):
+    """The operator string projecting dimension `dim` onto the digits of x:
+    "Dup" where x's digit is 1, "Ddn" where it is 0
+    (src/elementary_operators.jl:200-210).  compile_terms makes it an operator."""
+    vals = s.calculate_ind_values([x], [dim])
+    return [(1.0, {v: ("Dup" if vals[v] == 1 else "Ddn") for v in s.dimension_vertices(dim)})]
+
+
 def _shift_terms(s, dim, boundary, n, first, carry, isfwd):
     L = s.ndigits(dim)
     if L - n < 1:

@@ -24,7 +24,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .builders import cos_itn, exp_itn, random_itn, sin_itn
+from .builders import add_host, cos_itn, exp_itn, multiply_host, random_itn, sin_itn
 from .graphs import named_comb_tree, named_grid
 from .indsnetworkmap import continuous_siteinds
 from .operators import (first_derivative_operator, fredholm_operator, laplacian_operator,
@@ -45,7 +45,7 @@ class Workload:
 
 def sine_1d(npts: int = 100_000, seed: int = 0) -> Workload:
     s = continuous_siteinds(named_grid((12, 1)))
-    f = sin_itn(s, k=2 * math.pi) + cos_itn(s, k=6 * math.pi, c=0.5) + exp_itn(s, k=-1.0)
+    f = add_host(add_host(sin_itn(s, k=2 * math.pi), cos_itn(s, k=6 * math.pi, c=0.5)), exp_itn(s, k=-1.0))
     op = second_derivative_operator(s)
     xs = np.random.default_rng(seed).uniform(0.0, 1.0, (npts, 1))
     return Workload("sine_1d: 1D L=12 chain, D2 Dirichlet, cutoff 1e-12", s, f, op, xs, 1e-12, 0)
@@ -63,8 +63,8 @@ def laplacian_2d(npts: int = 1_000_000, seed: int = 1) -> Workload:
     b = rng.uniform(0.0, 3.0, 8)
     f = None
     for k in range(8):
-        term = sin_itn(s, k=w[k], c=a[k], dim=1) * exp_itn(s, k=-b[k], dim=2)
-        f = term if f is None else f + term
+        term = multiply_host(sin_itn(s, k=w[k], c=a[k], dim=1), exp_itn(s, k=-b[k], dim=2))
+        f = term if f is None else add_host(f, term)
     xs = rng.uniform(0.0, 1.0, (npts, 2))
     op = laplacian_operator(s)
     return Workload("laplacian_2d: 2D 2x20 bits interleaved chain, Laplacian, cutoff 1e-10 maxdim 128",

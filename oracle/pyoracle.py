@@ -245,3 +245,45 @@ def trunc_rule(p, cutoff, maxdim):
     err = C.c_double(0.0)
     m = lib().or_trunc_rule(pp, len(pa), float(cutoff), int(maxdim), C.byref(err))
     return int(m), err.value
+
+
+# ---------------------------------------------------------------------------
+# Function algebra on one layout, restated in numpy on the raw cores (the
+# per-vertex Kronecker product of src/elementary_operators.jl:266-276 and the
+# block direct sum of src/fixes.jl:55-95).
+# ---------------------------------------------------------------------------
+def hadamard(parent, sdim, coresG, coresF):
+    """Cores of (g f)(x) = g(x) f(x): core v is the outer product of the two
+    cores over every bond (g's index major), the site shared."""
+    ch = _children(parent)
+    out = []
+    for v, (A, B) in enumerate(zip(coresG, coresF)):
+        m = len(ch[v])
+        la, lb = "abcdefgh"[:m], "ijklmnop"[:m]
+        C = np.einsum(f"Ps{la},Qs{lb}->PQs" + "".join(x + y for x, y in zip(la, lb)), A, B)
+        shape = (A.shape[0] * B.shape[0], A.shape[1]) + tuple(A.shape[2 + k] * B.shape[2 + k] for k in range(m))
+        out.append(C.reshape(shape))
+    return out
+
+
+def direct_sum(parent, sdim, coresF, coresG):
+    """Cores of (f + g)(x): every bond is the sum of the two, f's block first;
+    the root's unit bond and the site are shared."""
+    ch = _children(parent)
+    out = []
+    for v, (A, B) in enumerate(zip(coresF, coresG)):
+        m = len(ch[v])
+        pa, pb = A.shape[0], B.shape[0]
+        p = 1 if v == 0 else pa + pb
+        shape = (p, A.shape[1]) + tuple(A.shape[2 + k] + B.shape[2 + k] for k in range(m))
+        C = np.zeros(shape)
+        if v == 0 and m == 0:
+            C[...] = A + B
+        else:
+            fa = (slice(0, 1) if v == 0 else slice(0, pa), slice(None)) + tuple(slice(0, A.shape[2 + k]) for k in range(m))
+            fb = (slice(0, 1) if v == 0 else slice(pa, pa + pb), slice(None)) + tuple(
+                slice(A.shape[2 + k], A.shape[2 + k] + B.shape[2 + k]) for k in range(m))
+            C[fa] = A
+            C[fb] = B
+        out.append(C)
+    return out

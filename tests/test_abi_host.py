@@ -94,10 +94,15 @@ def test_direct_sum_and_product_values(q, po, rng):
     g = q.exp_itn(s, k=-0.5, a=0.2, dim=2)
     xs = rng.uniform(0, 1, (50, 2))
     yf, yg = oracle_eval(po, f, xs), oracle_eval(po, g, xs)
-    assert np.allclose(oracle_eval(po, f + g, xs), yf + yg, rtol=1e-13, atol=1e-14)
-    assert np.allclose(oracle_eval(po, f * g, xs), yf * yg, rtol=1e-13, atol=1e-14)
+    t = f.topo
+    S = po.direct_sum(t.parent, t.sdim, f.cores, g.cores)
+    H = po.hadamard(t.parent, t.sdim, f.cores, g.cores)
+    ys, _ = po.evaluate(t.parent, t.sdim, S, t.vcoord, t.vdigit, t.Lc, xs)
+    yh, _ = po.evaluate(t.parent, t.sdim, H, t.vcoord, t.vdigit, t.Lc, xs)
+    assert np.allclose(ys, yf + yg, rtol=1e-13, atol=1e-14)
+    assert np.allclose(yh, yf * yg, rtol=1e-13, atol=1e-14)
     assert np.allclose(oracle_eval(po, 2.5 * f, xs), 2.5 * yf, rtol=1e-14)
-    assert (f + g).maxlinkdim() == 3 and (f * g).maxlinkdim() == 2
+    assert max(c.shape[0] for c in S) == 3 and max(c.shape[0] for c in H) == 2
 
 
 def test_operator_bonds_minimal(q):
