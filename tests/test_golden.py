@@ -126,3 +126,91 @@ def test_fixture_c1_engine(q, eng):
     assert q.integrate(exact) == pytest.approx(float(fx["integral_d2f"]), rel=1e-11)
     g = q.operate(op, f, cutoff=1e-12)  # the reference's cutoff: 1e-6 of the norm
     assert np.max(np.abs(q.evaluate(g, fx["xs"]) - fx["d2f"])) <= 1e-5 * np.max(np.abs(fx["d2f"]))
+
+
+# ---------------------------------------------------------------- network files
+def _read_qtn(path):
+    """The network file format restated for the test (include/qtt.h)."""
+    import struct
+    with open(path, "rb") as fh:
+        data = fh.read()
+    assert data[:4] == b"QTTN"
+    version, kind, nv, ncoord, flags = struct.unpack("<5I", data[4:24])
+    o = 24
+    ints = np.frombuffer(data, "<i4", 4 * nv + ncoord, o)
+    o += 4 * (4 * nv + ncoord)
+    parent, sdim = ints[:nv], ints[nv:2 * nv]
+    bond = np.frombuffer(data, "<i8", nv, o)
+    o += 8 * nv
+    names = []
+    for _ in range(nv if flags & 1 else 0):
+        (n,) = struct.unpack("<I", data[o:o + 4])
+        names.append(data[o + 4:o + 4 + n].decode())
+        o += 4 + n
+    ch = [[] for _ in range(nv)]
+    for v in range(1, nv):
+        ch[parent[v]].append(v)
+    cores = []
+    for v in range(nv):
+        shape = (int(bond[v]), int(sdim[v])) + ((int(sdim[v]),) if kind == 1 else ()) + tuple(int(bond[c]) for c in ch[v])
+        n = int(np.prod(shape))
+        cores.append(np.frombuffer(data, "<f8", n, o).reshape(shape))
+        o += 8 * n
+    assert o == len(data)
+    return kind, parent, names, cores
+
+
+def test_fixture_network_files(q):
+    from itensornumericalanalysis_jl_amd import files
+    c1 = _load("c1_sine.npz")
+    kind, parent, names, cores = _read_qtn(os.path.join(GOLDEN, "c1_sine.qtn"))
+    assert kind == 0 and np.array_equal(parent, c1["parent"])
+    for a, b in zip(cores, _cores(c1)):
+        assert np.array_equal(a, b)
+    # the layout rebuilt from the header is the one the network was built on
+    k, s, topo = files.layout_of(os.path.join(GOLDEN, "c1_sine.qtn"))
+    assert k == 0 and topo.same(q.Topology(q.continuous_siteinds(q.named_grid((12, 1)))))
+    cs = _load("comb_shifts_L6.npz")
+    kind, parent, names, cores = _read_qtn(os.path.join(GOLDEN, "comb_poly_L6.qtn"))
+    for a, b in zip(cores, _cores(cs)):
+        assert np.array_equal(a, b)
+    k, s, topo = files.layout_of(os.path.join(GOLDEN, "comb_poly_L6.qtn"))
+    assert topo.same(q.Topology(q.continuous_siteinds(q.named_comb_tree((2, 3)))))
+    assert names[0] == "(1, 1)"
+    # the operator file holds the Dirichlet second derivative
+    kind, parent, names, cores = _read_qtn(os.path.join(GOLDEN, "second_derivative_L6.qtn"))
+    assert kind == 1
+    k, s, topo = files.layout_of(os.path.join(GOLDEN, "second_derivative_L6.qtn"))
+    op = q.TreeOperator(s, [c.copy() for c in cores], topo)
+    ref = _load("shift_ops_L6.npz")["derivative_2"]
+    assert np.allclose(_digit_order_matrix(op, 6), ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+@pytest.mark.gpu
+def test_fixture_network_files_engine(q, eng, tmp_path):
+    for name in ("c1_sine.qtn", "comb_poly_L6.qtn", "second_derivative_L6.qtn"):
+        path = os.path.join(GOLDEN, name)
+        obj = q.load(path)
+        kind, parent, names, cores = _read_qtn(path)
+        for a, b in zip(obj.cores, cores):  # downloaded from the device
+            assert np.array_equal(a, b), name
+        out = tmp_path / name
+        q.save(obj, str(out))
+        assert out.read_bytes() == open(path, "rb").read(), name
+    c1 = _load("c1_sine.npz")
+    f = q.load(os.path.join(GOLDEN, "c1_sine.qtn"))
+    assert np.allclose(q.evaluate(f, c1["xs"]), c1["f"], rtol=1e-12, atol=1e-12)
+    cs = _load("comb_shifts_L6.npz")
+    g = q.load(os.path.join(GOLDEN, "comb_poly_L6.qtn"))
+    assert np.allclose(q.evaluate(g, cs["grid"][:, None]), cs["psi"], rtol=1e-13)
+    # a loaded operator applies as built: D2 of the polynomial is 0.5 inside the grid
+    d2 = q.load(os.path.join(GOLDEN, "second_derivative_L6.qtn"))
+    s = q.continuous_siteinds(q.named_grid((6, 1)))
+    p = q.poly_itn(s, [1.0, 0.5, 0.25])
+    y = q.evaluate(q.operate(d2, p, truncate=False), (np.arange(1, 63) / 64.0)[:, None])
+    assert np.allclose(y, 0.5, rtol=1e-9)
+    # broken files are refused
+    bad = tmp_path / "bad.qtn"
+    bad.write_bytes(open(os.path.join(GOLDEN, "c1_sine.qtn"), "rb").read()[:-8])
+    with pytest.raises(Exception):
+        q.load(str(bad))
