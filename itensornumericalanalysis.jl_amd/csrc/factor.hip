@@ -275,6 +275,73 @@ contract_batch_kernel(const ContractItem* __restrict__ items, const int64_t* __r
   }
 }
 
+// The batch again, split into segments of at most kSegElems elements of one
+// core: a workgroup takes one segment, so the network and the vertex are
+// uniform (no searches) and the index arithmetic is 32-bit.
+constexpr int kSegElems = 4096;
+struct ContractSeg {
+  int item, v, start, count;
+};
+
+template <int MODE>
+__global__ void __launch_bounds__(256)
+contract_seg_kernel(const ContractItem* __restrict__ items, const ContractSeg* __restrict__ segs) {
+  const ContractSeg sg = segs[blockIdx.x];
+  const ContractItem& it = items[sg.item];
+  const TopoView& t = it.t;
+  const int v = sg.v;
+  const int m = t.nch[v];
+  const int* ch = t.chs + t.choff[v];
+  const int d = t.sdim[v];
+  const int ap = it.abond[v];
+  const double* __restrict__ Wv = it.W + it.woff[v];
+  const double* __restrict__ Av = it.A + it.aoff[v];
+  double* __restrict__ Cv = it.C + it.coff[v];
+  int bwk[QTT_MAX_NDIM], bak[QTT_MAX_NDIM];
+  for (int k = 0; k < m; ++k) {
+    bwk[k] = it.wbond[ch[k]];
+    bak[k] = it.abond[ch[k]];
+  }
+  for (int q = threadIdx.x; q < sg.count; q += blockDim.x) {
+    unsigned local = (unsigned)(sg.start + q);
+    unsigned jw = 0, ja = 0, sw = 1, sa = 1;
+    for (int k = m - 1; k >= 0; --k) {
+      const unsigned bw = bwk[k], ba = bak[k], bb = bw * ba;
+      const unsigned idx = local % bb;
+      local /= bb;
+      jw += (idx / ba) * sw;
+      ja += (idx % ba) * sa;
+      sw *= bw;
+      sa *= ba;
+    }
+    const unsigned so = local % (unsigned)d;
+    local /= (unsigned)d;
+    const unsigned pw = local / (unsigned)ap, pa = local % (unsigned)ap;
+    double out = 0.0;
+    if (MODE == QTT_F64) {
+      for (int si = 0; si < d; ++si)
+        out += Wv[((int64_t)(pw * d + si) * d + so) * sw + jw] * Av[(int64_t)(pa * d + si) * sa + ja];
+    } else {
+      for (int si = 0; si < d; ++si) {
+        const double wv = Wv[((int64_t)(pw * d + si) * d + so) * sw + jw];
+        const double av = Av[(int64_t)(pa * d + si) * sa + ja];
+        if (wv == 0.0 || av == 0.0) continue;
+        const int ew = ilogb(wv), ea = ilogb(av);
+        double p;
+        if (MODE == QTT_F32_SPLIT) {
+          ffv tt = {0.0f, 0.0f};
+          ff_madd(tt, ff_split(ldexp(wv, -ew)), ff_split(ldexp(av, -ea)));
+          p = ff_value(tt);
+        } else {
+          p = (double)((float)ldexp(wv, -ew) * (float)ldexp(av, -ea));
+        }
+        out += ldexp(p, ew + ea);
+      }
+    }
+    Cv[sg.start + q] = out;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // truncation sweep
 // ---------------------------------------------------------------------------
@@ -2676,10 +2743,13 @@ int launch_contract(qtt_ctx* ctx, const qtt_mpo* op, const qtt_net* net, qtt_net
 // copy) and launch.
 size_t contract_batch_bytes(int n, qtt_net* const* outs) {
   if (n <= 0) return 0;
-  int64_t ctot = 0;
-  for (int i = 0; i < n; i++) ctot += outs[i]->t.nv + 1;
+  int64_t ctot = 0, nseg = 0;
+  for (int i = 0; i < n; i++) {
+    ctot += outs[i]->t.nv + 1;
+    for (int v = 0; v < outs[i]->t.nv; v++) nseg += (outs[i]->cap[v] + kSegElems - 1) / kSegElems;
+  }
   const size_t ib = ((size_t)n * sizeof(ContractItem) + 15) & ~size_t(15);
-  return ib + (size_t)(n + 1 + ctot) * sizeof(int64_t) + 64;
+  return ib + (size_t)(n + 1 + ctot) * sizeof(int64_t) + (size_t)nseg * sizeof(ContractSeg) + 64;
 }
 
 int contract_batch_stage(qtt_ctx* ctx, int n, const qtt_mpo* const* ops, const qtt_net* const* nets,
@@ -2716,12 +2786,45 @@ int contract_batch_stage(qtt_ctx* ctx, int n, const qtt_mpo* const* ops, const q
   }
   st.total = estart[n];
   st.aux = (char*)d_es;
-  return qtt_hip_check(ctx, hipMemcpyAsync(scr, st.host.data(), bytes, hipMemcpyHostToDevice, ctx->stream), "contract items");
+  // segments of the 32-bit kernel (cores below 2^31 elements), after the tables
+  bool small = true;
+  int64_t nseg = 0;
+  for (int i = 0; i < n && small; i++)
+    for (int v = 0; v < outs[i]->t.nv; v++) {
+      if (outs[i]->cap[v] >= (int64_t(1) << 31)) small = false;
+      nseg += (outs[i]->cap[v] + kSegElems - 1) / kSegElems;
+    }
+  st.nseg = 0;  // segments in use (0: the searching kernel)
+  if (small && nseg > 0 && nseg < (int64_t(1) << 31)) {  // contract_batch_bytes left room for them
+    const size_t off = st.host.size();
+    st.host.resize(off + (size_t)nseg * sizeof(ContractSeg));
+    ContractSeg* hs = (ContractSeg*)(st.host.data() + off);
+    int64_t j = 0;
+    for (int i = 0; i < n; i++)
+      for (int v = 0; v < outs[i]->t.nv; v++)
+        for (int64_t e0 = 0; e0 < outs[i]->cap[v]; e0 += kSegElems)
+          hs[j++] = {i, v, (int)e0, (int)std::min<int64_t>(kSegElems, outs[i]->cap[v] - e0)};
+    st.nseg = nseg;
+    st.seg = scr + bytes;
+  }
+  return qtt_hip_check(ctx, hipMemcpyAsync(scr, st.host.data(), st.host.size(), hipMemcpyHostToDevice, ctx->stream),
+                       "contract items");
 }
 
 int contract_batch_launch(qtt_ctx* ctx, const BatchStage& st) {
   if (st.n <= 0) return QTT_OK;
   const ContractItem* items = (const ContractItem*)st.dev;
+  if (st.nseg > 0) {
+    const ContractSeg* segs = (const ContractSeg*)st.seg;
+    const dim3 g((unsigned)st.nseg);
+    if (st.precision == QTT_F32)
+      contract_seg_kernel<QTT_F32><<<g, 256, 0, ctx->stream>>>(items, segs);
+    else if (st.precision == QTT_F32_SPLIT)
+      contract_seg_kernel<QTT_F32_SPLIT><<<g, 256, 0, ctx->stream>>>(items, segs);
+    else
+      contract_seg_kernel<QTT_F64><<<g, 256, 0, ctx->stream>>>(items, segs);
+    return qtt_hip_check(ctx, hipGetLastError(), "contract_seg_kernel");
+  }
   const int64_t* d_es = (const int64_t*)st.aux;
   const int64_t nb = std::min<int64_t>((st.total + 255) / 256, 65536);
   const dim3 grid((unsigned)std::max<int64_t>(nb, 1));

@@ -164,6 +164,8 @@ struct BatchStage {
   int maxnv = 1;
   char* dev = nullptr;  // device tables
   char* aux = nullptr;  // a second table inside them
+  char* seg = nullptr;  // contraction: the segment table (nseg segments)
+  int64_t nseg = 0;
   double* d_out = nullptr;
   std::vector<char> host;
   std::vector<int> cls;  // evaluate: class per network read back after the run
