@@ -117,6 +117,36 @@ int qtt_net_load(qtt_ctx* ctx, const char* path, qtt_net** out);
 int qtt_mpo_save(qtt_ctx* ctx, const qtt_mpo* mpo, const char* path, const char* const* names);
 int qtt_mpo_load(qtt_ctx* ctx, const char* path, qtt_mpo** out);
 
+/* ---- networks on graphs with loops ----------------------------------------
+ * evaluate(f, xs; alg="bp") on a loopy layout (src/itensornetworkfunction.jl:96-106
+ * -> ITensorNetworks.scalar(tn; alg="bp"); the grids of
+ * test/test_realitensorfunction.jl:41).  The graph has nv vertices and ne
+ * edges (eu[e], ev[e]) with bond ebond[e] (1..32); vertex v carries a binary
+ * digit (sdim 2, vcoord, vdigit as in qtt_topology) or none (sdim 1).  Core v
+ * is row-major [sdim[v]][legs...], its legs being its edges in increasing
+ * edge id.  Per point, belief propagation on the projected network: sweeps
+ * over the vertices in order, each vertex sending its messages over its legs
+ * in edge order, every message normalised to unit 2-norm and used at once;
+ * uniform start; stop when a sweep changes no message entry by more than tol,
+ * or after maxiter sweeps.  The value is prod_v Z_v / prod_e Z_e (Z_v: vertex
+ * v contracted with its incoming messages; Z_e: the two messages of edge e
+ * contracted), exact on trees.  iters_out (nullable) receives the largest
+ * sweep count over the points.  Host buffers; synchronous; invalid points
+ * give QTT_ERANGE. */
+typedef struct {
+  int32_t nv, ne;
+  const int32_t* eu;
+  const int32_t* ev;
+  const int64_t* ebond;
+  const int32_t* sdim;
+  const int32_t* vcoord;
+  const int32_t* vdigit;
+  int32_t ncoord;
+  const int32_t* ndigits;
+} qtt_graph;
+int qtt_bp_eval_host(qtt_ctx* ctx, const qtt_graph* g, const double* const* cores, const double* xs,
+                     int64_t npts, double* out, int maxiter, double tol, int* iters_out);
+
 /* ---- evaluate ------------------------------------------------------------
  * Replaces evaluate(f, xs, dims; alg) (src/itensornetworkfunction.jl:96-106:
  * calculate_ind_values + project + scalar), batched over points.

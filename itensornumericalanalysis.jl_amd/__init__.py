@@ -14,6 +14,7 @@ from .network import ITensorNetworkFunction, Topology, TreeOperator
 from .builders import (const_itn, cos_itn, cosh_itn, delta_kernel, delta_p, exp_itn, poly_itn, random_itn, scale,
                        sin_itn, sinh_itn)
 from .files import load, save
+from .loopy import LoopyNetworkFunction, evaluate_bp
 from .algebra import (add, map_to_zero_operator, map_to_zeros, multiply, operator_proj,
                       partial_integrate)
 from .operators import (add_operators, backward_shift_op, compile_terms, first_derivative_operator,
@@ -33,6 +34,7 @@ __all__ = [
     "sin_itn", "cos_itn", "cosh_itn", "sinh_itn", "random_itn", "poly_itn", "add", "multiply", "scale",
     "partial_integrate", "reduced_indsnetworkmap", "delta_p", "delta_kernel", "operator_proj",
     "map_to_zero_operator", "map_to_zeros", "point_to_opsum", "compile_terms", "save", "load",
+    "LoopyNetworkFunction", "evaluate_bp",
     "stencil", "forward_shift_op", "backward_shift_op", "first_derivative_operator",
     "second_derivative_operator", "third_derivative_operator", "fourth_derivative_operator",
     "laplacian_operator", "identity_operator", "fredholm_operator", "add_operators", "evaluate",

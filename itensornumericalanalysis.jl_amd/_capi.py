@@ -25,7 +25,7 @@ EXPORTS = [
     "qtt_eval_grad_host", "qtt_net_size", "qtt_apply_batch", "qtt_integrate_batch", "qtt_eval_batch",
     "qtt_tune_eval", "qtt_eval_noalloc", "qtt_apply_grad", "qtt_mpo_size", "qtt_batch_run",
     "qtt_multiply", "qtt_add", "qtt_partial_integrate", "qtt_operator_proj", "qtt_mpo_bonds", "qtt_mpo_read",
-    "qtt_net_save", "qtt_net_load", "qtt_mpo_save", "qtt_mpo_load",
+    "qtt_net_save", "qtt_net_load", "qtt_mpo_save", "qtt_mpo_load", "qtt_bp_eval_host",
 ]
 
 
@@ -33,6 +33,12 @@ class EngineError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"libqtt {_NAMES.get(code, code)}: {msg}")
         self.code = code
+
+
+class QttGraph(C.Structure):
+    _fields_ = [("nv", C.c_int32), ("ne", C.c_int32), ("eu", C.POINTER(C.c_int32)), ("ev", C.POINTER(C.c_int32)),
+                ("ebond", C.POINTER(C.c_int64)), ("sdim", C.POINTER(C.c_int32)), ("vcoord", C.POINTER(C.c_int32)),
+                ("vdigit", C.POINTER(C.c_int32)), ("ncoord", C.c_int32), ("ndigits", C.POINTER(C.c_int32))]
 
 
 class QttTopology(C.Structure):
@@ -96,6 +102,8 @@ def lib():
         L.qtt_net_load.argtypes = [vp, C.c_char_p, C.POINTER(vp)]
         L.qtt_mpo_save.argtypes = [vp, vp, C.c_char_p, cpp]
         L.qtt_mpo_load.argtypes = [vp, C.c_char_p, C.POINTER(vp)]
+        L.qtt_bp_eval_host.argtypes = [vp, C.POINTER(QttGraph), dpp, dp, C.c_int64, dp, C.c_int, C.c_double,
+                                       C.POINTER(C.c_int)]
         L.qtt_partial_integrate.argtypes = [vp, vp, i32p, C.c_double, C.POINTER(QttTopology), i32p,
                                             C.POINTER(vp)]
         L.qtt_batch_run.argtypes = [vp, C.c_int, C.POINTER(vp), C.POINTER(vp), C.c_double, C.c_int64, C.c_int,

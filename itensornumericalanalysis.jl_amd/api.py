@@ -108,7 +108,14 @@ def evaluate(f: ITensorNetworkFunction, xs, dims=None, precision: int = Precisio
     With a CUDA tensor of points the call is asynchronous: it is queued on
     torch's current stream and `out` (a CUDA tensor) is ready when that stream
     reaches it.  An invalid coordinate then yields NaN at that point and a
-    DigitMapError from the next `engine_context().sync()`."""
+    DigitMapError from the next `engine_context().sync()`.  A network on a
+    layout with loops (LoopyNetworkFunction) is evaluated by belief
+    propagation (evaluate(f, xs; alg="bp") of the reference), host points only."""
+    from .loopy import LoopyNetworkFunction, evaluate_bp
+    if isinstance(f, LoopyNetworkFunction):  # a layout with loops: belief propagation
+        if dims is not None:
+            raise ValueError("give every coordinate of a loopy network's points")
+        return evaluate_bp(f, xs)
     dn = device_network(f)
     ctx = dn.ctx
     try:

@@ -10,7 +10,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["capi.hip", "eval.hip", "sweep.hip", "factor.hip", "grad.hip", "algebra.hip", "io.hip"]
+SOURCES = ["capi.hip", "eval.hip", "sweep.hip", "factor.hip", "grad.hip", "algebra.hip", "io.hip", "bp.hip"]
 LIB = os.path.join(HERE, "libqtt.so")
 
 HIPCC_FLAGS = [

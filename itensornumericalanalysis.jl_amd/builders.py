@@ -52,7 +52,14 @@ def _product_state(s: IndsNetworkMap, site_factor, root_scale=1.0):
 
 def const_itn(s: IndsNetworkMap, c=1.0, linkdim: int = 1):
     """f = c.  With linkdim > 1 every core is a delta over its bonds (the
-    reference's c_tensor, src/utils.jl:42-47) and the root carries c / linkdim."""
+    reference's c_tensor, src/utils.jl:42-47) and the root carries c / linkdim.
+    On a layout with loops: bond 1, evaluated by belief propagation."""
+    if not s.graph.is_tree():
+        from .loopy import loopy_const
+        if linkdim != 1:
+            raise NotImplementedError("a constant with linkdim > 1 on a graph with loops needs exact "
+                                      "contraction, which the engine does not do (only belief propagation)")
+        return loopy_const(s, c)
     topo = Topology(s)
     chi = int(linkdim)
     bond = np.full(topo.nv, chi, dtype=np.int64)
@@ -137,7 +144,11 @@ def sinh_itn(s: IndsNetworkMap, k=1.0, a=0.0, c=1.0, dim: int = 1):
 def random_itn(s: IndsNetworkMap, link_space: int = 1, seed=None, rng=None, std=1.0,
                normalize_cores=False):
     """Dense random cores, entries N(0, std^2), every bond = link_space.  Cores are
-    drawn in internal vertex order from `numpy.random.default_rng(seed)`."""
+    drawn in internal vertex order from `numpy.random.default_rng(seed)`.  On a
+    layout with loops: a LoopyNetworkFunction (cores in graph vertex order)."""
+    if not s.graph.is_tree():
+        from .loopy import loopy_random
+        return loopy_random(s, link_space, rng if rng is not None else np.random.default_rng(seed), std)
     topo = Topology(s)
     if rng is None:
         rng = np.random.default_rng(seed)

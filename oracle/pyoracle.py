@@ -287,3 +287,83 @@ def direct_sum(parent, sdim, coresF, coresG):
             C[fb] = B
         out.append(C)
     return out
+
+
+# ---------------------------------------------------------------------------
+# Belief propagation on a network with loops, per point (the scalar of the
+# projected network by ITensorNetworks' BP, as evaluate(f; alg="bp") uses it,
+# src/itensornetworkfunction.jl:96-106), restated in numpy; and the exact
+# scalar of a projected network by brute force, for small graphs.
+# Cores: core v is [site][legs...], its legs being its edges in edge order.
+# ---------------------------------------------------------------------------
+def _project(nv, sdim, vcoord, vdigit, Lc, cores, x):
+    T = []
+    for v in range(nv):
+        d = digit(x[vcoord[v]], int(Lc[vcoord[v]]), int(vdigit[v])) if vcoord[v] >= 0 else 0
+        if d < 0:
+            return None
+        T.append(np.asarray(cores[v])[d])
+    return T
+
+
+def loopy_exact(nv, edges, sdim, vcoord, vdigit, Lc, cores, xs):
+    """Exact scalar of every projected network (einsum over all edges)."""
+    legs = [[e for e, (a, b) in enumerate(edges) if v in (a, b)] for v in range(nv)]
+    out = []
+    for x in np.atleast_2d(xs):
+        T = _project(nv, sdim, vcoord, vdigit, Lc, cores, x)
+        args = []
+        for v in range(nv):
+            args += [T[v], legs[v]]
+        out.append(float(np.einsum(*args, [], optimize="greedy")))
+    return np.array(out)
+
+
+def bp_evaluate(nv, edges, sdim, vcoord, vdigit, Lc, cores, xs, maxiter=200, tol=1e-13):
+    """BP estimate of every projected network: sweeps over the vertices in
+    order, each vertex sending its messages over its legs in edge order,
+    every message normalised to unit 2-norm and used at once; uniform start;
+    stop when no entry of a sweep changes by more than tol or after maxiter
+    sweeps; value prod_v Z_v / prod_e Z_e.  Returns (values, sweeps)."""
+    legs = [[e for e, (a, b) in enumerate(edges) if v in (a, b)] for v in range(nv)]
+    other = lambda e, v: edges[e][1] if edges[e][0] == v else edges[e][0]
+    vals, sweeps = [], 0
+    for x in np.atleast_2d(xs):
+        T = _project(nv, sdim, vcoord, vdigit, Lc, cores, x)
+        if T is None:
+            vals.append(np.nan)
+            continue
+        # msg[(e, src)]: the message sent by src over edge e
+        msg = {}
+        for e, (a, b) in enumerate(edges):
+            bd = T[a].shape[legs[a].index(e)]
+            msg[(e, a)] = np.ones(bd)
+            msg[(e, b)] = np.ones(bd)
+        it = 0
+        while it < maxiter:
+            change = 0.0
+            for v in range(nv):
+                for k, e in enumerate(legs[v]):
+                    args = [T[v], list(range(len(legs[v])))]
+                    for kk, e2 in enumerate(legs[v]):
+                        if kk != k:
+                            args += [msg[(e2, other(e2, v))], [kk]]
+                    m = np.einsum(*args, [k])
+                    nrm = np.linalg.norm(m)
+                    m = m / nrm if nrm > 0 else m * 0.0
+                    change = max(change, float(np.max(np.abs(m - msg[(e, v)]))))
+                    msg[(e, v)] = m
+            it += 1
+            if change <= tol:
+                break
+        sweeps = max(sweeps, it)
+        val = 1.0
+        for v in range(nv):
+            args = [T[v], list(range(len(legs[v])))]
+            for kk, e2 in enumerate(legs[v]):
+                args += [msg[(e2, other(e2, v))], [kk]]
+            val *= float(np.einsum(*args, []))
+        for e, (a, b) in enumerate(edges):
+            val /= float(msg[(e, a)] @ msg[(e, b)])
+        vals.append(val)
+    return np.array(vals), sweeps
