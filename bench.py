@@ -188,8 +188,14 @@ def main():
     # 16 (from 2^19 points), one up to 32, LDS vectors above
     ppl = tuned["points_per_lane"] if tuned and tuned["points_per_lane"] else (
         2 if chi <= 16 and args.npts >= (1 << 19) else 1)
-    kname = (f"eval_chain_reg2_kernel<{chi}>" if chi <= 16 and ppl == 2 else
-             f"eval_chain_reg_kernel<{chi}>" if chi <= 32 else "eval_chain_f64_kernel")
+    # environment tables: the tuned size, else the launcher's rule (from 2^15 points)
+    lg = max(0, int(np.floor(np.log2(args.npts))))
+    tbits = tuned["table_digits"] if tuned else (0 if args.npts < (1 << 15) else min(16, lg - 5))
+    if chi <= 32 and tbits > 0:
+        kname = f"env_table_kernel + eval_chain_env{'2' if chi <= 16 and ppl == 2 else ''}_kernel<{chi}>"
+    else:
+        kname = (f"eval_chain_reg2_kernel<{chi}>" if chi <= 16 and ppl == 2 else
+                 f"eval_chain_reg_kernel<{chi}>" if chi <= 32 else "eval_chain_f64_kernel")
     traffic = measured_traffic(args.npts)
 
     # parity of the timed result against the CPU oracle on a sample of points
@@ -261,6 +267,8 @@ def main():
                      "issued_on": "VALU FP64 FMA (no MFMA instruction; MI355X's vector FP64 peak equals "
                                   "its matrix FP64 peak, 78.6 TFLOP/s)",
                      "kernel": kname, "kernel_ms": k64, "launch_shape": tuned,
+                     "kernel_ms_note": "one evaluate call timed with HIP events on its stream: the table "
+                                       "launch and the chain launch together",
                      "hbm_gbs": bytes_pt * args.npts / (k64 * 1e-3) / 1e9},
         "roofline_operate": {"bound": "mfma", "achieved": op_flops / (op_ms * 1e-3) / 1e12,
                              "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
