@@ -296,13 +296,19 @@ int qtt_eval_noalloc(qtt_ctx* ctx, const qtt_net* net, const double* xs, int64_t
 /* ---- autotuning (no reference counterpart) --------------------------------
  * Times every launch shape of the float64 register chain evaluation on these
  * device points (xs [npts][ncoord]; out [npts] is overwritten by the runs):
- * digit sites per environment table (0 = no tables, else 8, 10, ... up to 20
+ * digit sites per environment table (0 = no tables, else 8, 10, ... up to 16
  * and half of the chain's digit sites), points per lane 1 or 2, workgroups of
  * 64, 128 or 256 threads.  The fastest is kept in the context for networks of
  * the same bond class and point counts of the same octave; later qtt_eval
  * calls use it.  cfg[3] receives {points per lane, workgroup size, table
  * digit sites} ({0, 0, 0} if the network does not take that kernel: not a
- * chain, bonds over 32), ms the time of one call.  Synchronous. */
+ * chain, bonds over 32), ms the time of one call once the network's tables
+ * exist.  Synchronous.
+ * Environment tables (the prefix and suffix products of a chain for every
+ * value of their digits) are built by the first qtt_eval of a network with a
+ * given table size and kept with the network, like its packed copy, until its
+ * cores change (a truncation) or it is destroyed: later calls only run the
+ * chain kernel. */
 int qtt_tune_eval(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t npts, double* out, int* cfg,
                   double* ms);
 

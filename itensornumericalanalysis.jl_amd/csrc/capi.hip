@@ -492,6 +492,7 @@ void qtt_net_destroy(qtt_net* n) {
   if (n->d_pack_off) (void)hipFree(n->d_pack_off);
   if (n->d_pack32) (void)hipFree(n->d_pack32);
   if (n->d_desc) (void)hipFree(n->d_desc);
+  if (n->d_env) (void)hipFree(n->d_env);
   qtt_ctx* c = n->ctx;
   delete n;
   ctx_release(c);

@@ -909,6 +909,7 @@ static int ensure_pack(qtt_ctx* ctx, qtt_net* net) {
   if (rc) return rc;
   net->pack_valid = true;
   net->pack32_valid = false;
+  net->env_valid = false;
   return QTT_OK;
 }
 
@@ -1064,14 +1065,32 @@ static int launch_eval_env(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t
   const EnvPlan pl = env_plan(net, tb);
   if (pl.bl == 0) return launch_eval_reg(ctx, net, xs, npts, out, maxb, ppl, wg);
   const int64_t nL = (int64_t(1) << pl.bl) * pl.lw, nR = (int64_t(1) << pl.br) * pl.rw;
-  double* scr = qtt_scratch(ctx, (size_t)(nL + nR) * sizeof(double));
-  if (!scr) return qtt_fail(ctx, QTT_ENOMEM, "evaluate: environment tables");
-  const EnvTableArgs a = env_table_args(net, pl, true, scr), b = env_table_args(net, pl, false, scr + nL);
   const TopoView tv = net->view();
-  env_table_kernel<<<a.nblk + b.nblk, 256, std::max(env_lds(a), env_lds(b)), ctx->stream>>>(
-      net->d_data, net->d_off, net->d_bond, tv, a, b);
-  int rc = qtt_hip_check(ctx, hipGetLastError(), "env_table_kernel");
-  if (rc) return rc;
+  // The tables depend only on the cores: build them on the first call with
+  // this size and keep them with the network (as the packed copy) until the
+  // cores change.
+  if (!(net->env_valid && net->env_tb == tb)) {
+    net->env_valid = false;
+    const int64_t need = nL + nR;
+    if (net->d_env && (net->env_elems < need || net->env_elems > 4 * need)) {
+      (void)hipFree(net->d_env);
+      net->d_env = nullptr;
+      net->env_elems = 0;
+    }
+    if (!net->d_env) {
+      int rc = qtt_hip_check(ctx, hipMalloc(&net->d_env, need * sizeof(double)), "hipMalloc environment tables");
+      if (rc) return rc;
+      net->env_elems = need;
+    }
+    const EnvTableArgs a = env_table_args(net, pl, true, net->d_env), b = env_table_args(net, pl, false, net->d_env + nL);
+    env_table_kernel<<<a.nblk + b.nblk, 256, std::max(env_lds(a), env_lds(b)), ctx->stream>>>(
+        net->d_data, net->d_off, net->d_bond, tv, a, b);
+    int rc = qtt_hip_check(ctx, hipGetLastError(), "env_table_kernel");
+    if (rc) return rc;
+    net->env_valid = true;
+    net->env_tb = tb;
+  }
+  double* scr = net->d_env;
   EnvView ev{scr, scr + nL, (int)pl.lw, (int)pl.rw, pl.ml, pl.mr};
   const SiteDesc* dd = (const SiteDesc*)net->d_desc;
   const int bsr = wg;
@@ -1103,11 +1122,12 @@ static int launch_eval_env(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t
 // Time every launch shape of the float64 register chain evaluation on these
 // points (one warm-up and five timed launches each, HIP events on the
 // context's stream) and keep the fastest for this bond class and point-count
-// octave.  Shapes: digit sites per environment table (0: none, else 8 .. 20,
+// octave.  Shapes: digit sites per environment table (0: none, else 8 .. 16,
 // at most half of the digit sites), points per lane 1 or 2, workgroups of 64,
 // 128 or 256 threads.  cfg receives {points per lane, workgroup size, table
 // digit sites} ({0, 0, 0}: the network does not take the register kernel), ms
-// the time of one call (table build included).
+// the time of one call once the network's tables exist (the warm-up builds
+// them).
 int tune_eval(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t npts, double* out, int* cfg, double* ms) {
   cfg[0] = cfg[1] = cfg[2] = 0;
   *ms = 0.0;
@@ -1119,8 +1139,11 @@ int tune_eval(qtt_ctx* ctx, qtt_net* net, const double* xs, int64_t npts, double
   if (rc) return rc;
   int D = 0;
   for (int k = 0; k < t.nv; k++) D += t.vcoord[k] >= 0;
+  // tables of at most 2^16 rows each (8 MiB at bond 16): with the tables kept
+  // on the network the timed launches below measure the steady state, where
+  // larger tables would always win by turning the chain into two lookups
   std::vector<int> tbs{0};
-  for (int tb = 8; tb <= std::min(20, D / 2); tb += 2) tbs.push_back(tb);
+  for (int tb = 8; tb <= std::min(16, D / 2); tb += 2) tbs.push_back(tb);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   rc = qtt_hip_check(ctx, hipEventCreate(&e0), "tune event");
   if (!rc) rc = qtt_hip_check(ctx, hipEventCreate(&e1), "tune event");

@@ -3062,6 +3062,7 @@ int truncate_batch_finish(qtt_ctx* ctx, TruncStaged& st, qtt_net* const* nets, d
     for (int v = 0; v < nets[i]->t.nv; v++) nets[i]->bond[v] = im->b32[i][v];
     nets[i]->pack_valid = false;
     nets[i]->pack32_valid = false;
+    nets[i]->env_valid = false;
     if (errs) errs[i] = im->herr[i];
   }
   int fl[2];

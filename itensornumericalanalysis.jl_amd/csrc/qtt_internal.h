@@ -101,6 +101,13 @@ struct qtt_net {
   // float32 copies for the mixed-precision evaluate
   float* d_pack32 = nullptr;
   bool pack32_valid = false;
+  // environment tables of the chain evaluate (left then right), built by the
+  // first evaluate with table size env_tb and reused while the cores are
+  // unchanged (cleared with pack_valid)
+  double* d_env = nullptr;
+  int64_t env_elems = 0;
+  int env_tb = -1;
+  bool env_valid = false;
   // Networks made by one apply batch share two device allocations (cores;
   // offsets, topology and bonds): d_data, d_off, d_topo and d_bond point into
   // them, and the last network destroyed frees them.  Null: own allocations.

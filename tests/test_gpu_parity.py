@@ -131,6 +131,47 @@ def test_eval_environment_tables(q, po, eng, rng, monkeypatch, tbits, case):
     assert np.max(np.abs(y - ref)) < TOL * np.max(np.abs(ref))
 
 
+def test_eval_tables_kept_until_cores_change(q, po, eng, rng, monkeypatch):
+    """The tables built by one evaluate serve the next ones (other points, the
+    same table size), are rebuilt for another size, and are dropped when a
+    truncation changes the cores in place."""
+    import ctypes
+    import torch
+    from itensornumericalanalysis_jl_amd import _capi
+    s = q.continuous_siteinds(q.named_grid((20, 1)), map_dimension=2)
+    f = q.random_itn(s, link_space=12, rng=rng, std=1.0 / math.sqrt(12))
+    monkeypatch.setenv("QTT_EVAL_TABLE_BITS", "6")
+    ctx = _capi.Context(0)
+    monkeypatch.delenv("QTT_EVAL_TABLE_BITS")
+    try:
+        dn = _capi.DeviceNetwork.upload(ctx, f.topo, f.cores)
+
+        def ev(xs):
+            xd = torch.tensor(xs, device="cuda")
+            out = torch.empty(len(xs), dtype=torch.float64, device="cuda")
+            ctx.check(_capi.lib().qtt_eval(ctx.handle, dn.handle, ctypes.c_void_p(xd.data_ptr()), len(xs),
+                                           ctypes.c_void_p(out.data_ptr()), 0))
+            ctx.check(_capi.lib().qtt_sync(ctx.handle))
+            return out.cpu().numpy()
+
+        for _ in range(3):  # first call builds the tables, the others reuse them
+            xs = rng.uniform(0, 1, (5000, 2))
+            assert rel_l2(ev(xs), oracle_eval(po, f, xs)) < TOL
+        # truncate in place: the cores (and bonds) change under the same handle
+        err = ctypes.c_double(0.0)
+        ctx.check(_capi.lib().qtt_truncate(ctx.handle, dn.handle, 0.0, 5, ctypes.byref(err)))
+        assert err.value > 0.0
+        cores = dn.download()
+        t = f.topo
+        xs = rng.uniform(0, 1, (5000, 2))
+        ref, rc = po.evaluate(t.parent, t.sdim, cores, t.vcoord, t.vdigit, t.Lc, xs)
+        assert rc == 0
+        assert rel_l2(ev(xs), ref) < TOL
+        dn._release()
+    finally:
+        ctx.close()
+
+
 def test_eval_environment_tables_invalid_point(q, eng, rng, monkeypatch):
     s = q.continuous_siteinds(q.named_grid((16, 1)), map_dimension=2)
     f = q.random_itn(s, link_space=6, rng=rng, std=0.4)
