@@ -42,6 +42,19 @@ extern "C" int qtt_debug_times(unsigned long long* out, int n) {
 // the move's shape in the two spare slots of its record
 #define QTT_SHAPE(step, rows, b, tr) \
   do { if (threadIdx.x == 0 && (step) < 8192) { qtt_phase_t[(step) * 8 + 6] = (rows); qtt_phase_t[(step) * 8 + 7] = (b) | ((tr) << 20); } } while (0)
+// Jacobi sweeps (low 16 bits) and calls (high bits) of the step opened last
+__device__ unsigned long long qtt_phase_sweeps[8192];
+extern "C" int qtt_debug_sweeps(unsigned long long* out, int n) {
+  return (int)hipMemcpyFromSymbol(out, HIP_SYMBOL(qtt_phase_sweeps), (size_t)n * sizeof(unsigned long long));
+}
+#define QTT_SWEEPS(s) \
+  do { if (threadIdx.x == 0 && qtt_phase_step < 8192) qtt_phase_sweeps[qtt_phase_step] += (unsigned long long)(s) + (1ull << 16); } while (0)
+#define QTT_SWEEPS_CLEAR(step) do { if (threadIdx.x == 0 && (step) < 8192) qtt_phase_sweeps[step] = 0; } while (0)
+#endif
+// Cycle counters inside the Jacobi rounds and QR panels (-DQTT_JACOBI_TIMING,
+// on top of the phase timestamps); each mark is a read-modify-write of global
+// memory by lane 0, so they slow the rounds they measure.
+#ifdef QTT_JACOBI_TIMING
 // cycles of lane 0 per segment of a Jacobi round, plus round / sweep counts
 __device__ unsigned long long qtt_jac_t[8];
 extern "C" int qtt_debug_jacobi(unsigned long long* out) {
@@ -72,11 +85,15 @@ extern "C" int qtt_debug_panel(unsigned long long* out) {
 #define QTT_PT_INIT do { } while (0)
 #define QTT_PT(i) do { } while (0)
 #define QTT_PANEL_COUNT (void)0
-#define QTT_PHASE(step, phase) do { } while (0)
-#define QTT_SHAPE(step, rows, b, tr) do { } while (0)
 #define QTT_JT_INIT do { } while (0)
 #define QTT_JT(i) do { } while (0)
 #define QTT_JC(i) do { } while (0)
+#endif
+#ifndef QTT_PHASE_TIMING
+#define QTT_PHASE(step, phase) do { } while (0)
+#define QTT_SHAPE(step, rows, b, tr) do { } while (0)
+#define QTT_SWEEPS(s) do { } while (0)
+#define QTT_SWEEPS_CLEAR(step) do { } while (0)
 #endif
 
 namespace {
@@ -755,7 +772,10 @@ __device__ __forceinline__ void qr_body(DP W, int rows, int cols, DP tau, DP Rb)
       for (int s = 0; s < NS; ++s) {
         const int c = gid + ng * s;
         if (c > j && c < cols) {
-          if (tj != 0.0) reg_apply_mem<GL, RPL>(a[s], VT, j, gl, tj);
+          // (tau 0 leaves the column as it is: the reflector is finite, so the
+          // update subtracts zeros; not testing it lets tau and the reflector
+          // load together)
+          reg_apply_mem<GL, RPL>(a[s], VT, j, gl, tj);
           if (c == j + 1 && c < k) reg_reflector<GL, RPL>(a[s], c, gl, VT, tau);
         }
       }
@@ -918,6 +938,14 @@ __device__ bool noise_block_exact(const double* sig, int n, double cut, int* slo
   return r != 0;
 }
 
+// Stopping rule of the register Jacobi: a sweep whose rotations all had
+// |cos(angle between the rows)| at most sqrt(tol) / 8 ends the iteration.
+// One-sided Jacobi converges quadratically, so the next sweep's angles would
+// be about the square of these, below the rotation threshold tol (tol =
+// eps max(n, b)): that sweep would rotate nothing, or by angles of order
+// eps, and is not run.  Returns the bound on g^2 / (a c).
+__device__ __forceinline__ double jacobi_stop2(double tol) { return tol * (1.0 / 64.0); }
+
 // One-sided Jacobi on the rows of X (n x b, ld b), rotations accumulated in J
 // (n x n; set to the identity first when init_j).  Returns false if it did
 // not converge.
@@ -1075,6 +1103,23 @@ __device__ __forceinline__ double rsq_nr(double x) {
   return __builtin_fma(0.5 * y, e, y);
 }
 
+// Rotation of a row pair with squared norms a, c and inner product g != 0
+// that makes the rows orthogonal: tan = t with
+//   t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)),  zeta = (c - a) / (2 g),
+// evaluated as t = +-|2 g| / (|c - a| + sqrt((c - a)^2 + 4 g^2)) (the same
+// value with one reciprocal fewer on the dependent chain of a round); then
+// cs = 1 / sqrt(1 + t^2), sn = cs t.  (a, c and g come from row entries, so
+// (c - a)^2 + 4 g^2 stays finite as a c does.)
+__device__ __forceinline__ void jacobi_rotation(double a, double c, double g, double& cs, double& sn) {
+  const double d = c - a, g2 = 2.0 * g;
+  const double r2 = __builtin_fma(d, d, g2 * g2);
+  const double h = r2 * rsq_nr(r2);                 // sqrt((c - a)^2 + 4 g^2)
+  const double t0 = fabs(g2) * rcp_nr(fabs(d) + h);  // |t|
+  const double tt = d * g >= 0.0 ? t0 : -t0;         // sign of zeta (+ at zeta = 0)
+  cs = rsq_nr(__builtin_fma(tt, tt, 1.0));
+  sn = cs * tt;
+}
+
 template <int JE>
 __device__ __forceinline__ void rotate4(double (&u)[JE], double (&w)[JE], double cs, double sn) {
 #pragma unroll
@@ -1109,6 +1154,7 @@ __device__ __noinline__ bool jacobi_rows_reg(F* X, int ldx_in, int n_in, int b_i
   const double fsum = block_sum(fro, red);
   const double small = fsum * DBL_EPSILON * DBL_EPSILON;
   const double noise = noise_level(fsum, cut, n);
+  const double stop2 = jacobi_stop2(tol);
   QTT_JT_INIT;
   for (int sweep = 0; sweep < max_sweeps; ++sweep) {
     if (tid == 0) *flag = 0;
@@ -1149,12 +1195,8 @@ __device__ __noinline__ bool jacobi_rows_reg(F* X, int ldx_in, int n_in, int b_i
         QTT_JT(0);
         // |g| > tol sqrt(a c), compared squared (a c stays finite for core entries)
         if (g != 0.0 && a > small && c > small && (a > noise || c > noise) && g * g > tol * tol * (a * c)) {
-          const double zeta = (c - a) * rcp_nr(2.0 * g);
-          const double q1 = __builtin_fma(zeta, zeta, 1.0);
-          const double r = q1 * rsq_nr(q1);  // sqrt(1 + zeta^2)
-          const double t0 = rcp_nr(fabs(zeta) + r);
-          const double tt = zeta >= 0.0 ? t0 : -t0;
-          const double cs = rsq_nr(__builtin_fma(tt, tt, 1.0)), sn = cs * tt;
+          double cs, sn;
+          jacobi_rotation(a, c, g, cs, sn);
           QTT_JT(1);
           rotate4(u, w, cs, sn);
           rotate4(ju, jw, cs, sn);
@@ -1162,7 +1204,7 @@ __device__ __noinline__ bool jacobi_rows_reg(F* X, int ldx_in, int n_in, int b_i
           store_row<G, JE>(xq, b, gl, w);
           store_row<G, JE>(jp, n, gl, ju);
           store_row<G, JE>(jq, n, gl, jw);
-          if (gl == 0) *flag = 1;
+          if (gl == 0 && g * g > stop2 * (a * c)) *flag = 1;
           QTT_JT(2);
         }
       }
@@ -1172,9 +1214,152 @@ __device__ __noinline__ bool jacobi_rows_reg(F* X, int ldx_in, int n_in, int b_i
     }
     const int rot = *flag;
     __syncthreads();
-    if (!rot) return true;
+    if (!rot) {
+      QTT_SWEEPS(sweep + 1);
+      return true;
+    }
   }
+  QTT_SWEEPS(max_sweeps);
   return false;
+}
+
+// The one-sided Jacobi of jacobi_rows_reg for small matrices, run by the first
+// wavefront alone: pair pi of each round goes to lanes [G pi, G pi + G), so
+// every pair of a round is in flight at once when n <= 128 / G.  LDS
+// operations of one wavefront complete in the order they are issued, so each
+// round reads the rows the round before it wrote with no barrier between
+// rounds; the other wavefronts wait at the barrier that ends the call.  A
+// round then costs one LDS round trip plus the rotation, not a block-wide
+// barrier behind the slowest wavefront.  The rows are padded with zeros to bp
+// (X) and np (J) entries, multiples of G of at most G JE and G JJ, so every
+// lane loads, rotates and stores whole blocks with no masks; the zeros stay
+// zero.  Rotations, tests and stopping rule are those of jacobi_rows_reg;
+// J must hold its start (the identity or the rotations so far).
+template <int G, int JE>
+__device__ __forceinline__ void wave_load(const lds_f64* r, int len, int gl, double (&x)[JE]) {
+#pragma unroll
+  for (int e = 0; e < JE; ++e) x[e] = (G * e < len) ? r[gl + G * e] : 0.0;
+}
+
+template <int G, int JE>
+__device__ __forceinline__ void wave_store(lds_f64* r, int len, int gl, const double (&x)[JE]) {
+#pragma unroll
+  for (int e = 0; e < JE; ++e)
+    if (G * e < len) r[gl + G * e] = x[e];
+}
+
+template <int G, int JE, int JJ>
+__device__ __noinline__ bool jacobi_wave(lds_f64* X, int ldx_in, int n_in, int b_in, int bp_in, lds_f64* J,
+                                         int ldj_in, int np_in, lds_i32* flag, double* red, double cut,
+                                         int max_sweeps_in) {
+  static_assert(JE % 2 == 0, "two partial sums per row");
+  const int ldx = uni(ldx_in), n = uni(n_in), b = uni(b_in), bp = uni(bp_in), ldj = uni(ldj_in), np = uni(np_in);
+  const int max_sweeps = uni(max_sweeps_in);
+  const int tid = threadIdx.x, bs = blockDim.x;
+  if (n < 2) return true;
+  const int nn = n + (n & 1);
+  const double tol = DBL_EPSILON * (double)(b > n ? b : n);
+  double fro = 0.0;
+  for (int e = tid; e < n * b; e += bs) {
+    const double x = X[(e / b) * ldx + e % b];
+    fro += x * x;
+  }
+  const double fsum = block_sum(fro, red);
+  const double small = fsum * DBL_EPSILON * DBL_EPSILON;
+  const double noise = noise_level(fsum, cut, n);
+  const double stop2 = jacobi_stop2(tol);
+  if (tid < 64) {
+    const int pi = tid / G;
+    const bool player = pi < nn / 2;
+    int sweeps = max_sweeps, conv = 0;
+    for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+      int rot = 0;
+      for (int t = 0; t < nn - 1; ++t) {
+        int p = rr_player(pi, t, nn), q = rr_player(nn - 1 - pi, t, nn);
+        if (p > q) {
+          const int tmp = p;
+          p = q;
+          q = tmp;
+        }
+        if (player && q < n) {
+          const int gl = tid & (G - 1);
+          lds_f64* xp = X + p * ldx;
+          lds_f64* xq = X + q * ldx;
+          lds_f64* jp = J + p * ldj;
+          lds_f64* jq = J + q * ldj;
+          double u[JE], w[JE], ju[JJ], jw[JJ];
+          wave_load<G, JE>(xp, bp, gl, u);
+          wave_load<G, JE>(xq, bp, gl, w);
+          wave_load<G, JJ>(jp, np, gl, ju);
+          wave_load<G, JJ>(jq, np, gl, jw);
+          double a0 = 0.0, c0 = 0.0, g0 = 0.0, a1 = 0.0, c1 = 0.0, g1 = 0.0;
+#pragma unroll
+          for (int e = 0; e < JE; e += 2) {
+            a0 = __builtin_fma(u[e], u[e], a0);
+            c0 = __builtin_fma(w[e], w[e], c0);
+            g0 = __builtin_fma(u[e], w[e], g0);
+            a1 = __builtin_fma(u[e + 1], u[e + 1], a1);
+            c1 = __builtin_fma(w[e + 1], w[e + 1], c1);
+            g1 = __builtin_fma(u[e + 1], w[e + 1], g1);
+          }
+          double a = a0 + a1, c = c0 + c1, g = g0 + g1;
+          if constexpr (G == 4) {  // sums over the 4 lanes of a quad
+            a += dpp_d<0xB1>(a);
+            c += dpp_d<0xB1>(c);
+            g += dpp_d<0xB1>(g);
+            a += dpp_d<0x4E>(a);
+            c += dpp_d<0x4E>(c);
+            g += dpp_d<0x4E>(g);
+          } else {
+            a = jsum<G>(a);
+            c = jsum<G>(c);
+            g = jsum<G>(g);
+          }
+          if (g != 0.0 && a > small && c > small && (a > noise || c > noise) && g * g > tol * tol * (a * c)) {
+            double cs, sn;
+            jacobi_rotation(a, c, g, cs, sn);
+            rotate4(u, w, cs, sn);
+            rotate4(ju, jw, cs, sn);
+            wave_store<G, JE>(xp, bp, gl, u);
+            wave_store<G, JE>(xq, bp, gl, w);
+            wave_store<G, JJ>(jp, np, gl, ju);
+            wave_store<G, JJ>(jq, np, gl, jw);
+            rot |= g * g > stop2 * (a * c) ? 1 : 0;
+          }
+        }
+        // keep the next round's loads after this round's stores (the LDS
+        // itself serves one wavefront's operations in order)
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      }
+      if (__builtin_amdgcn_ballot_w64(rot != 0) == 0) {
+        sweeps = sweep + 1;
+        conv = 1;
+        break;
+      }
+    }
+    if (tid == 0) *flag = conv;
+    QTT_SWEEPS(sweeps);
+  }
+  __syncthreads();
+  const int cv = *flag;
+  __syncthreads();
+  return cv != 0;
+}
+
+// Lanes per pair of the single-wavefront Jacobi for an n x b matrix, 0 when it
+// does not apply: 8 lanes with 4 entries of a row each, up to 16 rows of at
+// most 32 entries.  (Longer rows make a round issue-bound on the one SIMD that
+// runs the wavefront; the rounds spread over the block are then faster.)
+__device__ __forceinline__ int wave_group(int n, int b) {
+  return n <= 16 && b <= 32 ? 8 : 0;
+}
+
+// LDS the single-wavefront Jacobi needs for its padded copy of X and J
+__device__ __forceinline__ int64_t wave_jacobi_lds(int n, int b) {
+  const int gw = wave_group(n, b);
+  if (!gw) return 0;
+  const int bp = (b + gw - 1) / gw * gw, np = (n + gw - 1) / gw * gw;
+  return (int64_t)n * ((bp + 1) + (np + 1)) + 16;
 }
 
 // ---------------------------------------------------------------------------
@@ -1973,12 +2158,8 @@ __device__ __noinline__ bool coop_pairs(double* X, double* J, int n_in, int b_in
     }
     const double a = jsum<64>(a0), c = jsum<64>(c0), g = jsum<64>(g0);
     if (g != 0.0 && a > small && c > small && (a > noise || c > noise) && g * g > tol * tol * (a * c)) {
-      const double zeta = (c - a) * rcp_nr(2.0 * g);
-      const double q1 = __builtin_fma(zeta, zeta, 1.0);
-      const double r = q1 * rsq_nr(q1);
-      const double t0 = rcp_nr(fabs(zeta) + r);
-      const double tt = zeta >= 0.0 ? t0 : -t0;
-      const double cs = rsq_nr(__builtin_fma(tt, tt, 1.0)), sn = cs * tt;
+      double cs, sn;
+      jacobi_rotation(a, c, g, cs, sn);
       rotate4<E>(u, w, cs, sn);
       rotate4<E>(ju, jw, cs, sn);
       store_row<64, E>(xp, b, gl, u);
@@ -2048,12 +2229,8 @@ __device__ __noinline__ bool coop_block(double* X, double* J, int n_in, int b_in
         }
         const double a = jsum<64>(a0), c = jsum<64>(c0), g = jsum<64>(g0);
         if (g != 0.0 && a > small && c > small && (a > noise || c > noise) && g * g > tol * tol * (a * c)) {
-          const double zeta = (c - a) * rcp_nr(2.0 * g);
-          const double q1 = __builtin_fma(zeta, zeta, 1.0);
-          const double r = q1 * rsq_nr(q1);
-          const double t0 = rcp_nr(fabs(zeta) + r);
-          const double tt = zeta >= 0.0 ? t0 : -t0;
-          const double cs = rsq_nr(__builtin_fma(tt, tt, 1.0)), sn = cs * tt;
+          double cs, sn;
+          jacobi_rotation(a, c, g, cs, sn);
           rotate4<E>(u, w, cs, sn);
           rotate4<E>(ju, jw, cs, sn);
           store_row<64, E>(Xl + i * lx, b, gl, u);
@@ -2328,7 +2505,7 @@ __device__ bool mode_product_coop(const double* Rrows, const int* perm, bool use
 // scratch.  Two separate instantiations (one per memory) let the compiler use
 // LDS instructions on the LDS path instead of generic flat addressing.
 template <int G, bool REG>
-__device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job, const int nwg, double* jpad,
+__device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job, const int nwg, double* jpad, const bool jwave,
                                         lds_f64* pnl, const int pnl_cap, const int pmode, const int rows,
                                         const int b, const int tr, const int rpl, const int bx,
                                         const int64_t Au, const int64_t nu, const int64_t Bu, double* Xc,
@@ -2338,8 +2515,8 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
   const int k = rows < b ? rows : b;
   // register QR for the shapes it covers (the QR of a tall SVD matrix included)
   // register QR for the QR steps: rpl = rows per lane bound / 8 (8: rows <= 64,
-  // 16: rows <= 128, both 8-lane groups with 2 column slots; 24: rows <= 192,
-  // 16-lane groups with 3 column slots)
+  // 16: rows <= 128, both 8-lane groups with 2 column slots; 20: rows <= 160,
+  // 24: rows <= 192, 16-lane groups with 3 column slots)
   const bool use_reg = REG && G == 8 && rpl != 0;
   // workspace: W rows x b (or the reflector rows) | R k x b | J k x k (SVD only) |
   //            tau, sig, perm | 2 staging rows
@@ -2371,12 +2548,14 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
       lds_f64 *Wl = (lds_f64*)W, *taul = (lds_f64*)tau, *Rl = (lds_f64*)Rb;
       if (rpl == 8) qr_reg<8, 8, 2>(Wl, rows, b, taul, Rl);
       else if (rpl == 16) qr_reg<8, 16, 2>(Wl, rows, b, taul, Rl);
+      else if (rpl == 20) qr_reg<16, 10, 3>(Wl, rows, b, taul, Rl);
       else qr_reg<16, 12, 3>(Wl, rows, b, taul, Rl);
       __syncthreads();
       QTT_PHASE(-1, 2);  // factored
       // x <- Q (bond m)
       if (rpl == 8) q_form_reg<8, 8>(Wl, rows, k, taul, Xc, bx);
       else if (rpl == 16) q_form_reg<8, 16>(Wl, rows, k, taul, Xc, bx);
+      else if (rpl == 20) q_form_reg<16, 10>(Wl, rows, k, taul, Xc, bx);
       else q_form_reg<16, 12>(Wl, rows, k, taul, Xc, bx);
     } else if (pmode) {  // workspace in global memory: blocked QR, WY updates as MFMA tiles
       double* Tg = vbuf;
@@ -2415,6 +2594,7 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
         lds_f64 *Wl = (lds_f64*)W, *taul = (lds_f64*)tau, *Rl = (lds_f64*)Rb;
         if (rpl == 8) qr_reg<8, 8, 2>(Wl, rows, b, taul, Rl);
         else if (rpl == 16) qr_reg<8, 16, 2>(Wl, rows, b, taul, Rl);
+        else if (rpl == 20) qr_reg<16, 10, 3>(Wl, rows, b, taul, Rl);
         else qr_reg<16, 12, 3>(Wl, rows, b, taul, Rl);
       } else if (pmode) {  // blocked QR; the reflectors stay in W for U below
         if (pmode == 3) qr_tall(W, rows, b, b, tau, vbuf, pnl, job, nwg, iflag);
@@ -2442,7 +2622,28 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
     // J from the identity (else continue the rotations already in it)
     auto jacobi = [&](const double cut, const bool init_j) -> bool {
       bool cv;
-      if (REG && G == 8 && b <= 256) {
+      if (REG && G == 8 && jwave) {
+        // small matrices: the single-wavefront rounds on a zero-padded copy
+        const int gw = wave_group(n, b);
+        const int bp = (b + gw - 1) / gw * gw, np = (n + gw - 1) / gw * gw;
+        const int ldx = bp + 1, ldj = np + 1;  // odd strides: the rows of a round fall in different banks
+        lds_f64* Xl = (lds_f64*)jpad;
+        lds_f64* Jl = Xl + n * ldx;
+        for (int e = tid; e < n * bp; e += bs) {
+          const int r = e / bp, c = e - r * bp;
+          Xl[r * ldx + c] = c < b ? X[r * b + c] : 0.0;
+        }
+        for (int e = tid; e < n * np; e += bs) {
+          const int r = e / np, c = e - r * np;
+          Jl[r * ldj + c] = c >= n ? 0.0 : init_j ? (r == c ? 1.0 : 0.0) : Jb[r * n + c];
+        }
+        __syncthreads();
+        cv = jacobi_wave<8, 4, 2>((lds_f64*)Xl, ldx, n, b, bp, Jl, ldj, np, (lds_i32*)iflag, red, cut, max_sweeps);
+        __syncthreads();
+        for (int e = tid; e < n * b; e += bs) X[e] = Xl[(e / b) * ldx + e % b];
+        for (int e = tid; e < n * n; e += bs) Jb[e] = Jl[(e / n) * ldj + e % n];
+        __syncthreads();
+      } else if (REG && G == 8 && b <= 256) {
         // (REG: the workspace is in LDS.)  With room for it, the rounds run on a
         // copy whose row strides are odd numbers of doubles, so the rows that
         // the groups of a wavefront read together fall in different banks.
@@ -2550,6 +2751,7 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
       const lds_i32* pl = (const lds_i32*)perm;
       if (rpl == 8) u_form_reg<8, 8>(Wl, rows, b, taul, Jl, n, pl, m, Xc, bx);
       else if (rpl == 16) u_form_reg<8, 16>(Wl, rows, b, taul, Jl, n, pl, m, Xc, bx);
+      else if (rpl == 20) u_form_reg<16, 10>(Wl, rows, b, taul, Jl, n, pl, m, Xc, bx);
       else u_form_reg<16, 12>(Wl, rows, b, taul, Jl, n, pl, m, Xc, bx);
     } else {
     for (int e = tid; e < rows * m; e += bs) {
@@ -2646,14 +2848,16 @@ __global__ void __launch_bounds__(512) truncate_kernel(const SweepArgs* __restri
     const int rows = sx[0] * sx[2], b = sx[1];
     const int k = rows < b ? rows : b;
     QTT_SHAPE(st, rows, b, tr);
+    QTT_SWEEPS_CLEAR(st);
     const int64_t jsz = tr ? (int64_t)k * k : 0;
     const bool small = rows <= 256 && b <= 256;
     // register QR: rows <= 64 or <= 128 with columns <= 2 per 8-lane group, or
-    // rows <= 192 with columns <= 3 per 16-lane group (the reflector rows take
-    // k * 8 rpl doubles)
+    // rows <= 160 or <= 192 with columns <= 3 per 16-lane group (the reflector
+    // rows take k * 8 rpl doubles)
     int rpl = !small ? 0
                     : rows <= 64 && b <= 2 * (bs / 8) ? 8
                     : rows <= 128 && b <= 2 * (bs / 8) ? 16
+                    : rows <= 160 && b <= 3 * (bs / 16) ? 20
                     : rows <= 192 && b <= 3 * (bs / 16) ? 24 : 0;
     const int64_t base = (int64_t)k * b + jsz + 4 * (int64_t)k + 2 * (int64_t)rows + 16;
     int64_t need = max((int64_t)rows * b, rpl ? (int64_t)k * 8 * rpl : 0) + base;
@@ -2675,9 +2879,12 @@ __global__ void __launch_bounds__(512) truncate_kernel(const SweepArgs* __restri
     }
     // padded copy for the Jacobi rounds of an SVD move, when it fits after the workspace
     double* jpad = nullptr;
+    bool jwave = false;  // room for the single-wavefront Jacobi's copy at jpad
     if (tr) {
       const int nj = rows > b ? b : rows;
       if (need + (int64_t)nj * ((b | 1) + (nj | 1)) + 16 <= a.ws_lds) jpad = wsl + need;
+      const int64_t wl = wave_jacobi_lds(nj, b);
+      jwave = jpad && wl && need + wl <= a.ws_lds;
     }
     double* Xc = a.data + a.off[x];
     const double* Uc = a.data + a.off[u];
@@ -2685,17 +2892,17 @@ __global__ void __launch_bounds__(512) truncate_kernel(const SweepArgs* __restri
     int m;
     if (need <= a.ws_lds) {
       if (small)
-        m = run_move<8, true>(wsl, false, nullptr, 1, jpad, nullptr, 0, 0, rows, b, tr, rpl, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.max_sweeps, a.prec, a.status,
+        m = run_move<8, true>(wsl, false, nullptr, 1, jpad, jwave, nullptr, 0, 0, rows, b, tr, rpl, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.max_sweeps, a.prec, a.status,
                         iflag, red, dbc);
       else
-        m = run_move<64, false>(wsl, false, nullptr, 1, nullptr, nullptr, 0, 0, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.max_sweeps, a.prec, a.status,
+        m = run_move<64, false>(wsl, false, nullptr, 1, nullptr, false, nullptr, 0, 0, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.max_sweeps, a.prec, a.status,
                          iflag, red, dbc);
     } else {
       if (small)
-        m = run_move<8, false>(a.gscr, true, a.job, a.nwg, nullptr, pnl, pnl_cap, pmode, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.max_sweeps, a.prec, a.status,
+        m = run_move<8, false>(a.gscr, true, a.job, a.nwg, nullptr, false, pnl, pnl_cap, pmode, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.max_sweeps, a.prec, a.status,
                         iflag, red, dbc);
       else
-        m = run_move<64, false>(a.gscr, true, a.job, a.nwg, nullptr, pnl, pnl_cap, pmode, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.max_sweeps, a.prec, a.status,
+        m = run_move<64, false>(a.gscr, true, a.job, a.nwg, nullptr, false, pnl, pnl_cap, pmode, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.max_sweeps, a.prec, a.status,
                          iflag, red, dbc);
     }
     QTT_PHASE(st, 5);
