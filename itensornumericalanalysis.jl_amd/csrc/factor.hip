@@ -938,7 +938,7 @@ __device__ bool noise_block_exact(const double* sig, int n, double cut, int* slo
   return r != 0;
 }
 
-// Stopping rule of the register Jacobi: a sweep whose rotations all had
+// Stopping rule of every Jacobi variant here: a sweep whose rotations all had
 // |cos(angle between the rows)| at most sqrt(tol) / 8 ends the iteration.
 // One-sided Jacobi converges quadratically, so the next sweep's angles would
 // be about the square of these, below the rotation threshold tol (tol =
@@ -1033,7 +1033,7 @@ __device__ bool jacobi_rows(double* X, int n, int b, double* J, int* flag, doubl
             jp[i] = cs * u0 - sn * w0;
             jq[i] = sn * u0 + cs * w0;
           }
-          if (gl == 0) *flag = 1;
+          if (gl == 0 && g * g > jacobi_stop2(tol) * (a * c)) *flag = 1;
         }
       }
       __syncthreads();
@@ -2123,7 +2123,8 @@ __device__ __noinline__ void mode_product_mfma(const double* Rrows, const int* p
 
 // One round of a job on this workgroup: pair pi = wg + nwg * slot goes to
 // wavefront slot (64 lanes per pair, E entries per lane of a row of X and of
-// J), as in jacobi_rows_reg.  Returns whether any of its pairs rotated.
+// J), as in jacobi_rows_reg.  Returns whether any of its pairs rotated by
+// more than the stopping rule's angle (jacobi_stop2).
 template <int E>
 __device__ __noinline__ bool coop_pairs(double* X, double* J, int n_in, int b_in, int ldx_in, int ldj_in, int t_in,
                                         int nn_in, double tol, double small, double noise, int wg_in, int nwg_in) {
@@ -2166,7 +2167,7 @@ __device__ __noinline__ bool coop_pairs(double* X, double* J, int n_in, int b_in
       store_row<64, E>(xq, b, gl, w);
       store_row<64, E>(jp, n, gl, ju);
       store_row<64, E>(jq, n, gl, jw);
-      any = true;
+      any |= g * g > jacobi_stop2(tol) * (a * c);
     }
   }
   return any;
@@ -2175,7 +2176,7 @@ __device__ __noinline__ bool coop_pairs(double* X, double* J, int n_in, int b_in
 // Type 8: this workgroup's block pair of block round t: the 2 bsz rows of X
 // and of J are brought into LDS, their pairs rotated (one wavefront per pair,
 // both rows of X and J in registers), and written back.  Returns whether any
-// pair rotated.
+// pair rotated by more than the stopping rule's angle (jacobi_stop2).
 template <int E>
 __device__ __noinline__ bool coop_block(double* X, double* J, int n_in, int b_in, int bsz_in, int nblk_in, int t_in,
                                         double tol, double small, double noise, int wg_in, lds_f64* lw) {
@@ -2237,7 +2238,7 @@ __device__ __noinline__ bool coop_block(double* X, double* J, int n_in, int b_in
           store_row<64, E>(Xl + k * lx, b, gl, w);
           store_row<64, E>(Jl + i * lj, n, gl, ju);
           store_row<64, E>(Jl + k * lj, n, gl, jw);
-          if (lane == 0) fl[0] = 1;
+          if (lane == 0 && g * g > jacobi_stop2(tol) * (a * c)) fl[0] = 1;
         }
       }
     }
@@ -2467,8 +2468,12 @@ __device__ bool jacobi_coop(double* X, int n, int b, double* J, CoopJob* job, in
     const int st = sflag[2];
     __syncthreads();
     if (st & 2) return false;  // a wait failed: report non-convergence
-    if (!st) return true;
+    if (!st) {
+      QTT_SWEEPS(sweep + 1);
+      return true;
+    }
   }
+  QTT_SWEEPS(max_sweeps);
   return false;
 }
 
