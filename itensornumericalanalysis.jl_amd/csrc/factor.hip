@@ -380,6 +380,8 @@ struct SweepArgs {
   int* status;   // [1] <- non-convergence
   struct CoopJob* job;  // jobs for helper workgroups (single-network launches), or null
   int nwg;              // workgroups taking part in a job (1: no helpers)
+  double* gT;           // T factors of the QR moves whose Q is kept as reflectors: vertex v's at gT + v tstride
+  int tstride;
 };
 
 // A job posted by workgroup 0 of a single-network launch to the helper
@@ -1801,8 +1803,9 @@ __device__ __noinline__ void qr_tall(double* W, int rows_in, int cols_in, int ld
 }
 
 // x <- Q (rows x k) of qr_tall's reflectors, or with Jb x <- U = Q [J_perm^T; 0]
-// (rows x m), formed row-major in Mq (global) and then written into the core Xc
-// (rows split as (aa, bb), bb < bx).
+// (rows x m; perm null: the rows of J in order), formed row-major in Mq
+// (global) and then written into the core Xc (rows split as (aa, bb), bb < bx).
+// W is read until the last panel is applied, so Xc may be W's memory.
 __device__ __noinline__ void q_form_tall(const double* W, int rows_in, int k_in, int ld_in, const double* Tg,
                                          double* Mq, double* Xc, int bx_in, const double* Jb, int n_in,
                                          const int* perm, int m_in, lds_f64* L, CoopJob* job, int nwg_in,
@@ -1814,7 +1817,7 @@ __device__ __noinline__ void q_form_tall(const double* W, int rows_in, int k_in,
   lds_f64* Tl = L + 8 * TALL_REGION + 16 * 128;
   for (int64_t e = tid; e < (int64_t)rows * ncol; e += bs) {
     const int r = (int)(e / ncol), c = (int)(e - (int64_t)r * ncol);
-    Mq[e] = Jb ? (r < n ? Jb[(int64_t)perm[c] * n + r] : 0.0) : (r == c ? 1.0 : 0.0);
+    Mq[e] = Jb ? (r < n ? Jb[(int64_t)(perm ? perm[c] : c) * n + r] : 0.0) : (r == c ? 1.0 : 0.0);
   }
   __syncthreads();
   for (int p0 = ((k - 1) / PNB) * PNB; p0 >= 0; p0 -= PNB) {
@@ -2515,7 +2518,8 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
                                         const int b, const int tr, const int rpl, const int bx,
                                         const int64_t Au, const int64_t nu, const int64_t Bu, double* Xc,
                                         const double* Uc, double* Ud, double* gY, double cutoff, long long maxdim, int max_sweeps, int prec,
-                                        int* status, int* iflag, double* red, double* dbc) {
+                                        int* status, int* iflag, double* red, double* dbc, double* gscr_any,
+                                        lds_f64* pnl_any, double* lazyT, const double* uT) {
   const int tid = threadIdx.x, bs = blockDim.x;
   const int k = rows < b ? rows : b;
   // register QR for the shapes it covers (the QR of a tall SVD matrix included)
@@ -2573,7 +2577,19 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
       }
       __syncthreads();
       QTT_PHASE(-1, 2);  // factored
-      q_form_tall(W, rows, k, b, Tg, Mq, Xc, bx, nullptr, 0, nullptr, 0, pnl, job, nwg, iflag);  // x <- Q (bond m)
+      if (lazyT) {
+        // x keeps the reflectors (rows x k, ld k) and lazyT the panels' T: the
+        // move that consumes x applies Q to its own thin factor (4 rows k m
+        // flops for m columns) instead of Q being formed here (about 2 rows
+        // k^2) and multiplied later (2 rows k m)
+        for (int64_t e = tid; e < (int64_t)rows * k; e += bs) {
+          const int r = (int)(e / k), c = (int)(e - (int64_t)r * k);
+          Xc[e] = W[(int64_t)r * b + c];
+        }
+        for (int e = tid; e < (k + PNB - 1) / PNB * PNB * PNB; e += bs) lazyT[e] = Tg[e];
+      } else {
+        q_form_tall(W, rows, k, b, Tg, Mq, Xc, bx, nullptr, 0, nullptr, 0, pnl, job, nwg, iflag);  // x <- Q (bond m)
+      }
     } else {
       geqr2<G>(W, rows, b, b, tau);
       for (int e = tid; e < k * b; e += bs) {
@@ -2779,7 +2795,22 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
   QTT_PHASE(-1, 3);  // x written
   // u <- R (x)_{au} u : Y[(aa*m + i)*Bu + bb] = sum_c R[i][c] U[(aa*nu + c)*Bu + bb]
   const int ytot = (int)(Au * m * Bu);
-  if (!REG && gws && job && nwg > 1 && Au * Bu * nu * (int64_t)m >= (int64_t(1) << 22)) {
+  if (uT) {
+    // u holds the reflectors of its own QR toward x (Q kept, not formed):
+    // u <- Q_u [Z; 0], Z = R^T (nu x m, the rows of R in perm order), in FP64
+    const double* zs = Rrows;
+    const int* zp = use_perm ? perm : nullptr;
+    if (!gws) {  // R in LDS, which the apply works in: copied to global scratch first
+      for (int e = tid; e < m * b; e += bs) {
+        const int i = e / b, c = e - i * b;
+        gscr_any[e] = Rrows[(int64_t)(use_perm ? perm[i] : i) * b + c];
+      }
+      __syncthreads();
+      zs = gscr_any;
+      zp = nullptr;
+    }
+    q_form_tall(Uc, (int)(Au * Bu), (int)nu, (int)nu, uT, gY, Ud, (int)Bu, zs, b, zp, m, pnl_any, job, nwg, iflag);
+  } else if (!REG && gws && job && nwg > 1 && Au * Bu * nu * (int64_t)m >= (int64_t(1) << 22)) {
     // a large product in global memory: columns spread over the helper workgroups
     if (!mode_product_coop(Rrows, perm, use_perm, m, b, Uc, gY, Ud, (int)Au, (int)nu, (int)Bu, job, nwg, iflag, pnl,
                            prec)) {
@@ -2824,15 +2855,37 @@ __global__ void __launch_bounds__(512) truncate_kernel(const SweepArgs* __restri
   const SweepArgs& a = all[blockIdx.x];
   const TopoView& t = a.t;
   const int tid = threadIdx.x, bs = blockDim.x;
-  // LDS layout: [misc 64 doubles][bonds: nv ints][workspace]
+  // LDS layout: [misc 64 doubles][bonds: nv ints][kept-Q state: nv ints][workspace]
   double* red = lds;                      // 16 doubles
   int* iflag = (int*)(lds + 16);          // flags and broadcast ints
   double* dbc = lds + 24;                 // broadcast doubles
+  const int nvi = (t.nv + 1) & ~1;
   int* bond = (int*)(lds + 64);
-  double* wsl = lds + 64 + ((t.nv + 1) >> 1);
-  for (int v = tid; v < t.nv; v += bs) bond[v] = a.bond_g[v];
+  // kq[v] = w: v's core holds the reflectors of its QR toward w (Q not formed
+  // yet), its T factors at a.gT + v a.tstride; -1: v's core is plain
+  int* kq = bond + nvi;
+  double* wsl = lds + 64 + nvi;
+  // LDS of the blocked QR's panels and WY updates (TALL_LDS doubles), 16-byte aligned
+  const int poff = (64 + nvi + 1) & ~1;
+  lds_f64* pnl_any = (lds_f64*)(lds + poff);
+  for (int v = tid; v < t.nv; v += bs) {
+    bond[v] = a.bond_g[v];
+    kq[v] = -1;
+  }
   if (tid == 0) dbc[0] = 0.0;
   __syncthreads();
+  // Q of v formed from its kept reflectors, in v's core (gY as the scratch:
+  // a core's size is at most its capacity)
+  auto form_kept_q = [&](int v) {
+    int sh[3];
+    move_shape(t, bond, v, kq[v], sh);
+    double* Xv = a.data + a.off[v];
+    q_form_tall(Xv, sh[0] * sh[2], sh[1], sh[1], a.gT + (int64_t)v * a.tstride, a.gY, Xv, sh[2], nullptr, 0, nullptr,
+                0, pnl_any, a.job, a.nwg, iflag);
+    __syncthreads();
+    if (tid == 0) kq[v] = -1;
+    __syncthreads();
+  };
   for (int st = 0; st < a.nsteps; ++st) {
     const int x = a.steps[3 * st], u = a.steps[3 * st + 1], tr = a.steps[3 * st + 2];
     QTT_PHASE(st, 0);
@@ -2852,6 +2905,14 @@ __global__ void __launch_bounds__(512) truncate_kernel(const SweepArgs* __restri
     const int64_t Bx = sx[2], Au = su[0], nu = su[1], Bu = su[2];
     const int rows = sx[0] * sx[2], b = sx[1];
     const int k = rows < b ? rows : b;
+    // A kept Q is applied by the move along its edge that takes it as u, when
+    // that move's new bond is at most half the kept one (the apply then costs
+    // at most what forming Q and multiplying it would); any other use forms it
+    // first.
+    if (kq[x] >= 0) form_kept_q(x);
+    const long long mbound = tr ? min((long long)k, a.maxdim < 1 ? 1LL : a.maxdim) : (long long)k;
+    if (kq[u] >= 0 && (kq[u] != x || 2 * mbound > nu)) form_kept_q(u);
+    const double* uT = kq[u] == x ? a.gT + (int64_t)u * a.tstride : nullptr;
     QTT_SHAPE(st, rows, b, tr);
     QTT_SWEEPS_CLEAR(st);
     const int64_t jsz = tr ? (int64_t)k * k : 0;
@@ -2878,10 +2939,11 @@ __global__ void __launch_bounds__(512) truncate_kernel(const SweepArgs* __restri
     int pmode = 0;
     if (need > a.ws_lds && TALL_LDS + 64 <= a.ws_lds) pmode = (int64_t)rows * PLD <= 8 * TALL_REGION ? 1 : 3;
     if (pmode) {
-      const int off = (64 + ((t.nv + 1) >> 1) + 1) & ~1;  // 16-byte aligned for the row loads
-      pnl = (lds_f64*)(lds + off);
-      pnl_cap = QTT_LDS_DOUBLES - off;
+      pnl = pnl_any;
+      pnl_cap = QTT_LDS_DOUBLES - poff;
     }
+    // a QR move on the blocked path keeps its reflectors instead of forming Q
+    double* lazyT = (pmode && !tr && a.gT) ? a.gT + (int64_t)x * a.tstride : nullptr;
     // padded copy for the Jacobi rounds of an SVD move, when it fits after the workspace
     double* jpad = nullptr;
     bool jwave = false;  // room for the single-wavefront Jacobi's copy at jpad
@@ -2898,24 +2960,28 @@ __global__ void __launch_bounds__(512) truncate_kernel(const SweepArgs* __restri
     if (need <= a.ws_lds) {
       if (small)
         m = run_move<8, true>(wsl, false, nullptr, 1, jpad, jwave, nullptr, 0, 0, rows, b, tr, rpl, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.max_sweeps, a.prec, a.status,
-                        iflag, red, dbc);
+                        iflag, red, dbc, a.gscr, pnl_any, lazyT, uT);
       else
         m = run_move<64, false>(wsl, false, nullptr, 1, nullptr, false, nullptr, 0, 0, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.max_sweeps, a.prec, a.status,
-                         iflag, red, dbc);
+                         iflag, red, dbc, a.gscr, pnl_any, lazyT, uT);
     } else {
       if (small)
         m = run_move<8, false>(a.gscr, true, a.job, a.nwg, nullptr, false, pnl, pnl_cap, pmode, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.max_sweeps, a.prec, a.status,
-                        iflag, red, dbc);
+                        iflag, red, dbc, a.gscr, pnl_any, lazyT, uT);
       else
         m = run_move<64, false>(a.gscr, true, a.job, a.nwg, nullptr, false, pnl, pnl_cap, pmode, rows, b, tr, 0, (int)Bx, Au, nu, Bu, Xc, Uc, Ud, a.gY, a.cutoff, a.maxdim, a.max_sweeps, a.prec, a.status,
-                         iflag, red, dbc);
+                         iflag, red, dbc, a.gscr, pnl_any, lazyT, uT);
     }
     QTT_PHASE(st, 5);
     if (tid == 0) {
       if (t.parent[x] == u) bond[x] = m; else bond[u] = m;
+      if (uT) kq[u] = -1;
+      if (lazyT) kq[x] = u;
     }
     __syncthreads();
   }
+  for (int v = 0; v < t.nv; ++v)
+    if (kq[v] >= 0) form_kept_q(v);
   for (int v = tid; v < t.nv; v += bs) a.bond_g[v] = bond[v];
   if (tid == 0) a.err_out[0] = dbc[0];
   if (a.nwg > 1 && tid == 0) __hip_atomic_store(&a.job->quit, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -3082,6 +3148,10 @@ struct TruncStaged::Impl {
 TruncStaged::TruncStaged() : impl(new Impl) {}
 TruncStaged::~TruncStaged() { delete impl; }
 
+// doubles of T factors per vertex for a kept Q: one PNB x PNB block per panel
+// of at most maxb columns
+static int64_t trunc_tstride(int64_t maxb) { return (maxb + PNB - 1) / PNB * PNB * PNB; }
+
 static size_t trunc_layout(qtt_net* const* nets, int nnet, int nwg, std::vector<int64_t>* ws_out,
                            std::vector<int64_t>* soff_out, std::vector<std::vector<int>>* steps_out,
                            int64_t* tot_doubles_out, int64_t* tot_steps_out) {
@@ -3097,8 +3167,9 @@ static size_t trunc_layout(qtt_net* const* nets, int nnet, int nwg, std::vector<
       maxb = std::max(maxb, net->bond[v]);
     }
     // workspace bound: W (or the reflector rows, <= max(2 core, 64 maxb)), R, J,
-    // staging rows (<= 2 core), vectors; plus the Y buffer
-    ws[i] = 4 * maxcap + 2 * maxb * maxb + 68 * maxb + 64 + maxcap;
+    // staging rows (<= 2 core), vectors; then the T factors of kept Q's
+    // (trunc_tstride per vertex) and the Y buffer
+    ws[i] = 4 * maxcap + 2 * maxb * maxb + 68 * maxb + 64 + (int64_t)net->t.nv * trunc_tstride(maxb) + maxcap;
     soff[i] = tot_steps;
     tot_doubles += ws[i];
     tot_steps += (int64_t)steps[i].size();
@@ -3195,10 +3266,15 @@ int truncate_batch_stage(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double cu
     sa.max_sweeps = ctx->jacobi_max_sweeps;
     sa.prec = precision;
     sa.gscr = wbase[i];
-    int64_t maxcap = 1;
-    for (int v = 0; v < net->t.nv; v++) maxcap = std::max(maxcap, net->cap[v]);
+    int64_t maxcap = 1, maxb = 1;
+    for (int v = 0; v < net->t.nv; v++) {
+      maxcap = std::max(maxcap, net->cap[v]);
+      maxb = std::max(maxb, net->bond[v]);
+    }
     sa.gY = wbase[i] + ws[i] - maxcap;
-    sa.ws_lds = lds_doubles - 64 - (net->t.nv + 1) / 2;
+    sa.tstride = (int)trunc_tstride(maxb);
+    sa.gT = sa.gY - (int64_t)net->t.nv * sa.tstride;
+    sa.ws_lds = lds_doubles - 64 - ((net->t.nv + 1) & ~1);
     sa.err_out = d_err + i;
     sa.status = ctx->d_flag + 2;  // truncation is synchronous: its own status slots
     sa.job = nwg > 1 ? d_job : nullptr;

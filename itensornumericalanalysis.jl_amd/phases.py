@@ -3,10 +3,12 @@ of the sweep split into its phases, read from the timestamps that the engine
 built with -DQTT_PHASE_TIMING writes (wall clock, 100 MHz), with the Jacobi
 sweeps of each SVD move.  Workloads: laplacian_2d (2 x 20 interleaved digits
 on a 40-site chain, Laplacian, cutoff 1e-10, maxdim 128; the default) and
-comb_3d (comb 3 x 16, bond 128, d/dx, maxdim 128).
+comb_3d (comb 3 x 16, bond 128, d/dx, maxdim 128) and fredholm_member (the
+costliest network of the 512-function Fredholm batch: 23 sites, bond 95,
+the bond-3 kernel operator, cutoff 1e-10).
 
     python itensornumericalanalysis.jl_amd/build.py --phase-timing
-    python itensornumericalanalysis.jl_amd/phases.py [laplacian_2d|comb_3d]
+    python itensornumericalanalysis.jl_amd/phases.py [laplacian_2d|comb_3d|fredholm_member]
 
 Phases of a move (csrc/factor.hip, run_move):
   gather   the move's matrix copied from the network into the workspace
@@ -51,15 +53,20 @@ def main():
         w = workloads.comb_3d()
         fc = w.f
         reps = 2
+    elif which == "fredholm_member":
+        batch = workloads.fredholm_batch()
+        w = batch[97]
+        fc = w.f
+        reps = 3
     else:
         raise SystemExit(f"unknown workload {which}")
     for _ in range(2):
-        q.operate(w.op, fc, cutoff=w.cutoff, maxdim=w.maxdim)
+        q.operate(w.op, fc, cutoff=w.cutoff, maxdim=w.maxdim or None)
     times = []
     for _ in range(reps):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        g = q.operate(w.op, fc, cutoff=w.cutoff, maxdim=w.maxdim)
+        g = q.operate(w.op, fc, cutoff=w.cutoff, maxdim=w.maxdim or None)
         torch.cuda.synchronize()
         times.append((time.perf_counter() - t0) * 1e3)
     L = capi.lib()
