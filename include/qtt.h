@@ -45,7 +45,10 @@ extern "C" {
 #define QTT_EHIP 4
 #define QTT_ENOCONV 5
 
-/* arithmetic of the bulk contractions (truncation is always FP64) */
+/* arithmetic of the bulk contractions: the exact contraction and the mode
+ * products of the truncation sweep (its QR, SVD and cutoff are always FP64, and
+ * so is the apply of a kept Q: a QR move of a matrix beyond LDS keeps its
+ * reflectors and the move consuming that core applies them in FP64) */
 #define QTT_F64 0       /* float64 everywhere */
 #define QTT_F32_SPLIT 1 /* float32 hi/lo split operands, float-float accumulation */
 #define QTT_F32 2       /* plain float32 operands and accumulation */
