@@ -489,6 +489,45 @@ def test_truncate_lossless_on_low_rank(q, po, rng):
     assert err < 1e-15
 
 
+def _dense_tt_svd_from_right(T, cutoff, maxdim):
+    """Textbook TT-SVD of a dense tensor (Oseledets 2011, Alg. 1), run from the
+    last axis to the first, each unfolding truncated by the ITensors rule
+    (relative cutoff on squared singular values, then maxdim); returns the
+    dense tensor of the truncated train.  Independent of the oracle."""
+    dims = T.shape
+    C = T.reshape(-1, dims[-1])  # rows (d_0 .. d_{k-1}), columns (d_k, m)
+    R = None                     # the train right of the cut, (m, d_k, ..., d_last)
+    for k in range(len(dims) - 1, 0, -1):
+        U, S, Vt = np.linalg.svd(C, full_matrices=False)
+        p = S ** 2
+        m, terr = len(S), 0.0
+        while m > maxdim:
+            terr += p[m - 1]
+            m -= 1
+        while m > 1 and terr + p[m - 1] <= cutoff * p.sum():
+            terr += p[m - 1]
+            m -= 1
+        if R is None:
+            R = Vt[:m]
+        else:
+            R = np.tensordot(Vt[:m].reshape(m, dims[k], -1), R, axes=([2], [0]))
+        C = (U[:, :m] * S[:m]).reshape(-1, dims[k - 1] * m)
+    return np.tensordot(C.reshape(dims[0], -1), R, axes=([1], [0])).reshape(dims)
+
+
+@pytest.mark.parametrize("cutoff,maxdim", [(0.0, 3), (1e-3, 1 << 20), (1e-4, 4)])
+def test_truncate_chain_is_dense_tt_svd(q, po, rng, cutoff, maxdim):
+    # the truncation sweep of a chain (QR moves toward the last vertex, then SVD
+    # moves back) is the TT-SVD of the dense tensor from its last axis
+    s = q.continuous_siteinds(q.named_grid((8, 1)))
+    f = q.random_itn(s, link_space=5, rng=rng)
+    Tf = dense.dense_tensor(f.topo.parent, f.cores)
+    out, err = po.truncate(f.topo.parent, f.topo.sdim, f.cores, cutoff, maxdim)
+    Tg = dense.dense_tensor(f.topo.parent, out)
+    Tr = _dense_tt_svd_from_right(Tf, cutoff, maxdim)
+    assert np.max(np.abs(Tg - Tr)) <= 1e-12 * np.max(np.abs(Tf))
+
+
 @pytest.mark.parametrize("graph", ["chain", "comb"])
 def test_truncate_error_bound(q, po, rng, graph):
     g = q.named_grid((10, 1)) if graph == "chain" else q.named_comb_tree((3, 4))
