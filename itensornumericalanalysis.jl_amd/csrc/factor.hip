@@ -2480,6 +2480,53 @@ __device__ bool jacobi_coop(double* X, int n, int b, double* J, CoopJob* job, in
   return false;
 }
 
+// One-sided Jacobi of a wide matrix X (n x b, n <= b, ld b, global memory)
+// preconditioned by two QRs (Drmac and Veselic; here without pivoting):
+// X^T = Q1 R1 and R1^T = Q2 R2, so X = Q2 R2 Q1^T.  The rounds run on the rows
+// of R2 (n x n) with J starting from Q2^T; X's rotated rows are then
+// (J R2) Q1^T, formed as (Q1 (J R2)^T)^T.  A truncation sweep's wide matrices
+// are numerically rank deficient (their smallest singular values are round-off
+// of the bigger ones) and their rows keep rotating for many sweeps unless
+// preconditioned: on a 256 x 384 matrix of the 3D comb operate a CPU replay of
+// the block rounds takes 35 sweeps on X, 8 on R2.  A (b x n) receives X^T and
+// keeps Q1's reflectors; Tg1, Tg2 the panels' T of the two QRs; Mq (b x n) is
+// scratch.  Same result as the Jacobi on X up to round-off (the rotations act
+// on the same row space).
+__device__ __noinline__ bool jacobi_precond(double* X, int n_in, int b_in, double* A, double* J, double* tau,
+                                            double* Tg1, double* Tg2, double* Mq, lds_f64* pnl, int pnl_cap,
+                                            CoopJob* job, int nwg_in, double* red, int* iflag, double cut,
+                                            int max_sweeps) {
+  const int tid = threadIdx.x, bs = blockDim.x;
+  const int n = uni(n_in), b = uni(b_in), nwg = uni(nwg_in);
+  for (int64_t e = tid; e < (int64_t)n * b; e += bs) {  // A = X^T
+    const int i = (int)(e / b), c = (int)(e - (int64_t)i * b);
+    A[(int64_t)c * n + i] = X[e];
+  }
+  __syncthreads();
+  if (b * PLD <= 8 * TALL_REGION) qr_panel(A, b, n, n, tau, Tg1, pnl, job, nwg, iflag);
+  else qr_tall(A, b, n, n, tau, Tg1, pnl, job, nwg, iflag);
+  __syncthreads();
+  for (int e = tid; e < n * n; e += bs) {  // X <- R1^T (n x n, ld n)
+    const int i = e / n, j = e - i * n;
+    X[e] = j <= i ? A[(int64_t)j * n + i] : 0.0;
+  }
+  __syncthreads();
+  if (n * PLD <= 8 * TALL_REGION) qr_panel(X, n, n, n, tau, Tg2, pnl, job, nwg, iflag);
+  else qr_tall(X, n, n, n, tau, Tg2, pnl, job, nwg, iflag);
+  __syncthreads();
+  // J <- Q2^T: Q2 formed and written transposed (rows of Q2 as columns, bx = n)
+  q_form_tall(X, n, n, n, Tg2, Mq, J, n, nullptr, 0, nullptr, 0, pnl, job, nwg, iflag);
+  for (int e = tid; e < n * n; e += bs) {  // X <- R2 (upper triangle of the second QR)
+    const int i = e / n, j = e - i * n;
+    if (j < i) X[e] = 0.0;
+  }
+  __syncthreads();
+  const bool cv = jacobi_coop(X, n, n, J, job, nwg, red, iflag, pnl, pnl_cap, cut, false, max_sweeps);
+  // X (n x b) <- (J R2) Q1^T: Q1 applied to [(J R2)^T; 0], written transposed (bx = b)
+  q_form_tall(A, b, n, n, Tg1, Mq, X, b, X, n, nullptr, n, pnl, job, nwg, iflag);
+  return cv;
+}
+
 // u <- R (x) u with the helper workgroups: the columns (aa, bb) split over all
 // workgroups into Y, then Y copied into u's core (a second job: the core's
 // new layout overlaps columns of other workgroups).
@@ -2713,7 +2760,14 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
       }
       return cv;
     };
-    bool conv = jacobi(cutoff, true);
+    // wide matrices of at least 192 rows on the global workspace with helper
+    // workgroups: the QR-preconditioned rounds (3D comb operate: 256 x 384
+    // moves 25.3 -> 12.7 ms; at 128 x 128 the two QRs cost what the saved
+    // sweeps do)
+    const bool precond = !REG && gws && job && nwg > 1 && !tall && n >= 192 && b <= 512 && pnl != nullptr;
+    bool conv = precond ? jacobi_precond(X, n, b, Rb, Jb, tau, vbuf, vbuf + ((int64_t)n + PNB) * PNB, gY, pnl,
+                                         pnl_cap, job, nwg, red, iflag, cutoff, max_sweeps)
+                        : jacobi(cutoff, true);
     row_norms<G>(X, n, b, sig);
     __syncthreads();
     if (conv && cutoff > 0.0 && cutoff < 1.0 && !noise_block_exact(sig, n, cutoff, iflag + 10)) {
