@@ -356,6 +356,22 @@ def test_truncate_large_matrices(q, po, eng, rng, bond, cutoff):
     assert err == pytest.approx(rerr, rel=1e-8, abs=1e-20)
 
 
+def test_truncate_preconditioned_wide_moves(q, po, eng, rng):
+    # an 18-site chain of bond 256 truncated to maxdim 100: cores beyond LDS,
+    # so the launch carries helper workgroups, and the SVD moves back from the
+    # last site meet 200 x 256 matrices (wide, >= 192 rows), whose Jacobi runs
+    # QR-preconditioned (X^T = Q1 R1, R1^T = Q2 R2, rounds on R2); the QR moves
+    # keep their reflectors for the SVD moves to apply
+    s = q.continuous_siteinds(q.named_grid((18, 1)))
+    f = q.random_itn(s, link_space=256, rng=rng, std=1.0 / math.sqrt(256))
+    tr, err = q.truncate(f, cutoff=0.0, maxdim=100, return_error=True)
+    ref, rerr = po.truncate(f.topo.parent, f.topo.sdim, f.cores, 0.0, 100)
+    assert list(tr.bonds()) == [c.shape[0] for c in ref]
+    assert max(tr.bonds()) == 100
+    assert _compare_networks(q, po, tr, ref, f, rng) < 1e-11
+    assert err == pytest.approx(rerr, rel=1e-9)
+
+
 # ------------------------------------------------------------------ comb tree (config-3 shape, reduced)
 def test_comb_operate_and_integrate(q, po, eng, rng):
     from itensornumericalanalysis_jl_amd import workloads
