@@ -425,6 +425,11 @@ struct CoopJob {
   // reading -U from Ug.
   int nb;
   int bsz, nblk;  // type 8: block round t of a block Jacobi sweep (blocks of bsz rows)
+  // type 9: a whole block Jacobi sweep; each workgroup takes its block pair of
+  // every round as soon as the round before has written both blocks back:
+  // ver[b] = rbase + (rounds of this sweep that block b has been through)
+  int rbase;
+  int* ver;
   double dcol[16];
   double* Gp;  // [workgroup][16]
   double* Pg;  // [unit][256]
@@ -2301,6 +2306,38 @@ __device__ void coop_work(CoopJob* job, int wg, int nwg, lds_f64* lw) {
     else if (e <= 6) r = coop_block<6>(job->X, job->J, job->n, job->b, job->bsz, job->nblk, job->t, job->tol, job->small, job->noise, wg, lw);
     else r = coop_block<8>(job->X, job->J, job->n, job->b, job->bsz, job->nblk, job->t, job->tol, job->small, job->noise, wg, lw);
     if (r && tid == 0) __hip_atomic_store(&job->rot, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else if (type == 9) {  // a whole block Jacobi sweep, rounds ordered block by block
+    const int nblk = job->nblk, rbase = job->rbase, e = job->e;
+    bool r = false;
+    for (int t = 0; wg < nblk / 2 && t < nblk - 1; ++t) {
+      const int P = rr_player(wg, t, nblk), Q = rr_player(nblk - 1 - wg, t, nblk);
+      if (tid == 0) {  // both blocks written back by round t - 1 (bounded wait)
+        int spins = 0;
+        while (__hip_atomic_load(&job->ver[P], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < rbase + t ||
+               __hip_atomic_load(&job->ver[Q], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < rbase + t) {
+          __builtin_amdgcn_s_sleep(1);
+          if (++spins > COOP_SPIN_MASTER) {
+            __hip_atomic_store(&job->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+        }
+      }
+      __syncthreads();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      bool rt;
+      if (e <= 2) rt = coop_block<2>(job->X, job->J, job->n, job->b, job->bsz, nblk, t, job->tol, job->small, job->noise, wg, lw);
+      else if (e <= 4) rt = coop_block<4>(job->X, job->J, job->n, job->b, job->bsz, nblk, t, job->tol, job->small, job->noise, wg, lw);
+      else if (e <= 6) rt = coop_block<6>(job->X, job->J, job->n, job->b, job->bsz, nblk, t, job->tol, job->small, job->noise, wg, lw);
+      else rt = coop_block<8>(job->X, job->J, job->n, job->b, job->bsz, nblk, t, job->tol, job->small, job->noise, wg, lw);
+      r = r || rt;
+      // (coop_block ends with a barrier after its stores: every wavefront's
+      // writes are done before thread 0 releases the two blocks)
+      if (tid == 0) {
+        __hip_atomic_store(&job->ver[P], rbase + t + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&job->ver[Q], rbase + t + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    if (r && tid == 0) __hip_atomic_store(&job->rot, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   } else if (type == 4 || type == 5) {  // a tall-panel pass over this workgroup's rows
     const int mr = job->n, nb = job->nb, j = job->t;
     const int start = type == 4 ? 1 : j + 1;
@@ -2441,13 +2478,19 @@ __device__ bool jacobi_coop(double* X, int n, int b, double* J, CoopJob* job, in
   int nblk = (n + bsz - 1) / bsz;
   nblk += nblk & 1;
   const int need = 2 * bsz * ldsum + 8;
-  const bool blocked = bsz <= 16 && nblk >= 4 && need <= cap;
+  const bool blocked = bsz <= 16 && nblk >= 4 && nblk <= 256 && need <= cap;  // (ver holds 256 blocks)
   const int rounds = blocked ? nblk - 1 : nn - 1;
+  // block sweeps are one job each (type 9), the rounds ordered by per-block
+  // counters instead of a hand-off through workgroup 0 per round
+  int rbase = 0;
+  if (blocked)
+    for (int b2 = tid; b2 < nblk; b2 += bs) job->ver[b2] = 0;  // published by the first post
   for (int sweep = 0; sweep < max_sweeps; ++sweep) {
     if (tid == 0) __hip_atomic_store(&job->rot, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (int t = 0; t < rounds; ++t) {
+    for (int t = 0; t < (blocked ? 1 : rounds); ++t) {
       if (tid == 0) {
-        job->type = blocked ? 8 : 1;
+        job->type = blocked ? 9 : 1;
+        job->rbase = rbase;
         job->X = X;
         job->J = J;
         job->n = n;
@@ -2465,6 +2508,7 @@ __device__ bool jacobi_coop(double* X, int n, int b, double* J, CoopJob* job, in
       }
       if (!coop_post_run(job, nwg, iflag, lw)) break;
     }
+    rbase += rounds;
     if (tid == 0) sflag[2] = __hip_atomic_load(&job->rot, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) |
                              __hip_atomic_load(&job->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) * 2;
     __syncthreads();
@@ -3230,8 +3274,9 @@ static size_t trunc_layout(qtt_net* const* nets, int nnet, int nwg, std::vector<
   }
   const size_t args_bytes = (size_t)nnet * sizeof(SweepArgs);
   // the tail slack covers the whole-block row reads of the Jacobi rounds (< 512 doubles past a row)
-  // helpers' exchange buffers after the job: Gp [32][16], Pg [512][256], Ug [8][256]
-  const size_t coop_doubles = 32 * 16 + (size_t)COOP_PG_UNITS * 256 + 8 * 256;
+  // helpers' exchange buffers after the job: Gp [32][16], Pg [512][256], Ug [8][256],
+  // then the block counters of the Jacobi sweeps (at most 128 blocks; 256 ints)
+  const size_t coop_doubles = 32 * 16 + (size_t)COOP_PG_UNITS * 256 + 8 * 256 + 128;
   const size_t bytes = (tot_doubles + nnet + 8) * sizeof(double) + ((args_bytes + 255) & ~size_t(255)) +
                        ((tot_steps * sizeof(int) + 255) & ~size_t(255)) + 4096 + sizeof(CoopJob) + 256 +
                        (nwg > 1 ? coop_doubles * sizeof(double) : 0);
@@ -3342,6 +3387,7 @@ int truncate_batch_stage(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double cu
     im->hjob.Gp = cb;
     im->hjob.Pg = cb + 32 * 16;
     im->hjob.Ug = im->hjob.Pg + (size_t)COOP_PG_UNITS * 256;
+    im->hjob.ver = (int*)(im->hjob.Ug + 8 * 256);
   }
   if (!rc && nwg > 1)
     rc = qtt_hip_check(ctx, hipMemcpyAsync(d_job, &im->hjob, sizeof(CoopJob), hipMemcpyHostToDevice, ctx->stream), "coop job");
