@@ -2198,14 +2198,23 @@ __device__ __noinline__ bool coop_block(double* X, double* J, int n_in, int b_in
   lds_f64* Xl = lw;
   lds_f64* Jl = lw + nl * lx;
   lds_i32* fl = (lds_i32*)(Jl + nl * lj);
-  for (int e = tid; e < nl * lx; e += bs) {
-    const int i = e / lx, c = e - i * lx, r = grow(i);
-    Xl[e] = (r < n && c < b) ? X[(int64_t)r * b + c] : 0.0;
-  }
-  for (int e = tid; e < nl * lj; e += bs) {
-    const int i = e / lj, c = e - i * lj, r = grow(i);
-    Jl[e] = (r < n && c < n) ? J[(int64_t)r * n + c] : 0.0;
-  }
+  // the two blocks of X and of J into LDS, 8 loads per thread in flight per wait
+  auto stage_in = [&](lds_f64* dst, const double* src, int ld, int len, int cols) {
+    for (int e0 = tid; e0 < nl * ld; e0 += 8 * bs) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + u * bs;
+        const int i = e / ld, c = e - i * ld, r = e < nl * ld ? grow(i) : n;
+        v[u] = (r < n && c < cols) ? src[(int64_t)r * len + c] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (e0 + u * bs < nl * ld) dst[e0 + u * bs] = v[u];
+    }
+  };
+  stage_in(Xl, X, lx, b, b);
+  stage_in(Jl, J, lj, n, n);
   if (tid == 0) fl[0] = 0;
   __syncthreads();
   // round 0 sweeps all local pairs (so every pair inside a block is met once
