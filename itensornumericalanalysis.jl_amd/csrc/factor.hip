@@ -2631,18 +2631,25 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
   //            tau, sig, perm | 2 staging rows
   const int64_t jsz = tr ? (int64_t)k * k : 0;
   const int64_t wsz = use_reg ? max((int64_t)rows * b, (int64_t)k * 8 * rpl) : (int64_t)rows * b;
-  double* W = ws;                               // rows x b
-  double* Rb = W + wsz;                         // k x b
+  // A tall QR move on the blocked path whose bond to u is the core's last leg
+  // (bx = 1) has its matrix already row-major in x's core: it is factored
+  // there (no copy into the workspace, and a kept Q's reflectors are then in
+  // place too), which saves two passes over the core in global memory.
+  const bool inplace = gws && !tr && pmode && bx == 1 && rows >= b;
+  double* W = inplace ? Xc : ws;                // rows x b
+  double* Rb = ws + wsz;                        // k x b
   double* Jb = Rb + (int64_t)k * b;             // k x k
   double* tau = Jb + jsz;                       // k
   double* sig = tau + k;                        // k
   int* perm = (int*)(sig + k);                  // k ints (2k doubles reserved)
   double* vbuf = sig + 3 * (int64_t)k;          // 2 rows
   // gather the matrix: M[a*Bx + bb][i] = X[(a*b + i)*Bx + bb] (sizes < 2^31, checked on the host)
-  for (int e = tid; e < rows * b; e += bs) {
-    const int row = e / b, i = e - row * b;
-    const int aa = row / bx, bb = row - aa * bx;
-    W[e] = Xc[((int64_t)aa * b + i) * bx + bb];
+  if (!inplace) {
+    for (int e = tid; e < rows * b; e += bs) {
+      const int row = e / b, i = e - row * b;
+      const int aa = row / bx, bb = row - aa * bx;
+      W[e] = Xc[((int64_t)aa * b + i) * bx + bb];
+    }
   }
   __syncthreads();
   QTT_PHASE(-1, 1);  // gathered
@@ -2681,10 +2688,13 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
         // x keeps the reflectors (rows x k, ld k) and lazyT the panels' T: the
         // move that consumes x applies Q to its own thin factor (4 rows k m
         // flops for m columns) instead of Q being formed here (about 2 rows
-        // k^2) and multiplied later (2 rows k m)
-        for (int64_t e = tid; e < (int64_t)rows * k; e += bs) {
-          const int r = (int)(e / k), c = (int)(e - (int64_t)r * k);
-          Xc[e] = W[(int64_t)r * b + c];
+        // k^2) and multiplied later (2 rows k m).  (In place, k = b: they are
+        // there already.)
+        if (!inplace) {
+          for (int64_t e = tid; e < (int64_t)rows * k; e += bs) {
+            const int r = (int)(e / k), c = (int)(e - (int64_t)r * k);
+            Xc[e] = W[(int64_t)r * b + c];
+          }
         }
         for (int e = tid; e < (k + PNB - 1) / PNB * PNB * PNB; e += bs) lazyT[e] = Tg[e];
       } else {
