@@ -86,7 +86,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); the engine has no "
-              "multi-GPU path in this build.", file=sys.stderr)
+              "multi-GPU path (its sharding is a design, not code).", file=sys.stderr)
         sys.exit(1)
 
     import torch
