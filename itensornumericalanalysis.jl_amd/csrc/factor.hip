@@ -1712,7 +1712,10 @@ __device__ __noinline__ void tall_wy(double* M, int ldm_in, int mr_in, int c0_in
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
   const int ldm = uni(ldm_in), mr = uni(mr_in), c0 = uni(c0_in), c1 = uni(c1_in), ldv = uni(ldv_in),
             nb = uni(nb_in), nwg = uni(nwg_in);
-  const bool coop = job != nullptr && nwg > 1;
+  // helper workgroups only for a large trailing matrix: below 2^15 entries
+  // the two hand-offs of a shared update cost more than the update on
+  // workgroup 0 alone
+  const bool coop = job != nullptr && nwg > 1 && (int64_t)mr * (c1 - c0) >= (int64_t(1) << 15);
   lds_f64* Pw = L;                  // partial W tiles: [unit][e * 64 + lane]
   lds_f64* Wt = L + 32 * 256;       // W tiles [tile][j][c]
   lds_f64* Ut = Wt + 8 * 256;       // U tiles [tile][j][c]
