@@ -108,6 +108,12 @@ __device__ inline double dpp_d(double v) {
   return __hiloint2double(hi, lo);
 }
 
+__device__ inline double readfirstlane_d(double v) {
+  const int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
+  const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
+  return __hiloint2double(hi, lo);
+}
+
 __device__ inline double readlane_d(double v, int l) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
   const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
@@ -400,7 +406,7 @@ struct CoopJob {
   int err;   // a wait exceeded its limit
   unsigned int done;
   int pad;
-  int type;  // 1: Jacobi round, 2: mode product columns, 3: copy
+  int type;  // 1: Jacobi round, 2: mode product columns, 3: copy, ... (below)
   // type 1: one round t of a one-sided Jacobi sweep on the rows of X (n x b,
   // ld ldx) with the rotations kept in J (n x n, ld ldj), both in global memory
   double* X;
@@ -409,6 +415,8 @@ struct CoopJob {
   double tol, small, noise;
   // type 2: Y = R (x) U over the columns (aa, bb), each workgroup a range of
   // them (mode_product_mfma); type 3: D[0, ytot) <- Y, each a range
+  // type 10: a move's matrix gathered from its core, D <- U (gather_tiles
+  // with Au blocks, rb = b, Bu = bx)
   const double* R;
   const int* perm;
   const double* U;
@@ -1371,68 +1379,17 @@ __device__ __forceinline__ int64_t wave_jacobi_lds(int n, int b) {
 
 // ---------------------------------------------------------------------------
 // Blocked Householder QR for matrices in global memory (the move's workspace
-// does not fit LDS): panels of PNB columns are factored in LDS, and the rest
-// of the matrix is updated once per panel in the compact WY form
-// H_0 ... H_{nb-1} = I - V T V^T (LAPACK dlarft / dlarfb), one lane per column
-// with the panel's reflectors broadcast from LDS.  Q (or U of a tall SVD move)
-// is formed the same way, panels in reverse order.
+// does not fit LDS): panels of PNB columns are factored in registers (up to
+// 1024 rows) or in place in global memory (taller), and the rest of the
+// matrix is updated once per panel in the compact WY form
+// H_0 ... H_{nb-1} = I - V T V^T (LAPACK dlarft / dlarfb) with FP64 MFMA
+// tiles.  Q (or U of a tall SVD move) is formed the same way, panels in
+// reverse order.
 // ---------------------------------------------------------------------------
 constexpr int PNB = 16;
-constexpr int PLD = PNB + 1;  // row stride of the panel that is read by columns: odd, so no bank conflicts
+constexpr int PLD = PNB + 1;  // row stride of a panel staged in LDS: odd, so no bank conflicts
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
-
-// T (nb x nb upper triangular, ld PNB) of the reflectors V (unit lower
-// trapezoidal, mr x nb, ld PLD, reflector entries below the diagonal of the
-// panel) and their scalars tau.  Y is PNB x PNB scratch, S 1024 doubles.
-__device__ void build_T(const lds_f64* V, int mr, int nb, const lds_f64* tau, lds_f64* T, lds_f64* Y, lds_f64* S) {
-  const int tid = threadIdx.x;
-  nb = PNB;  // padded: the extra reflectors have tau = 0, so their rows and columns of T are 0
-  // Y = V^T V with v_j unit at row j and zero above it: FP64 MFMA over the
-  // rows (k-steps of 4 rows), the operand of lane (li, lk) is v_li[row] for
-  // both A and B; wavefront w takes the k-steps w, w + 8, ...  (512 threads)
-  const int lane = tid & 63, wv = tid >> 6, li = lane & 15, lk = lane >> 4;
-  f64x4 acc = {0.0, 0.0, 0.0, 0.0};
-  const int nk = (mr + 3) >> 2;
-  for (int ks = wv; ks < nk; ks += 8) {
-    const int r = 4 * ks + lk;
-    const double v = r >= mr ? 0.0 : r > li ? V[r * PLD + li] : (r == li ? 1.0 : 0.0);
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(v, v, acc, 0, 0, 0);
-  }
-  // the 8 partials summed in a fixed order through S (1024 doubles):
-  // (w0 + w4) + (w1 + w5) and (w2 + w6) + (w3 + w7)
-  if (wv >= 4) {
-#pragma unroll
-    for (int e = 0; e < 4; ++e) S[(wv - 4) * 256 + e * 64 + lane] = acc[e];
-  }
-  __syncthreads();
-  if (wv < 4) {
-#pragma unroll
-    for (int e = 0; e < 4; ++e) S[wv * 256 + e * 64 + lane] += acc[e];
-  }
-  __syncthreads();
-  if (tid < 256) {  // element e of lane ln is Y[ln / 16 + 4 e][ln % 16]
-    const int e = tid >> 6, ln = tid & 63, o = e * 64 + ln;
-    Y[((ln >> 4) + 4 * e) * PNB + (ln & 15)] = (S[o] + S[256 + o]) + (S[512 + o] + S[768 + o]);
-  }
-  __syncthreads();
-  if (tid < nb) {  // T[:, i] = -tau_i T[:, 0:i] Y[0:i, i]; lane r owns row r
-    const int r = tid;
-    for (int i = 0; i < nb; ++i) {
-      double t = 0.0;
-      if (i == r) {
-        t = tau[i];
-      } else if (i > r && tau[i] != 0.0) {  // padding reflectors (tau 0) keep zero columns
-        double acc = 0.0;
-        for (int j = r; j < i; ++j) acc = __builtin_fma(T[r * PNB + j], Y[j * PNB + i], acc);
-        t = -tau[i] * acc;
-      }
-      T[r * PNB + i] = t;
-    }
-  }
-  __syncthreads();
-}
-
 typedef double f64x2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(3))) f64x2 lds_f64x2;
 
@@ -1845,45 +1802,237 @@ __device__ __noinline__ void q_form_tall(const double* W, int rows_in, int k_in,
   __syncthreads();
 }
 
+// ---------------------------------------------------------------------------
+// Panel factorization in registers (panels of up to 512 RPT rows, 512
+// threads): thread t holds rows t + 512 k (k < RPT) of the panel's PNB
+// columns.  Step j applies reflector j to those rows and, in the same pass,
+// forms this thread's share of what step j+1 needs: s_c = the sum over the
+// rows r > j+1 of a[r][j+1] a[r][c], for every column c -- for c > j+1 the
+// dot products with column j+1, for c == j+1 its norm^2, and for c <= j its
+// dot products with the earlier reflectors (stored in place), which give
+// column j+1 of T.  The 16 sums are reduced within each wavefront by four
+// halving exchanges (every lane ends with the sum of one column over its
+// quad) and across the wavefronts through LDS: one barrier per column.
+// Row j+1, read by every thread in step j+1, travels through LDS with them.
+// ---------------------------------------------------------------------------
+
+// column whose sum lane l holds after wave_sum16: 8 b5 + 4 b4 + 2 b3 + b2
+__device__ __forceinline__ int sum16_col(int lane) {
+  return ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 + ((lane >> 2) & 1);
+}
+
+// v_permlane32_swap on both halves of a and b, then the sum: lanes 0-31 get
+// a + a of lanes 32-63, lanes 32-63 get b + b of lanes 0-31.
+__device__ __forceinline__ double swap32_add(double a, double b) {
+  const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)__double2loint(a), (unsigned)__double2loint(b), false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)__double2hiint(a), (unsigned)__double2hiint(b), false, false);
+  return __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
+}
+// v_permlane16_swap (odd 16-lane rows of a with even rows of b), then the
+// sum: rows 0 and 2 get a + a of the next row, rows 1 and 3 b + b of the row before.
+__device__ __forceinline__ double swap16_add(double a, double b) {
+  const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)__double2loint(a), (unsigned)__double2loint(b), false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)__double2hiint(a), (unsigned)__double2hiint(b), false, false);
+  return __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
+}
+
+// Sum p[0..15] over the wavefront; lane l returns the sum of column
+// sum16_col(l).  Each exchange pairs lanes that hold the same column set and
+// differ in one lane bit (5, 4, 3, 2): the lane keeps the half of its set
+// that the bit selects and adds the partner's copy of that half.  The last
+// two steps sum the quad.  Fixed order: the same sums on every run.
+__device__ __forceinline__ double wave_sum16(const double (&p)[PNB]) {
+  const int lane = threadIdx.x & 63;
+  const bool h3 = lane & 8, h2 = lane & 4;
+  double q8[8], q4[4], q2[2];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) q8[i] = swap32_add(p[i], p[i + 8]);   // bit 5: columns i / i + 8
+#pragma unroll
+  for (int i = 0; i < 4; ++i) q4[i] = swap16_add(q8[i], q8[i + 4]);  // bit 4: + 0 / + 4
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {  // row_mirror: lane 15 - m of the 16-lane row (bit 3 differs)
+    const double keep = h3 ? q4[i + 2] : q4[i], send = h3 ? q4[i] : q4[i + 2];
+    q2[i] = keep + dpp_d<0x140>(send);
+  }
+  // row_half_mirror: lane 7 - m of the 8-lane half (bit 2 differs, bit 3 kept)
+  double q = (h2 ? q2[1] : q2[0]) + dpp_d<0x141>(h2 ? q2[0] : q2[1]);
+  q += dpp_d<0xB1>(q);  // quad_perm [1,0,3,2]
+  q += dpp_d<0x4E>(q);  // quad_perm [2,3,0,1]
+  return q;
+}
+
+// The 16 column sums of p over the workgroup (8 wavefronts); lane l returns
+// the sum of column l & 15.  red: 8 x PNB doubles of LDS.  Every wavefront
+// adds the 8 partials in the same order, so all threads see the same sums.
+__device__ __forceinline__ double panel_sums16(const double (&p)[PNB], lds_f64* red) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const double q = wave_sum16(p);
+  if ((lane & 3) == 0) red[wv * PNB + sum16_col(lane)] = q;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int w = 0; w < 8; ++w) t += red[w * PNB + (lane & 15)];
+  return t;
+}
+
+// Factor the panel P = W[p0:, p0:p0+nb] (global, ld; mr = rows - p0 rows,
+// nb <= mr): R on and above its diagonal, the reflectors below (unit diagonal
+// implied), tau[0..nb), and the panel's T (PNB x PNB, zero rows and columns
+// past nb) into Tl (LDS) and Tg.  L: LDS, the panel staged through its first
+// mr * PLD doubles (coalesced global reads and writes); red: 2 x 8 x PNB sums
+// and 2 x PNB row doubles of LDS, disjoint from the staging area and Tl.
+template <int RPT>
+__device__ __noinline__ void reg_panel(double* W, int rows_in, int ld_in, int p0_in, int nb_in, double* tau,
+                                       double* Tg, lds_f64* Tl, lds_f64* L, lds_f64* red) {
+  const int tid = threadIdx.x;
+  const int rows = uni(rows_in), ld = uni(ld_in), p0 = uni(p0_in), nb = uni(nb_in);
+  const int mr = rows - p0, ne = mr * PNB;
+  double* P = W + (int64_t)p0 * ld + p0;
+  lds_f64* rowb = red + 2 * 8 * PNB;
+  // global -> LDS (8 loads in flight per thread) -> registers
+  for (int e0 = 0; e0 < ne; e0 += 8 * 512) {
+    double t[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = e0 + u * 512 + tid, r = e >> 4, c = e & 15;
+      t[u] = (e < ne && c < nb) ? P[(int64_t)r * ld + c] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = e0 + u * 512 + tid;
+      if (e < ne) L[(e >> 4) * PLD + (e & 15)] = t[u];
+    }
+  }
+  __syncthreads();
+  double x[RPT][PNB];
+#pragma unroll
+  for (int k = 0; k < RPT; ++k) {
+    const int r = tid + 512 * k;
+#pragma unroll
+    for (int c = 0; c < PNB; ++c) x[k][c] = r < mr ? L[r * PLD + c] : 0.0;
+  }
+  // column layout: lane l holds, for column l & 15, the sum S (over the
+  // rows below the current row j) and row j's entry R
+  double S, R, Trow[PNB];
+  const int cl = threadIdx.x & 15;
+  {
+    double p[PNB];
+#pragma unroll
+    for (int c = 0; c < PNB; ++c) p[c] = 0.0;
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+      const double a = (tid + 512 * k > 0) ? x[k][0] : 0.0;
+#pragma unroll
+      for (int c = 0; c < PNB; ++c) p[c] = __builtin_fma(a, x[k][c], p[c]);
+    }
+    if (tid == 0) {
+#pragma unroll
+      for (int c = 0; c < PNB; ++c) rowb[c] = x[0][c];
+    }
+    S = panel_sums16(p, red);
+    R = rowb[cl];
+  }
+  const int ti = tid & 15;  // the row of T this thread keeps (written by threads 0..15)
+#pragma unroll
+  for (int j = 0; j < PNB; ++j) {
+    // reflector j from alpha = a[j][j] and S[j] = the norm^2 below it (zero
+    // columns past nb give tau 0)
+    const double alpha = readlane_d(R, j), sg = readlane_d(S, j);
+    double tj = 0.0, beta = alpha, scale = 0.0;
+    if (sg != 0.0) {
+      beta = -copysign(sqrt(alpha * alpha + sg), alpha);
+      tj = (beta - alpha) / beta;
+      scale = 1.0 / (alpha - beta);
+    }
+    // z_c = v_j . a_c = a[j][c] + scale * S[c]: for c < j the y_c of T (a_c
+    // holds reflector c there), for c > j, times tau_j, what reflector j
+    // takes from column c
+    double zl = __builtin_fma(scale, S, R);
+    zl = cl > j ? tj * zl : (cl < j ? zl : 0.0);
+    double z[PNB];
+#pragma unroll
+    for (int c = 0; c < PNB; ++c) z[c] = readlane_d(zl, c);
+    // T[:, j] = -tau_j T[:, 0:j] y
+    {
+      double acc = 0.0;
+#pragma unroll
+      for (int l = 0; l < PNB; ++l)
+        if (l < j) acc = __builtin_fma(Trow[l], z[l], acc);
+      Trow[j] = ti == j ? tj : (ti < j ? -tj * acc : 0.0);
+    }
+    if (tid == 0 && j < nb) tau[j] = tj;
+    double p[PNB];
+#pragma unroll
+    for (int c = 0; c < PNB; ++c) p[c] = 0.0;
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+      const int r = tid + 512 * k;
+      const double v = r > j ? scale * x[k][j] : (r == j ? 1.0 : 0.0);
+#pragma unroll
+      for (int c = 0; c < PNB; ++c)  // constant trip counts: every loop unrolls, the arrays stay in registers
+        if (c > j) x[k][c] = __builtin_fma(-z[c], v, x[k][c]);
+      x[k][j] = r > j ? v : (r == j ? beta : x[k][j]);
+      if (j + 1 < PNB) {
+        const double a = r > j + 1 ? x[k][j + 1] : 0.0;
+#pragma unroll
+        for (int c = 0; c < PNB; ++c) p[c] = __builtin_fma(a, x[k][c], p[c]);
+      }
+    }
+    if (j + 1 < PNB) {
+      lds_f64* rb = rowb + ((j + 1) & 1) * PNB;
+      if (tid == j + 1) {
+#pragma unroll
+        for (int c = 0; c < PNB; ++c) rb[c] = x[0][c];
+      }
+      S = panel_sums16(p, red + ((j + 1) & 1) * 8 * PNB);
+      R = rb[cl];
+    }
+  }
+  if (tid < PNB) {
+#pragma unroll
+    for (int c = 0; c < PNB; ++c) {
+      Tl[tid * PNB + c] = Trow[c];
+      Tg[tid * PNB + c] = Trow[c];
+    }
+  }
+  // registers -> LDS -> global
+#pragma unroll
+  for (int k = 0; k < RPT; ++k) {
+    const int r = tid + 512 * k;
+    if (r < mr) {
+#pragma unroll
+      for (int c = 0; c < PNB; ++c) L[r * PLD + c] = x[k][c];
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < ne; e += 512) {
+    const int r = e >> 4, c = e & 15;
+    if (c < nb) P[(int64_t)r * ld + c] = L[r * PLD + c];
+  }
+  __syncthreads();
+}
+
 // QR of W (global rows x cols, ld) whose panels fit LDS (rows * PLD <= 16384):
-// each panel is factored in LDS (a wavefront per column), its reflectors and
-// T written back (T of every panel to Tg, for q_form_tall), and the trailing
+// each panel is factored in registers (reg_panel), its reflectors and T
+// written back (T of every panel to Tg, for q_form_tall), and the trailing
 // columns updated by tall_wy with the reflectors read from W.  L: the tall
-// layout (TALL_LDS doubles); the panel lives in its first part, which tall_wy
-// reuses once the panel is written back.
+// layout (TALL_LDS doubles); the panel is staged through its first part,
+// which tall_wy reuses once the panel is written back.
 __device__ __noinline__ void qr_panel(double* W, int rows_in, int cols_in, int ld_in, double* tau, double* Tg,
                                       lds_f64* L, CoopJob* job, int nwg_in, int* iflag) {
-  const int tid = threadIdx.x, bs = blockDim.x;
   const int nwg = uni(nwg_in);
   const int rows = uni(rows_in), cols = uni(cols_in), ld = uni(ld_in);
   const int k = rows < cols ? rows : cols;
-  lds_f64* Pp = L;                             // mr x PLD
-  lds_f64* Yl = L + 8 * TALL_REGION;           // PNB x PNB
-  lds_f64* tl = Yl + PNB * PNB;                // PNB
+  lds_f64* red = L + 8 * TALL_REGION;          // 2 x 8 x PNB sums, 2 x PNB rows
   lds_f64* Tl = L + 8 * TALL_REGION + 16 * 128;
   QTT_PT_INIT;
   for (int p0 = 0; p0 < k; p0 += PNB) {
     const int nb = k - p0 < PNB ? k - p0 : PNB, mr = rows - p0;
-    for (int e = tid; e < mr * PNB; e += bs) {
-      const int r = e / PNB, c = e - r * PNB;
-      Pp[r * PLD + c] = c < nb ? W[(int64_t)(p0 + r) * ld + p0 + c] : 0.0;
-    }
-    __syncthreads();
-    QTT_PT(0);
-    geqr2<64, lds_f64*>(Pp, mr, nb, PLD, tl);  // a wavefront per column: few rows per lane
-    __syncthreads();
+    if (mr <= 512)
+      reg_panel<1>(W, rows, ld, p0, nb, tau + p0, Tg + (p0 / PNB) * PNB * PNB, Tl, L, red);
+    else
+      reg_panel<2>(W, rows, ld, p0, nb, tau + p0, Tg + (p0 / PNB) * PNB * PNB, Tl, L, red);
     QTT_PT(1);
-    for (int e = tid; e < mr * nb; e += bs) {
-      const int r = e / nb, c = e - r * nb;
-      W[(int64_t)(p0 + r) * ld + p0 + c] = Pp[r * PLD + c];
-    }
-    for (int e = tid; e < nb; e += bs) tau[p0 + e] = tl[e];
-    for (int e = nb + tid; e < PNB; e += bs) tl[e] = 0.0;  // padding reflectors
-    __syncthreads();
-    QTT_PT(2);
-    build_T(Pp, mr, nb, tl, Tl, Yl, tl + PNB);  // partial sums after tl, below Tl
-    for (int e = tid; e < PNB * PNB; e += bs) Tg[(p0 / PNB) * PNB * PNB + e] = Tl[e];
-    QTT_PT(3);
     if (p0 + nb < cols)
       tall_wy(W + (int64_t)p0 * ld, ld, mr, p0 + nb, cols, W + (int64_t)p0 * ld + p0, ld, nb, Tl, true, L, job, nwg,
               iflag);
@@ -2277,6 +2426,39 @@ __device__ __noinline__ bool coop_block(double* X, double* J, int n_in, int b_in
   return any;
 }
 
+// W[(aa bx + bb) b + i] = X[(aa b + i) bx + bb] for aa < na: the move's
+// matrix gathered from its core, for each aa a transpose of the b x bx block
+// X[aa] into W[aa].  64 x 64 tiles through LDS (lw: 64 x 65 doubles), so the
+// reads (along bb) and the writes (along i) are both coalesced; this
+// workgroup takes tiles wg, wg + nwg, ...
+__device__ void gather_tiles(const double* X, double* W, int na, int b, int bx, int wg, int nwg, lds_f64* lw) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int ti = (b + 63) >> 6, tb = (bx + 63) >> 6;
+  const int64_t per = (int64_t)ti * tb, ntile = (int64_t)na * per;
+  for (int64_t t = wg; t < ntile; t += nwg) {
+    const int aa = (int)(t / per), rem = (int)(t - (int64_t)aa * per);
+    const int i0 = (rem / tb) * 64, b0 = (rem - (rem / tb) * tb) * 64;
+    const double* S = X + (int64_t)aa * b * bx;
+    double* D = W + (int64_t)aa * bx * b;
+    double v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int i = i0 + wv + 8 * q, c = b0 + lane;
+      v[q] = (i < b && c < bx) ? S[(int64_t)i * bx + c] : 0.0;
+    }
+    __syncthreads();  // the previous tile's reads of lw are done
+#pragma unroll
+    for (int q = 0; q < 8; ++q) lw[(wv + 8 * q) * 65 + lane] = v[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int c = b0 + wv + 8 * q, i = i0 + lane;
+      if (c < bx && i < b) D[(int64_t)c * b + i] = lw[lane * 65 + wv + 8 * q];
+    }
+  }
+  __syncthreads();
+}
+
 // This workgroup's share of the posted job, then its completion count.  lw:
 // LDS the job may use (the row blocks of a block Jacobi round).
 __device__ void coop_work(CoopJob* job, int wg, int nwg, lds_f64* lw) {
@@ -2310,6 +2492,8 @@ __device__ void coop_work(CoopJob* job, int wg, int nwg, lds_f64* lw) {
     const long long chunk = ((tot + nwg - 1) / nwg + 63) & ~63LL;
     const long long e0 = min(tot, (long long)wg * chunk), e1 = min(tot, e0 + chunk);
     for (long long x = e0 + tid; x < e1; x += blockDim.x) job->D[x] = job->Y[x];
+  } else if (type == 10) {
+    gather_tiles(job->U, job->D, job->Au, job->rb, job->Bu, wg, nwg, lw);
   } else if (type == 8) {  // a block round of a block Jacobi sweep
     const int e = job->e;
     bool r;
@@ -2647,7 +2831,22 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
   int* perm = (int*)(sig + k);                  // k ints (2k doubles reserved)
   double* vbuf = sig + 3 * (int64_t)k;          // 2 rows
   // gather the matrix: M[a*Bx + bb][i] = X[(a*b + i)*Bx + bb] (sizes < 2^31, checked on the host)
-  if (!inplace) {
+  if (!inplace && gws && bx > 1) {
+    // tiled transposes, with the helper workgroups when the matrix is large
+    if (job != nullptr && nwg > 1 && (int64_t)rows * b >= (int64_t(1) << 15)) {
+      if (tid == 0) {
+        job->type = 10;
+        job->U = Xc;
+        job->D = W;
+        job->Au = rows / bx;
+        job->rb = b;
+        job->Bu = bx;
+      }
+      coop_post_run(job, nwg, iflag, pnl_any);
+    } else {
+      gather_tiles(Xc, W, rows / bx, b, bx, 0, 1, pnl_any);
+    }
+  } else if (!inplace) {
     for (int e = tid; e < rows * b; e += bs) {
       const int row = e / b, i = e - row * b;
       const int aa = row / bx, bb = row - aa * bx;
