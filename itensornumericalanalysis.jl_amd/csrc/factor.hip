@@ -69,12 +69,13 @@ extern "C" int qtt_debug_jacobi(unsigned long long* out) {
   } while (0)
 #define QTT_JC(i) do { if (threadIdx.x == 0) qtt_jac_t[i] += 1; } while (0)
 // cycles of thread 0 per segment of a blocked-QR panel
-__device__ unsigned long long qtt_pnl_t[8];
+__device__ unsigned long long qtt_pnl_t[16];
 extern "C" int qtt_debug_panel(unsigned long long* out) {
-  return (int)hipMemcpyFromSymbol(out, HIP_SYMBOL(qtt_pnl_t), 8 * sizeof(unsigned long long));
+  return (int)hipMemcpyFromSymbol(out, HIP_SYMBOL(qtt_pnl_t), 16 * sizeof(unsigned long long));
 }
 #define QTT_PT_INIT unsigned long long pt_last_ = clock64()
 #define QTT_PANEL_COUNT (qtt_pnl_t[7] += 1)
+#define QTT_WY_COUNT(coop) (qtt_pnl_t[(coop) ? 0 : 6] += 1)
 #define QTT_PT(i)                                                   \
   do {                                                              \
     const unsigned long long c_ = clock64();                        \
@@ -85,6 +86,7 @@ extern "C" int qtt_debug_panel(unsigned long long* out) {
 #define QTT_PT_INIT do { } while (0)
 #define QTT_PT(i) do { } while (0)
 #define QTT_PANEL_COUNT (void)0
+#define QTT_WY_COUNT(coop) (void)0
 #define QTT_JT_INIT do { } while (0)
 #define QTT_JT(i) do { } while (0)
 #define QTT_JC(i) do { } while (0)
@@ -1676,6 +1678,8 @@ __device__ __noinline__ void tall_wy(double* M, int ldm_in, int mr_in, int c0_in
   lds_f64* Pw = L;                  // partial W tiles: [unit][e * 64 + lane]
   lds_f64* Wt = L + 32 * 256;       // W tiles [tile][j][c]
   lds_f64* Ut = Wt + 8 * 256;       // U tiles [tile][j][c]
+  QTT_PT_INIT;
+  if (threadIdx.x == 0) QTT_WY_COUNT(coop);
   for (int cg = c0; cg < c1; cg += 128) {
     const int nct = (min(128, c1 - cg) + 15) >> 4;  // 16-column tiles in this group
     // row chunks per tile: enough units for every wavefront (of all workgroups with helpers)
@@ -1721,6 +1725,7 @@ __device__ __noinline__ void tall_wy(double* M, int ldm_in, int mr_in, int c0_in
       Wt[t * 256 + ((ln >> 4) + 4 * e) * 16 + (ln & 15)] = sum;
     }
     __syncthreads();
+    QTT_PT(coop ? 2 : 8);
     // U = op(T) W, stored negated for the update
     for (int x = tid; x < nct * 256; x += blockDim.x) {
       const int t = x >> 8, i = (x >> 4) & 15, c = x & 15;
@@ -1731,6 +1736,7 @@ __device__ __noinline__ void tall_wy(double* M, int ldm_in, int mr_in, int c0_in
       if (coop) job->Ug[t * 256 + i * 16 + c] = -sum;
     }
     __syncthreads();
+    QTT_PT(coop ? 3 : 9);
     // M -= V U, one (tile, 16-row block) at a time
     const int nrb = (mr + 15) >> 4;
     if (coop) {
@@ -1743,6 +1749,7 @@ __device__ __noinline__ void tall_wy(double* M, int ldm_in, int mr_in, int c0_in
       }
       __syncthreads();
     }
+    QTT_PT(coop ? 5 : 10);
   }
 }
 
@@ -2509,8 +2516,8 @@ __device__ void coop_work(CoopJob* job, int wg, int nwg, lds_f64* lw) {
       const int P = rr_player(wg, t, nblk), Q = rr_player(nblk - 1 - wg, t, nblk);
       if (tid == 0) {  // both blocks written back by round t - 1 (bounded wait)
         int spins = 0;
-        while (__hip_atomic_load(&job->ver[P], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < rbase + t ||
-               __hip_atomic_load(&job->ver[Q], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < rbase + t) {
+        while (__hip_atomic_load(&job->ver[P], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < rbase + t ||
+               __hip_atomic_load(&job->ver[Q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < rbase + t) {
           __builtin_amdgcn_s_sleep(1);
           if (++spins > COOP_SPIN_MASTER) {
             __hip_atomic_store(&job->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -2593,7 +2600,7 @@ __device__ bool coop_post_run(CoopJob* job, int nwg, int* iflag, lds_f64* lw) {
   if (tid == 0) {
     const unsigned int target = (unsigned int)s * (unsigned int)nwg;
     int spins = 0, bad = 0;
-    while (__hip_atomic_load(&job->done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    while (__hip_atomic_load(&job->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
       __builtin_amdgcn_s_sleep(1);
       if (++spins > COOP_SPIN_MASTER) {
         __hip_atomic_store(&job->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -2617,8 +2624,8 @@ __device__ void coop_helper(CoopJob* job, int wg, int nwg, lds_i32* sflag, lds_f
     if (tid == 0) {
       int spins = 0, st = 0;
       for (;;) {
-        if (__hip_atomic_load(&job->quit, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) { st = 1; break; }
-        if (__hip_atomic_load(&job->seq, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= expected) break;
+        if (__hip_atomic_load(&job->quit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { st = 1; break; }
+        if (__hip_atomic_load(&job->seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= expected) break;
         if (spins < 64) __builtin_amdgcn_s_sleep(2);
         else __builtin_amdgcn_s_sleep(64);
         if (++spins > COOP_SPIN_HELPER) {  // give up; a later job would then time out in workgroup 0
