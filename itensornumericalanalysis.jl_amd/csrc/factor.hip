@@ -76,6 +76,7 @@ extern "C" int qtt_debug_panel(unsigned long long* out) {
 #define QTT_PT_INIT unsigned long long pt_last_ = clock64()
 #define QTT_PANEL_COUNT (qtt_pnl_t[7] += 1)
 #define QTT_WY_COUNT(coop) (qtt_pnl_t[(coop) ? 0 : 6] += 1)
+#define QTT_WY_COUNT_COLS (qtt_pnl_t[12] += 1)
 #define QTT_PT(i)                                                   \
   do {                                                              \
     const unsigned long long c_ = clock64();                        \
@@ -87,6 +88,7 @@ extern "C" int qtt_debug_panel(unsigned long long* out) {
 #define QTT_PT(i) do { } while (0)
 #define QTT_PANEL_COUNT (void)0
 #define QTT_WY_COUNT(coop) (void)0
+#define QTT_WY_COUNT_COLS (void)0
 #define QTT_JT_INIT do { } while (0)
 #define QTT_JT(i) do { } while (0)
 #define QTT_JC(i) do { } while (0)
@@ -447,6 +449,9 @@ struct CoopJob {
   double* wy_M;
   const double* wy_V;
   int wy_ldm, wy_ldv, wy_mr, wy_cg, wy_c1, wy_nct, wy_nch, wy_chunk;
+  // type 11: a whole WY update (wy_cols) over columns [wy_cg, wy_c1), T in
+  // Ug[0, 256), op(T) = T^T if wy_trans
+  int wy_trans;
 };
 constexpr int COOP_PG_UNITS = 512;  // partial tiles Pg holds
 
@@ -1433,6 +1438,9 @@ __device__ __forceinline__ void tall_reduce16(double acc, lds_f64* buf, lds_f64*
 }
 
 __device__ bool coop_post_run(CoopJob* job, int nwg, int* iflag, lds_f64* lw);
+__device__ int coop_post(CoopJob* job, int* iflag);
+__device__ bool coop_wait(CoopJob* job, int s, int nwg, int* iflag);
+__device__ void coop_wy_cols(CoopJob* job, int wg, int nwg, lds_f64* lw);
 
 // The two kinds of pass over panel rows [r0, r1) (lane layout: a 16-lane group
 // per row, a lane per column c = lane & 15; rows step by 4 per wavefront).
@@ -1675,6 +1683,28 @@ __device__ __noinline__ void tall_wy(double* M, int ldm_in, int mr_in, int c0_in
   // the two hand-offs of a shared update cost more than the update on
   // workgroup 0 alone
   const bool coop = job != nullptr && nwg > 1 && (int64_t)mr * (c1 - c0) >= (int64_t(1) << 15);
+  if (coop && mr <= 1024) {  // whole column tiles per workgroup: one job
+    if (tid == 0) {
+      job->type = 11;
+      job->wy_M = M;
+      job->wy_ldm = ldm;
+      job->wy_mr = mr;
+      job->wy_cg = c0;
+      job->wy_c1 = c1;
+      job->wy_V = V;
+      job->wy_ldv = ldv;
+      job->nb = nb;
+      job->wy_trans = trans ? 1 : 0;
+    }
+    for (int e = tid; e < PNB * PNB; e += blockDim.x) job->Ug[e] = Tl[e];
+    QTT_PT_INIT;
+    const int s = coop_post(job, iflag);
+    coop_wy_cols(job, 0, nwg, L);
+    coop_wait(job, s, nwg, iflag);
+    QTT_PT(11);
+    if (threadIdx.x == 0) QTT_WY_COUNT_COLS;
+    return;
+  }
   lds_f64* Pw = L;                  // partial W tiles: [unit][e * 64 + lane]
   lds_f64* Wt = L + 32 * 256;       // W tiles [tile][j][c]
   lds_f64* Ut = Wt + 8 * 256;       // U tiles [tile][j][c]
@@ -2466,6 +2496,57 @@ __device__ void gather_tiles(const double* X, double* W, int na, int b, int bx, 
   __syncthreads();
 }
 
+// M[0:mr, c0:c1] <- (I - V op(T) V^T) M[0:mr, c0:c1] for mr <= 1024, whole
+// 16-column tiles per workgroup (tiles wg, wg + nwg, ...): W = V^T M_tile
+// with the rows split over the wavefronts (16-row blocks w, w + 8, ...),
+// the 8 partial tiles summed through LDS, U = op(T) W, then M_tile -= V U
+// (the tile read again, from L2).  One job per update, no partial tiles
+// through global memory.  Few registers (two blocks in flight): the sweep
+// kernel keeps values across calls in the registers its callees leave
+// alone.  T: 16 x 16 row-major; L: 2560 doubles of LDS.
+__device__ __noinline__ void wy_cols(double* M, int ldm, int mr, int c0, int c1, const double* V, int ldv, int nb, const double* T,
+                        bool trans, int wg, int nwg, lds_f64* L) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, li = lane & 15, lk = lane >> 4;
+  const int ntile = (c1 - c0 + 15) >> 4;
+  lds_f64* Pw = L;              // [wavefront][e * 64 + lane]
+  lds_f64* Wt = L + 8 * 256;    // W [j][c]
+  lds_f64* Ut = Wt + 256;       // -U [j][c]
+  for (int t = wg; t < ntile; t += nwg) {
+    const int col0 = c0 + 16 * t;
+    f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 2
+    for (int s = 0; s < 8; ++s) {
+      const int rb = 16 * (wv + 8 * s);
+      if (rb < mr) acc += wy_p1_unit(M, ldm, mr, col0, c1, V, ldv, nb, rb, min(mr, rb + 16));
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) Pw[wv * 256 + e * 64 + lane] = acc[e];
+    __syncthreads();
+    if (tid < 256) {  // element e of lane ln is W[ln / 16 + 4 e][ln % 16]
+      const int e = tid >> 6, ln = tid & 63;
+      double w = 0.0;
+#pragma unroll
+      for (int h = 0; h < 8; ++h) w += Pw[h * 256 + tid];
+      Wt[((ln >> 4) + 4 * e) * 16 + (ln & 15)] = w;
+    }
+    __syncthreads();
+    if (tid < 256) {
+      const int i = tid >> 4, c = tid & 15;
+      double u = 0.0;
+#pragma unroll
+      for (int j = 0; j < PNB; ++j) u = __builtin_fma(trans ? T[j * PNB + i] : T[i * PNB + j], Wt[j * 16 + c], u);
+      Ut[i * 16 + c] = -u;
+    }
+    __syncthreads();
+#pragma unroll 2
+    for (int s = 0; s < 8; ++s) {
+      const int rb = 16 * (wv + 8 * s);
+      if (rb < mr) wy_p3_unit(M, ldm, mr, col0, c1, V, ldv, nb, rb, (const lds_f64*)Ut);
+    }
+    __syncthreads();  // Pw, Wt and Ut are reused by the next tile
+  }
+}
+
 // This workgroup's share of the posted job, then its completion count.  lw:
 // LDS the job may use (the row blocks of a block Jacobi round).
 __device__ void coop_work(CoopJob* job, int wg, int nwg, lds_f64* lw) {
@@ -2586,6 +2667,14 @@ __device__ void coop_work(CoopJob* job, int wg, int nwg, lds_f64* lw) {
 // share, and wait until every workgroup has done its share.  Returns false if
 // the wait exceeded its limit.
 __device__ bool coop_post_run(CoopJob* job, int nwg, int* iflag, lds_f64* lw) {
+  const int s = coop_post(job, iflag);
+  coop_work(job, 0, nwg, lw);
+  return coop_wait(job, s, nwg, iflag);
+}
+
+// Post the job whose fields this workgroup has written; returns its sequence
+// number.
+__device__ int coop_post(CoopJob* job, int* iflag) {
   const int tid = threadIdx.x;
   lds_i32* sflag = (lds_i32*)iflag;
   __syncthreads();  // job fields written by any thread of this workgroup come before the release
@@ -2595,8 +2684,14 @@ __device__ bool coop_post_run(CoopJob* job, int nwg, int* iflag, lds_f64* lw) {
     sflag[1] = s;
   }
   __syncthreads();
-  const int s = sflag[1];
-  coop_work(job, 0, nwg, lw);
+  return sflag[1];
+}
+
+// Wait until every workgroup has done its share of job s (workgroup 0's
+// included); false if the wait exceeded its limit.
+__device__ bool coop_wait(CoopJob* job, int s, int nwg, int* iflag) {
+  const int tid = threadIdx.x;
+  lds_i32* sflag = (lds_i32*)iflag;
   if (tid == 0) {
     const unsigned int target = (unsigned int)s * (unsigned int)nwg;
     int spins = 0, bad = 0;
@@ -2615,6 +2710,16 @@ __device__ bool coop_post_run(CoopJob* job, int nwg, int* iflag, lds_f64* lw) {
   const int bad = sflag[3];
   __syncthreads();
   return !bad;
+}
+
+// This workgroup's share of a type-11 job (kept out of coop_work, which the
+// sweep calls in many places, so that its registers stay few), then its
+// completion count.
+__device__ void coop_wy_cols(CoopJob* job, int wg, int nwg, lds_f64* lw) {
+  wy_cols(job->wy_M, job->wy_ldm, job->wy_mr, job->wy_cg, job->wy_c1, job->wy_V, job->wy_ldv, job->nb, job->Ug,
+          job->wy_trans != 0, wg, nwg, lw);
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_fetch_add(&job->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Helper workgroups: take every posted job until workgroup 0 quits.
@@ -2640,7 +2745,8 @@ __device__ void coop_helper(CoopJob* job, int wg, int nwg, lds_i32* sflag, lds_f
     __syncthreads();
     if (st) return;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    coop_work(job, wg, nwg, lw);
+    if (job->type == 11) coop_wy_cols(job, wg, nwg, lw);
+    else coop_work(job, wg, nwg, lw);
   }
 }
 
