@@ -406,6 +406,12 @@ struct SweepArgs {
 struct CoopJob {
   int seq;
   int quit;
+  // the posted job: seq << 32 | nact, one atomic word, so a helper reads both
+  // together; workgroups 0 .. nact-1 take part, the others skip it and do not
+  // count it.  seq and dtarget (the completions posted so far) are workgroup
+  // 0's own.
+  unsigned long long post;
+  unsigned int dtarget;
   int rot;   // a rotation happened in the current Jacobi sweep
   int err;   // a wait exceeded its limit
   unsigned int done;
@@ -1437,8 +1443,8 @@ __device__ __forceinline__ void tall_reduce16(double acc, lds_f64* buf, lds_f64*
   __syncthreads();
 }
 
-__device__ bool coop_post_run(CoopJob* job, int nwg, int* iflag, lds_f64* lw);
-__device__ int coop_post(CoopJob* job, int* iflag);
+__device__ bool coop_post_run(CoopJob* job, int nwg, int* iflag, lds_f64* lw, int nact = 0);
+__device__ int coop_post(CoopJob* job, int* iflag, int nact);
 __device__ bool coop_wait(CoopJob* job, int s, int nwg, int* iflag);
 __device__ void coop_wy_cols(CoopJob* job, int wg, int nwg, lds_f64* lw);
 
@@ -1673,16 +1679,20 @@ __device__ __forceinline__ void wy_p3_unit(double* M, int ldm, int mr, int col0,
   }
 }
 
+constexpr int64_t WY_COOP_MIN = int64_t(1) << 15;  // entries of a row-chunk (two-job) shared update
+constexpr int64_t WY_COLS_MIN = int64_t(1) << 13;  // entries of a column-tile (one-job) shared update
+
 __device__ __noinline__ void tall_wy(double* M, int ldm_in, int mr_in, int c0_in, int c1_in, const double* V,
                                      int ldv_in, int nb_in, const lds_f64* Tl, bool trans, lds_f64* L,
                                      CoopJob* job, int nwg_in, int* iflag) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
   const int ldm = uni(ldm_in), mr = uni(mr_in), c0 = uni(c0_in), c1 = uni(c1_in), ldv = uni(ldv_in),
             nb = uni(nb_in), nwg = uni(nwg_in);
-  // helper workgroups only for a large trailing matrix: below 2^15 entries
-  // the two hand-offs of a shared update cost more than the update on
-  // workgroup 0 alone
-  const bool coop = job != nullptr && nwg > 1 && (int64_t)mr * (c1 - c0) >= (int64_t(1) << 15);
+  // helper workgroups only for a large trailing matrix: below WY_COOP_MIN
+  // (WY_COLS_MIN for a column-tile job) entries the hand-offs of a shared
+  // update cost more than the update on workgroup 0 alone
+  const int64_t ent = (int64_t)mr * (c1 - c0);
+  const bool coop = job != nullptr && nwg > 1 && ent >= (mr <= 1024 ? WY_COLS_MIN : WY_COOP_MIN);
   if (coop && mr <= 1024) {  // whole column tiles per workgroup: one job
     if (tid == 0) {
       job->type = 11;
@@ -1698,7 +1708,7 @@ __device__ __noinline__ void tall_wy(double* M, int ldm_in, int mr_in, int c0_in
     }
     for (int e = tid; e < PNB * PNB; e += blockDim.x) job->Ug[e] = Tl[e];
     QTT_PT_INIT;
-    const int s = coop_post(job, iflag);
+    const int s = coop_post(job, iflag, min(nwg, (c1 - c0 + 15) >> 4));
     coop_wy_cols(job, 0, nwg, L);
     coop_wait(job, s, nwg, iflag);
     QTT_PT(11);
@@ -2666,34 +2676,40 @@ __device__ void coop_work(CoopJob* job, int wg, int nwg, lds_f64* lw) {
 // Workgroup 0: post the job whose fields this workgroup has written, take its own
 // share, and wait until every workgroup has done its share.  Returns false if
 // the wait exceeded its limit.
-__device__ bool coop_post_run(CoopJob* job, int nwg, int* iflag, lds_f64* lw) {
-  const int s = coop_post(job, iflag);
+// nact: the workgroups taking part (0: all).
+__device__ bool coop_post_run(CoopJob* job, int nwg, int* iflag, lds_f64* lw, int nact) {
+  const int s = coop_post(job, iflag, nact > 0 ? min(nact, nwg) : nwg);
   coop_work(job, 0, nwg, lw);
   return coop_wait(job, s, nwg, iflag);
 }
 
-// Post the job whose fields this workgroup has written; returns its sequence
-// number.
-__device__ int coop_post(CoopJob* job, int* iflag) {
+// Post the job whose fields this workgroup has written, for workgroups
+// 0 .. nact-1; returns its sequence number.
+__device__ int coop_post(CoopJob* job, int* iflag, int nact) {
   const int tid = threadIdx.x;
   lds_i32* sflag = (lds_i32*)iflag;
   __syncthreads();  // job fields written by any thread of this workgroup come before the release
   if (tid == 0) {
     const int s = job->seq + 1;
-    __hip_atomic_store(&job->seq, s, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    job->seq = s;
+    job->dtarget += (unsigned int)nact;
+    __hip_atomic_store(&job->post, ((unsigned long long)s << 32) | (unsigned int)nact, __ATOMIC_RELEASE,
+                       __HIP_MEMORY_SCOPE_AGENT);
     sflag[1] = s;
   }
   __syncthreads();
   return sflag[1];
 }
 
-// Wait until every workgroup has done its share of job s (workgroup 0's
-// included); false if the wait exceeded its limit.
+// Wait until every workgroup taking part has done its share of the posted
+// job (workgroup 0's included); false if the wait exceeded its limit.
 __device__ bool coop_wait(CoopJob* job, int s, int nwg, int* iflag) {
   const int tid = threadIdx.x;
   lds_i32* sflag = (lds_i32*)iflag;
+  (void)s;
+  (void)nwg;
   if (tid == 0) {
-    const unsigned int target = (unsigned int)s * (unsigned int)nwg;
+    const unsigned int target = job->dtarget;
     int spins = 0, bad = 0;
     while (__hip_atomic_load(&job->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
       __builtin_amdgcn_s_sleep(1);
@@ -2722,15 +2738,23 @@ __device__ void coop_wy_cols(CoopJob* job, int wg, int nwg, lds_f64* lw) {
   if (threadIdx.x == 0) __hip_atomic_fetch_add(&job->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Helper workgroups: take every posted job until workgroup 0 quits.
+// Helper workgroups: take every posted job they take part in until
+// workgroup 0 quits.  A job of workgroups 0 .. nact-1 is not followed by the
+// next before they have all counted it, so a helper that takes part always
+// sees its job; one that does not may see a later job and skips to it.
 __device__ void coop_helper(CoopJob* job, int wg, int nwg, lds_i32* sflag, lds_f64* lw) {
   const int tid = threadIdx.x;
-  for (int expected = 1;; ++expected) {
+  for (int expected = 1;;) {
     if (tid == 0) {
-      int spins = 0, st = 0;
+      int spins = 0, st = 0, seen = 0, part = 0;
       for (;;) {
         if (__hip_atomic_load(&job->quit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { st = 1; break; }
-        if (__hip_atomic_load(&job->seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= expected) break;
+        const unsigned long long w = __hip_atomic_load(&job->post, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        seen = (int)(w >> 32);
+        if (seen >= expected) {
+          part = wg < (int)(unsigned int)w;
+          break;
+        }
         if (spins < 64) __builtin_amdgcn_s_sleep(2);
         else __builtin_amdgcn_s_sleep(64);
         if (++spins > COOP_SPIN_HELPER) {  // give up; a later job would then time out in workgroup 0
@@ -2739,11 +2763,15 @@ __device__ void coop_helper(CoopJob* job, int wg, int nwg, lds_i32* sflag, lds_f
         }
       }
       sflag[0] = st;
+      sflag[4] = seen;
+      sflag[5] = part;
     }
     __syncthreads();
-    const int st = sflag[0];
+    const int st = sflag[0], seen = sflag[4], part = sflag[5];
     __syncthreads();
     if (st) return;
+    expected = seen + 1;
+    if (!part) continue;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     if (job->type == 11) coop_wy_cols(job, wg, nwg, lw);
     else coop_work(job, wg, nwg, lw);
@@ -2815,7 +2843,7 @@ __device__ bool jacobi_coop(double* X, int n, int b, double* J, CoopJob* job, in
         job->bsz = bsz;
         job->nblk = nblk;
       }
-      if (!coop_post_run(job, nwg, iflag, lw)) break;
+      if (!coop_post_run(job, nwg, iflag, lw, blocked ? nblk / 2 : 0)) break;
     }
     rbase += rounds;
     if (tid == 0) sflag[2] = __hip_atomic_load(&job->rot, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) |
@@ -2955,7 +2983,8 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
         job->rb = b;
         job->Bu = bx;
       }
-      coop_post_run(job, nwg, iflag, pnl_any);
+      const int64_t ntile = (int64_t)(rows / bx) * ((b + 63) >> 6) * ((bx + 63) >> 6);
+      coop_post_run(job, nwg, iflag, pnl_any, (int)min<int64_t>(ntile, nwg));
     } else {
       gather_tiles(Xc, W, rows / bx, b, bx, 0, 1, pnl_any);
     }
