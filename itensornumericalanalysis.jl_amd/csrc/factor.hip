@@ -2301,7 +2301,9 @@ __device__ __noinline__ void mode_product_mfma(const double* Rrows, const int* p
     const double* rpa = Rrows + (int64_t)(aon ? (use_perm ? perm[ra] : ra) : 0) * b;
     const double* rpb = Rrows + (int64_t)(bon ? (use_perm ? perm[rb] : rb) : 0) * b;
     f64x4 da = {0.0, 0.0, 0.0, 0.0}, db = {0.0, 0.0, 0.0, 0.0};
-    for (int c0 = 0; c0 < nu; c0 += 32) {  // 8 k-steps: all 24 loads in flight before the MFMAs
+    // without a permutation the rows are a QR's R (upper triangular, its zeros
+    // stored): the k-steps left of the tile's first row add nothing
+    for (int c0 = use_perm ? 0 : i0; c0 < nu; c0 += 32) {  // 8 k-steps: all 24 loads in flight before the MFMAs
       double bv[8], av[8], aw[8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
