@@ -3063,7 +3063,9 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
       }
     }
   } else {
-    tall = rows > b;
+    // a square move that can drop nothing takes the tall path as well (QR
+    // first), so that it can keep Q R below
+    tall = rows > b || (rows == b && cutoff <= 0.0 && maxdim >= b);
     double* X = W;
     n = rows;
     if (tall) {
@@ -3174,9 +3176,28 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
     // moves 25.3 -> 12.7 ms; at 128 x 128 the two QRs cost what the saved
     // sweeps do)
     const bool precond = !REG && gws && job && nwg > 1 && !tall && n >= 192 && b <= 512 && pnl != nullptr;
-    bool conv = precond ? jacobi_precond(X, n, b, Rb, Jb, tau, vbuf, vbuf + ((int64_t)n + PNB) * PNB, gY, pnl,
-                                         pnl_cap, job, nwg, red, iflag, cutoff, max_sweeps)
-                        : jacobi(cutoff, true);
+    // A tall move that can drop nothing (cutoff <= 0: at most exactly zero singular
+    // values would go; maxdim >= n) and whose R has no zero on its diagonal
+    // (nonsingular, so none of its singular values is zero) keeps all n: Q R
+    // is then already the factorization the truncation would produce, up to
+    // an orthogonal change of basis on the new bond.  The rows of R stand for
+    // S V^T (J = I), sorted by norm below, and the Jacobi is not run.
+    bool keep_all = false;
+    if (tall && cutoff <= 0.0 && maxdim >= n) {
+      if (tid == 0) iflag[11] = 1;
+      __syncthreads();
+      for (int i = tid; i < n; i += bs)
+        if (X[(int64_t)i * b + i] == 0.0) iflag[11] = 0;
+      __syncthreads();
+      keep_all = iflag[11] != 0;
+      if (keep_all)
+        for (int e = tid; e < n * n; e += bs) Jb[e] = (e / n == e - (e / n) * n) ? 1.0 : 0.0;
+      __syncthreads();
+    }
+    bool conv = keep_all ? true
+                : precond ? jacobi_precond(X, n, b, Rb, Jb, tau, vbuf, vbuf + ((int64_t)n + PNB) * PNB, gY, pnl,
+                                           pnl_cap, job, nwg, red, iflag, cutoff, max_sweeps)
+                          : jacobi(cutoff, true);
     row_norms<G>(X, n, b, sig);
     __syncthreads();
     if (conv && cutoff > 0.0 && cutoff < 1.0 && !noise_block_exact(sig, n, cutoff, iflag + 10)) {

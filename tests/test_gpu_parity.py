@@ -343,10 +343,12 @@ def test_truncate_random(q, po, eng, rng, graph):
     assert err == pytest.approx(rerr, rel=1e-9)
 
 
-@pytest.mark.parametrize("bond,cutoff", [(130, 1e-10), (100, 1e-3)])
+@pytest.mark.parametrize("bond,cutoff", [(130, 1e-10), (100, 1e-3), (130, 0.0)])
 def test_truncate_large_matrices(q, po, eng, rng, bond, cutoff):
     # moves whose matrix does not fit LDS (e.g. 260 x 130): blocked QR with panels in LDS,
-    # U of the tall SVD moves formed the same way
+    # U of the tall SVD moves formed the same way.  Cutoff 0 with no maxdim: the
+    # tall SVD moves can drop nothing, so they keep Q R (no Jacobi); the oracle
+    # runs the SVD, and the bonds, values and error must still agree
     s = q.continuous_siteinds(q.named_grid((12, 1)))
     f = q.random_itn(s, link_space=bond, rng=rng, std=1.0 / math.sqrt(bond))
     tr, err = q.truncate(f, cutoff=cutoff, return_error=True)
