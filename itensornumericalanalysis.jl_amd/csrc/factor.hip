@@ -1793,6 +1793,161 @@ __device__ __noinline__ void tall_wy(double* M, int ldm_in, int mr_in, int c0_in
   }
 }
 
+// Two panels' trailing update in one pass over M, on this workgroup alone:
+// M <- (I - V2 T2^T V2^T)(I - V1 T1^T V1^T) M for M[0:mr, c0:c1] (global,
+// ldm).  V1: the first panel's reflectors (rows from 0), V2: the second's
+// (rows from 16, so zero above), both with leading dimension ldv.  With
+// W1 = V1^T M and W2 = V2^T M read in the same pass, U1 = T1^T W1,
+// U2 = T2^T (W2 - G U1) with G = V2^T V1, and M -= V1 U1 + V2 U2 in a second
+// pass: M is read twice and written once per 32 columns instead of per 16.
+// L: LDS [0, 16384) for the partial and final tiles; T1, G: 256 doubles each.
+__device__ __forceinline__ void wy2_p1_unit(const double* M, int ldm, int mr, int col0, int c1, const double* V1,
+                                            int nb1, const double* V2, int nb2, int ldv, int rs, int re, f64x4& w1,
+                                            f64x4& w2) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  const int col = col0 + li;
+  const bool con = col < c1;
+  f64x4 a1 = {0.0, 0.0, 0.0, 0.0}, a2 = {0.0, 0.0, 0.0, 0.0};
+  for (int r = rs; r < re; r += 16) {  // 16 rows (four steps) in flight per wait
+    double v1[4], v2[4], b[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int rr = r + 4 * q + lk;
+      const bool rin = rr < re;
+      v1[q] = rin ? vref(V1, ldv, rr, li, mr, nb1) : 0.0;
+      v2[q] = rin ? vref(V2, ldv, rr - PNB, li, mr - PNB, nb2) : 0.0;
+      b[q] = (rin && con) ? M[(int64_t)rr * ldm + col] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      a1 = __builtin_amdgcn_mfma_f64_16x16x4f64(v1[q], b[q], a1, 0, 0, 0);
+      a2 = __builtin_amdgcn_mfma_f64_16x16x4f64(v2[q], b[q], a2, 0, 0, 0);
+    }
+  }
+  w1 = a1;
+  w2 = a2;
+}
+
+__device__ __noinline__ void tall_wy2(double* M, int ldm_in, int mr_in, int c0_in, int c1_in, const double* V1,
+                                      const double* V2, int ldv_in, int nb1_in, int nb2_in, const lds_f64* T1,
+                                      const lds_f64* T2, lds_f64* G, lds_f64* L) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int ldm = uni(ldm_in), mr = uni(mr_in), c0 = uni(c0_in), c1 = uni(c1_in), ldv = uni(ldv_in),
+            nb1 = uni(nb1_in), nb2 = uni(nb2_in);
+  lds_f64* P1 = L;                // [unit][e * 64 + lane], 16 units
+  lds_f64* P2 = L + 16 * 256;
+  lds_f64* W1 = L + 32 * 256;     // [tile][j][c], 8 tiles
+  lds_f64* W2 = L + 40 * 256;
+  lds_f64* U1 = L + 48 * 256;     // -U1 [tile][j][c]
+  lds_f64* U2 = L + 56 * 256;     // -U2
+  // G = V2^T V1 over the rows from 16: partials per wavefront in P1, summed
+  {
+    f64x4 g = {0.0, 0.0, 0.0, 0.0};
+    for (int r = PNB + 4 * wv; r < mr; r += 4 * nwv) {
+      const int rr = r + lk;
+      const double a = rr < mr ? vref(V2, ldv, rr - PNB, li, mr - PNB, nb2) : 0.0;
+      const double bb = rr < mr ? vref(V1, ldv, rr, li, mr, nb1) : 0.0;
+      g = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, g, 0, 0, 0);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) P1[wv * 256 + e * 64 + lane] = g[e];
+    __syncthreads();
+    if (tid < 256) {  // element e of lane ln is G[ln / 16 + 4 e][ln % 16]
+      const int e = tid >> 6, ln = tid & 63;
+      double sum = 0.0;
+      for (int w = 0; w < nwv; ++w) sum += P1[w * 256 + tid];
+      G[((ln >> 4) + 4 * e) * 16 + (ln & 15)] = sum;
+    }
+    __syncthreads();
+  }
+  for (int cg = c0; cg < c1; cg += 128) {
+    const int nct = (min(128, c1 - cg) + 15) >> 4;
+    const int nch = max(1, min(16 / nct, (mr + 63) / 64));
+    const int nunit = nct * nch, chunk = (((mr + nch - 1) / nch) + 15) & ~15;
+    for (int u = wv; u < nunit; u += nwv) {
+      const int t = u / nch, h = u - t * nch;
+      const int rs = h * chunk, re = min(mr, rs + chunk);
+      f64x4 w1, w2;
+      wy2_p1_unit(M, ldm, mr, cg + t * 16, c1, V1, nb1, V2, nb2, ldv, rs, re, w1, w2);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        P1[u * 256 + e * 64 + lane] = w1[e];
+        P2[u * 256 + e * 64 + lane] = w2[e];
+      }
+    }
+    __syncthreads();
+    for (int x = tid; x < nct * 256; x += blockDim.x) {
+      const int t = x >> 8, y = x & 255;
+      const int e = y >> 6, ln = y & 63;
+      double s1 = 0.0, s2 = 0.0;
+      for (int h = 0; h < nch; ++h) {
+        s1 += P1[(t * nch + h) * 256 + y];
+        s2 += P2[(t * nch + h) * 256 + y];
+      }
+      const int o = t * 256 + ((ln >> 4) + 4 * e) * 16 + (ln & 15);
+      W1[o] = s1;
+      W2[o] = s2;
+    }
+    __syncthreads();
+    // U1 = T1^T W1 (stored negated)
+    for (int x = tid; x < nct * 256; x += blockDim.x) {
+      const int t = x >> 8, i = (x >> 4) & 15, c = x & 15;
+      double sum = 0.0;
+#pragma unroll
+      for (int j = 0; j < PNB; ++j) sum = __builtin_fma(T1[j * PNB + i], W1[t * 256 + j * 16 + c], sum);
+      U1[t * 256 + i * 16 + c] = -sum;
+    }
+    __syncthreads();
+    // W2 <- W2 - G U1 (U1 is stored negated)
+    for (int x = tid; x < nct * 256; x += blockDim.x) {
+      const int t = x >> 8, i = (x >> 4) & 15, c = x & 15;
+      double sum = W2[t * 256 + i * 16 + c];
+#pragma unroll
+      for (int j = 0; j < PNB; ++j) sum = __builtin_fma(G[i * 16 + j], U1[t * 256 + j * 16 + c], sum);
+      P2[t * 256 + i * 16 + c] = sum;
+    }
+    __syncthreads();
+    // U2 = T2^T W2 (stored negated)
+    for (int x = tid; x < nct * 256; x += blockDim.x) {
+      const int t = x >> 8, i = (x >> 4) & 15, c = x & 15;
+      double sum = 0.0;
+#pragma unroll
+      for (int j = 0; j < PNB; ++j) sum = __builtin_fma(T2[j * PNB + i], P2[t * 256 + j * 16 + c], sum);
+      U2[t * 256 + i * 16 + c] = -sum;
+    }
+    __syncthreads();
+    // M -= V1 U1 + V2 U2, one (tile, 16-row block) at a time
+    const int nrb = (mr + 15) >> 4;
+    for (int x = wv; x < nct * nrb; x += nwv) {
+      const int t = x / nrb, rb = (x - t * nrb) * 16;
+      const int col = cg + t * 16 + li;
+      const bool con = col < c1;
+      double b1[4], b2[4], a1[4], a2[4];
+      f64x4 cm;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        b1[q] = U1[t * 256 + (4 * q + lk) * 16 + li];
+        b2[q] = U2[t * 256 + (4 * q + lk) * 16 + li];
+        a1[q] = vref(V1, ldv, rb + li, 4 * q + lk, mr, nb1);
+        a2[q] = vref(V2, ldv, rb + li - PNB, 4 * q + lk, mr - PNB, nb2);
+        const int rr = rb + lk + 4 * q;
+        cm[q] = (con && rr < mr) ? M[(int64_t)rr * ldm + col] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) cm = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[q], b1[q], cm, 0, 0, 0);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) cm = __builtin_amdgcn_mfma_f64_16x16x4f64(a2[q], b2[q], cm, 0, 0, 0);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int rr = rb + lk + 4 * e;
+        if (con && rr < mr) M[(int64_t)rr * ldm + col] = cm[e];
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // QR of W (global rows x cols, ld) for the tall case: as qr_panel, plus every
 // panel's T in Tg (PNB x PNB each).
 __device__ __noinline__ void qr_tall(double* W, int rows_in, int cols_in, int ld_in, double* tau, double* Tg,
@@ -2072,17 +2227,38 @@ __device__ __noinline__ void qr_panel(double* W, int rows_in, int cols_in, int l
   const int k = rows < cols ? rows : cols;
   lds_f64* red = L + 8 * TALL_REGION;          // 2 x 8 x PNB sums, 2 x PNB rows
   lds_f64* Tl = L + 8 * TALL_REGION + 16 * 128;
+  // on this workgroup alone, panels go in pairs: the second panel's columns
+  // are updated by the first, and the rest by both in one pass (tall_wy2)
+  const bool pairs = !(job != nullptr && nwg > 1);
+  lds_f64* T1 = red + 512;   // the first panel's T while the second is factored
+  lds_f64* Gp = red + 768;   // V2^T V1
   QTT_PT_INIT;
-  for (int p0 = 0; p0 < k; p0 += PNB) {
-    const int nb = k - p0 < PNB ? k - p0 : PNB, mr = rows - p0;
-    if (mr <= 512)
+  auto panel = [&](int p0, int nb) {
+    if (rows - p0 <= 512)
       reg_panel<1>(W, rows, ld, p0, nb, tau + p0, Tg + (p0 / PNB) * PNB * PNB, Tl, L, red);
     else
       reg_panel<2>(W, rows, ld, p0, nb, tau + p0, Tg + (p0 / PNB) * PNB * PNB, Tl, L, red);
+  };
+  for (int p0 = 0; p0 < k;) {
+    const int nb = k - p0 < PNB ? k - p0 : PNB, mr = rows - p0;
+    panel(p0, nb);
     QTT_PT(1);
-    if (p0 + nb < cols)
-      tall_wy(W + (int64_t)p0 * ld, ld, mr, p0 + nb, cols, W + (int64_t)p0 * ld + p0, ld, nb, Tl, true, L, job, nwg,
-              iflag);
+    const int p1 = p0 + nb;
+    double* M = W + (int64_t)p0 * ld;
+    const double* V1 = W + (int64_t)p0 * ld + p0;
+    if (pairs && p1 < k && p1 + PNB < cols) {
+      const int nb2 = k - p1 < PNB ? k - p1 : PNB;
+      tall_wy(M, ld, mr, p1, p1 + nb2, V1, ld, nb, Tl, true, L, nullptr, 1, iflag);
+      __syncthreads();
+      for (int e = threadIdx.x; e < PNB * PNB; e += blockDim.x) T1[e] = Tl[e];
+      __syncthreads();
+      panel(p1, nb2);
+      tall_wy2(M, ld, mr, p1 + nb2, cols, V1, W + (int64_t)p1 * ld + p1, ld, nb, nb2, T1, Tl, Gp, L);
+      p0 = p1 + nb2;
+    } else {
+      if (p1 < cols) tall_wy(M, ld, mr, p1, cols, V1, ld, nb, Tl, true, L, job, nwg, iflag);
+      p0 = p1;
+    }
     __syncthreads();
     QTT_PT(4);
     if (threadIdx.x == 0) QTT_PANEL_COUNT;
