@@ -281,10 +281,10 @@ int qtt_ctx_create(int device, qtt_ctx** out) {
   const char* tbits = std::getenv("QTT_EVAL_TABLE_BITS");
   if (tbits) c->eval_tbits = std::max(0, std::min(24, std::atoi(tbits)));
   // QTT_TRUNC_HELPERS: helper workgroups of a single-network truncation whose
-  // cores exceed LDS (default and at most 31, the exchange buffers' size; 0:
-  // workgroup 0 alone)
+  // cores exceed LDS (default 31, at most QTT_MAX_HELPERS and one per other
+  // CU; 0: workgroup 0 alone)
   const char* th = std::getenv("QTT_TRUNC_HELPERS");
-  if (th) c->trunc_helpers = std::min(31, std::max(0, std::atoi(th)));
+  if (th) c->trunc_helpers = std::min(QTT_MAX_HELPERS, std::max(0, std::atoi(th)));
   // QTT_JACOBI_MAX_SWEEPS: sweeps before an SVD reports QTT_ENOCONV (default 100)
   const char* js = std::getenv("QTT_JACOBI_MAX_SWEEPS");
   if (js) c->jacobi_max_sweeps = std::min(1000, std::max(1, std::atoi(js)));

@@ -3836,9 +3836,9 @@ static size_t trunc_layout(qtt_net* const* nets, int nnet, int nwg, std::vector<
   }
   const size_t args_bytes = (size_t)nnet * sizeof(SweepArgs);
   // the tail slack covers the whole-block row reads of the Jacobi rounds (< 512 doubles past a row)
-  // helpers' exchange buffers after the job: Gp [32][16], Pg [512][256], Ug [8][256],
+  // helpers' exchange buffers after the job: Gp [QTT_MAX_HELPERS + 1][16], Pg [512][256], Ug [8][256],
   // then the block counters of the Jacobi sweeps (at most 128 blocks; 256 ints)
-  const size_t coop_doubles = 32 * 16 + (size_t)COOP_PG_UNITS * 256 + 8 * 256 + 128;
+  const size_t coop_doubles = (QTT_MAX_HELPERS + 1) * 16 + (size_t)COOP_PG_UNITS * 256 + 8 * 256 + 128;
   const size_t bytes = (tot_doubles + nnet + 8) * sizeof(double) + ((args_bytes + 255) & ~size_t(255)) +
                        ((tot_steps * sizeof(int) + 255) & ~size_t(255)) + 4096 + sizeof(CoopJob) + 256 +
                        (nwg > 1 ? coop_doubles * sizeof(double) : 0);
@@ -3947,7 +3947,7 @@ int truncate_batch_stage(qtt_ctx* ctx, qtt_net* const* nets, int nnet, double cu
   if (nwg > 1) {
     double* cb = (double*)((char*)d_job + ((sizeof(CoopJob) + 255) & ~size_t(255)));
     im->hjob.Gp = cb;
-    im->hjob.Pg = cb + 32 * 16;
+    im->hjob.Pg = cb + (QTT_MAX_HELPERS + 1) * 16;
     im->hjob.Ug = im->hjob.Pg + (size_t)COOP_PG_UNITS * 256;
     im->hjob.ver = (int*)(im->hjob.Ug + 8 * 256);
   }

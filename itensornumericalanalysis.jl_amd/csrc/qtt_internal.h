@@ -15,6 +15,10 @@
 #define QTT_MAX_NDIM 10     // core rank (parent + site + up to 8 children)
 #define QTT_LDS_DOUBLES 20224  // 158 KiB of the 160 KiB LDS per CU
 
+// most helper workgroups a single-network truncation can take (the size of
+// their exchange buffers)
+constexpr int QTT_MAX_HELPERS = 255;
+
 struct qtt_ctx {
   int device = 0;
   hipStream_t own_stream = nullptr;
