@@ -3386,11 +3386,12 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
     if (!conv) {
       if (tid == 0) status[1] = 1;
     }
-    for (int i = tid; i < n; i += bs) {
+    for (int i = tid; i < n; i += bs) {  // by decreasing norm (any order when all are kept)
       int r = 0;
       const double si = sig[i];
-      for (int j = 0; j < n; ++j) r += (sig[j] > si || (sig[j] == si && j < i)) ? 1 : 0;
-      perm[r] = i;
+      if (!keep_all)
+        for (int j = 0; j < n; ++j) r += (sig[j] > si || (sig[j] == si && j < i)) ? 1 : 0;
+      perm[keep_all ? i : r] = i;
     }
     __syncthreads();
     if (tid == 0) {
@@ -3422,11 +3423,11 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
     __syncthreads();
     m = iflag[1];
     Rrows = X;
-    use_perm = true;
+    use_perm = !keep_all;  // kept all: U = Q and the rows of R in order (triangular)
     // x <- U (bond m): U[row][i] = sum_j Q[row][j] J[perm[i]][j]  (or J[perm[i]][row])
     if (tall && pmode) {
-      q_form_tall(W, rows, b, b, vbuf, vbuf + (((int64_t)b + PNB) * PNB), Xc, bx, Jb, n, perm, m, pnl, job, nwg,
-                  iflag);
+      q_form_tall(W, rows, b, b, vbuf, vbuf + (((int64_t)b + PNB) * PNB), Xc, bx, keep_all ? nullptr : Jb, n,
+                  keep_all ? nullptr : perm, m, pnl, job, nwg, iflag);
     } else if (tall && use_reg) {
       const lds_f64 *Wl = (const lds_f64*)W, *taul = (const lds_f64*)tau, *Jl = (const lds_f64*)Jb;
       const lds_i32* pl = (const lds_i32*)perm;
