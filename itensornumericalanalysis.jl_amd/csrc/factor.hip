@@ -13,7 +13,9 @@
 //     * SVD move: Householder QR if the matrix is tall, then one-sided Jacobi
 //       on the rows of the (square or wide) factor with the rotations kept in J,
 //       singular values sorted, the ITensors cutoff rule applied, x <- U (an
-//       exact product of orthogonal factors), u <- S V^T (x)_bond u.
+//       exact product of orthogonal factors), u <- S V^T (x)_bond u.  A tall
+//       or square move that can drop nothing (cutoff <= 0, maxdim >= its
+//       rank, R nonsingular) keeps x <- Q, u <- R (x)_bond u instead.
 //   The matrix lives in LDS when it fits (158 KiB), otherwise in a global
 //   scratch area; all arithmetic is FP64.
 //   Work is spread over lane groups of G lanes (G = 8 for matrices of up to
