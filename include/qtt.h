@@ -183,7 +183,10 @@ int qtt_apply(qtt_ctx* ctx, const qtt_mpo* op, const qtt_net* net, double cutoff
  * ITensorNetworks.truncate): for every edge in post-order from vertex 0,
  * orthogonalise toward the child (Householder QR moves), SVD the edge
  * (one-sided Jacobi, FP64) and keep singular values by the ITensors rule on
- * squared values (relative cutoff, then maxdim).  In place. */
+ * squared values (relative cutoff, then maxdim).  An edge that can drop
+ * nothing (cutoff <= 0, maxdim >= its rank, R of its QR nonsingular) keeps
+ * its QR factors instead: the same function, bonds and error, the bond's
+ * basis differing by an orthogonal change.  In place. */
 int qtt_truncate(qtt_ctx* ctx, qtt_net* net, double cutoff, int64_t maxdim, double* trunc_err);
 
 /* ---- adjoint (no reference counterpart: the rrule of evaluate) ------------
