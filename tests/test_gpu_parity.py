@@ -343,6 +343,21 @@ def test_truncate_random(q, po, eng, rng, graph):
     assert err == pytest.approx(rerr, rel=1e-9)
 
 
+@pytest.mark.parametrize("graph,maxdim", [("chain", None), ("comb", None), ("comb", 8)])
+def test_truncate_keep_all_moves(q, po, eng, rng, graph, maxdim):
+    # cutoff 0: the tall and square SVD moves whose rank the bound does not
+    # cut keep Q R without a Jacobi (cores in LDS here); with maxdim 8 the
+    # moves of rank above 8 still take the SVD.  Against the oracle's SVD.
+    g = q.named_grid((14, 1)) if graph == "chain" else q.named_comb_tree((3, 5))
+    s = q.continuous_siteinds(g)
+    f = q.random_itn(s, link_space=12, rng=rng)
+    tr, err = q.truncate(f, cutoff=0.0, maxdim=maxdim, return_error=True)
+    ref, rerr = po.truncate(f.topo.parent, f.topo.sdim, f.cores, 0.0, maxdim or (1 << 30))
+    assert list(tr.bonds()) == [c.shape[0] for c in ref]
+    assert _compare_networks(q, po, tr, ref, f, rng) < 1e-11
+    assert err == pytest.approx(rerr, rel=1e-9, abs=1e-24)
+
+
 @pytest.mark.parametrize("bond,cutoff", [(130, 1e-10), (100, 1e-3), (130, 0.0)])
 def test_truncate_large_matrices(q, po, eng, rng, bond, cutoff):
     # moves whose matrix does not fit LDS (e.g. 260 x 130): blocked QR with panels in LDS,
