@@ -1832,7 +1832,7 @@ __device__ __forceinline__ void wy2_p1_unit(const double* M, int ldm, int mr, in
 
 __device__ __noinline__ void tall_wy2(double* M, int ldm_in, int mr_in, int c0_in, int c1_in, const double* V1,
                                       const double* V2, int ldv_in, int nb1_in, int nb2_in, const lds_f64* T1,
-                                      const lds_f64* T2, lds_f64* G, lds_f64* L) {
+                                      const lds_f64* T2, lds_f64* G, lds_f64* L, bool trans, bool rev) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const int ldm = uni(ldm_in), mr = uni(mr_in), c0 = uni(c0_in), c1 = uni(c1_in), ldv = uni(ldv_in),
@@ -1892,31 +1892,40 @@ __device__ __noinline__ void tall_wy2(double* M, int ldm_in, int mr_in, int c0_i
       W2[o] = s2;
     }
     __syncthreads();
-    // U1 = T1^T W1 (stored negated)
+    // the panel applied first (V1, or V2 if rev): U_f = op(T_f) W_f; then
+    // W_s <- W_s - (V_s^T V_f) U_f and U_s = op(T_s) W_s for the second, with
+    // V_s^T V_f = G (or G^T if rev); op(T) = T^T if trans.  U stored negated.
+    const lds_f64* Tf = rev ? T2 : T1;
+    const lds_f64* Ts = rev ? T1 : T2;
+    lds_f64* Wf = rev ? W2 : W1;
+    lds_f64* Ws = rev ? W1 : W2;
+    lds_f64* Uf = rev ? U2 : U1;
+    lds_f64* Us = rev ? U1 : U2;
     for (int x = tid; x < nct * 256; x += blockDim.x) {
       const int t = x >> 8, i = (x >> 4) & 15, c = x & 15;
       double sum = 0.0;
 #pragma unroll
-      for (int j = 0; j < PNB; ++j) sum = __builtin_fma(T1[j * PNB + i], W1[t * 256 + j * 16 + c], sum);
-      U1[t * 256 + i * 16 + c] = -sum;
+      for (int j = 0; j < PNB; ++j)
+        sum = __builtin_fma(trans ? Tf[j * PNB + i] : Tf[i * PNB + j], Wf[t * 256 + j * 16 + c], sum);
+      Uf[t * 256 + i * 16 + c] = -sum;
     }
     __syncthreads();
-    // W2 <- W2 - G U1 (U1 is stored negated)
-    for (int x = tid; x < nct * 256; x += blockDim.x) {
+    for (int x = tid; x < nct * 256; x += blockDim.x) {  // Uf is stored negated
       const int t = x >> 8, i = (x >> 4) & 15, c = x & 15;
-      double sum = W2[t * 256 + i * 16 + c];
+      double sum = Ws[t * 256 + i * 16 + c];
 #pragma unroll
-      for (int j = 0; j < PNB; ++j) sum = __builtin_fma(G[i * 16 + j], U1[t * 256 + j * 16 + c], sum);
+      for (int j = 0; j < PNB; ++j)
+        sum = __builtin_fma(rev ? G[j * 16 + i] : G[i * 16 + j], Uf[t * 256 + j * 16 + c], sum);
       P2[t * 256 + i * 16 + c] = sum;
     }
     __syncthreads();
-    // U2 = T2^T W2 (stored negated)
     for (int x = tid; x < nct * 256; x += blockDim.x) {
       const int t = x >> 8, i = (x >> 4) & 15, c = x & 15;
       double sum = 0.0;
 #pragma unroll
-      for (int j = 0; j < PNB; ++j) sum = __builtin_fma(T2[j * PNB + i], P2[t * 256 + j * 16 + c], sum);
-      U2[t * 256 + i * 16 + c] = -sum;
+      for (int j = 0; j < PNB; ++j)
+        sum = __builtin_fma(trans ? Ts[j * PNB + i] : Ts[i * PNB + j], P2[t * 256 + j * 16 + c], sum);
+      Us[t * 256 + i * 16 + c] = -sum;
     }
     __syncthreads();
     // M -= V1 U1 + V2 U2, one (tile, 16-row block) at a time
@@ -1989,13 +1998,29 @@ __device__ __noinline__ void q_form_tall(const double* W, int rows_in, int k_in,
     Mq[e] = Jb ? (r < n ? Jb[(int64_t)(perm ? perm[c] : c) * n + r] : 0.0) : (r == c ? 1.0 : 0.0);
   }
   __syncthreads();
-  for (int p0 = ((k - 1) / PNB) * PNB; p0 >= 0; p0 -= PNB) {
+  // panels in pairs (tall_wy2, one pass over Mq per two panels) where every
+  // update would run on this workgroup alone anyway
+  const bool pairs = !(job != nullptr && nwg > 1) || (int64_t)rows * ncol < WY_COLS_MIN;
+  lds_f64* T1 = L + 8 * TALL_REGION + 512;  // the earlier panel's T of a pair
+  lds_f64* Gp = L + 8 * TALL_REGION + 768;  // V_later^T V_earlier
+  for (int p0 = ((k - 1) / PNB) * PNB; p0 >= 0;) {
     const int nb = k - p0 < PNB ? k - p0 : PNB;
     for (int e = tid; e < PNB * PNB; e += bs) Tl[e] = Tg[(p0 / PNB) * PNB * PNB + e];
-    __syncthreads();
-    // Q: only columns >= p0 differ from the identity yet; U: every column
-    tall_wy(Mq + (int64_t)p0 * ncol, ncol, rows - p0, Jb ? 0 : p0, ncol, W + (int64_t)p0 * ld + p0, ld, nb, Tl,
-            false, L, job, nwg, iflag);
+    if (pairs && p0 >= PNB) {
+      // panels pa = p0 - 16 (V1) and p0 (V2, 16 rows further), p0's applied first
+      const int pa = p0 - PNB;
+      for (int e = tid; e < PNB * PNB; e += bs) T1[e] = Tg[(pa / PNB) * PNB * PNB + e];
+      __syncthreads();
+      tall_wy2(Mq + (int64_t)pa * ncol, ncol, rows - pa, Jb ? 0 : pa, ncol, W + (int64_t)pa * ld + pa,
+               W + (int64_t)p0 * ld + p0, ld, PNB, nb, T1, Tl, Gp, L, false, true);
+      p0 = pa - PNB;
+    } else {
+      __syncthreads();
+      // Q: only columns >= p0 differ from the identity yet; U: every column
+      tall_wy(Mq + (int64_t)p0 * ncol, ncol, rows - p0, Jb ? 0 : p0, ncol, W + (int64_t)p0 * ld + p0, ld, nb, Tl,
+              false, L, job, nwg, iflag);
+      p0 -= PNB;
+    }
   }
   for (int64_t e = tid; e < (int64_t)rows * ncol; e += bs) {
     const int bb = (int)(e % bx);
@@ -2255,7 +2280,7 @@ __device__ __noinline__ void qr_panel(double* W, int rows_in, int cols_in, int l
       for (int e = threadIdx.x; e < PNB * PNB; e += blockDim.x) T1[e] = Tl[e];
       __syncthreads();
       panel(p1, nb2);
-      tall_wy2(M, ld, mr, p1 + nb2, cols, V1, W + (int64_t)p1 * ld + p1, ld, nb, nb2, T1, Tl, Gp, L);
+      tall_wy2(M, ld, mr, p1 + nb2, cols, V1, W + (int64_t)p1 * ld + p1, ld, nb, nb2, T1, Tl, Gp, L, true, false);
       p0 = p1 + nb2;
     } else {
       if (p1 < cols) tall_wy(M, ld, mr, p1, cols, V1, ld, nb, Tl, true, L, job, nwg, iflag);
