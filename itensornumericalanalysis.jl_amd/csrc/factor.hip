@@ -1449,6 +1449,7 @@ __device__ bool coop_post_run(CoopJob* job, int nwg, int* iflag, lds_f64* lw, in
 __device__ int coop_post(CoopJob* job, int* iflag, int nact);
 __device__ bool coop_wait(CoopJob* job, int s, int nwg, int* iflag);
 __device__ void coop_wy_cols(CoopJob* job, int wg, int nwg, lds_f64* lw);
+__device__ void gather_tiles(const double* X, double* W, int na, int b, int bx, int wg, int nwg, lds_f64* lw);
 
 // The two kinds of pass over panel rows [r0, r1) (lane layout: a 16-lane group
 // per row, a lane per column c = lane & 15; rows step by 4 per wavefront).
@@ -1993,9 +1994,12 @@ __device__ __noinline__ void q_form_tall(const double* W, int rows_in, int k_in,
             nwg = uni(nwg_in);
   const int ncol = Jb ? m : k;
   lds_f64* Tl = L + 8 * TALL_REGION + 16 * 128;
-  for (int64_t e = tid; e < (int64_t)rows * ncol; e += bs) {
-    const int r = (int)(e / ncol), c = (int)(e - (int64_t)r * ncol);
-    Mq[e] = Jb ? (r < n ? Jb[(int64_t)(perm ? perm[c] : c) * n + r] : 0.0) : (r == c ? 1.0 : 0.0);
+  {  // a row per wavefront, the columns over its lanes (no divisions per entry)
+    const int lane = tid & 63, wv = tid >> 6, nwv = bs >> 6;
+    for (int r = wv; r < rows; r += nwv)
+      for (int c = lane; c < ncol; c += 64)
+        Mq[(int64_t)r * ncol + c] =
+            Jb ? (r < n ? Jb[(int64_t)(perm ? perm[c] : c) * n + r] : 0.0) : (r == c ? 1.0 : 0.0);
   }
   __syncthreads();
   // panels in pairs (tall_wy2, one pass over Mq per two panels) where every
@@ -2022,11 +2026,23 @@ __device__ __noinline__ void q_form_tall(const double* W, int rows_in, int k_in,
       p0 -= PNB;
     }
   }
-  for (int64_t e = tid; e < (int64_t)rows * ncol; e += bs) {
-    const int bb = (int)(e % bx);
-    const int64_t t2 = e / bx;
-    const int c = (int)(t2 % ncol), aa = (int)(t2 / ncol);
-    Xc[e] = Mq[(int64_t)(aa * bx + bb) * ncol + c];
+  // Xc[aa] (ncol x bx) = Mq[aa]^T (Mq[aa]: bx x ncol): tiled transposes, with
+  // the helper workgroups for a large result
+  if (bx == 1) {
+    for (int e = tid; e < rows * ncol; e += bs) Xc[e] = Mq[e];
+  } else if (job != nullptr && nwg > 1 && (int64_t)rows * ncol >= (int64_t(1) << 15)) {
+    if (tid == 0) {
+      job->type = 10;
+      job->U = Mq;
+      job->D = Xc;
+      job->Au = rows / bx;
+      job->rb = bx;
+      job->Bu = ncol;
+    }
+    const int64_t ntile = (int64_t)(rows / bx) * ((bx + 63) >> 6) * ((ncol + 63) >> 6);
+    coop_post_run(job, nwg, iflag, L, (int)min<int64_t>(ntile, nwg));
+  } else {
+    gather_tiles(Mq, Xc, rows / bx, bx, ncol, 0, 1, L);
   }
   __syncthreads();
 }
@@ -3240,10 +3256,9 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
         // k^2) and multiplied later (2 rows k m).  (In place, k = b: they are
         // there already.)
         if (!inplace) {
-          for (int64_t e = tid; e < (int64_t)rows * k; e += bs) {
-            const int r = (int)(e / k), c = (int)(e - (int64_t)r * k);
-            Xc[e] = W[(int64_t)r * b + c];
-          }
+          const int lane = tid & 63, wv = tid >> 6, nwv = bs >> 6;
+          for (int r = wv; r < rows; r += nwv)
+            for (int c = lane; c < k; c += 64) Xc[(int64_t)r * k + c] = W[(int64_t)r * b + c];
         }
         for (int e = tid; e < (k + PNB - 1) / PNB * PNB * PNB; e += bs) lazyT[e] = Tg[e];
       } else {
