@@ -3100,10 +3100,16 @@ __device__ __noinline__ bool jacobi_precond(double* X, int n_in, int b_in, doubl
                                             int max_sweeps) {
   const int tid = threadIdx.x, bs = blockDim.x;
   const int n = uni(n_in), b = uni(b_in), nwg = uni(nwg_in);
-  for (int64_t e = tid; e < (int64_t)n * b; e += bs) {  // A = X^T
-    const int i = (int)(e / b), c = (int)(e - (int64_t)i * b);
-    A[(int64_t)c * n + i] = X[e];
+  // A = X^T: tiled transposes shared with the helper workgroups
+  if (tid == 0) {
+    job->type = 10;
+    job->U = X;
+    job->D = A;
+    job->Au = 1;
+    job->rb = n;
+    job->Bu = b;
   }
+  coop_post_run(job, nwg, iflag, pnl, (int)min<int64_t>((int64_t)((n + 63) >> 6) * ((b + 63) >> 6), nwg));
   __syncthreads();
   if (b * PLD <= 8 * TALL_REGION) qr_panel(A, b, n, n, tau, Tg1, pnl, job, nwg, iflag);
   else qr_tall(A, b, n, n, tau, Tg1, pnl, job, nwg, iflag);
