@@ -1449,6 +1449,7 @@ __device__ bool coop_post_run(CoopJob* job, int nwg, int* iflag, lds_f64* lw, in
 __device__ int coop_post(CoopJob* job, int* iflag, int nact);
 __device__ bool coop_wait(CoopJob* job, int s, int nwg, int* iflag);
 __device__ void coop_wy_cols(CoopJob* job, int wg, int nwg, lds_f64* lw);
+__device__ void coop_wy_rows(CoopJob* job, int wg, int nwg);
 __device__ void gather_tiles(const double* X, double* W, int na, int b, int bx, int wg, int nwg, lds_f64* lw);
 
 // The two kinds of pass over panel rows [r0, r1) (lane layout: a 16-lane group
@@ -1682,6 +1683,38 @@ __device__ __forceinline__ void wy_p3_unit(double* M, int ldm, int mr, int col0,
   }
 }
 
+// Two wy_p3_unit's with all loads of both issued before the MFMAs.
+template <typename UP>
+__device__ __forceinline__ void wy_p3_unit2(double* M, int ldm, int mr, int col0a, int rba, UP Uta, int col0b,
+                                            int rbb, UP Utb, int c1, const double* V, int ldv, int nb) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  const int cola = col0a + li, colb = col0b + li;
+  const bool cona = cola < c1, conb = colb < c1;
+  double bua[4], aa[4], bub[4], ab[4];
+  f64x4 cma, cmb;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    bua[q] = Uta[(4 * q + lk) * 16 + li];
+    bub[q] = Utb[(4 * q + lk) * 16 + li];
+    aa[q] = vref(V, ldv, rba + li, 4 * q + lk, mr, nb);
+    ab[q] = vref(V, ldv, rbb + li, 4 * q + lk, mr, nb);
+    const int ra = rba + lk + 4 * q, rb = rbb + lk + 4 * q;
+    cma[q] = (cona && ra < mr) ? M[(int64_t)ra * ldm + cola] : 0.0;
+    cmb[q] = (conb && rb < mr) ? M[(int64_t)rb * ldm + colb] : 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    cma = __builtin_amdgcn_mfma_f64_16x16x4f64(aa[q], bua[q], cma, 0, 0, 0);
+    cmb = __builtin_amdgcn_mfma_f64_16x16x4f64(ab[q], bub[q], cmb, 0, 0, 0);
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int ra = rba + lk + 4 * e, rb = rbb + lk + 4 * e;
+    if (cona && ra < mr) M[(int64_t)ra * ldm + cola] = cma[e];
+    if (conb && rb < mr) M[(int64_t)rb * ldm + colb] = cmb[e];
+  }
+}
+
 constexpr int64_t WY_COOP_MIN = int64_t(1) << 15;  // entries of a row-chunk (two-job) shared update
 constexpr int64_t WY_COLS_MIN = int64_t(1) << 13;  // entries of a column-tile (one-job) shared update
 
@@ -1745,7 +1778,11 @@ __device__ __noinline__ void tall_wy(double* M, int ldm_in, int mr_in, int c0_in
         job->wy_nch = nch;
         job->wy_chunk = chunk;
       }
-      coop_post_run(job, nwg, iflag, nullptr);
+      {
+        const int s = coop_post(job, iflag, nwg);
+        coop_wy_rows(job, 0, nwg);
+        coop_wait(job, s, nwg, iflag);
+      }
     } else {
       for (int u = wv; u < nunit; u += nwv) {
         const int t = u / nch, h = u - t * nch;
@@ -1784,7 +1821,11 @@ __device__ __noinline__ void tall_wy(double* M, int ldm_in, int mr_in, int c0_in
     const int nrb = (mr + 15) >> 4;
     if (coop) {
       if (tid == 0) job->type = 7;  // the other wy_* fields stay from the first phase
-      coop_post_run(job, nwg, iflag, nullptr);
+      {
+        const int s = coop_post(job, iflag, nwg);
+        coop_wy_rows(job, 0, nwg);
+        coop_wait(job, s, nwg, iflag);
+      }
     } else {
       for (int x = wv; x < nct * nrb; x += nwv) {
         const int t = x / nrb, rb = (x - t * nrb) * 16;
@@ -2869,26 +2910,6 @@ __device__ void coop_work(CoopJob* job, int wg, int nwg, lds_f64* lw) {
     }
     tall_reduce16(acc, lw, lw + 128);
     if (tid < PNB) job->Gp[wg * PNB + tid] = lw[128 + tid];
-  } else if (type == 6) {  // W partials of the units of this workgroup's wavefronts
-    const int lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
-    const int nch = job->wy_nch, chunk = job->wy_chunk, mr = job->wy_mr;
-    const int nunit = job->wy_nct * nch;
-    for (int u = wg * nwv + wv; u < nunit; u += nwg * nwv) {
-      const int t = u / nch, h = u - t * nch;
-      const int rs = h * chunk, re = min(mr, rs + chunk);
-      const f64x4 acc = wy_p1_unit(job->wy_M, job->wy_ldm, mr, job->wy_cg + t * 16, job->wy_c1, job->wy_V,
-                                   job->wy_ldv, job->nb, rs, re);
-#pragma unroll
-      for (int e = 0; e < 4; ++e) job->Pg[(int64_t)u * 256 + e * 64 + lane] = acc[e];
-    }
-  } else if (type == 7) {  // M -= V U on the (tile, 16-row block) units of this workgroup's wavefronts
-    const int wv = tid >> 6, nwv = blockDim.x >> 6;
-    const int mr = job->wy_mr, nrb = (mr + 15) >> 4, nct = job->wy_nct;
-    for (int x = wg * nwv + wv; x < nct * nrb; x += nwg * nwv) {
-      const int t = x / nrb, rb = (x - t * nrb) * 16;
-      wy_p3_unit(job->wy_M, job->wy_ldm, mr, job->wy_cg + t * 16, job->wy_c1, job->wy_V, job->wy_ldv, job->nb, rb,
-                 (const double*)(job->Ug + t * 256));
-    }
   }
   __syncthreads();  // every wavefront's writes are done before thread 0 releases them
   if (tid == 0) __hip_atomic_fetch_add(&job->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -2959,6 +2980,70 @@ __device__ void coop_wy_cols(CoopJob* job, int wg, int nwg, lds_f64* lw) {
   if (threadIdx.x == 0) __hip_atomic_fetch_add(&job->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// W partial of one (tile, row chunk) unit as wy_p1_unit, 32 rows (eight
+// steps) in flight per wait: the row-chunk jobs stream tall matrices from
+// memory, one unit per wavefront.
+__device__ __forceinline__ f64x4 wy_p1_unit32(const double* M, int ldm, int mr, int col0, int c1, const double* V,
+                                              int ldv, int nb, int rs, int re) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  const int col = col0 + li;
+  const bool con = col < c1;
+  f64x4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+  int r = rs;
+  for (; r + 32 <= re; r += 32) {
+    double a[8], b[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int rr = r + 4 * q + lk;
+      a[q] = vref(V, ldv, rr, li, mr, nb);
+      b[q] = con ? M[(int64_t)rr * ldm + col] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q += 2) {
+      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], b[q], acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q + 1], b[q + 1], acc1, 0, 0, 0);
+    }
+  }
+  return acc0 + acc1 + wy_p1_unit(M, ldm, mr, col0, c1, V, ldv, nb, r, re);
+}
+
+// This workgroup's share of a row-chunk WY job (type 6: the W partials of its
+// wavefronts' units; type 7: M -= V U on its (tile, 16-row block) units, two
+// per wait), then its completion count.  Kept out of coop_work, which the
+// sweep calls in many places, so that its registers stay few.
+__device__ __noinline__ void coop_wy_rows(CoopJob* job, int wg, int nwg) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
+  const int mr = job->wy_mr;
+  if (job->type == 6) {
+    const int nch = job->wy_nch, chunk = job->wy_chunk;
+    const int nunit = job->wy_nct * nch;
+    for (int u = wg * nwv + wv; u < nunit; u += nwg * nwv) {
+      const int t = u / nch, h = u - t * nch;
+      const int rs = h * chunk, re = min(mr, rs + chunk);
+      const f64x4 acc = wy_p1_unit32(job->wy_M, job->wy_ldm, mr, job->wy_cg + t * 16, job->wy_c1, job->wy_V,
+                                     job->wy_ldv, job->nb, rs, re);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) job->Pg[(int64_t)u * 256 + e * 64 + lane] = acc[e];
+    }
+  } else {
+    const int nrb = (mr + 15) >> 4, nct = job->wy_nct, nu = nct * nrb, stride = nwg * nwv;
+    for (int x = wg * nwv + wv; x < nu; x += 2 * stride) {
+      const int t = x / nrb, rb = (x - t * nrb) * 16, x2 = x + stride;
+      if (x2 < nu) {
+        const int t2 = x2 / nrb, rb2 = (x2 - t2 * nrb) * 16;
+        wy_p3_unit2(job->wy_M, job->wy_ldm, mr, job->wy_cg + t * 16, rb, (const double*)(job->Ug + t * 256),
+                    job->wy_cg + t2 * 16, rb2, (const double*)(job->Ug + t2 * 256), job->wy_c1, job->wy_V,
+                    job->wy_ldv, job->nb);
+      } else {
+        wy_p3_unit(job->wy_M, job->wy_ldm, mr, job->wy_cg + t * 16, job->wy_c1, job->wy_V, job->wy_ldv, job->nb,
+                   rb, (const double*)(job->Ug + t * 256));
+      }
+    }
+  }
+  __syncthreads();
+  if (tid == 0) __hip_atomic_fetch_add(&job->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Helper workgroups: take every posted job they take part in until
 // workgroup 0 quits.  A job of workgroups 0 .. nact-1 is not followed by the
 // next before they have all counted it, so a helper that takes part always
@@ -2995,6 +3080,7 @@ __device__ void coop_helper(CoopJob* job, int wg, int nwg, lds_i32* sflag, lds_f
     if (!part) continue;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     if (job->type == 11) coop_wy_cols(job, wg, nwg, lw);
+    else if (job->type == 6 || job->type == 7) coop_wy_rows(job, wg, nwg);
     else coop_work(job, wg, nwg, lw);
   }
 }
