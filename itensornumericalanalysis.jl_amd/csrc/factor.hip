@@ -3347,7 +3347,16 @@ __device__ __forceinline__ int run_move(double* ws, const bool gws, CoopJob* job
         // flops for m columns) instead of Q being formed here (about 2 rows
         // k^2) and multiplied later (2 rows k m).  (In place, k = b: they are
         // there already.)
-        if (!inplace) {
+        if (!inplace && k == b && job != nullptr && nwg > 1 && (int64_t)rows * k >= (int64_t(1) << 15)) {
+          // a plain copy, spread over the helper workgroups
+          if (tid == 0) {
+            job->type = 3;
+            job->D = Xc;
+            job->Y = W;
+            job->ytot = (long long)rows * k;
+          }
+          coop_post_run(job, nwg, iflag, pnl_any);
+        } else if (!inplace) {
           const int lane = tid & 63, wv = tid >> 6, nwv = bs >> 6;
           for (int r = wv; r < rows; r += nwv)
             for (int c = lane; c < k; c += 64) Xc[(int64_t)r * k + c] = W[(int64_t)r * b + c];
